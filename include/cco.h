@@ -1,0 +1,142 @@
+/*
+ * cco.h -- C ABI of the MI355X (gfx950) stealthy collective-coordinate library.
+ *
+ * Plain pointers and sizes only.  Every function returns 0 on success or one
+ * of the CCO_ERR_* codes; cco_last_error() gives the message of the last
+ * failure on the calling thread.  Host pointers ("host") are C-contiguous
+ * float64/int64 buffers borrowed for the duration of the call; device
+ * pointers ("dev") are device allocations on the context's GPU and calls on
+ * them are asynchronous on the context stream.
+ *
+ * Each declaration names the reference interface it replaces
+ * (paths are in the sdawley1/xla-cco tree).
+ */
+#ifndef CCO_H
+#define CCO_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define CCO_OK 0
+#define CCO_ERR_ARG 1       /* bad argument or shape  -> ValueError   */
+#define CCO_ERR_DTYPE 2     /* not float64            -> TypeError    */
+#define CCO_ERR_DEVICE 3    /* HIP runtime failure    -> RuntimeError */
+#define CCO_ERR_BOUND 4     /* relax round bound hit  -> RuntimeError */
+#define CCO_ERR_STATE 5     /* call out of sequence   -> RuntimeError */
+
+#define CCO_METHOD_DESCENT 0
+#define CCO_METHOD_LBFGS 1
+
+typedef struct cco_ctx cco_ctx;
+typedef struct cco_energy cco_energy;
+typedef struct cco_relax cco_relax;
+
+/* Library version (major*10000 + minor*100 + patch). */
+int cco_version(void);
+
+/* Message of the last failing call on this thread ("" if none). */
+const char* cco_last_error(void);
+
+/* Number of visible GPUs.  Replaces the device probe of
+ * src/xla_cco/_device.py:137 (require_gpu); no CPU fallback exists. */
+int cco_device_count(int* count);
+
+/* One HIP stream on one device; calls on a context are serialised and a
+ * context is not thread-safe. */
+int cco_ctx_create(int device, cco_ctx** out);
+int cco_ctx_destroy(cco_ctx* ctx);
+int cco_ctx_synchronize(cco_ctx* ctx);
+
+/* Device memory helpers for the *_dev entry points. */
+int cco_dev_alloc(cco_ctx* ctx, size_t bytes, void** out);
+int cco_dev_free(cco_ctx* ctx, void* ptr);
+int cco_memcpy_h2d(cco_ctx* ctx, void* dst_dev, const void* src_host, size_t bytes);
+int cco_memcpy_d2h(cco_ctx* ctx, void* dst_host, const void* src_dev, size_t bytes);
+
+/* Upload a half-space k-set: n (M x d int64 host), weights w (M host),
+ * box volume, padding multiple.  Replaces make_stealthy_energy
+ * (src/xla_cco/energy.py:361-414) and the StealthyEnergy fields (:158-162).
+ * M == 0 -> CCO_ERR_ARG ("empty k-set"). */
+int cco_energy_create(cco_ctx* ctx, const int64_t* n, const double* w, int M, int d,
+                      double volume, int pad_to, cco_energy** out);
+int cco_energy_destroy(cco_energy* e);
+/* M, M_pad, d, volume, max |n_a| per axis. */
+int cco_energy_info(const cco_energy* e, int* M, int* M_pad, int* d, double* volume,
+                    int* nmax3);
+
+/* Batched Phi, dPhi/ds, Re rho, Im rho for B configurations of N particles.
+ * s: B x N x d; phi: B; grad: B x N x d (NULL: energy only); re, im:
+ * B x M_pad (NULL: not returned; padding rows give re = N, im = 0).
+ * Replaces StealthyEnergy.value_grad_density / value_and_grad / __call__
+ * (src/xla_cco/energy.py:186-225) and collective_density (:327-358). */
+int cco_value_grad_density(cco_energy* e, const double* s_host, int B, int N,
+                           double* phi_host, double* grad_host, double* re_host,
+                           double* im_host);
+int cco_value_grad_density_dev(cco_energy* e, const double* s_dev, int B, int N,
+                               double* phi_dev, double* grad_dev, double* re_dev,
+                               double* im_dev);
+
+/* Settings of the two minimizers (src/xla_cco/minimize/descent.py:52-63,
+ * src/xla_cco/minimize/lbfgs.py:55-67). */
+typedef struct {
+  double step_init, step_min, shrink, grow, rel_tol, energy_tol;
+  int64_t max_evals;
+  int32_t record_every, max_records;
+} cco_descent_settings;
+
+typedef struct {
+  int32_t memory;
+  int32_t n_rescales;
+  double rescales[8];
+  int64_t max_evals_per_pass;
+  double energy_tol, grad_tol;
+  int32_t stagnation_patience, max_linesearch_steps;
+  int32_t record_every, max_records;
+} cco_lbfgs_settings;
+
+/* A batched relaxation of B configurations (N particles, d dims) held on the
+ * device.  e == NULL selects an external energy: the caller evaluates the
+ * trial points itself (cco_relax_pending / cco_relax_submit).  Replaces
+ * relax (src/xla_cco/minimize/__init__.py:102-151) with init_batch /
+ * chunk_batch / finalize_batch of descent.py:204-215 and lbfgs.py:336-350. */
+int cco_relax_create(cco_ctx* ctx, cco_energy* e, int B, int N, int d, int method,
+                     const cco_descent_settings* ds, const cco_lbfgs_settings* ls,
+                     cco_relax** out);
+int cco_relax_destroy(cco_relax* r);
+/* Load starting points (B x N x d) and mark every lane for its first evaluation. */
+int cco_relax_init(cco_relax* r, const double* s0_host);
+int cco_relax_init_dev(cco_relax* r, const double* s0_dev);
+/* Built-in energy: run until no lane is running, syncing with the host every
+ * sync_every evaluations; returns CCO_ERR_BOUND if max_evals evaluations
+ * pass first (the reference's round bound). */
+int cco_relax_run(cco_relax* r, int64_t max_evals, int sync_every);
+/* Same, stopping after at most n_evals evaluations; *n_active = lanes left. */
+int cco_relax_step(cco_relax* r, int64_t n_evals, int* n_active);
+/* External energy: lanes awaiting an evaluation and their trial points
+ * (lanes: n_active ints, s_trial: n_active x N x d), then submit phi and
+ * grad (dPhi/ds) for those lanes in the same order. */
+int cco_relax_pending(cco_relax* r, int* n_active, int32_t* lanes_host, double* s_trial_host);
+int cco_relax_submit(cco_relax* r, const double* phi_host, const double* grad_host);
+/* Results in input order: wrapped s (B x N x d), energy, grad_norm, n_evals,
+ * n_iters, status (codes of src/xla_cco/minimize/result.py:35-42). */
+int cco_relax_result(cco_relax* r, double* s_host, double* energy_host, double* grad_norm_host,
+                     int64_t* n_evals_host, int64_t* n_iters_host, int64_t* status_host);
+int cco_relax_result_dev(cco_relax* r, double* s_dev, double* energy_dev,
+                         double* grad_norm_dev, int64_t* n_evals_dev, int64_t* n_iters_dev,
+                         int64_t* status_dev);
+/* Recorded rows per lane (count), then iteration, s (wrapped), energy, step
+ * for B x max_records rows (src/xla_cco/minimize/_loop.py:189-248). */
+int cco_relax_trajectory(cco_relax* r, int64_t* count_host, int64_t* iteration_host,
+                         double* s_host, double* energy_host, double* step_host);
+/* Total energy evaluations issued so far (sum over lanes of evaluation steps). */
+int cco_relax_stats(cco_relax* r, int64_t* steps, int64_t* lane_evals);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* CCO_H */

@@ -1,0 +1,653 @@
+"""NumPy restatement of the stealthy hot path -- TEST INFRASTRUCTURE ONLY.
+
+Only ``tests/``, ``__graft_entry__.smoke()`` and ``bench.py``'s
+``cpu_baseline`` leg may import this module, and only as the checker.
+
+Each function cites the reference file:line (paths relative to the
+sdawley1/xla-cco tree) whose algorithm it restates.
+
+Parity pins (what checks each function; tests/test_oracle_*.py):
+
+* ``halfspace_kvectors`` / ``ka_from_chi`` / ``k_outer_from_ka``: the
+  reference's known answers (Ka=0.5963271031078479, K=2.6093, M=13 at d=3
+  N=100; M=28/61/89/101/128 at chi 0.1/0.2/0.3/0.35/0.45), the C++ K bits
+  ``0x1.24329cee12d94p+0`` and the 50 n-vectors of the C++ MD fixture, and the
+  closed-form count of a cubic lattice shell.
+* ``energy_grad`` / ``energy_loops``: the C++ energies of the 15 stored MD
+  configurations (tests/golden/md_cxx_*.npz, relative 1e-12), the
+  single-particle closed form Phi = sum(w)/V, a lattice below its first shell
+  (Phi ~ 0), a central finite difference of the energy, translation
+  invariance (sum of gradients ~ 0) and the C++ Cartesian force relation.
+* ``descent``: an independent transcription of the C++ loop (the one that
+  recomputes f1 and g every iteration, test_descent.py:52-89), the 40-step
+  floor (1e-3 * 0.5^40 < 1e-15) and the budget rules (max_evals // 2).
+* ``lbfgs_precondition``: the dense BFGS inverse-Hessian recursion
+  H <- (I - rho s y^T) H (I - rho y s^T) + rho s s^T from H0 = gamma I
+  (a closed form the two-loop recursion must equal).
+* ``cubicmin`` / ``quadmin``: exact minimisers of polynomials built with known
+  roots, and scipy.optimize's ``_cubicmin``/``_quadmin``.
+* ``zoom_linesearch``: the accepted step satisfies the strong-Wolfe or the
+  approximate-decrease condition it claims; a convex quadratic's exact
+  minimiser along the line is bracketed.
+* ``lbfgs``: the reference's outcome-level tests (SPD quadratic to 1e-18 and
+  within 1e-8 of the minimum, d=2 N=64 chi=0.25 below 1e-20, three passes equal
+  three chained single passes, exact evaluation count, stagnation and
+  line-search-failure statuses).  The per-iterate arithmetic inside optax
+  0.2.8 (not vendored, not installed) is **parity unpinned**: this oracle
+  restates its published algorithm (Nocedal & Wright Alg. 3.5/3.6 with
+  Hager-Zhang approximate decrease) and the HIP path is checked against this
+  restatement.
+"""
+
+from __future__ import annotations
+
+import dataclasses
+import itertools
+import math
+from typing import Callable, Dict, List, Optional, Tuple
+
+import numpy as np
+
+TWO_PI = 2.0 * math.pi
+
+STATUS_RUNNING = 0
+STATUS_ENERGY_TOL = 1
+STATUS_STEP_FLOOR = 2
+STATUS_BUDGET = 3
+STATUS_STAGNATED = 4
+STATUS_LINESEARCH_FAIL = 5
+STATUS_GRAD_TOL = 6
+STATUS_DRIFT = 7
+
+ValueGrad = Callable[[np.ndarray], Tuple[float, np.ndarray]]
+
+# ============================================================
+# K-SET (kvectors.py:60-169, tests/oracle.py:106-150)
+# ============================================================
+
+_CPP_UNIT_BALL = {1: 2.0, 2: math.pi, 3: 4.1887902047863909846, 4: 4.9348022005446793094}
+
+
+def unit_ball_volume(dim: int) -> float:
+    """C++ HyperSphere_C literals (kvectors.py:131-149)."""
+    if dim in _CPP_UNIT_BALL:
+        return _CPP_UNIT_BALL[dim]
+    return float(math.pi ** (dim / 2.0) / math.gamma(dim / 2.0 + 1.0))
+
+
+def k_outer_from_ka(ka: float, phi: float, dim: int) -> float:
+    """kvectors.py:152-156."""
+    return float(ka / (phi / unit_ball_volume(dim)) ** (1.0 / dim))
+
+
+def ka_from_chi(chi: float, phi: float, dim: int) -> float:
+    """kvectors.py:159-163."""
+    arg = 2.0 ** (dim + 1) * dim * chi * math.gamma(dim / 2.0 + 1.0) ** 2 * phi
+    return float(arg ** (1.0 / dim))
+
+
+def modulus2(k: np.ndarray) -> float:
+    """|k|^2 summed in index order (tests/oracle.py:106-111)."""
+    acc = 0.0
+    for c in range(k.shape[0]):
+        acc += float(k[c]) * float(k[c])
+    return acc
+
+
+def halfspace_kvectors(
+    basis: np.ndarray, k_inner: float, k_outer: float
+) -> Tuple[np.ndarray, np.ndarray]:
+    """Brute-force integer scan (tests/oracle.py:114-150), sorted like the KSet.
+
+    The Cartesian k is summed in index order (kvectors.py:82-85) so the strict
+    shell test sees the same float64 value as the reference; rows are then
+    ordered by (k_sq, n lexicographic) (kvectors.py:91-93).
+    """
+    dim = basis.shape[0]
+    recip = TWO_PI * np.linalg.inv(basis).T
+    bounds = [
+        int(math.floor(k_outer * float(np.linalg.norm(basis[i])) / TWO_PI)) + 1
+        for i in range(dim)
+    ]
+    rows: List[Tuple[float, Tuple[int, ...]]] = []
+    for n in itertools.product(*[range(-b, b + 1) for b in bounds]):
+        nz = [c for c in n if c != 0]
+        if not nz or nz[0] < 0:
+            continue
+        k = np.zeros(dim)
+        for j in range(dim):
+            acc = 0.0
+            for i in range(dim):
+                acc = acc + float(n[i]) * float(recip[i, j])
+            k[j] = acc
+        q = modulus2(k)
+        if k_inner * k_inner < q < k_outer * k_outer:
+            rows.append((q, tuple(n)))
+    rows.sort()
+    n_out = np.array([r[1] for r in rows], dtype=np.int64).reshape(-1, dim)
+    return n_out, np.array([r[0] for r in rows], dtype=np.float64)
+
+
+# ============================================================
+# ENERGY AND FORCES (energy.py:45-50, 186-225, 301-324)
+# ============================================================
+
+
+def energy_loops(s: np.ndarray, n: np.ndarray, w: np.ndarray, volume: float) -> float:
+    """Pure-Python double loop (tests/oracle.py:38-68); tiny inputs only."""
+    total = 0.0
+    for kk in range(n.shape[0]):
+        rho = 0.0 + 0.0j
+        for j in range(s.shape[0]):
+            ph = TWO_PI * float(np.dot(n[kk], s[j]))
+            rho += complex(math.cos(ph), math.sin(ph))
+        total += float(w[kk]) * (rho.real**2 + rho.imag**2)
+    return total / volume
+
+
+def density(s: np.ndarray, n: np.ndarray, chunk: int = 512) -> Tuple[np.ndarray, np.ndarray]:
+    """(Re rho, Im rho) per k row with rho_k = sum_j exp(2 pi i n_k.s_j).
+
+    energy.py:45-50 (``_density_block``): theta = 2 pi (n @ s^T), row sums of
+    cos and sin.  Chunked over k so the (c, N) block stays small.
+    """
+    n = np.asarray(n, dtype=np.float64)
+    re = np.empty(n.shape[0])
+    im = np.empty(n.shape[0])
+    for lo in range(0, n.shape[0], chunk):
+        th = TWO_PI * (n[lo : lo + chunk] @ s.T)
+        re[lo : lo + chunk] = np.cos(th).sum(axis=1)
+        im[lo : lo + chunk] = np.sin(th).sum(axis=1)
+    return re, im
+
+
+def energy_grad(
+    s: np.ndarray, n: np.ndarray, w: np.ndarray, volume: float, chunk: int = 512
+) -> Tuple[float, np.ndarray, np.ndarray, np.ndarray]:
+    """(Phi, dPhi/ds, Re rho, Im rho): energy.py:206-225 with ``_gradient`` :301-324.
+
+    dPhi/ds_j = (2 (2 pi) / V) sum_k w_k n_k (Im rho_k cos th_kj - Re rho_k sin th_kj).
+    """
+    n = np.asarray(n, dtype=np.float64)
+    re, im = density(s, n, chunk)
+    value = float(np.sum(w * (re * re + im * im)) / volume)
+    acc = np.zeros_like(s)
+    for lo in range(0, n.shape[0], chunk):
+        nc = n[lo : lo + chunk]
+        th = TWO_PI * (s @ nc.T)  # (N, c)
+        coef = w[lo : lo + chunk] * (im[lo : lo + chunk] * np.cos(th) - re[lo : lo + chunk] * np.sin(th))
+        acc = acc + coef @ nc
+    return value, (2.0 * TWO_PI / volume) * acc, re, im
+
+
+def cartesian_forces(
+    s: np.ndarray, n: np.ndarray, w: np.ndarray, basis: np.ndarray, volume: float
+) -> np.ndarray:
+    """C++ forces F = -dPhi/dr (tests/oracle.py:71-98), loop over k."""
+    recip = TWO_PI * np.linalg.inv(basis).T
+    k = np.asarray(n, dtype=np.float64) @ recip
+    r = s @ basis
+    out = np.zeros_like(r)
+    for kk in range(k.shape[0]):
+        ph = r @ k[kk]
+        rho = complex(np.cos(ph).sum(), np.sin(ph).sum())
+        coef = rho.imag * np.cos(ph) - rho.real * np.sin(ph)
+        out += -(2.0 * float(w[kk]) / volume) * coef[:, None] * k[kk][None, :]
+    return out
+
+
+def make_value_grad(n: np.ndarray, w: np.ndarray, volume: float) -> ValueGrad:
+    """Closure (s) -> (Phi, dPhi/ds) over one k-set."""
+
+    def vg(s: np.ndarray) -> Tuple[float, np.ndarray]:
+        f, g, _, _ = energy_grad(s, n, w, volume)
+        return f, g
+
+    return vg
+
+
+def wrap_fractional(s: np.ndarray) -> np.ndarray:
+    """box.py:127-139."""
+    return np.minimum(s - np.floor(s), np.nextafter(1.0, 0.0))
+
+
+# ============================================================
+# ADAPTIVE DESCENT (minimize/descent.py:52-232)
+# ============================================================
+
+
+@dataclasses.dataclass(frozen=True)
+class DescentSettings:
+    step_init: float = 1e-3
+    step_min: float = 1e-15
+    shrink: float = 0.5
+    grow: float = 1.0
+    rel_tol: float = 0.01
+    max_evals: int = 10_000
+    energy_tol: float = 0.0
+    record_every: int = 0
+    max_records: int = 0
+
+
+def _descent_status(f: float, g: np.ndarray, step: float, it: int, st: DescentSettings) -> int:
+    """Precedence energy tol > zero grad > step floor > budget (descent.py:92-104)."""
+    code = STATUS_BUDGET if it >= st.max_evals // 2 else STATUS_RUNNING
+    if step <= st.step_min:
+        code = STATUS_STEP_FLOOR
+    if np.all(g == 0.0):
+        code = STATUS_GRAD_TOL
+    if f < st.energy_tol:
+        code = STATUS_ENERGY_TOL
+    return code
+
+
+@dataclasses.dataclass
+class RelaxOut:
+    s: np.ndarray
+    energy: float
+    grad_norm: float
+    n_evals: int
+    n_iters: int
+    status: int
+    traj: Optional[Dict[str, np.ndarray]] = None
+
+
+def descent(vg: ValueGrad, s0: np.ndarray, st: DescentSettings) -> RelaxOut:
+    """One configuration (descent.py:121-197): init, iterate to a status, finalize."""
+    s = s0.copy()
+    f, g = vg(s)
+    step = float(st.step_init)
+    it = 0
+    n_evals = 0
+    status = _descent_status(f, g, step, it, st)
+    rec = _Recorder(st.record_every, st.max_records, s, f, step)
+    while status == STATUS_RUNNING:
+        norm = math.sqrt(float(np.sum(g * g)))
+        gn = g / norm if norm > 0.0 else 0.0 * g
+        xn = s - gn * step
+        f2, g2 = vg(xn)
+        df = f - f2
+        df2 = step * norm
+        with np.errstate(invalid="ignore", divide="ignore"):
+            reject = bool(abs(df - df2) / df2 > st.rel_tol)  # NaN accepts
+        if reject:
+            step = step * st.shrink
+        else:
+            step = step * st.grow
+            s, f, g = xn, f2, g2
+        it += 1
+        n_evals += 2
+        status = _descent_status(f, g, step, it, st)
+        rec.add(it, s, f, step)
+    return RelaxOut(
+        s=wrap_fractional(s),
+        energy=f,
+        grad_norm=math.sqrt(float(np.sum(g * g))),
+        n_evals=n_evals,
+        n_iters=it,
+        status=status,
+        traj=rec.result(),
+    )
+
+
+class _Recorder:
+    """Trajectory rows every ``every`` iterations (minimize/_loop.py:189-248)."""
+
+    def __init__(self, every: int, cap: int, s: np.ndarray, f: float, step: float):
+        self.on = every > 0 and cap > 0
+        self.every, self.cap = every, cap
+        self.rows: List[Tuple[int, np.ndarray, float, float]] = []
+        if self.on:
+            self.rows.append((0, wrap_fractional(s), f, step))
+
+    def add(self, it: int, s: np.ndarray, f: float, step: float) -> None:
+        if self.on and it % self.every == 0 and len(self.rows) < self.cap:
+            self.rows.append((it, wrap_fractional(s), f, step))
+
+    def result(self) -> Optional[Dict[str, np.ndarray]]:
+        if not self.on:
+            return None
+        return {
+            "iteration": np.array([r[0] for r in self.rows], dtype=np.int64),
+            "s": np.array([r[1] for r in self.rows]),
+            "energy": np.array([r[2] for r in self.rows]),
+            "step": np.array([r[3] for r in self.rows]),
+            "count": len(self.rows),
+        }
+
+
+# ============================================================
+# L-BFGS (minimize/lbfgs.py:55-350 + optax 0.2.8 lbfgs / zoom line search)
+# ============================================================
+
+
+@dataclasses.dataclass(frozen=True)
+class LBFGSSettings:
+    memory: int = 100
+    rescales: Tuple[float, ...] = (1e4, 1.0, 1e-2)
+    max_evals_per_pass: int = 500_000
+    energy_tol: float = 0.0
+    grad_tol: float = 0.0
+    stagnation_patience: int = 50
+    max_linesearch_steps: int = 30
+    record_every: int = 0
+    max_records: int = 0
+
+
+def vdot(a: np.ndarray, b: np.ndarray) -> float:
+    return float(np.dot(a.ravel(), b.ravel()))
+
+
+def lbfgs_precondition(
+    g: np.ndarray,
+    S: np.ndarray,
+    Y: np.ndarray,
+    rho: np.ndarray,
+    gamma: float,
+    memory_idx: int,
+) -> np.ndarray:
+    """Two-loop recursion over a ring of m pairs (optax ``_precondition_by_lbfgs``).
+
+    Slots are visited from ``memory_idx`` (oldest) around the ring; the first
+    loop runs newest -> oldest, the second oldest -> newest.  Unused slots have
+    rho = 0 and zero vectors, so they contribute exactly nothing.
+    """
+    m = rho.shape[0]
+    order = [(memory_idx + t) % m for t in range(m)]
+    q = g.copy()
+    alphas = np.zeros(m)
+    for t in reversed(range(m)):
+        i = order[t]
+        a = rho[i] * vdot(S[i], q)
+        q = q + (-a) * Y[i]
+        alphas[t] = a
+    q = gamma * q
+    for t in range(m):
+        i = order[t]
+        b = rho[i] * vdot(Y[i], q)
+        q = q + (alphas[t] - b) * S[i]
+    return q
+
+
+def cubicmin(a: float, fa: float, fpa: float, b: float, fb: float, c: float, fc: float) -> float:
+    """Critical point of the cubic through (a,fa),(b,fb),(c,fc) with slope fpa at a.
+
+    The same closed form as scipy.optimize's line-search helper; NaN when the
+    radical is negative or the points are degenerate (the caller then rejects it).
+    """
+    with np.errstate(all="ignore"):
+        C = fpa
+        db = b - a
+        dc = c - a
+        denom = (db * dc) ** 2 * (db - dc)
+        d1 = np.array([[dc**2, -(db**2)], [-(dc**3), db**3]])
+        d2 = np.array([fb - fa - C * db, fc - fa - C * dc])
+        A, B = (d1 @ d2) / denom
+        radical = B * B - 3.0 * A * C
+        return float(a + (-B + np.sqrt(radical)) / (3.0 * A))
+
+
+def quadmin(a: float, fa: float, fpa: float, b: float, fb: float) -> float:
+    """Minimiser of the parabola through (a,fa),(b,fb) with slope fpa at a."""
+    with np.errstate(all="ignore"):
+        D = fa
+        C = fpa
+        db = b - a
+        B = (fb - D - C * db) / (db * db)
+        return float(a - C / (2.0 * B))
+
+
+@dataclasses.dataclass
+class LineSearchOut:
+    stepsize: float
+    x: np.ndarray
+    value: float
+    grad: np.ndarray
+    num_steps: int
+    decrease_error: float
+
+
+# optax.scale_by_zoom_linesearch defaults as restated for this path.
+LS_SLOPE_RTOL = 1e-4
+LS_CURV_RTOL = 0.9
+LS_APPROX_DEC_RTOL = 1e-6
+LS_INCREASE = 2.0
+LS_INTERVAL_MIN = 1e-5
+LS_TOL = 0.0
+
+
+def _decrease_error(a: float, va: float, sa: float, v0: float, s0: float) -> float:
+    """Armijo error, relaxed by the Hager-Zhang approximate decrease; >= 0."""
+    with np.errstate(all="ignore"):
+        dec = va - (v0 + LS_SLOPE_RTOL * a * s0)
+        approx = sa - (2.0 * LS_SLOPE_RTOL - 1.0) * s0
+        delta = va - v0 - LS_APPROX_DEC_RTOL * abs(v0)
+        approx = _nanmax(approx, delta)
+        dec = _nanmin(approx, dec)
+        return _nanmax(dec, 0.0)
+
+
+def _curvature_error(sa: float, s0: float) -> float:
+    return _nanmax(abs(sa) - LS_CURV_RTOL * abs(s0), 0.0)
+
+
+def _nanmax(a: float, b: float) -> float:
+    return float("nan") if (math.isnan(a) or math.isnan(b)) else max(a, b)
+
+
+def _nanmin(a: float, b: float) -> float:
+    return float("nan") if (math.isnan(a) or math.isnan(b)) else min(a, b)
+
+
+def zoom_linesearch(
+    fg: ValueGrad,
+    x: np.ndarray,
+    u: np.ndarray,
+    f0: float,
+    g0: np.ndarray,
+    max_steps: int,
+) -> LineSearchOut:
+    """Strong-Wolfe zoom line search with initial guess 1 (optax 0.2.8 semantics).
+
+    Bracketing phase (N&W Alg. 3.5) doubles the step from 1; the zoom phase
+    (Alg. 3.6) picks cubic, then quadratic, then bisection trial points with
+    the 0.2 / 0.1 interior safeguards.  One value-and-gradient per step.  On
+    failure the last step with sufficient decrease is used when one exists
+    (reporting decrease error 0), else the last trial point.
+    """
+    slope0 = vdot(u, g0)
+    count = 0
+    interval_found = done = failed = False
+    prev = (0.0, f0, slope0)
+    low = (0.0, f0, slope0)
+    high = (0.0, f0, slope0)
+    cref = (0.0, f0)
+    safe: Tuple[float, float, np.ndarray, np.ndarray] = (0.0, f0, g0, x)
+    last: Tuple[float, float, np.ndarray, np.ndarray, float] = (0.0, f0, g0, x, math.inf)
+    while not (done or failed):
+        if not interval_found:
+            a = 1.0 if count == 0 else LS_INCREASE * prev[0]
+            xa = x + a * u
+            va, ga = fg(xa)
+            sa = vdot(ga, u)
+            dec = _decrease_error(a, va, sa, f0, slope0)
+            err = _nanmax(dec, _curvature_error(sa, slope0))
+            if dec <= LS_TOL:
+                safe = (a, va, ga, xa)
+            set_high = (dec > 0.0) or (va >= prev[1] and count > 0)
+            set_low = (sa >= 0.0) and not set_high
+            if set_low:
+                low, high = (a, va, sa), prev
+            else:
+                low, high = prev, (a, va, sa)
+            interval_found = set_high or set_low or err <= LS_TOL
+            done = err <= LS_TOL
+            cref = (low[0], low[1])
+            prev = (a, va, sa)
+            count += 1
+            failed = count >= max_steps and not done
+        else:
+            delta = abs(high[0] - low[0])
+            left, right = min(high[0], low[0]), max(high[0], low[0])
+            mc = cubicmin(low[0], low[1], low[2], high[0], high[1], cref[0], cref[1])
+            use_cubic = (mc > left + 0.2 * delta) and (mc < right - 0.2 * delta)
+            mq = quadmin(low[0], low[1], low[2], high[0], high[1])
+            use_quad = (not use_cubic) and (mq > left + 0.1 * delta) and (mq < right - 0.1 * delta)
+            if use_cubic:
+                a = mc
+            elif use_quad:
+                a = mq
+            else:
+                a = (low[0] + high[0]) / 2.0
+            xa = x + a * u
+            va, ga = fg(xa)
+            sa = vdot(ga, u)
+            dec = _decrease_error(a, va, sa, f0, slope0)
+            err = _nanmax(dec, _curvature_error(sa, slope0))
+            if dec <= LS_TOL and va < safe[1]:
+                safe = (a, va, ga, xa)
+            done = err <= LS_TOL
+            hi_to_mid = (dec > 0.0) or (va >= low[1])
+            hi_to_low = (sa * (high[0] - low[0]) >= 0.0) and not hi_to_mid
+            old_low, old_high = low, high
+            if hi_to_mid:
+                high = (a, va, sa)
+            elif hi_to_low:
+                high = old_low
+            if not hi_to_mid:
+                low = (a, va, sa)
+            cref = (old_high[0], old_high[1]) if (hi_to_mid or hi_to_low) else (old_low[0], old_low[1])
+            prev = (a, va, sa)
+            count += 1
+            failed = (count >= max_steps or delta <= LS_INTERVAL_MIN) and not done
+        last = (a, va, ga, xa, dec)
+    if failed and safe[0] > 0.0:
+        return LineSearchOut(safe[0], safe[3], safe[1], safe[2], count, 0.0)
+    return LineSearchOut(last[0], last[3], last[1], last[2], count, last[4])
+
+
+def lbfgs(vg: ValueGrad, s0: np.ndarray, st: LBFGSSettings) -> RelaxOut:
+    """Three-pass L-BFGS on x = R s (minimize/lbfgs.py:163-313).
+
+    Per iteration: memory update and two-loop (optax ``scale_by_lbfgs``),
+    direction u = -H g, zoom line search on Phi(x / R), then the reference's
+    pass logic: a failed search keeps the previous point, stagnation counter,
+    per-pass budget, rescale advance with a fresh optimizer state.
+    """
+    rescales = [float(r) for r in st.rescales]
+    n_pass = len(rescales)
+    m = st.memory
+
+    def phi_x(r: float) -> ValueGrad:
+        def f(x: np.ndarray) -> Tuple[float, np.ndarray]:
+            v, gs = vg(x / r)
+            return v, gs / r
+
+        return f
+
+    s = s0.copy()
+    r = rescales[0]
+    f, gs = vg(s0)
+    x = s0 * r
+    gx = gs / r
+    status = STATUS_ENERGY_TOL if f < st.energy_tol else STATUS_RUNNING
+    n_evals, it, pass_idx, pass_evals, stag = 1, 0, 0, 0, 0
+    last_step = 0.0
+    rec = _Recorder(st.record_every, st.max_records, s0, f, 0.0)
+
+    def fresh() -> Dict[str, object]:
+        return {
+            "count": 0,
+            "px": np.zeros_like(s0),
+            "pg": np.zeros_like(s0),
+            "S": np.zeros((m,) + s0.shape),
+            "Y": np.zeros((m,) + s0.shape),
+            "rho": np.zeros(m),
+        }
+
+    opt = fresh()
+    while status == STATUS_RUNNING:
+        r = rescales[pass_idx]
+        count = int(opt["count"])  # type: ignore[arg-type]
+        S, Y, rho = opt["S"], opt["Y"], opt["rho"]
+        if count > 0:
+            ds = x - opt["px"]  # type: ignore[operator]
+            dy = gx - opt["pg"]  # type: ignore[operator]
+            sy = vdot(dy, ds)
+            slot = (count - 1) % m
+            S[slot] = ds  # type: ignore[index]
+            Y[slot] = dy  # type: ignore[index]
+            rho[slot] = 0.0 if sy == 0.0 else 1.0 / sy  # type: ignore[index]
+            yy = vdot(dy, dy)
+            gamma = sy / yy if yy > 0.0 else 1.0
+        else:
+            gnorm0 = math.sqrt(vdot(gx, gx))
+            if math.isnan(gnorm0):
+                gamma = float("nan")
+            else:
+                gamma = min(1.0, 1.0 / gnorm0) if gnorm0 > 0.0 else 1.0
+        q = lbfgs_precondition(gx, S, Y, rho, gamma, count % m)  # type: ignore[arg-type]
+        opt["px"], opt["pg"], opt["count"] = x, gx, count + 1
+        u = -q
+        ls = zoom_linesearch(phi_x(r), x, u, f, gx, st.max_linesearch_steps)
+        fail = (ls.decrease_error > 0.0) or not math.isfinite(ls.value)
+        evals = max(ls.num_steps, 1)
+        f_old = f
+        if not fail:
+            x, s, f, gx = ls.x, ls.x / r, ls.value, ls.grad
+        stag = 0 if f < f_old else stag + 1
+        pass_evals += evals
+        gnorm = math.sqrt(vdot(gx, gx)) * r
+        glob = STATUS_GRAD_TOL if gnorm < st.grad_tol else STATUS_RUNNING
+        if f < st.energy_tol:
+            glob = STATUS_ENERGY_TOL
+        local = STATUS_BUDGET if pass_evals >= st.max_evals_per_pass else STATUS_RUNNING
+        if stag >= st.stagnation_patience:
+            local = STATUS_STAGNATED
+        if fail:
+            local = STATUS_LINESEARCH_FAIL
+        last = pass_idx >= n_pass - 1
+        pass_end = local != STATUS_RUNNING and glob == STATUS_RUNNING
+        advance = pass_end and not last
+        status = glob if glob != STATUS_RUNNING else STATUS_RUNNING
+        if pass_end and last:
+            status = local
+        if advance:
+            r_next = rescales[pass_idx + 1]
+            x = s * r_next
+            gx = gx * r / r_next
+            opt = fresh()
+            pass_idx += 1
+            pass_evals = 0
+            stag = 0
+        last_step = ls.stepsize
+        it += 1
+        n_evals += evals
+        rec.add(it, s, f, last_step)
+    r = rescales[pass_idx]
+    return RelaxOut(
+        s=wrap_fractional(s),
+        energy=f,
+        grad_norm=math.sqrt(vdot(gx, gx)) * r,
+        n_evals=n_evals,
+        n_iters=it,
+        status=status,
+        traj=rec.result(),
+    )
+
+
+def relax(
+    vg: ValueGrad,
+    s0: np.ndarray,
+    settings: object,
+) -> List[RelaxOut]:
+    """Batched relax == per-configuration minimizer, in input order.
+
+    minimize/__init__.py:102-151: lanes are independent; host compaction only
+    changes which lanes are stepped, never their arithmetic.
+    """
+    if isinstance(settings, DescentSettings):
+        return [descent(vg, s0[i], settings) for i in range(s0.shape[0])]
+    if isinstance(settings, LBFGSSettings):
+        return [lbfgs(vg, s0[i], settings) for i in range(s0.shape[0])]
+    raise TypeError(f"unsupported settings type {type(settings).__name__}")

@@ -1,0 +1,881 @@
+// C-ABI layer: contexts, k-set geometry upload, batched energy and relax
+// sessions.  See include/cco.h for the contract of every entry point.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <tuple>
+#include <vector>
+
+#include "../../include/cco.h"
+#include "cco_internal.h"
+#include "minimize_kernels.h"
+
+namespace cco {
+
+static thread_local std::string g_last_error;
+
+void set_last_error(const std::string& msg) { g_last_error = msg; }
+
+static int fail(int code, const std::string& msg) {
+  set_last_error(msg);
+  return code;
+}
+
+}  // namespace cco
+
+using namespace cco;
+
+struct cco_ctx {
+  int device;
+  hipStream_t stream;
+};
+
+// RAII device scope: make the context's device current for the call.
+struct DevScope {
+  explicit DevScope(int dev) { (void)hipSetDevice(dev); }
+};
+
+struct DevBuf {
+  void* p = nullptr;
+  size_t bytes = 0;
+};
+
+static int dev_reserve(DevBuf& b, size_t bytes) {
+  if (bytes <= b.bytes && b.p != nullptr) return CCO_OK;
+  if (b.p != nullptr) {
+    CCO_HIP(hipFree(b.p));
+    b.p = nullptr;
+    b.bytes = 0;
+  }
+  if (bytes == 0) return CCO_OK;
+  CCO_HIP(hipMalloc(&b.p, bytes));
+  b.bytes = bytes;
+  return CCO_OK;
+}
+
+static void dev_release(DevBuf& b) {
+  if (b.p != nullptr) (void)hipFree(b.p);
+  b.p = nullptr;
+  b.bytes = 0;
+}
+
+struct cco_energy {
+  cco_ctx* ctx;
+  Geo geo;
+  int nmax[3];
+  DevBuf tables;
+  // batch workspace for cco_value_grad_density*
+  DevBuf s, phi, grad, re, im, part, cbox, active, n_active;
+  int ident_B = 0;
+};
+
+namespace {
+
+constexpr int kSplitLen = 128;  // particles per rho split (fixed per lane shape)
+
+int n_splits_for(int N) { return (N + kSplitLen - 1) / kSplitLen; }
+int n_ftiles_for(int N) { return (N + kForceTile - 1) / kForceTile; }
+
+struct ColKey {
+  int n2, n3;
+  bool operator<(const ColKey& o) const { return std::tie(n2, n3) < std::tie(o.n2, o.n3); }
+};
+
+// Build the column box and traversal tables of a k-set (host), upload them.
+int build_geometry(cco_energy* E, const int64_t* n, const double* w, int M, int d, double volume,
+                   int pad_to) {
+  Geo& g = E->geo;
+  g.d = d;
+  g.M = M;
+  g.M_pad = ((M + pad_to - 1) / pad_to) * pad_to;
+  g.volume = volume;
+  g.pref = (2.0 * (2.0 * M_PI)) / volume;
+  // canonical rows: n1 >= 0 (a row with n1 < 0 is negated and flagged conj;
+  // |rho|^2 and the gradient term of -n equal those of n)
+  std::vector<int> n1(M), n2(M), n3(M);
+  std::vector<char> flip(M, 0);
+  int P1 = 0, P2 = 0, P3 = 0;
+  for (int k = 0; k < M; ++k) {
+    int a = (int)n[(size_t)k * d + 0];
+    int b = d >= 2 ? (int)n[(size_t)k * d + 1] : 0;
+    int c = d >= 3 ? (int)n[(size_t)k * d + 2] : 0;
+    if (a < 0) {
+      a = -a;
+      b = -b;
+      c = -c;
+      flip[k] = 1;
+    }
+    n1[k] = a;
+    n2[k] = b;
+    n3[k] = c;
+    P1 = std::max(P1, a);
+    P2 = std::max(P2, std::abs(b));
+    P3 = std::max(P3, std::abs(c));
+  }
+  g.P1 = P1;
+  g.P2 = P2;
+  g.P3 = P3;
+  E->nmax[0] = P1;
+  E->nmax[1] = P2;
+  E->nmax[2] = P3;
+  std::map<ColKey, int> colL;
+  for (int k = 0; k < M; ++k) {
+    auto it = colL.find({n2[k], n3[k]});
+    if (it == colL.end())
+      colL[{n2[k], n3[k]}] = n1[k] + 1;
+    else
+      it->second = std::max(it->second, n1[k] + 1);
+  }
+  struct Col {
+    int n2, n3, L, off;
+  };
+  std::vector<Col> cols;
+  for (auto& kv : colL) cols.push_back({kv.first.n2, kv.first.n3, kv.second, 0});
+  std::stable_sort(cols.begin(), cols.end(), [](const Col& a, const Col& b) {
+    if (a.L != b.L) return a.L > b.L;
+    return std::tie(a.n2, a.n3) < std::tie(b.n2, b.n3);
+  });
+  int off = 0;
+  std::map<ColKey, int> col_index;
+  for (size_t c = 0; c < cols.size(); ++c) {
+    cols[c].off = off;
+    off += cols[c].L;
+    col_index[{cols[c].n2, cols[c].n3}] = (int)c;
+  }
+  g.Mbox = off;
+  g.Cn = (int)cols.size();
+  g.CT = std::min(256, ((g.Cn + kWave - 1) / kWave) * kWave);
+  g.n_coltiles = (g.Cn + g.CT - 1) / g.CT;
+  g.n_segs = (P1 + 1 + kRowSeg - 1) / kRowSeg;
+  std::vector<int4> col_tab(cols.size());
+  for (size_t c = 0; c < cols.size(); ++c)
+    col_tab[c] = make_int4(cols[c].off, cols[c].L, cols[c].n2, cols[c].n3);
+  std::vector<int> tile_lmax(g.n_coltiles, 0);
+  for (size_t c = 0; c < cols.size(); ++c) {
+    int t = (int)c / g.CT;
+    tile_lmax[t] = std::max(tile_lmax[t], cols[c].L);
+  }
+  std::vector<int> kbox(M);
+  std::vector<double> wbox(g.Mbox, 0.0);
+  for (int k = 0; k < M; ++k) {
+    const Col& c = cols[col_index[{n2[k], n3[k]}]];
+    const int i = c.off + n1[k];
+    kbox[k] = flip[k] ? ~i : i;
+    wbox[i] += w[k];
+  }
+  // force traversal: groups (u = |n2|, v = |n3|) with four sign members
+  auto member = [&](int a2, int a3) -> int4 {
+    auto it = col_index.find({a2, a3});
+    if (it == col_index.end()) return make_int4(0, 0, 0, 0);
+    const Col& c = cols[it->second];
+    return make_int4(c.off, c.L, c.n2, c.n3);
+  };
+  std::vector<int2> ustart;
+  std::vector<int4> gmem;
+  int ngroups = 0;
+  for (int u = 0; u <= P2; ++u) {
+    int vmax = -1;
+    for (int v = 0; v <= P3; ++v) {
+      int4 m[4] = {member(u, v), v > 0 ? member(u, -v) : make_int4(0, 0, 0, 0),
+                   u > 0 ? member(-u, -v) : make_int4(0, 0, 0, 0),
+                   (u > 0 && v > 0) ? member(-u, v) : make_int4(0, 0, 0, 0)};
+      if (m[0].y || m[1].y || m[2].y || m[3].y) vmax = v;
+    }
+    ustart.push_back(make_int2(ngroups, vmax + 1));
+    for (int v = 0; v <= vmax; ++v) {
+      gmem.push_back(member(u, v));
+      gmem.push_back(v > 0 ? member(u, -v) : make_int4(0, 0, 0, 0));
+      gmem.push_back(u > 0 ? member(-u, -v) : make_int4(0, 0, 0, 0));
+      gmem.push_back((u > 0 && v > 0) ? member(-u, v) : make_int4(0, 0, 0, 0));
+      ++ngroups;
+    }
+  }
+  g.n_u = P2 + 1;
+  if (rho_lds_bytes(g) > 64 * 1024)
+    return fail(CCO_ERR_ARG, "k-set too wide for the rho kernel's LDS tables (max |n| too large)");
+  // one device allocation holding every table
+  auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  const size_t o_col = 0;
+  const size_t o_tl = align(o_col + sizeof(int4) * col_tab.size());
+  const size_t o_kb = align(o_tl + sizeof(int) * tile_lmax.size());
+  const size_t o_wb = align(o_kb + sizeof(int) * kbox.size());
+  const size_t o_us = align(o_wb + sizeof(double) * wbox.size());
+  const size_t o_gm = align(o_us + sizeof(int2) * ustart.size());
+  const size_t total = align(o_gm + sizeof(int4) * std::max<size_t>(gmem.size(), 1));
+  std::vector<char> host(total, 0);
+  std::memcpy(host.data() + o_col, col_tab.data(), sizeof(int4) * col_tab.size());
+  std::memcpy(host.data() + o_tl, tile_lmax.data(), sizeof(int) * tile_lmax.size());
+  std::memcpy(host.data() + o_kb, kbox.data(), sizeof(int) * kbox.size());
+  std::memcpy(host.data() + o_wb, wbox.data(), sizeof(double) * wbox.size());
+  std::memcpy(host.data() + o_us, ustart.data(), sizeof(int2) * ustart.size());
+  if (!gmem.empty()) std::memcpy(host.data() + o_gm, gmem.data(), sizeof(int4) * gmem.size());
+  int rc = dev_reserve(E->tables, total);
+  if (rc) return rc;
+  CCO_HIP(hipMemcpy(E->tables.p, host.data(), total, hipMemcpyHostToDevice));
+  char* base = (char*)E->tables.p;
+  g.col = (const int4*)(base + o_col);
+  g.tile_lmax = (const int*)(base + o_tl);
+  g.kbox = (const int*)(base + o_kb);
+  g.wbox = (const double*)(base + o_wb);
+  g.ustart = (const int2*)(base + o_us);
+  g.gmem = (const int4*)(base + o_gm);
+  return CCO_OK;
+}
+
+__global__ void iota_kernel(int* a, int n, int* count) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = i;
+  if (i == 0) *count = n;
+}
+
+int energy_batch(cco_energy* E, const double* s_dev, int B, int N, double* phi_dev,
+                 double* grad_dev, double* re_dev, double* im_dev) {
+  const Geo& g = E->geo;
+  const int nsp = n_splits_for(N);
+  int rc;
+  if ((rc = dev_reserve(E->part, sizeof(double2) * (size_t)B * nsp * g.Mbox))) return rc;
+  if ((rc = dev_reserve(E->cbox, sizeof(double2) * (size_t)B * g.Mbox))) return rc;
+  if ((rc = dev_reserve(E->active, sizeof(int) * (size_t)B))) return rc;
+  if ((rc = dev_reserve(E->n_active, sizeof(int)))) return rc;
+  hipStream_t st = E->ctx->stream;
+  if (E->ident_B != B) {
+    hipLaunchKernelGGL(iota_kernel, dim3((B + 255) / 256), dim3(256), 0, st, (int*)E->active.p, B,
+                       (int*)E->n_active.p);
+    CCO_HIP(hipGetLastError());
+    E->ident_B = B;
+  }
+  Batch b;
+  b.active = (const int*)E->active.p;
+  b.n_active = (const int*)E->n_active.p;
+  b.N = N;
+  b.lane_stride = N * g.d;
+  b.n_splits = nsp;
+  b.split_len = kSplitLen;
+  b.n_ftiles = n_ftiles_for(N);
+  b.s = s_dev;
+  b.grad = grad_dev;
+  b.phi = phi_dev;
+  b.re = re_dev;
+  b.im = im_dev;
+  b.part = (double2*)E->part.p;
+  b.cbox = (double2*)E->cbox.p;
+  launch_energy(g, b, B, grad_dev != nullptr, st);
+  CCO_HIP(hipGetLastError());
+  return CCO_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// relax session
+// ===========================================================================
+struct cco_relax {
+  cco_ctx* ctx;
+  cco_energy* E;
+  int B, N, d, method;
+  DescentCfg dc;
+  LbfgsCfg lc;
+  RelaxDev dev;
+  std::vector<DevBuf> bufs;
+  DevBuf part, cbox, out_s, out_scalars, pack, pack_idx;
+  int slots_ub = 0;
+  int64_t steps = 0;
+  int64_t lane_evals = 0;
+  bool ready = false;
+  int* host_count = nullptr;  // pinned
+};
+
+namespace {
+
+template <typename T>
+int alloc_into(cco_relax* R, T*& out, size_t count) {
+  DevBuf b;
+  int rc = dev_reserve(b, std::max<size_t>(count, 1) * sizeof(T));
+  if (rc) return rc;
+  CCO_HIP(hipMemsetAsync(b.p, 0, b.bytes, R->ctx->stream));
+  R->bufs.push_back(b);
+  out = (T*)b.p;
+  return CCO_OK;
+}
+
+__global__ void gather_rows_kernel(const double* __restrict__ src, const int* __restrict__ active,
+                                   const int* __restrict__ n_active, double* __restrict__ dst,
+                                   int nd, int stride) {
+  const int slot = blockIdx.x;
+  if (slot >= *n_active) return;
+  const int lane = active[slot];
+  for (int i = threadIdx.x; i < nd; i += blockDim.x)
+    dst[(size_t)slot * nd + i] = src[(size_t)lane * stride + i];
+}
+
+__global__ void scatter_eval_kernel(const double* __restrict__ phi_p,
+                                    const double* __restrict__ grad_p,
+                                    const int* __restrict__ active,
+                                    const int* __restrict__ n_active, double* __restrict__ phi,
+                                    double* __restrict__ grad, int nd, int stride) {
+  const int slot = blockIdx.x;
+  if (slot >= *n_active) return;
+  const int lane = active[slot];
+  if (threadIdx.x == 0) phi[lane] = phi_p[slot];
+  for (int i = threadIdx.x; i < nd; i += blockDim.x)
+    grad[(size_t)lane * stride + i] = grad_p[(size_t)slot * nd + i];
+}
+
+__global__ void relax_init_kernel(RelaxDev r, int method) {
+  const int lane = blockIdx.x * blockDim.x + threadIdx.x;
+  if (lane >= r.B) return;
+  r.status[lane] = ST_RUNNING;
+  r.active[lane] = lane;
+  if (lane == 0) *r.n_active = r.B;
+  if (method == CCO_METHOD_DESCENT) {
+    DescentLane L = {};
+    L.mode = MODE_INIT;
+    r.dl[lane] = L;
+  } else {
+    LbfgsLane L = {};
+    L.mode = MODE_INIT;
+    r.ll[lane] = L;
+  }
+}
+
+int relax_eval_builtin(cco_relax* R) {
+  const Geo& g = R->E->geo;
+  Batch b;
+  b.active = R->dev.active;
+  b.n_active = R->dev.n_active;
+  b.N = R->N;
+  b.lane_stride = R->dev.stride;
+  b.n_splits = n_splits_for(R->N);
+  b.split_len = kSplitLen;
+  b.n_ftiles = n_ftiles_for(R->N);
+  b.s = R->dev.s_eval;
+  b.grad = R->dev.g_eval;
+  b.phi = R->dev.phi_eval;
+  b.re = nullptr;
+  b.im = nullptr;
+  b.part = (double2*)R->part.p;
+  b.cbox = (double2*)R->cbox.p;
+  launch_energy(g, b, R->slots_ub, true, R->ctx->stream);
+  return CCO_OK;
+}
+
+int relax_advance(cco_relax* R) {
+  hipStream_t st = R->ctx->stream;
+  if (R->method == CCO_METHOD_DESCENT)
+    launch_descent_advance(R->dev, R->dc, R->slots_ub, st);
+  else
+    launch_lbfgs_advance(R->dev, R->lc, R->slots_ub, st);
+  launch_compact(R->dev, st);
+  CCO_HIP(hipGetLastError());
+  return CCO_OK;
+}
+
+int relax_sync_count(cco_relax* R, int* n_active) {
+  CCO_HIP(hipMemcpyAsync(R->host_count, R->dev.n_active, sizeof(int), hipMemcpyDeviceToHost,
+                         R->ctx->stream));
+  CCO_HIP(hipStreamSynchronize(R->ctx->stream));
+  R->slots_ub = *R->host_count;
+  if (n_active) *n_active = R->slots_ub;
+  return CCO_OK;
+}
+
+}  // namespace
+
+// ===========================================================================
+// extern "C"
+// ===========================================================================
+extern "C" {
+
+int cco_version(void) { return 100; }
+
+const char* cco_last_error(void) { return g_last_error.c_str(); }
+
+int cco_device_count(int* count) {
+  if (!count) return fail(CCO_ERR_ARG, "count is NULL");
+  int c = 0;
+  hipError_t e = hipGetDeviceCount(&c);
+  if (e != hipSuccess) c = 0;
+  *count = c;
+  return CCO_OK;
+}
+
+int cco_ctx_create(int device, cco_ctx** out) {
+  if (!out) return fail(CCO_ERR_ARG, "out is NULL");
+  int c = 0;
+  if (hipGetDeviceCount(&c) != hipSuccess || c == 0)
+    return fail(CCO_ERR_DEVICE, "no GPU visible to the HIP runtime");
+  if (device < 0 || device >= c) return fail(CCO_ERR_ARG, "device index out of range");
+  CCO_HIP(hipSetDevice(device));
+  cco_ctx* ctx = new cco_ctx();
+  ctx->device = device;
+  hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
+  if (e != hipSuccess) {
+    delete ctx;
+    return fail(CCO_ERR_DEVICE, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+  }
+  *out = ctx;
+  return CCO_OK;
+}
+
+int cco_ctx_destroy(cco_ctx* ctx) {
+  if (!ctx) return CCO_OK;
+  DevScope ds(ctx->device);
+  (void)hipStreamSynchronize(ctx->stream);
+  (void)hipStreamDestroy(ctx->stream);
+  delete ctx;
+  return CCO_OK;
+}
+
+int cco_ctx_synchronize(cco_ctx* ctx) {
+  if (!ctx) return fail(CCO_ERR_ARG, "ctx is NULL");
+  DevScope ds(ctx->device);
+  CCO_HIP(hipStreamSynchronize(ctx->stream));
+  return CCO_OK;
+}
+
+int cco_dev_alloc(cco_ctx* ctx, size_t bytes, void** out) {
+  if (!ctx || !out) return fail(CCO_ERR_ARG, "NULL argument");
+  DevScope ds(ctx->device);
+  CCO_HIP(hipMalloc(out, std::max<size_t>(bytes, 1)));
+  return CCO_OK;
+}
+
+int cco_dev_free(cco_ctx* ctx, void* ptr) {
+  if (!ctx) return fail(CCO_ERR_ARG, "ctx is NULL");
+  DevScope ds(ctx->device);
+  if (ptr) CCO_HIP(hipFree(ptr));
+  return CCO_OK;
+}
+
+int cco_memcpy_h2d(cco_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  if (!ctx) return fail(CCO_ERR_ARG, "ctx is NULL");
+  DevScope ds(ctx->device);
+  CCO_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, ctx->stream));
+  CCO_HIP(hipStreamSynchronize(ctx->stream));
+  return CCO_OK;
+}
+
+int cco_memcpy_d2h(cco_ctx* ctx, void* dst, const void* src, size_t bytes) {
+  if (!ctx) return fail(CCO_ERR_ARG, "ctx is NULL");
+  DevScope ds(ctx->device);
+  CCO_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, ctx->stream));
+  CCO_HIP(hipStreamSynchronize(ctx->stream));
+  return CCO_OK;
+}
+
+int cco_energy_create(cco_ctx* ctx, const int64_t* n, const double* w, int M, int d,
+                      double volume, int pad_to, cco_energy** out) {
+  if (!ctx || !out) return fail(CCO_ERR_ARG, "NULL argument");
+  if (M <= 0) return fail(CCO_ERR_ARG, "empty k-set: no k-vectors in the shell");
+  if (d < 1 || d > 3) return fail(CCO_ERR_ARG, "d must be 1, 2 or 3");
+  if (!n || !w) return fail(CCO_ERR_ARG, "n and w must not be NULL");
+  if (pad_to < 1) return fail(CCO_ERR_ARG, "pad_to must be >= 1");
+  if (!(volume > 0.0) || !std::isfinite(volume))
+    return fail(CCO_ERR_ARG, "volume must be positive and finite");
+  DevScope ds(ctx->device);
+  cco_energy* E = new cco_energy();
+  E->ctx = ctx;
+  int rc = build_geometry(E, n, w, M, d, volume, pad_to);
+  if (rc) {
+    dev_release(E->tables);
+    delete E;
+    return rc;
+  }
+  *out = E;
+  return CCO_OK;
+}
+
+int cco_energy_destroy(cco_energy* E) {
+  if (!E) return CCO_OK;
+  DevScope ds(E->ctx->device);
+  (void)hipStreamSynchronize(E->ctx->stream);
+  for (DevBuf* b : {&E->tables, &E->s, &E->phi, &E->grad, &E->re, &E->im, &E->part, &E->cbox,
+                    &E->active, &E->n_active})
+    dev_release(*b);
+  delete E;
+  return CCO_OK;
+}
+
+int cco_energy_info(const cco_energy* E, int* M, int* M_pad, int* d, double* volume,
+                    int* nmax3) {
+  if (!E) return fail(CCO_ERR_ARG, "energy is NULL");
+  if (M) *M = E->geo.M;
+  if (M_pad) *M_pad = E->geo.M_pad;
+  if (d) *d = E->geo.d;
+  if (volume) *volume = E->geo.volume;
+  if (nmax3) {
+    nmax3[0] = E->nmax[0];
+    nmax3[1] = E->nmax[1];
+    nmax3[2] = E->nmax[2];
+  }
+  return CCO_OK;
+}
+
+int cco_value_grad_density_dev(cco_energy* E, const double* s_dev, int B, int N, double* phi_dev,
+                               double* grad_dev, double* re_dev, double* im_dev) {
+  if (!E || !s_dev || !phi_dev) return fail(CCO_ERR_ARG, "NULL argument");
+  if (B < 1 || N < 1) return fail(CCO_ERR_ARG, "need B >= 1 and N >= 1");
+  if ((re_dev == nullptr) != (im_dev == nullptr))
+    return fail(CCO_ERR_ARG, "re and im must both be given or both be NULL");
+  DevScope ds(E->ctx->device);
+  return energy_batch(E, s_dev, B, N, phi_dev, grad_dev, re_dev, im_dev);
+}
+
+int cco_value_grad_density(cco_energy* E, const double* s_host, int B, int N, double* phi_host,
+                           double* grad_host, double* re_host, double* im_host) {
+  if (!E || !s_host || !phi_host) return fail(CCO_ERR_ARG, "NULL argument");
+  if (B < 1 || N < 1) return fail(CCO_ERR_ARG, "need B >= 1 and N >= 1");
+  if ((re_host == nullptr) != (im_host == nullptr))
+    return fail(CCO_ERR_ARG, "re and im must both be given or both be NULL");
+  DevScope ds(E->ctx->device);
+  const Geo& g = E->geo;
+  const size_t nsd = (size_t)B * N * g.d;
+  int rc;
+  if ((rc = dev_reserve(E->s, nsd * sizeof(double)))) return rc;
+  if ((rc = dev_reserve(E->phi, (size_t)B * sizeof(double)))) return rc;
+  if (grad_host && (rc = dev_reserve(E->grad, nsd * sizeof(double)))) return rc;
+  if (re_host) {
+    if ((rc = dev_reserve(E->re, (size_t)B * g.M_pad * sizeof(double)))) return rc;
+    if ((rc = dev_reserve(E->im, (size_t)B * g.M_pad * sizeof(double)))) return rc;
+  }
+  hipStream_t st = E->ctx->stream;
+  CCO_HIP(hipMemcpyAsync(E->s.p, s_host, nsd * sizeof(double), hipMemcpyHostToDevice, st));
+  rc = energy_batch(E, (const double*)E->s.p, B, N, (double*)E->phi.p,
+                    grad_host ? (double*)E->grad.p : nullptr, re_host ? (double*)E->re.p : nullptr,
+                    re_host ? (double*)E->im.p : nullptr);
+  if (rc) return rc;
+  CCO_HIP(hipMemcpyAsync(phi_host, E->phi.p, (size_t)B * sizeof(double), hipMemcpyDeviceToHost,
+                         st));
+  if (grad_host)
+    CCO_HIP(hipMemcpyAsync(grad_host, E->grad.p, nsd * sizeof(double), hipMemcpyDeviceToHost, st));
+  if (re_host) {
+    CCO_HIP(hipMemcpyAsync(re_host, E->re.p, (size_t)B * g.M_pad * sizeof(double),
+                           hipMemcpyDeviceToHost, st));
+    CCO_HIP(hipMemcpyAsync(im_host, E->im.p, (size_t)B * g.M_pad * sizeof(double),
+                           hipMemcpyDeviceToHost, st));
+  }
+  CCO_HIP(hipStreamSynchronize(st));
+  return CCO_OK;
+}
+
+int cco_relax_create(cco_ctx* ctx, cco_energy* E, int B, int N, int d, int method,
+                     const cco_descent_settings* ds_, const cco_lbfgs_settings* ls_,
+                     cco_relax** out) {
+  if (!ctx || !out) return fail(CCO_ERR_ARG, "NULL argument");
+  if (B < 1 || N < 1) return fail(CCO_ERR_ARG, "need B >= 1 and N >= 1");
+  if (d < 1 || d > 3) return fail(CCO_ERR_ARG, "d must be 1, 2 or 3");
+  if (E && E->geo.d != d) return fail(CCO_ERR_ARG, "dimension mismatch");
+  if (method == CCO_METHOD_DESCENT && !ds_) return fail(CCO_ERR_ARG, "descent settings missing");
+  if (method == CCO_METHOD_LBFGS && !ls_) return fail(CCO_ERR_ARG, "L-BFGS settings missing");
+  if (method != CCO_METHOD_DESCENT && method != CCO_METHOD_LBFGS)
+    return fail(CCO_ERR_ARG, "unknown method");
+  if (method == CCO_METHOD_DESCENT && ds_->max_evals < 0)
+    return fail(CCO_ERR_ARG, "max_evals must be >= 0");
+  if (method == CCO_METHOD_LBFGS) {
+    if (ls_->memory < 1 || ls_->memory > 1024) return fail(CCO_ERR_ARG, "memory must be in [1, 1024]");
+    if (ls_->n_rescales < 1 || ls_->n_rescales > kMaxRescales)
+      return fail(CCO_ERR_ARG, "need 1 to 8 rescales");
+    if (ls_->max_linesearch_steps < 1)
+      return fail(CCO_ERR_ARG, "max_linesearch_steps must be >= 1");
+  }
+  DevScope dscope(ctx->device);
+  cco_relax* R = new cco_relax();
+  R->ctx = ctx;
+  R->E = E;
+  R->B = B;
+  R->N = N;
+  R->d = d;
+  R->method = method;
+  std::memset(&R->dc, 0, sizeof(R->dc));
+  std::memset(&R->lc, 0, sizeof(R->lc));
+  int cap = 0;
+  if (method == CCO_METHOD_DESCENT) {
+    const cco_descent_settings& s = *ds_;
+    R->dc.step_init = s.step_init;
+    R->dc.step_min = s.step_min;
+    R->dc.shrink = s.shrink;
+    R->dc.grow = s.grow;
+    R->dc.rel_tol = s.rel_tol;
+    R->dc.energy_tol = s.energy_tol;
+    R->dc.max_iters = s.max_evals / 2;
+    R->dc.record_every = s.record_every;
+    R->dc.record_cap = (s.record_every > 0) ? s.max_records : 0;
+    cap = R->dc.record_cap;
+  } else {
+    const cco_lbfgs_settings& s = *ls_;
+    for (int i = 0; i < s.n_rescales; ++i) R->lc.rescales[i] = s.rescales[i];
+    R->lc.n_rescales = s.n_rescales;
+    R->lc.memory = s.memory;
+    R->lc.max_evals_per_pass = s.max_evals_per_pass;
+    R->lc.energy_tol = s.energy_tol;
+    R->lc.grad_tol = s.grad_tol;
+    R->lc.stagnation_patience = s.stagnation_patience;
+    R->lc.max_linesearch_steps = s.max_linesearch_steps;
+    R->lc.record_every = s.record_every;
+    R->lc.record_cap = (s.record_every > 0) ? s.max_records : 0;
+    cap = R->lc.record_cap;
+  }
+  RelaxDev& v = R->dev;
+  v.B = B;
+  v.nd = N * d;
+  v.stride = N * d;
+  const size_t vec = (size_t)B * v.stride;
+  int rc = 0;
+  auto A = [&](auto*& p, size_t cnt) {
+    if (rc == 0) rc = alloc_into(R, p, cnt);
+  };
+  A(v.status, B);
+  A(v.active, B);
+  A(v.n_active, 1);
+  A(v.s_eval, vec);
+  A(v.g_eval, vec);
+  A(v.phi_eval, B);
+  A(v.s, vec);
+  A(v.g, vec);
+  if (method == CCO_METHOD_DESCENT) {
+    A(v.dl, B);
+  } else {
+    const size_t m = R->lc.memory;
+    A(v.ll, B);
+    A(v.x, vec);
+    A(v.px, vec);
+    A(v.pg, vec);
+    A(v.u, vec);
+    A(v.safe_g, vec);
+    A(v.S, vec * m);
+    A(v.Y, vec * m);
+    A(v.rho, (size_t)B * m);
+  }
+  if (cap > 0) {
+    A(v.tr_s, vec * cap);
+    A(v.tr_e, (size_t)B * cap);
+    A(v.tr_step, (size_t)B * cap);
+    A(v.tr_it, (size_t)B * cap);
+  }
+  if (rc == 0 && E) {
+    rc = dev_reserve(R->part, sizeof(double2) * (size_t)B * n_splits_for(N) * E->geo.Mbox);
+    if (rc == 0) rc = dev_reserve(R->cbox, sizeof(double2) * (size_t)B * E->geo.Mbox);
+  }
+  if (rc == 0 && hipHostMalloc((void**)&R->host_count, sizeof(int)) != hipSuccess)
+    rc = fail(CCO_ERR_DEVICE, "hipHostMalloc failed");
+  if (rc) {
+    cco_relax_destroy(R);
+    return rc;
+  }
+  *out = R;
+  return CCO_OK;
+}
+
+int cco_relax_destroy(cco_relax* R) {
+  if (!R) return CCO_OK;
+  DevScope ds(R->ctx->device);
+  (void)hipStreamSynchronize(R->ctx->stream);
+  for (DevBuf& b : R->bufs) dev_release(b);
+  for (DevBuf* b : {&R->part, &R->cbox, &R->out_s, &R->out_scalars, &R->pack, &R->pack_idx})
+    dev_release(*b);
+  if (R->host_count) (void)hipHostFree(R->host_count);
+  delete R;
+  return CCO_OK;
+}
+
+int cco_relax_init_dev(cco_relax* R, const double* s0_dev) {
+  if (!R || !s0_dev) return fail(CCO_ERR_ARG, "NULL argument");
+  DevScope ds(R->ctx->device);
+  hipStream_t st = R->ctx->stream;
+  const size_t bytes = sizeof(double) * (size_t)R->B * R->dev.stride;
+  CCO_HIP(hipMemcpyAsync(R->dev.s, s0_dev, bytes, hipMemcpyDeviceToDevice, st));
+  CCO_HIP(hipMemcpyAsync(R->dev.s_eval, s0_dev, bytes, hipMemcpyDeviceToDevice, st));
+  hipLaunchKernelGGL(relax_init_kernel, dim3((R->B + 255) / 256), dim3(256), 0, st, R->dev,
+                     R->method);
+  CCO_HIP(hipGetLastError());
+  R->slots_ub = R->B;
+  R->steps = 0;
+  R->lane_evals = 0;
+  R->ready = true;
+  return CCO_OK;
+}
+
+int cco_relax_init(cco_relax* R, const double* s0_host) {
+  if (!R || !s0_host) return fail(CCO_ERR_ARG, "NULL argument");
+  DevScope ds(R->ctx->device);
+  hipStream_t st = R->ctx->stream;
+  const size_t bytes = sizeof(double) * (size_t)R->B * R->dev.stride;
+  CCO_HIP(hipMemcpyAsync(R->dev.s, s0_host, bytes, hipMemcpyHostToDevice, st));
+  CCO_HIP(hipMemcpyAsync(R->dev.s_eval, R->dev.s, bytes, hipMemcpyDeviceToDevice, st));
+  hipLaunchKernelGGL(relax_init_kernel, dim3((R->B + 255) / 256), dim3(256), 0, st, R->dev,
+                     R->method);
+  CCO_HIP(hipGetLastError());
+  CCO_HIP(hipStreamSynchronize(st));
+  R->slots_ub = R->B;
+  R->steps = 0;
+  R->lane_evals = 0;
+  R->ready = true;
+  return CCO_OK;
+}
+
+int cco_relax_step(cco_relax* R, int64_t n_evals, int* n_active) {
+  if (!R) return fail(CCO_ERR_ARG, "relax is NULL");
+  if (!R->ready) return fail(CCO_ERR_STATE, "relax not initialised");
+  if (!R->E) return fail(CCO_ERR_STATE, "external-energy relax: use pending/submit");
+  DevScope ds(R->ctx->device);
+  for (int64_t i = 0; i < n_evals && R->slots_ub > 0; ++i) {
+    int rc = relax_eval_builtin(R);
+    if (rc) return rc;
+    rc = relax_advance(R);
+    if (rc) return rc;
+    R->steps += 1;
+  }
+  return relax_sync_count(R, n_active);
+}
+
+int cco_relax_run(cco_relax* R, int64_t max_evals, int sync_every) {
+  if (!R) return fail(CCO_ERR_ARG, "relax is NULL");
+  if (sync_every < 1) return fail(CCO_ERR_ARG, "sync_every must be >= 1");
+  int n_active = R->slots_ub;
+  while (n_active > 0) {
+    if (R->steps >= max_evals)
+      return fail(CCO_ERR_BOUND, "relax did not finish within the round bound");
+    const int64_t chunk = std::min<int64_t>(sync_every, max_evals - R->steps);
+    int rc = cco_relax_step(R, chunk, &n_active);
+    if (rc) return rc;
+  }
+  return CCO_OK;
+}
+
+int cco_relax_pending(cco_relax* R, int* n_active, int32_t* lanes_host, double* s_trial_host) {
+  if (!R || !n_active) return fail(CCO_ERR_ARG, "NULL argument");
+  if (!R->ready) return fail(CCO_ERR_STATE, "relax not initialised");
+  DevScope ds(R->ctx->device);
+  int rc = relax_sync_count(R, n_active);
+  if (rc) return rc;
+  const int na = *n_active;
+  if (na == 0) return CCO_OK;
+  hipStream_t st = R->ctx->stream;
+  if (lanes_host)
+    CCO_HIP(hipMemcpyAsync(lanes_host, R->dev.active, sizeof(int) * na, hipMemcpyDeviceToHost, st));
+  if (s_trial_host) {
+    const int nd = R->dev.nd;
+    if ((rc = dev_reserve(R->pack, sizeof(double) * (size_t)R->B * nd))) return rc;
+    hipLaunchKernelGGL(gather_rows_kernel, dim3(na), dim3(256), 0, st, R->dev.s_eval,
+                       R->dev.active, R->dev.n_active, (double*)R->pack.p, nd, R->dev.stride);
+    CCO_HIP(hipGetLastError());
+    CCO_HIP(hipMemcpyAsync(s_trial_host, R->pack.p, sizeof(double) * (size_t)na * nd,
+                           hipMemcpyDeviceToHost, st));
+  }
+  CCO_HIP(hipStreamSynchronize(st));
+  return CCO_OK;
+}
+
+int cco_relax_submit(cco_relax* R, const double* phi_host, const double* grad_host) {
+  if (!R || !phi_host || !grad_host) return fail(CCO_ERR_ARG, "NULL argument");
+  if (!R->ready) return fail(CCO_ERR_STATE, "relax not initialised");
+  DevScope ds(R->ctx->device);
+  const int na = R->slots_ub;
+  if (na == 0) return fail(CCO_ERR_STATE, "no lane is awaiting an evaluation");
+  hipStream_t st = R->ctx->stream;
+  const int nd = R->dev.nd;
+  int rc;
+  if ((rc = dev_reserve(R->pack, sizeof(double) * (size_t)R->B * nd))) return rc;
+  if ((rc = dev_reserve(R->pack_idx, sizeof(double) * (size_t)R->B))) return rc;
+  CCO_HIP(hipMemcpyAsync(R->pack.p, grad_host, sizeof(double) * (size_t)na * nd,
+                         hipMemcpyHostToDevice, st));
+  CCO_HIP(hipMemcpyAsync(R->pack_idx.p, phi_host, sizeof(double) * na, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(scatter_eval_kernel, dim3(na), dim3(256), 0, st, (const double*)R->pack_idx.p,
+                     (const double*)R->pack.p, R->dev.active, R->dev.n_active, R->dev.phi_eval,
+                     R->dev.g_eval, nd, R->dev.stride);
+  CCO_HIP(hipGetLastError());
+  rc = relax_advance(R);
+  if (rc) return rc;
+  R->steps += 1;
+  CCO_HIP(hipStreamSynchronize(st));
+  return CCO_OK;
+}
+
+int cco_relax_result_dev(cco_relax* R, double* s_dev, double* energy_dev, double* grad_norm_dev,
+                         int64_t* n_evals_dev, int64_t* n_iters_dev, int64_t* status_dev) {
+  if (!R || !s_dev || !energy_dev || !grad_norm_dev || !n_evals_dev || !n_iters_dev || !status_dev)
+    return fail(CCO_ERR_ARG, "NULL argument");
+  if (!R->ready) return fail(CCO_ERR_STATE, "relax not initialised");
+  DevScope ds(R->ctx->device);
+  RelaxOutDev o;
+  o.s = s_dev;
+  o.energy = energy_dev;
+  o.grad_norm = grad_norm_dev;
+  o.n_evals = (long long*)n_evals_dev;
+  o.n_iters = (long long*)n_iters_dev;
+  o.status = (long long*)status_dev;
+  launch_finalize(R->dev, R->method, R->lc, o, R->ctx->stream);
+  CCO_HIP(hipGetLastError());
+  return CCO_OK;
+}
+
+int cco_relax_result(cco_relax* R, double* s_host, double* energy_host, double* grad_norm_host,
+                     int64_t* n_evals_host, int64_t* n_iters_host, int64_t* status_host) {
+  if (!R || !s_host || !energy_host || !grad_norm_host || !n_evals_host || !n_iters_host ||
+      !status_host)
+    return fail(CCO_ERR_ARG, "NULL argument");
+  DevScope ds(R->ctx->device);
+  const size_t vec = (size_t)R->B * R->dev.stride;
+  int rc;
+  if ((rc = dev_reserve(R->out_s, sizeof(double) * vec))) return rc;
+  if ((rc = dev_reserve(R->out_scalars, sizeof(double) * 6 * (size_t)R->B))) return rc;
+  double* sc = (double*)R->out_scalars.p;
+  const size_t B = R->B;
+  rc = cco_relax_result_dev(R, (double*)R->out_s.p, sc, sc + B, (int64_t*)(sc + 2 * B),
+                            (int64_t*)(sc + 3 * B), (int64_t*)(sc + 4 * B));
+  if (rc) return rc;
+  hipStream_t st = R->ctx->stream;
+  CCO_HIP(hipMemcpyAsync(s_host, R->out_s.p, sizeof(double) * vec, hipMemcpyDeviceToHost, st));
+  CCO_HIP(hipMemcpyAsync(energy_host, sc, sizeof(double) * B, hipMemcpyDeviceToHost, st));
+  CCO_HIP(hipMemcpyAsync(grad_norm_host, sc + B, sizeof(double) * B, hipMemcpyDeviceToHost, st));
+  CCO_HIP(hipMemcpyAsync(n_evals_host, sc + 2 * B, sizeof(int64_t) * B, hipMemcpyDeviceToHost, st));
+  CCO_HIP(hipMemcpyAsync(n_iters_host, sc + 3 * B, sizeof(int64_t) * B, hipMemcpyDeviceToHost, st));
+  CCO_HIP(hipMemcpyAsync(status_host, sc + 4 * B, sizeof(int64_t) * B, hipMemcpyDeviceToHost, st));
+  CCO_HIP(hipStreamSynchronize(st));
+  return CCO_OK;
+}
+
+int cco_relax_trajectory(cco_relax* R, int64_t* count_host, int64_t* iteration_host,
+                         double* s_host, double* energy_host, double* step_host) {
+  if (!R || !count_host) return fail(CCO_ERR_ARG, "NULL argument");
+  DevScope ds(R->ctx->device);
+  const int cap = R->method == CCO_METHOD_DESCENT ? R->dc.record_cap : R->lc.record_cap;
+  hipStream_t st = R->ctx->stream;
+  const size_t B = R->B;
+  if (R->method == CCO_METHOD_DESCENT) {
+    std::vector<DescentLane> L(B);
+    CCO_HIP(hipMemcpyAsync(L.data(), R->dev.dl, sizeof(DescentLane) * B, hipMemcpyDeviceToHost, st));
+    CCO_HIP(hipStreamSynchronize(st));
+    for (size_t i = 0; i < B; ++i) count_host[i] = L[i].rec_count;
+  } else {
+    std::vector<LbfgsLane> L(B);
+    CCO_HIP(hipMemcpyAsync(L.data(), R->dev.ll, sizeof(LbfgsLane) * B, hipMemcpyDeviceToHost, st));
+    CCO_HIP(hipStreamSynchronize(st));
+    for (size_t i = 0; i < B; ++i) count_host[i] = L[i].rec_count;
+  }
+  if (cap == 0) return CCO_OK;
+  const size_t rows = B * cap;
+  if (iteration_host)
+    CCO_HIP(hipMemcpyAsync(iteration_host, R->dev.tr_it, sizeof(int64_t) * rows,
+                           hipMemcpyDeviceToHost, st));
+  if (s_host)
+    CCO_HIP(hipMemcpyAsync(s_host, R->dev.tr_s, sizeof(double) * rows * R->dev.stride,
+                           hipMemcpyDeviceToHost, st));
+  if (energy_host)
+    CCO_HIP(hipMemcpyAsync(energy_host, R->dev.tr_e, sizeof(double) * rows, hipMemcpyDeviceToHost, st));
+  if (step_host)
+    CCO_HIP(hipMemcpyAsync(step_host, R->dev.tr_step, sizeof(double) * rows, hipMemcpyDeviceToHost,
+                           st));
+  CCO_HIP(hipStreamSynchronize(st));
+  return CCO_OK;
+}
+
+int cco_relax_stats(cco_relax* R, int64_t* steps, int64_t* lane_evals) {
+  if (!R) return fail(CCO_ERR_ARG, "relax is NULL");
+  if (steps) *steps = R->steps;
+  if (lane_evals) *lane_evals = R->lane_evals;
+  return CCO_OK;
+}
+
+}  // extern "C"

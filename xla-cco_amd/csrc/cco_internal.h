@@ -1,0 +1,120 @@
+// Internal declarations shared by the gfx950 kernels and the C-ABI layer.
+//
+// Layout conventions (all device memory, all FP64 unless noted):
+//   positions  s[lane][j][a]      lane stride = lane_stride doubles (>= N*d)
+//   gradients  g[lane][j][a]      same strides as positions
+//   rho box    part[lane][split][i], cbox[lane][i]   i < Mbox (complex)
+// The k-set is stored as a "box" of columns: a column is one tail
+// (n_2, ..., n_d) of the integer vector, and holds rows n_1 = 0 .. L-1.
+// rho over the box is the complex contraction
+//   rho[c][r] = sum_j z1_j^r * B_c(j),   B_c(j) = z2_j^{n2(c)} z3_j^{n3(c)},
+//   z_a = exp(2 pi i s_a),
+// built by phase recurrence along r, so no trig is evaluated per (k, j).
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <cstdio>
+#include <string>
+
+namespace cco {
+
+constexpr int kWave = 64;
+constexpr int kRowSeg = 32;      // rows of one rho accumulator segment
+constexpr int kRhoChunk = 64;    // particles staged in LDS per rho chunk
+constexpr int kForceTile = 64;   // particles per force workgroup (one wave)
+
+// Geometry of one k-set, as the kernels read it.  Pointers are device memory.
+struct Geo {
+  int d;
+  int M;        // real k count
+  int M_pad;    // padded k count (multiple of pad_to)
+  int Mbox;     // entries in the column box
+  int Cn;       // number of columns
+  int P1, P2, P3;  // max |n_a| per axis (P2 = P3 = 0 when absent)
+  int CT;       // rho column tile width (threads per rho workgroup)
+  int n_coltiles;
+  int n_segs;   // row segments of kRowSeg
+  int n_u;      // force traversal: u = 0 .. n_u-1 (u = |n2|)
+  double volume;
+  double pref;  // 2 * (2 pi) / V
+  const int4* col;       // rho order: {box offset, L, n2, n3}
+  const int* tile_lmax;  // per column tile: max L
+  const int* kbox;       // k -> box index (M); ~i marks a conjugated row
+  const double* wbox;    // summed weight of the k rows on each box entry (Mbox)
+  const int2* ustart;    // per u: {first group, number of v values}
+  const int4* gmem;      // per group: 4 members {box offset, L, n2, n3}
+};
+
+// Per-launch batch description for the energy kernels.
+struct Batch {
+  const int* active;     // slot -> lane
+  const int* n_active;   // device count (slots >= count exit)
+  int N;                 // particles per lane
+  int lane_stride;       // doubles between lanes in s and grad
+  int n_splits;          // rho particle splits per lane
+  int split_len;         // particles per split
+  int n_ftiles;          // force tiles per lane
+  const double* s;       // positions
+  double* grad;          // gradient out (may be null for energy only)
+  double* phi;           // energy out [lane]
+  double* re;            // Re rho out [lane][M_pad] (may be null)
+  double* im;            // Im rho out
+  double2* part;         // rho partials [lane][split][Mbox]
+  double2* cbox;         // w * rho per box entry [lane][Mbox]
+};
+
+void launch_energy(const Geo& g, const Batch& b, int slots_ub, bool with_grad,
+                   hipStream_t stream);
+size_t rho_lds_bytes(const Geo& g);
+
+// ---------------- small helpers ----------------
+
+struct Error {
+  int code;
+  std::string msg;
+};
+
+void set_last_error(const std::string& msg);
+
+#define CCO_HIP(call)                                                        \
+  do {                                                                       \
+    hipError_t e_ = (call);                                                  \
+    if (e_ != hipSuccess) {                                                  \
+      ::cco::set_last_error(std::string(#call) + ": " + hipGetErrorString(e_)); \
+      return CCO_ERR_DEVICE;                                                 \
+    }                                                                        \
+  } while (0)
+
+__device__ __forceinline__ double2 cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+
+// Deterministic sum over a workgroup of up to 1024 threads (fixed tree order).
+__device__ __forceinline__ double wave_sum(double v) {
+  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
+  return v;
+}
+
+template <int BLOCK>
+__device__ __forceinline__ double block_sum(double v, double* lds) {
+  constexpr int nw = BLOCK / kWave;
+  v = wave_sum(v);
+  const int wid = threadIdx.x / kWave;
+  const int lid = threadIdx.x % kWave;
+  __syncthreads();
+  if (lid == 0) lds[wid] = v;
+  __syncthreads();
+  double t = 0.0;
+  if (nw == 1) {
+    t = lds[0];
+  } else {
+#pragma unroll
+    for (int i = 0; i < nw; ++i) t += lds[i];
+  }
+  return t;
+}
+
+}  // namespace cco

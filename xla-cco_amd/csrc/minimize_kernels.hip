@@ -1,0 +1,566 @@
+// Batched relax on the device: adaptive descent and three-pass L-BFGS.
+//
+// Each lane is a state machine advanced once per energy evaluation, so all
+// active lanes share every energy launch whatever stage of a line search
+// they are in.  One workgroup per active lane runs the vector work (norms,
+// memory update, two-loop recursion) with deterministic block reductions;
+// thread 0 runs the scalar line-search logic.  A lane whose status leaves
+// RUNNING is dropped from the active list by compact_kernel and is never
+// written again (bit-exact freeze, src/xla_cco/minimize/_loop.py:142-152).
+//
+// Restated algorithms:
+//   descent  : src/xla_cco/minimize/descent.py:92-197
+//   L-BFGS   : src/xla_cco/minimize/lbfgs.py:163-329 over optax 0.2.8
+//              lbfgs(memory, scale_by_zoom_linesearch(max_steps, "one"))
+// Floating-point contraction is off in this file so the minimizer arithmetic
+// rounds exactly like its NumPy statement (x + a*u, q - a*y, ...).
+#pragma clang fp contract(off)
+
+#include "minimize_kernels.h"
+
+namespace cco {
+
+__device__ __forceinline__ double nanmax(double a, double b) {
+  if (a != a || b != b) return __longlong_as_double(0x7ff8000000000000LL);
+  return a > b ? a : b;
+}
+__device__ __forceinline__ double nanmin(double a, double b) {
+  if (a != a || b != b) return __longlong_as_double(0x7ff8000000000000LL);
+  return a < b ? a : b;
+}
+
+__device__ __forceinline__ double wrap01(double v) {
+  const double w = v - floor(v);
+  const double below_one = 0x1.fffffffffffffp-1;
+  return w < below_one ? w : below_one;
+}
+
+// ---------------------------------------------------------------------------
+// compaction: active list = lanes with status RUNNING, in lane order
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(1024) void compact_kernel(const int* __restrict__ status,
+                                                       int* __restrict__ active,
+                                                       int* __restrict__ n_active, int B) {
+  __shared__ int cnt[1024];
+  const int t = threadIdx.x;
+  const int per = (B + 1023) / 1024;
+  const int lo = min(B, t * per);
+  const int hi = min(B, lo + per);
+  int c = 0;
+  for (int i = lo; i < hi; ++i) c += (status[i] == ST_RUNNING);
+  cnt[t] = c;
+  __syncthreads();
+  for (int off = 1; off < 1024; off <<= 1) {
+    const int v = (t >= off) ? cnt[t - off] : 0;
+    __syncthreads();
+    cnt[t] += v;
+    __syncthreads();
+  }
+  int pos = cnt[t] - c;
+  for (int i = lo; i < hi; ++i)
+    if (status[i] == ST_RUNNING) active[pos++] = i;
+  if (t == 1023) *n_active = cnt[1023];
+}
+
+void launch_compact(const RelaxDev& r, hipStream_t st) {
+  hipLaunchKernelGGL(compact_kernel, dim3(1), dim3(1024), 0, st, r.status, r.active, r.n_active,
+                     r.B);
+}
+
+// ---------------------------------------------------------------------------
+// adaptive descent (descent.py:92-104 status, :153-175 step)
+// ---------------------------------------------------------------------------
+__device__ int descent_status(double f, bool all_zero, double step, long long it,
+                              const DescentCfg& c) {
+  int code = it >= c.max_iters ? ST_BUDGET : ST_RUNNING;
+  if (step <= c.step_min) code = ST_STEP_FLOOR;
+  if (all_zero) code = ST_GRAD_TOL;
+  if (f < c.energy_tol) code = ST_ENERGY_TOL;
+  return code;
+}
+
+__global__ __launch_bounds__(kAdvBlock) void descent_advance_kernel(RelaxDev r, DescentCfg c) {
+  __shared__ double red[16];
+  const int slot = blockIdx.x;
+  if (slot >= *r.n_active) return;
+  const int lane = r.active[slot];
+  const int nd = r.nd;
+  const size_t off = (size_t)lane * r.stride;
+  double* s = r.s + off;
+  double* g = r.g + off;
+  double* se = r.s_eval + off;
+  const double* ge = r.g_eval + off;
+  DescentLane L = r.dl[lane];
+  const double fe = r.phi_eval[lane];
+
+  if (L.mode == MODE_INIT) {
+    // init_one (descent.py:121-141): s = s0 already, f, g from the first evaluation
+    for (int i = threadIdx.x; i < nd; i += blockDim.x) g[i] = ge[i];
+    L.f = fe;
+    L.step = c.step_init;
+    L.it = 0;
+    L.n_evals = 0;
+    L.mode = MODE_EVAL;
+  } else {
+    const double f2 = fe;
+    const double df = L.f - f2;
+    const double df2 = L.step * L.norm;
+    const bool reject = fabs(df - df2) / df2 > c.rel_tol;  // NaN accepts
+    if (reject) {
+      L.step = L.step * c.shrink;
+    } else {
+      L.step = L.step * c.grow;
+      for (int i = threadIdx.x; i < nd; i += blockDim.x) {
+        s[i] = se[i];
+        g[i] = ge[i];
+      }
+      L.f = f2;
+    }
+    L.it += 1;
+    L.n_evals += 2;
+  }
+  // norm and the all-zero test of the (new) gradient
+  double ss = 0.0, nz = 0.0;
+  for (int i = threadIdx.x; i < nd; i += blockDim.x) {
+    const double v = g[i];
+    ss += v * v;
+    nz += (v != 0.0) ? 1.0 : 0.0;
+  }
+  ss = block_sum<kAdvBlock>(ss, red);
+  nz = block_sum<kAdvBlock>(nz, red);
+  const int code = descent_status(L.f, nz == 0.0, L.step, L.it, c);
+  // trajectory row (minimize/_loop.py:221-235), iteration 0 written at init
+  if (c.record_every > 0 && c.record_cap > 0) {
+    const bool init_row = (L.it == 0 && L.rec_count == 0);
+    const bool due = init_row || ((L.it % c.record_every) == 0 && L.rec_count < c.record_cap &&
+                                  L.it > 0);
+    if (due) {
+      const size_t row = (size_t)lane * c.record_cap + L.rec_count;
+      for (int i = threadIdx.x; i < nd; i += blockDim.x) r.tr_s[row * r.stride + i] = wrap01(s[i]);
+      if (threadIdx.x == 0) {
+        r.tr_e[row] = L.f;
+        r.tr_step[row] = L.step;
+        r.tr_it[row] = L.it;
+      }
+      L.rec_count += 1;
+    }
+  }
+  if (code == ST_RUNNING) {
+    const double norm = sqrt(ss);
+    L.norm = norm;
+    const double step = L.step;
+    for (int i = threadIdx.x; i < nd; i += blockDim.x) {
+      const double gn = norm > 0.0 ? g[i] / norm : 0.0 * g[i];
+      se[i] = s[i] - gn * step;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    r.dl[lane] = L;
+    r.status[lane] = code;
+  }
+}
+
+void launch_descent_advance(const RelaxDev& r, const DescentCfg& c, int slots_ub,
+                            hipStream_t st) {
+  if (slots_ub <= 0) return;
+  hipLaunchKernelGGL(descent_advance_kernel, dim3(slots_ub), dim3(kAdvBlock), 0, st, r, c);
+}
+
+// ---------------------------------------------------------------------------
+// L-BFGS with zoom line search
+// ---------------------------------------------------------------------------
+constexpr double kSlopeRtol = 1e-4;
+constexpr double kCurvRtol = 0.9;
+constexpr double kApproxDecRtol = 1e-6;
+constexpr double kIncrease = 2.0;
+constexpr double kIntervalMin = 1e-5;
+
+__device__ double ls_decrease_error(double a, double va, double sa, double v0, double s0) {
+  double dec = va - (v0 + kSlopeRtol * a * s0);
+  double approx = sa - (2.0 * kSlopeRtol - 1.0) * s0;
+  const double delta = va - v0 - kApproxDecRtol * fabs(v0);
+  approx = nanmax(approx, delta);
+  dec = nanmin(approx, dec);
+  return nanmax(dec, 0.0);
+}
+
+__device__ double ls_curvature_error(double sa, double s0) {
+  return nanmax(fabs(sa) - kCurvRtol * fabs(s0), 0.0);
+}
+
+__device__ double cubicmin(double a, double fa, double fpa, double b, double fb, double c,
+                           double fc) {
+  const double C = fpa;
+  const double db = b - a;
+  const double dc = c - a;
+  const double denom = (db * dc) * (db * dc) * (db - dc);
+  const double d2a = fb - fa - C * db;
+  const double d2b = fc - fa - C * dc;
+  const double A = (dc * dc * d2a + (-(db * db)) * d2b) / denom;
+  const double Bq = ((-(dc * dc * dc)) * d2a + db * db * db * d2b) / denom;
+  const double radical = Bq * Bq - 3.0 * A * C;
+  return a + (-Bq + sqrt(radical)) / (3.0 * A);
+}
+
+__device__ double quadmin(double a, double fa, double fpa, double b, double fb) {
+  const double db = b - a;
+  const double Bq = (fb - fa - fpa * db) / (db * db);
+  return a - fpa / (2.0 * Bq);
+}
+
+// Propose the next trial step of a running line search.
+__device__ void ls_propose(LsState& L) {
+  if (!L.interval_found) {
+    L.a = L.count == 0 ? 1.0 : kIncrease * L.prev_a;
+    L.delta = 0.0;
+    return;
+  }
+  const double delta = fabs(L.high_a - L.low_a);
+  const double left = fmin(L.high_a, L.low_a);
+  const double right = fmax(L.high_a, L.low_a);
+  const double mc = cubicmin(L.low_a, L.low_v, L.low_s, L.high_a, L.high_v, L.cref_a, L.cref_v);
+  const bool use_cubic = (mc > left + 0.2 * delta) && (mc < right - 0.2 * delta);
+  const double mq = quadmin(L.low_a, L.low_v, L.low_s, L.high_a, L.high_v);
+  const bool use_quad = !use_cubic && (mq > left + 0.1 * delta) && (mq < right - 0.1 * delta);
+  L.a = use_cubic ? mc : (use_quad ? mq : (L.low_a + L.high_a) / 2.0);
+  L.delta = delta;
+}
+
+// Consume the value and slope at the current trial step.  Returns flags:
+// bit 0 = record as safe step, bit 1 = search finished, bit 2 = failed.
+__device__ int ls_consume(LsState& L, double va, double sa, int max_steps) {
+  const double a = L.a;
+  const double dec = ls_decrease_error(a, va, sa, L.f0, L.slope0);
+  const double err = nanmax(dec, ls_curvature_error(sa, L.slope0));
+  bool safe, done, failed;
+  if (!L.interval_found) {
+    safe = dec <= 0.0;
+    const bool set_high = (dec > 0.0) || (va >= L.prev_v && L.count > 0);
+    const bool set_low = (sa >= 0.0) && !set_high;
+    if (set_low) {
+      L.high_a = L.prev_a; L.high_v = L.prev_v; L.high_s = L.prev_s;
+      L.low_a = a; L.low_v = va; L.low_s = sa;
+    } else {
+      L.low_a = L.prev_a; L.low_v = L.prev_v; L.low_s = L.prev_s;
+      L.high_a = a; L.high_v = va; L.high_s = sa;
+    }
+    L.interval_found = (set_high || set_low || err <= 0.0) ? 1 : 0;
+    done = err <= 0.0;
+    L.cref_a = L.low_a;
+    L.cref_v = L.low_v;
+    L.count += 1;
+    failed = (L.count >= max_steps) && !done;
+  } else {
+    safe = (dec <= 0.0) && (va < L.safe_v);
+    done = err <= 0.0;
+    const bool hi_to_mid = (dec > 0.0) || (va >= L.low_v);
+    const bool hi_to_low = (sa * (L.high_a - L.low_a) >= 0.0) && !hi_to_mid;
+    const double ola = L.low_a, olv = L.low_v, ols = L.low_s;
+    const double oha = L.high_a, ohv = L.high_v;
+    if (hi_to_mid) {
+      L.high_a = a; L.high_v = va; L.high_s = sa;
+    } else if (hi_to_low) {
+      L.high_a = ola; L.high_v = olv; L.high_s = ols;
+    }
+    if (!hi_to_mid) {
+      L.low_a = a; L.low_v = va; L.low_s = sa;
+    }
+    if (hi_to_mid || hi_to_low) {
+      L.cref_a = oha; L.cref_v = ohv;
+    } else {
+      L.cref_a = ola; L.cref_v = olv;
+    }
+    L.count += 1;
+    failed = ((L.count >= max_steps) || (L.delta <= kIntervalMin)) && !done;
+  }
+  L.prev_a = a; L.prev_v = va; L.prev_s = sa;
+  L.last_dec = dec;
+  if (safe) {
+    L.safe_a = a;
+    L.safe_v = va;
+  }
+  const bool finished = done || failed;
+  if (!finished) ls_propose(L);
+  return (safe ? 1 : 0) | (finished ? 2 : 0) | (failed ? 4 : 0);
+}
+
+// Start an L-BFGS iteration: memory update, two-loop recursion, line-search
+// init, first trial point x + 1 * u.  All threads call it.
+__device__ void lbfgs_start_iteration(const RelaxDev& r, const LbfgsCfg& c, int lane,
+                                      LbfgsLane& L, double* red) {
+  const int nd = r.nd;
+  const size_t off = (size_t)lane * r.stride;
+  const int m = c.memory;
+  const double* x = r.x + off;
+  const double* gx = r.g + off;
+  double* px = r.px + off;
+  double* pg = r.pg + off;
+  double* u = r.u + off;
+  double* S = r.S + (size_t)lane * m * r.stride;
+  double* Y = r.Y + (size_t)lane * m * r.stride;
+  double* rho = r.rho + (size_t)lane * m;
+  const double rs = c.rescales[L.pass_idx];
+  const int count = L.opt_count;
+  double gamma;
+  if (count > 0) {
+    const int slot = (count - 1) % m;
+    double* Ss = S + (size_t)slot * r.stride;
+    double* Ys = Y + (size_t)slot * r.stride;
+    double sy = 0.0, yy = 0.0;
+    for (int i = threadIdx.x; i < nd; i += blockDim.x) {
+      const double ds = x[i] - px[i];
+      const double dy = gx[i] - pg[i];
+      Ss[i] = ds;
+      Ys[i] = dy;
+      sy += dy * ds;
+      yy += dy * dy;
+    }
+    sy = block_sum<kAdvBlock>(sy, red);
+    yy = block_sum<kAdvBlock>(yy, red);
+    if (threadIdx.x == 0) rho[slot] = (sy == 0.0) ? 0.0 : 1.0 / sy;
+    gamma = (yy > 0.0) ? sy / yy : 1.0;
+  } else {
+    double gg = 0.0;
+    for (int i = threadIdx.x; i < nd; i += blockDim.x) gg += gx[i] * gx[i];
+    gg = block_sum<kAdvBlock>(gg, red);
+    gamma = nanmin(1.0, 1.0 / sqrt(gg));
+  }
+  __syncthreads();
+  // two-loop recursion over the written slots, newest first (q lives in u)
+  const int nvalid = count < m ? count : m;
+  const int midx = count % m;
+  __shared__ double alphas[1024];
+  for (int i = threadIdx.x; i < nd; i += blockDim.x) u[i] = gx[i];
+  for (int t = m - 1; t >= m - nvalid; --t) {
+    const int sl = (midx + t) % m;
+    const double* Si = S + (size_t)sl * r.stride;
+    const double* Yi = Y + (size_t)sl * r.stride;
+    double dot = 0.0;
+    for (int i = threadIdx.x; i < nd; i += blockDim.x) dot += Si[i] * u[i];
+    dot = block_sum<kAdvBlock>(dot, red);
+    const double a = rho[sl] * dot;
+    if (threadIdx.x == 0) alphas[t] = a;
+    for (int i = threadIdx.x; i < nd; i += blockDim.x) u[i] = u[i] + (-a) * Yi[i];
+  }
+  for (int i = threadIdx.x; i < nd; i += blockDim.x) u[i] = gamma * u[i];
+  __syncthreads();
+  for (int t = m - nvalid; t < m; ++t) {
+    const int sl = (midx + t) % m;
+    const double* Si = S + (size_t)sl * r.stride;
+    const double* Yi = Y + (size_t)sl * r.stride;
+    double dot = 0.0;
+    for (int i = threadIdx.x; i < nd; i += blockDim.x) dot += Yi[i] * u[i];
+    dot = block_sum<kAdvBlock>(dot, red);
+    const double bcoef = rho[sl] * dot;
+    const double coef = alphas[t] - bcoef;
+    for (int i = threadIdx.x; i < nd; i += blockDim.x) u[i] = u[i] + coef * Si[i];
+  }
+  // direction u = -H g, remember (x, g) for the next memory update
+  double slope = 0.0;
+  for (int i = threadIdx.x; i < nd; i += blockDim.x) {
+    const double ui = -u[i];
+    u[i] = ui;
+    px[i] = x[i];
+    pg[i] = gx[i];
+    slope += ui * gx[i];
+  }
+  slope = block_sum<kAdvBlock>(slope, red);
+  L.opt_count = count + 1;
+  LsState& Q = L.ls;
+  Q.f0 = L.f;
+  Q.slope0 = slope;
+  Q.count = 0;
+  Q.interval_found = 0;
+  Q.prev_a = 0.0; Q.prev_v = L.f; Q.prev_s = slope;
+  Q.low_a = 0.0; Q.low_v = L.f; Q.low_s = slope;
+  Q.high_a = 0.0; Q.high_v = L.f; Q.high_s = slope;
+  Q.cref_a = 0.0; Q.cref_v = L.f;
+  Q.safe_a = 0.0; Q.safe_v = L.f;
+  Q.last_dec = __longlong_as_double(0x7ff0000000000000LL);
+  ls_propose(Q);
+  const double a = Q.a;
+  double* se = r.s_eval + off;
+  for (int i = threadIdx.x; i < nd; i += blockDim.x) se[i] = (x[i] + a * u[i]) / rs;
+}
+
+__global__ __launch_bounds__(kAdvBlock) void lbfgs_advance_kernel(RelaxDev r, LbfgsCfg c) {
+  __shared__ double red[16];
+  const int slot = blockIdx.x;
+  if (slot >= *r.n_active) return;
+  const int lane = r.active[slot];
+  const int nd = r.nd;
+  const size_t off = (size_t)lane * r.stride;
+  double* s = r.s + off;
+  double* x = r.x + off;
+  double* gx = r.g + off;
+  double* u = r.u + off;
+  double* sg = r.safe_g + off;
+  double* ge = r.g_eval + off;
+  LbfgsLane L = r.ll[lane];
+  const double fe = r.phi_eval[lane];
+  int code = ST_RUNNING;
+
+  if (L.mode == MODE_INIT) {
+    // init_one (lbfgs.py:163-197)
+    const double r0 = c.rescales[0];
+    for (int i = threadIdx.x; i < nd; i += blockDim.x) {
+      x[i] = s[i] * r0;
+      gx[i] = ge[i] / r0;
+    }
+    L.f = fe;
+    L.last_step = 0.0;
+    L.it = 0;
+    L.n_evals = 1;
+    L.pass_idx = 0;
+    L.pass_evals = 0;
+    L.stag = 0;
+    L.opt_count = 0;
+    L.mode = MODE_EVAL;
+    code = (fe < c.energy_tol) ? ST_ENERGY_TOL : ST_RUNNING;
+    if (c.record_every > 0 && c.record_cap > 0 && L.rec_count == 0) {
+      for (int i = threadIdx.x; i < nd; i += blockDim.x) r.tr_s[(size_t)lane * c.record_cap * r.stride + i] = wrap01(s[i]);
+      if (threadIdx.x == 0) {
+        r.tr_e[(size_t)lane * c.record_cap] = L.f;
+        r.tr_step[(size_t)lane * c.record_cap] = 0.0;
+        r.tr_it[(size_t)lane * c.record_cap] = 0;
+      }
+      L.rec_count = 1;
+    }
+    __syncthreads();
+    if (code == ST_RUNNING) lbfgs_start_iteration(r, c, lane, L, red);
+  } else {
+    const double rs = c.rescales[L.pass_idx];
+    // gradient w.r.t. x at the trial point, and its slope along u
+    double sa = 0.0;
+    for (int i = threadIdx.x; i < nd; i += blockDim.x) {
+      const double gi = ge[i] / rs;
+      ge[i] = gi;
+      sa += gi * u[i];
+    }
+    sa = block_sum<kAdvBlock>(sa, red);
+    // every thread runs the same scalar update, so L stays uniform
+    const int flags = ls_consume(L.ls, fe, sa, c.max_linesearch_steps);
+    if (flags & 1) {
+      for (int i = threadIdx.x; i < nd; i += blockDim.x) sg[i] = ge[i];
+    }
+    if (flags & 2) {
+      const bool use_safe = (flags & 4) && (L.ls.safe_a > 0.0);
+      const double alpha = use_safe ? L.ls.safe_a : L.ls.prev_a;
+      const double value = use_safe ? L.ls.safe_v : fe;
+      const double dec = use_safe ? 0.0 : L.ls.last_dec;
+      const double* gnew = use_safe ? sg : ge;
+      const bool fail = (dec > 0.0) || !isfinite(value);
+      const long long evals = L.ls.count > 1 ? L.ls.count : 1;
+      const double f_old = L.f;
+      __syncthreads();
+      if (!fail) {
+        for (int i = threadIdx.x; i < nd; i += blockDim.x) {
+          const double xi = x[i] + alpha * u[i];
+          x[i] = xi;
+          s[i] = xi / rs;
+          gx[i] = gnew[i];
+        }
+        L.f = value;
+      }
+      L.stag = (L.f < f_old) ? 0 : L.stag + 1;
+      L.pass_evals += evals;
+      double gg = 0.0;
+      for (int i = threadIdx.x; i < nd; i += blockDim.x) gg += gx[i] * gx[i];
+      gg = block_sum<kAdvBlock>(gg, red);
+      const double gnorm = sqrt(gg) * rs;
+      int glob = (gnorm < c.grad_tol) ? ST_GRAD_TOL : ST_RUNNING;
+      if (L.f < c.energy_tol) glob = ST_ENERGY_TOL;
+      int local = (L.pass_evals >= c.max_evals_per_pass) ? ST_BUDGET : ST_RUNNING;
+      if (L.stag >= c.stagnation_patience) local = ST_STAGNATED;
+      if (fail) local = ST_LINESEARCH_FAIL;
+      const bool last = L.pass_idx >= c.n_rescales - 1;
+      const bool pass_end = (local != ST_RUNNING) && (glob == ST_RUNNING);
+      const bool advance = pass_end && !last;
+      code = (glob != ST_RUNNING) ? glob : ST_RUNNING;
+      if (pass_end && last) code = local;
+      if (advance) {
+        const double rn = c.rescales[L.pass_idx + 1];
+        for (int i = threadIdx.x; i < nd; i += blockDim.x) {
+          x[i] = s[i] * rn;
+          gx[i] = gx[i] * rs / rn;
+        }
+        L.opt_count = 0;
+        L.pass_idx += 1;
+        L.pass_evals = 0;
+        L.stag = 0;
+      }
+      L.last_step = alpha;
+      L.it += 1;
+      L.n_evals += evals;
+      if (c.record_every > 0 && c.record_cap > 0 && (L.it % c.record_every) == 0 &&
+          L.rec_count < c.record_cap) {
+        const size_t row = (size_t)lane * c.record_cap + L.rec_count;
+        for (int i = threadIdx.x; i < nd; i += blockDim.x) r.tr_s[row * r.stride + i] = wrap01(s[i]);
+        if (threadIdx.x == 0) {
+          r.tr_e[row] = L.f;
+          r.tr_step[row] = L.last_step;
+          r.tr_it[row] = L.it;
+        }
+        L.rec_count += 1;
+      }
+      __syncthreads();
+      if (code == ST_RUNNING) lbfgs_start_iteration(r, c, lane, L, red);
+    } else {
+      const double a = L.ls.a;
+      double* se = r.s_eval + off;
+      for (int i = threadIdx.x; i < nd; i += blockDim.x) se[i] = (x[i] + a * u[i]) / rs;
+    }
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    r.ll[lane] = L;
+    r.status[lane] = code;
+  }
+}
+
+void launch_lbfgs_advance(const RelaxDev& r, const LbfgsCfg& c, int slots_ub, hipStream_t st) {
+  if (slots_ub <= 0) return;
+  hipLaunchKernelGGL(lbfgs_advance_kernel, dim3(slots_ub), dim3(kAdvBlock), 0, st, r, c);
+}
+
+// ---------------------------------------------------------------------------
+// finalize (descent.py:186-197, lbfgs.py:316-329): wrapped s and summaries
+// ---------------------------------------------------------------------------
+__global__ __launch_bounds__(kAdvBlock) void finalize_kernel(RelaxDev r, int method, LbfgsCfg lc,
+                                                             RelaxOutDev o) {
+  __shared__ double red[16];
+  const int lane = blockIdx.x;
+  const size_t off = (size_t)lane * r.stride;
+  const double* s = r.s + off;
+  const double* g = r.g + off;
+  double gg = 0.0;
+  for (int i = threadIdx.x; i < r.nd; i += blockDim.x) {
+    o.s[off + i] = wrap01(s[i]);
+    gg += g[i] * g[i];
+  }
+  gg = block_sum<kAdvBlock>(gg, red);
+  if (threadIdx.x == 0) {
+    if (method == 0) {
+      const DescentLane L = r.dl[lane];
+      o.energy[lane] = L.f;
+      o.grad_norm[lane] = sqrt(gg);
+      o.n_evals[lane] = L.n_evals;
+      o.n_iters[lane] = L.it;
+    } else {
+      const LbfgsLane L = r.ll[lane];
+      o.energy[lane] = L.f;
+      o.grad_norm[lane] = sqrt(gg) * lc.rescales[L.pass_idx];
+      o.n_evals[lane] = L.n_evals;
+      o.n_iters[lane] = L.it;
+    }
+    o.status[lane] = r.status[lane];
+  }
+}
+
+void launch_finalize(const RelaxDev& r, int method, const LbfgsCfg& lc, const RelaxOutDev& o,
+                     hipStream_t st) {
+  hipLaunchKernelGGL(finalize_kernel, dim3(r.B), dim3(kAdvBlock), 0, st, r, method, lc, o);
+}
+
+}  // namespace cco

@@ -1,0 +1,51 @@
+"""xla_cco (MI355X): stealthy collective-coordinate energy, batched relax and
+MD steps on gfx950 HIP kernels, with the host interface of sdawley1/xla-cco.
+
+All arrays are float64.  The compute path is the HIP library
+``_native/libcco_hip.so`` (C ABI in include/cco.h); there is no CPU fallback.
+"""
+
+from xla_cco._types import require_float64
+from xla_cco.box import Box, cubic_box, wrap_fractional
+from xla_cco.energy import (
+    AbstractEnergy,
+    StealthyEnergy,
+    collective_density,
+    make_stealthy_energy,
+)
+from xla_cco.kvectors import (
+    KSet,
+    chi_from_count,
+    enumerate_kvectors,
+    k_outer_from_ka,
+    ka_from_chi,
+    unit_ball_volume,
+)
+from xla_cco.minimize import relax
+from xla_cco.minimize.descent import DescentSettings, adaptive_descent
+from xla_cco.minimize.lbfgs import LBFGSSettings, lbfgs
+from xla_cco.minimize.result import RelaxResult, Trajectory
+
+__all__ = [
+    "require_float64",
+    "Box",
+    "cubic_box",
+    "wrap_fractional",
+    "AbstractEnergy",
+    "StealthyEnergy",
+    "collective_density",
+    "make_stealthy_energy",
+    "KSet",
+    "chi_from_count",
+    "enumerate_kvectors",
+    "k_outer_from_ka",
+    "ka_from_chi",
+    "unit_ball_volume",
+    "relax",
+    "DescentSettings",
+    "adaptive_descent",
+    "LBFGSSettings",
+    "lbfgs",
+    "RelaxResult",
+    "Trajectory",
+]

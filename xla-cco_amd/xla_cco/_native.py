@@ -1,0 +1,202 @@
+"""ctypes binding of the gfx950 HIP library (``_native/libcco_hip.so``).
+
+The library is the only compute path: there is no CPU fallback.  Loading
+fails with ``NativeLibraryError`` when the library has not been built, and
+creating a context fails with ``RuntimeError`` when no GPU is visible
+(the reference's fail-fast device guard, src/xla_cco/_device.py:137).
+"""
+
+from __future__ import annotations
+
+import ctypes as C
+import threading
+from pathlib import Path
+from typing import Dict, Optional
+
+import numpy as np
+
+LIB_PATH = Path(__file__).resolve().parent / "_native" / "libcco_hip.so"
+
+CCO_OK = 0
+CCO_ERR_ARG = 1
+CCO_ERR_DTYPE = 2
+CCO_ERR_DEVICE = 3
+CCO_ERR_BOUND = 4
+CCO_ERR_STATE = 5
+
+METHOD_DESCENT = 0
+METHOD_LBFGS = 1
+
+
+class NativeLibraryError(ImportError):
+    """The HIP library is missing or does not export the expected ABI."""
+
+
+class DescentSettingsC(C.Structure):
+    _fields_ = [
+        ("step_init", C.c_double),
+        ("step_min", C.c_double),
+        ("shrink", C.c_double),
+        ("grow", C.c_double),
+        ("rel_tol", C.c_double),
+        ("energy_tol", C.c_double),
+        ("max_evals", C.c_int64),
+        ("record_every", C.c_int32),
+        ("max_records", C.c_int32),
+    ]
+
+
+class LBFGSSettingsC(C.Structure):
+    _fields_ = [
+        ("memory", C.c_int32),
+        ("n_rescales", C.c_int32),
+        ("rescales", C.c_double * 8),
+        ("max_evals_per_pass", C.c_int64),
+        ("energy_tol", C.c_double),
+        ("grad_tol", C.c_double),
+        ("stagnation_patience", C.c_int32),
+        ("max_linesearch_steps", C.c_int32),
+        ("record_every", C.c_int32),
+        ("max_records", C.c_int32),
+    ]
+
+
+_P = C.c_void_p
+_PD = C.POINTER(C.c_double)
+_PI64 = C.POINTER(C.c_int64)
+_PI32 = C.POINTER(C.c_int32)
+_PI = C.POINTER(C.c_int)
+
+# name -> (restype, argtypes); every symbol declared in include/cco.h
+SIGNATURES = {
+    "cco_version": (C.c_int, []),
+    "cco_last_error": (C.c_char_p, []),
+    "cco_device_count": (C.c_int, [_PI]),
+    "cco_ctx_create": (C.c_int, [C.c_int, C.POINTER(_P)]),
+    "cco_ctx_destroy": (C.c_int, [_P]),
+    "cco_ctx_synchronize": (C.c_int, [_P]),
+    "cco_dev_alloc": (C.c_int, [_P, C.c_size_t, C.POINTER(_P)]),
+    "cco_dev_free": (C.c_int, [_P, _P]),
+    "cco_memcpy_h2d": (C.c_int, [_P, _P, _P, C.c_size_t]),
+    "cco_memcpy_d2h": (C.c_int, [_P, _P, _P, C.c_size_t]),
+    "cco_energy_create": (
+        C.c_int,
+        [_P, _PI64, _PD, C.c_int, C.c_int, C.c_double, C.c_int, C.POINTER(_P)],
+    ),
+    "cco_energy_destroy": (C.c_int, [_P]),
+    "cco_energy_info": (C.c_int, [_P, _PI, _PI, _PI, _PD, _PI]),
+    "cco_value_grad_density": (C.c_int, [_P, _PD, C.c_int, C.c_int, _PD, _PD, _PD, _PD]),
+    "cco_value_grad_density_dev": (C.c_int, [_P, _P, C.c_int, C.c_int, _P, _P, _P, _P]),
+    "cco_relax_create": (
+        C.c_int,
+        [
+            _P,
+            _P,
+            C.c_int,
+            C.c_int,
+            C.c_int,
+            C.c_int,
+            C.POINTER(DescentSettingsC),
+            C.POINTER(LBFGSSettingsC),
+            C.POINTER(_P),
+        ],
+    ),
+    "cco_relax_destroy": (C.c_int, [_P]),
+    "cco_relax_init": (C.c_int, [_P, _PD]),
+    "cco_relax_init_dev": (C.c_int, [_P, _P]),
+    "cco_relax_run": (C.c_int, [_P, C.c_int64, C.c_int]),
+    "cco_relax_step": (C.c_int, [_P, C.c_int64, _PI]),
+    "cco_relax_pending": (C.c_int, [_P, _PI, _PI32, _PD]),
+    "cco_relax_submit": (C.c_int, [_P, _PD, _PD]),
+    "cco_relax_result": (C.c_int, [_P, _PD, _PD, _PD, _PI64, _PI64, _PI64]),
+    "cco_relax_result_dev": (C.c_int, [_P, _P, _P, _P, _P, _P, _P]),
+    "cco_relax_trajectory": (C.c_int, [_P, _PI64, _PI64, _PD, _PD, _PD]),
+    "cco_relax_stats": (C.c_int, [_P, _PI64, _PI64]),
+}
+
+_lib: Optional[C.CDLL] = None
+_lock = threading.Lock()
+
+
+def lib() -> C.CDLL:
+    """Load the HIP library once (raises NativeLibraryError when missing)."""
+    global _lib
+    with _lock:
+        if _lib is not None:
+            return _lib
+        if not LIB_PATH.exists():
+            raise NativeLibraryError(
+                f"HIP library not built: {LIB_PATH} is missing "
+                "(run `python xla-cco_amd/build_native.py`)"
+            )
+        handle = C.CDLL(str(LIB_PATH))
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(handle, name, None)
+            if fn is None:
+                raise NativeLibraryError(f"{LIB_PATH} does not export {name}")
+            fn.restype = res
+            fn.argtypes = args
+        _lib = handle
+        return handle
+
+
+def check(rc: int) -> None:
+    """Map a C-ABI return code to the reference's exception types."""
+    if rc == CCO_OK:
+        return
+    msg = lib().cco_last_error().decode(errors="replace")
+    if rc == CCO_ERR_ARG:
+        raise ValueError(msg)
+    if rc == CCO_ERR_DTYPE:
+        raise TypeError(msg)
+    raise RuntimeError(msg or f"cco error {rc}")
+
+
+def dptr(a: np.ndarray) -> "C._Pointer[C.c_double]":
+    assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_PD)
+
+
+def i64ptr(a: np.ndarray) -> "C._Pointer[C.c_int64]":
+    assert a.dtype == np.int64 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(_PI64)
+
+
+class Context:
+    """One device and its stream (cco_ctx)."""
+
+    def __init__(self, device: int = 0) -> None:
+        L = lib()
+        count = C.c_int(0)
+        check(L.cco_device_count(C.byref(count)))
+        if count.value == 0:
+            raise RuntimeError("no GPU visible to the HIP runtime; the HIP path has no CPU fallback")
+        h = _P()
+        check(L.cco_ctx_create(device, C.byref(h)))
+        self.handle = h
+        self.device = device
+
+    def synchronize(self) -> None:
+        check(lib().cco_ctx_synchronize(self.handle))
+
+    def __del__(self) -> None:  # pragma: no cover - interpreter teardown order
+        try:
+            if getattr(self, "handle", None) and _lib is not None:
+                _lib.cco_ctx_destroy(self.handle)
+        except Exception:
+            pass
+
+
+_contexts: Dict[int, Context] = {}
+
+
+def context(device: int = 0) -> Context:
+    """The process-wide context of ``device`` (created on first use)."""
+    with _lock:
+        ctx = _contexts.get(device)
+    if ctx is None:
+        ctx = Context(device)
+        with _lock:
+            _contexts.setdefault(device, ctx)
+            ctx = _contexts[device]
+    return ctx
