@@ -1,0 +1,252 @@
+"""Benchmark: converged configurations per second of the batched FP64 relax on one MI355X.
+
+Workload (one "step"): relax 256 configurations of N=1000 particles in d=2
+(chi=0.45, M=894 half-space k-vectors, unit density) from uniform random
+positions with three-pass L-BFGS until E < 1e-20 (or another stop), all in
+FP64 on the GPU.  value = configurations with E < 1e-20 per second of wall
+time, inputs already resident in device memory when the timed region starts.
+
+    python bench.py [--gpus 1] [--steps 2] [--warmup 1]
+
+Prints one JSON line.  Per-kernel device times come from HIP events recorded
+on the library's stream around every launch inside the timed region; the
+roofline object describes the kernel with the largest total time.  The CPU
+baseline times the oracle's C restatement (OpenMP, one configuration per
+thread) on a bounded sample of the same workload.
+"""
+
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+import numpy as np
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "xla-cco_amd"))
+sys.path.insert(0, str(ROOT))
+
+PEAK_FP64_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
+PEAK_HBM_GBS = 8000.0
+METRIC = "converged configs/s (E<1e-20), mixed vs FP64 on 1 GPU; energy+force evals/s"
+STAGES = ("rho_kernel", "reduce_kernel", "force_kernel", "lbfgs_advance_kernel", "compact_kernel")
+
+
+def build_workload(d: int, n: int, chi: float):
+    from xla_cco import cubic_box, enumerate_kvectors, k_outer_from_ka, ka_from_chi, make_stealthy_energy
+
+    box = cubic_box(n, d)
+    ks = enumerate_kvectors(box.basis, 0.0, k_outer_from_ka(ka_from_chi(chi, 0.05, d), 0.05, d))
+    return box, ks, make_stealthy_energy(box, ks)
+
+
+def cpu_baseline(ks, volume: float, d: int, n: int, evals_per_converged: float, seconds: float):
+    """Time the C oracle's L-BFGS on `threads` lanes of the same workload for a bounded sample."""
+    from oracle import c_oracle, oracle as O
+
+    try:
+        threads = len(os.sched_getaffinity(0))
+    except AttributeError:  # pragma: no cover
+        threads = os.cpu_count() or 1
+    threads = max(1, min(threads, 16))
+    s0 = np.random.default_rng(99).uniform(size=(threads, n, d))
+    # calibrate the per-lane evaluation cap to ~`seconds` of wall time
+    t0 = time.perf_counter()
+    c_oracle.relax(s0[:1], ks.n, ks.weights, volume, O.LBFGSSettings(energy_tol=1e-20), eval_cap=3)
+    per_eval = (time.perf_counter() - t0) / 3.0
+    cap = max(4, int(seconds / max(per_eval, 1e-6)))
+    t0 = time.perf_counter()
+    _, _, ne, _, _ = c_oracle.relax(
+        s0, ks.n, ks.weights, volume, O.LBFGSSettings(energy_tol=1e-20), eval_cap=cap, threads=threads
+    )
+    wall = time.perf_counter() - t0
+    evals = float(ne.sum() + 0 * threads)  # n_evals counts the initial evaluation as well
+    eps = evals / wall
+    return {
+        "value": eps / evals_per_converged,
+        "unit": "configs/s",
+        "cores": threads,
+        "kind": "port",
+        "evals_per_s": eps,
+        "sample": (
+            f"C oracle (oracle/cco_oracle.c, trig per (k,j), same L-BFGS) on {threads} configurations "
+            f"of the workload, one per OpenMP thread, capped at {cap} energy+force evaluations each "
+            f"({wall:.1f} s wall); converged configs/s = measured evals/s / {evals_per_converged:.1f} "
+            f"evaluations per converged configuration (the GPU run's mean, same algorithm)"
+        ),
+    }
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=2)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--lanes", type=int, default=256)
+    ap.add_argument("--n", type=int, default=1000)
+    ap.add_argument("--d", type=int, default=2)
+    ap.add_argument("--chi", type=float, default=0.45)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-profile", action="store_true", help="skip per-kernel event timing")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(
+            f"bench.py runs on one GPU only (asked for {args.gpus}): lanes are independent, "
+            "so a multi-GPU run is G copies of this one; see DESIGN.md for the sharding plan.",
+            file=sys.stderr,
+        )
+        return 2
+
+    from xla_cco import _native
+    from xla_cco.minimize import LBFGSSettings, RelaxSession
+
+    L = _native.lib()
+    ctx = _native.context(0)
+    d, n, chi, B = args.d, args.n, args.chi, args.lanes
+    box, ks, energy = build_workload(d, n, chi)
+    M = int(ks.n.shape[0])
+    tol = 1e-20
+    settings = LBFGSSettings(energy_tol=tol)
+    sess = RelaxSession(energy, B, n, d, settings)
+    bound = 1 + 3 * (settings.max_evals_per_pass + settings.max_linesearch_steps + 1)
+
+    inputs = [
+        _native.DeviceArray.from_host(ctx, np.random.default_rng(1000 + i).uniform(size=(B, n, d)))
+        for i in range(args.warmup + args.steps)
+    ]
+    out_s = _native.DeviceArray(ctx, (B, n, d))
+    out_e = _native.DeviceArray(ctx, (B,))
+    out_g = _native.DeviceArray(ctx, (B,))
+    out_i = [_native.DeviceArray(ctx, (B,), np.int64) for _ in range(3)]
+
+    def one_step(dev_s0) -> None:
+        _native.check(L.cco_relax_init_dev(sess.handle, dev_s0.ptr))
+        _native.check(L.cco_relax_run(sess.handle, bound, 256))
+        _native.check(
+            L.cco_relax_result_dev(
+                sess.handle, out_s.ptr, out_e.ptr, out_g.ptr, out_i[0].ptr, out_i[1].ptr, out_i[2].ptr
+            )
+        )
+
+    for w in range(args.warmup):
+        one_step(inputs[w])
+    ctx.synchronize()
+
+    _native.check(L.cco_relax_set_profiling(sess.handle, 0 if args.no_profile else 1))
+    converged = 0
+    lane_evals = 0
+    iters = 0
+    pairs = 0
+    steps_total = 0
+    elapsed = 0.0
+    for k in range(args.steps):
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        one_step(inputs[args.warmup + k])
+        ctx.synchronize()
+        elapsed += time.perf_counter() - t0
+        e = out_e.to_host()
+        converged += int(np.sum(e < tol))
+        cnt = (C.c_int64 * 4)()
+        _native.check(L.cco_relax_counters(sess.handle, cnt))
+        lane_evals += cnt[0]
+        iters += cnt[1]
+        pairs += cnt[2]
+        steps_total += cnt[3]
+    ms = (C.c_double * 5)()
+    launches = (C.c_int64 * 5)()
+    _native.check(L.cco_relax_profile(sess.handle, ms, launches))
+
+    value = converged / elapsed
+    evals_per_conv = lane_evals / max(converged, 1)
+    nd = n * d
+    # algorithmic work per launch of each kernel
+    lane_evals_per_launch = lane_evals / max(steps_total, 1)
+    work = {
+        "rho_kernel": ("mfma", 8.0 * M * n * lane_evals_per_launch),
+        "force_kernel": ("mfma", 8.0 * M * n * d * lane_evals_per_launch),
+        "lbfgs_advance_kernel": ("hbm", (32.0 * nd * pairs + 8.0 * nd * (3 * lane_evals + 12 * iters)) / max(steps_total, 1)),
+    }
+    roofline = None
+    stage_avg = {}
+    if not args.no_profile and launches[0] > 0:
+        stage_avg = {STAGES[i]: ms[i] / max(launches[i], 1) for i in range(5)}
+        dominant = max(range(5), key=lambda i: ms[i])
+        name = STAGES[dominant]
+        avg_ms = stage_avg[name]
+        bound_kind, per_launch = work.get(name, ("hbm", 0.0))
+        if bound_kind == "mfma":
+            achieved = per_launch / (avg_ms * 1e-3) / 1e12
+            peak, unit = PEAK_FP64_TFLOPS, "TFLOP/s"
+        else:
+            achieved = per_launch / (avg_ms * 1e-3) / 1e9
+            peak, unit = PEAK_HBM_GBS, "GB/s"
+        traffic = None
+        tfile = ROOT / "profiles" / "traffic.json"
+        if tfile.exists():
+            try:
+                tdat = json.loads(tfile.read_text()).get(name)
+                if tdat and tdat.get("workload") == f"d{d}_n{n}_chi{chi}_b{B}":
+                    traffic = tdat.get("bytes_per_launch")
+            except (ValueError, AttributeError):
+                traffic = None
+        roofline = {
+            "kernel": name,
+            "bound": bound_kind,
+            "achieved": achieved,
+            "peak": peak,
+            "unit": unit,
+            "frac": achieved / peak,
+            "traffic": traffic,
+            "avg_launch_ms": avg_ms,
+            "work_per_launch": per_launch,
+        }
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(ks, box.volume, d, n, evals_per_conv, args.cpu_seconds)
+
+    line = {
+        "metric": METRIC,
+        "value": value,
+        "unit": "configs/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": elapsed / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"batched FP64 L-BFGS relax to E<1e-20, d={d} N={n} chi={chi} (M={M}), {B} uniform-random configurations",
+            "d": d,
+            "N": n,
+            "chi": chi,
+            "M": M,
+            "lanes": B,
+            "method": "lbfgs (memory 100, rescales 1e4/1/1e-2, zoom line search)",
+            "energy_tol": tol,
+            "precision": "fp64",
+        },
+        "converged": f"{converged}/{B * args.steps}",
+        "energy_force_evals_per_s": lane_evals / elapsed,
+        "evals_per_converged": evals_per_conv,
+        "kernel_avg_ms": stage_avg,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+    sess.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -134,6 +134,26 @@ int cco_relax_trajectory(cco_relax* r, int64_t* count_host, int64_t* iteration_h
                          double* s_host, double* energy_host, double* step_host);
 /* Total energy evaluations issued so far (sum over lanes of evaluation steps). */
 int cco_relax_stats(cco_relax* r, int64_t* steps, int64_t* lane_evals);
+/* Sums over lanes: energy evaluations (descent: iterations + 1), iterations,
+ * two-loop (s, y) pairs visited (L-BFGS), evaluation steps of the session. */
+int cco_relax_counters(cco_relax* r, int64_t* out4);
+/* Tuning knobs (no reference counterpart).  CCO_OPT_ADVANCE_VARIANT: 0 auto,
+ * 1 L-BFGS vectors in global memory, 2 register-resident (N*d <= 8192). */
+#define CCO_OPT_ADVANCE_VARIANT 1
+int cco_relax_set_option(cco_relax* r, int key, int64_t value);
+
+/* Measurement (no reference counterpart; the reference times with
+ * block_until_ready, benchmarks/bench_energy.py:104-115).  Per-stage device
+ * time from HIP events recorded on the context stream around every launch:
+ * stages rho, reduce, force, advance, compact (ms summed, launches counted). */
+int cco_relax_set_profiling(cco_relax* r, int on);
+int cco_relax_profile(cco_relax* r, double* ms_sum5, int64_t* launches5);
+/* reps timed evaluations of B x N on device data: avg ms of the rho, reduce
+ * and force kernels and of the whole evaluation (avg_ms[0..3]). */
+int cco_energy_profile(cco_energy* e, const double* s_dev, int B, int N, int reps,
+                       double* avg_ms4);
+/* Plain device copy kernel (calibration of the HBM traffic counters). */
+int cco_copy_dev(cco_ctx* ctx, void* dst_dev, const void* src_dev, size_t bytes, double* ms);
 
 #ifdef __cplusplus
 }

@@ -285,3 +285,31 @@ def test_relax_d3_batch_to_ground_state(gpu_ctx) -> None:
     assert np.all(out.energy < 9.99999968e-21)
     phi = e.energy_batch(out.s)
     assert np.all(phi < 1e-19)
+
+
+@pytest.mark.parametrize("n,d", [(64, 2), (700, 3)])
+def test_lbfgs_advance_variants_agree(gpu_ctx, n: int, d: int) -> None:
+    """Global-memory and register-resident L-BFGS advance kernels: same path to ~1e-9."""
+    from xla_cco import _native
+    from xla_cco.minimize import RelaxSession
+
+    e, vg, _ = _poisson(n, d, 0.2)
+    s0 = np.random.default_rng(11).uniform(size=(3, n, d))
+    st = LBFGSSettings(rescales=(1.0,), max_evals_per_pass=60, record_every=1, max_records=12)
+    outs = []
+    for variant in (1, 2):
+        sess = RelaxSession(e, 3, n, d, st)
+        _native.check(_native.lib().cco_relax_set_option(sess.handle, 1, variant))
+        sess.init(s0)
+        sess.run(10_000, 64)
+        outs.append(sess.result())
+        sess.close()
+    a, b = outs
+    assert np.array_equal(a.trajectory.count, b.trajectory.count)
+    assert np.allclose(a.trajectory.energy, b.trajectory.energy, rtol=1e-9, atol=1e-15)
+    assert np.allclose(a.trajectory.step, b.trajectory.step, rtol=1e-6)
+    for i in range(3):
+        ref = O.lbfgs(vg, s0[i], O.LBFGSSettings(rescales=(1.0,), max_evals_per_pass=60,
+                                                  record_every=1, max_records=12))
+        k = int(b.trajectory.count[i])
+        assert np.allclose(b.trajectory.energy[i, :k], ref.traj["energy"][:k], rtol=1e-8)

@@ -270,6 +270,8 @@ int energy_batch(cco_energy* E, const double* s_dev, int B, int N, double* phi_d
 // ===========================================================================
 // relax session
 // ===========================================================================
+constexpr int kStages = 5;  // rho, reduce, force, advance, compact
+
 struct cco_relax {
   cco_ctx* ctx;
   cco_energy* E;
@@ -284,6 +286,12 @@ struct cco_relax {
   int64_t lane_evals = 0;
   bool ready = false;
   int* host_count = nullptr;  // pinned
+  // optional per-stage timing with events on the context stream
+  bool profile = false;
+  int advance_variant = 0;  // 0 auto, 1 global-memory vectors, 2 register-resident
+  std::vector<hipEvent_t> evs;
+  double stage_ms[kStages] = {0, 0, 0, 0, 0};
+  int64_t stage_n[kStages] = {0, 0, 0, 0, 0};
 };
 
 namespace {
@@ -339,8 +347,7 @@ __global__ void relax_init_kernel(RelaxDev r, int method) {
   }
 }
 
-int relax_eval_builtin(cco_relax* R) {
-  const Geo& g = R->E->geo;
+Batch relax_batch(cco_relax* R) {
   Batch b;
   b.active = R->dev.active;
   b.n_active = R->dev.n_active;
@@ -356,17 +363,26 @@ int relax_eval_builtin(cco_relax* R) {
   b.im = nullptr;
   b.part = (double2*)R->part.p;
   b.cbox = (double2*)R->cbox.p;
-  launch_energy(g, b, R->slots_ub, true, R->ctx->stream);
-  return CCO_OK;
+  return b;
+}
+
+void relax_stage(cco_relax* R, int stage) {
+  hipStream_t st = R->ctx->stream;
+  if (stage <= 2) {
+    launch_energy_stage(stage, R->E->geo, relax_batch(R), R->slots_ub, st);
+  } else if (stage == 3) {
+    if (R->method == CCO_METHOD_DESCENT)
+      launch_descent_advance(R->dev, R->dc, R->slots_ub, st);
+    else
+      launch_lbfgs_advance(R->dev, R->lc, R->slots_ub, st, R->advance_variant);
+  } else {
+    launch_compact(R->dev, st);
+  }
 }
 
 int relax_advance(cco_relax* R) {
-  hipStream_t st = R->ctx->stream;
-  if (R->method == CCO_METHOD_DESCENT)
-    launch_descent_advance(R->dev, R->dc, R->slots_ub, st);
-  else
-    launch_lbfgs_advance(R->dev, R->lc, R->slots_ub, st);
-  launch_compact(R->dev, st);
+  relax_stage(R, 3);
+  relax_stage(R, 4);
   CCO_HIP(hipGetLastError());
   return CCO_OK;
 }
@@ -675,6 +691,7 @@ int cco_relax_destroy(cco_relax* R) {
   for (DevBuf* b : {&R->part, &R->cbox, &R->out_s, &R->out_scalars, &R->pack, &R->pack_idx})
     dev_release(*b);
   if (R->host_count) (void)hipHostFree(R->host_count);
+  for (hipEvent_t e : R->evs) (void)hipEventDestroy(e);
   delete R;
   return CCO_OK;
 }
@@ -719,14 +736,173 @@ int cco_relax_step(cco_relax* R, int64_t n_evals, int* n_active) {
   if (!R->ready) return fail(CCO_ERR_STATE, "relax not initialised");
   if (!R->E) return fail(CCO_ERR_STATE, "external-energy relax: use pending/submit");
   DevScope ds(R->ctx->device);
-  for (int64_t i = 0; i < n_evals && R->slots_ub > 0; ++i) {
-    int rc = relax_eval_builtin(R);
-    if (rc) return rc;
-    rc = relax_advance(R);
-    if (rc) return rc;
-    R->steps += 1;
+  hipStream_t st = R->ctx->stream;
+  int64_t done = 0;
+  const size_t need = R->profile ? (size_t)(kStages + 1) * (size_t)std::max<int64_t>(n_evals, 0) : 0;
+  while (R->evs.size() < need) {
+    hipEvent_t ev;
+    CCO_HIP(hipEventCreate(&ev));
+    R->evs.push_back(ev);
   }
-  return relax_sync_count(R, n_active);
+  for (int64_t i = 0; i < n_evals && R->slots_ub > 0; ++i) {
+    for (int stage = 0; stage < kStages; ++stage) {
+      if (R->profile && stage == 0) CCO_HIP(hipEventRecord(R->evs[(kStages + 1) * i], st));
+      relax_stage(R, stage);
+      if (R->profile) CCO_HIP(hipEventRecord(R->evs[(kStages + 1) * i + stage + 1], st));
+    }
+    CCO_HIP(hipGetLastError());
+    R->steps += 1;
+    ++done;
+  }
+  int rc = relax_sync_count(R, n_active);
+  if (rc) return rc;
+  if (R->profile) {
+    for (int64_t i = 0; i < done; ++i) {
+      for (int stage = 0; stage < kStages; ++stage) {
+        float ms = 0.0f;
+        CCO_HIP(hipEventElapsedTime(&ms, R->evs[(kStages + 1) * i + stage],
+                                    R->evs[(kStages + 1) * i + stage + 1]));
+        R->stage_ms[stage] += ms;
+        R->stage_n[stage] += 1;
+      }
+    }
+  }
+  return CCO_OK;
+}
+
+int cco_relax_set_option(cco_relax* R, int key, int64_t value) {
+  if (!R) return fail(CCO_ERR_ARG, "relax is NULL");
+  if (key == CCO_OPT_ADVANCE_VARIANT) {
+    if (value < 0 || value > 2) return fail(CCO_ERR_ARG, "advance variant must be 0, 1 or 2");
+    if (value == 2 && lbfgs_register_ept(R->dev.nd) == 0)
+      return fail(CCO_ERR_ARG, "register-resident advance needs N*d <= 8192");
+    R->advance_variant = (int)value;
+    return CCO_OK;
+  }
+  return fail(CCO_ERR_ARG, "unknown option key");
+}
+
+int cco_relax_counters(cco_relax* R, int64_t* out4) {
+  if (!R || !out4) return fail(CCO_ERR_ARG, "NULL argument");
+  DevScope ds(R->ctx->device);
+  const size_t B = R->B;
+  int64_t ev = 0, it = 0, pairs = 0;
+  if (R->method == CCO_METHOD_DESCENT) {
+    std::vector<DescentLane> L(B);
+    CCO_HIP(hipMemcpy(L.data(), R->dev.dl, sizeof(DescentLane) * B, hipMemcpyDeviceToHost));
+    for (auto& l : L) {
+      ev += l.it + 1;
+      it += l.it;
+    }
+  } else {
+    std::vector<LbfgsLane> L(B);
+    CCO_HIP(hipMemcpy(L.data(), R->dev.ll, sizeof(LbfgsLane) * B, hipMemcpyDeviceToHost));
+    for (auto& l : L) {
+      ev += l.n_evals;
+      it += l.it;
+      pairs += l.pairs;
+    }
+  }
+  out4[0] = ev;
+  out4[1] = it;
+  out4[2] = pairs;
+  out4[3] = R->steps;
+  return CCO_OK;
+}
+
+int cco_relax_set_profiling(cco_relax* R, int on) {
+  if (!R) return fail(CCO_ERR_ARG, "relax is NULL");
+  R->profile = on != 0;
+  for (int s = 0; s < kStages; ++s) {
+    R->stage_ms[s] = 0.0;
+    R->stage_n[s] = 0;
+  }
+  return CCO_OK;
+}
+
+int cco_relax_profile(cco_relax* R, double* ms_sum, int64_t* launches) {
+  if (!R || !ms_sum || !launches) return fail(CCO_ERR_ARG, "NULL argument");
+  for (int s = 0; s < kStages; ++s) {
+    ms_sum[s] = R->stage_ms[s];
+    launches[s] = R->stage_n[s];
+  }
+  return CCO_OK;
+}
+
+int cco_energy_profile(cco_energy* E, const double* s_dev, int B, int N, int reps,
+                       double* avg_ms) {
+  if (!E || !s_dev || !avg_ms || reps < 1 || B < 1 || N < 1)
+    return fail(CCO_ERR_ARG, "bad argument");
+  DevScope ds(E->ctx->device);
+  const Geo& g = E->geo;
+  int rc;
+  const size_t nsd = (size_t)B * N * g.d;
+  if ((rc = dev_reserve(E->phi, (size_t)B * sizeof(double)))) return rc;
+  if ((rc = dev_reserve(E->grad, nsd * sizeof(double)))) return rc;
+  // one untimed evaluation sets up the workspace and the lane list
+  rc = energy_batch(E, s_dev, B, N, (double*)E->phi.p, (double*)E->grad.p, nullptr, nullptr);
+  if (rc) return rc;
+  Batch b;
+  b.active = (const int*)E->active.p;
+  b.n_active = (const int*)E->n_active.p;
+  b.N = N;
+  b.lane_stride = N * g.d;
+  b.n_splits = n_splits_for(N);
+  b.split_len = kSplitLen;
+  b.n_ftiles = n_ftiles_for(N);
+  b.s = s_dev;
+  b.grad = (double*)E->grad.p;
+  b.phi = (double*)E->phi.p;
+  b.re = nullptr;
+  b.im = nullptr;
+  b.part = (double2*)E->part.p;
+  b.cbox = (double2*)E->cbox.p;
+  hipStream_t st = E->ctx->stream;
+  std::vector<hipEvent_t> ev(4 * (size_t)reps);
+  for (auto& e : ev) CCO_HIP(hipEventCreate(&e));
+  for (int r = 0; r < reps; ++r) {
+    CCO_HIP(hipEventRecord(ev[4 * r], st));
+    for (int stage = 0; stage < 3; ++stage) {
+      launch_energy_stage(stage, g, b, B, st);
+      CCO_HIP(hipEventRecord(ev[4 * r + stage + 1], st));
+    }
+  }
+  CCO_HIP(hipGetLastError());
+  CCO_HIP(hipStreamSynchronize(st));
+  double acc[4] = {0, 0, 0, 0};
+  for (int r = 0; r < reps; ++r) {
+    for (int stage = 0; stage < 3; ++stage) {
+      float ms = 0.0f;
+      CCO_HIP(hipEventElapsedTime(&ms, ev[4 * r + stage], ev[4 * r + stage + 1]));
+      acc[stage] += ms;
+    }
+    float tot = 0.0f;
+    CCO_HIP(hipEventElapsedTime(&tot, ev[4 * r], ev[4 * r + 3]));
+    acc[3] += tot;
+  }
+  for (auto& e : ev) (void)hipEventDestroy(e);
+  for (int i = 0; i < 4; ++i) avg_ms[i] = acc[i] / reps;
+  return CCO_OK;
+}
+
+int cco_copy_dev(cco_ctx* ctx, void* dst_dev, const void* src_dev, size_t bytes, double* ms) {
+  if (!ctx || !dst_dev || !src_dev) return fail(CCO_ERR_ARG, "NULL argument");
+  if (bytes % 32 != 0) return fail(CCO_ERR_ARG, "bytes must be a multiple of 32");
+  DevScope ds(ctx->device);
+  hipEvent_t a, b;
+  CCO_HIP(hipEventCreate(&a));
+  CCO_HIP(hipEventCreate(&b));
+  CCO_HIP(hipEventRecord(a, ctx->stream));
+  launch_copy(src_dev, dst_dev, bytes, ctx->stream);
+  CCO_HIP(hipEventRecord(b, ctx->stream));
+  CCO_HIP(hipGetLastError());
+  CCO_HIP(hipStreamSynchronize(ctx->stream));
+  float t = 0.0f;
+  CCO_HIP(hipEventElapsedTime(&t, a, b));
+  if (ms) *ms = t;
+  (void)hipEventDestroy(a);
+  (void)hipEventDestroy(b);
+  return CCO_OK;
 }
 
 int cco_relax_run(cco_relax* R, int64_t max_evals, int sync_every) {

@@ -67,6 +67,8 @@ struct Batch {
 
 void launch_energy(const Geo& g, const Batch& b, int slots_ub, bool with_grad,
                    hipStream_t stream);
+void launch_energy_stage(int stage, const Geo& g, const Batch& b, int slots_ub, hipStream_t st);
+void launch_copy(const void* src, void* dst, size_t bytes, hipStream_t st);
 size_t rho_lds_bytes(const Geo& g);
 
 // ---------------- small helpers ----------------

@@ -271,26 +271,45 @@ size_t rho_lds_bytes(const Geo& g) {
   return sizeof(double2) * (size_t)kRhoChunk * (2 + (g.P2 + 1) + (g.P3 + 1));
 }
 
+// Launch one stage (0 = rho, 1 = reduce, 2 = force) of an evaluation.
 template <int D>
-static void launch_d(const Geo& g, const Batch& b, int slots_ub, bool with_grad,
-                     hipStream_t st) {
-  const int rho_tiles = g.n_coltiles * g.n_segs * b.n_splits;
-  const int ct = g.CT;
-  hipLaunchKernelGGL(rho_kernel<D>, dim3(slots_ub * rho_tiles), dim3(ct), rho_lds_bytes(g), st,
-                     g, b);
-  hipLaunchKernelGGL(reduce_kernel, dim3(slots_ub), dim3(256), 0, st, g, b);
-  if (with_grad) {
+static void launch_stage(int stage, const Geo& g, const Batch& b, int slots_ub, hipStream_t st) {
+  if (stage == 0) {
+    const int rho_tiles = g.n_coltiles * g.n_segs * b.n_splits;
+    hipLaunchKernelGGL(rho_kernel<D>, dim3(slots_ub * rho_tiles), dim3(g.CT), rho_lds_bytes(g),
+                       st, g, b);
+  } else if (stage == 1) {
+    hipLaunchKernelGGL(reduce_kernel, dim3(slots_ub), dim3(256), 0, st, g, b);
+  } else {
     hipLaunchKernelGGL(force_kernel<D>, dim3(slots_ub * b.n_ftiles), dim3(kForceTile), 0, st, g,
                        b);
   }
 }
 
+void launch_energy_stage(int stage, const Geo& g, const Batch& b, int slots_ub, hipStream_t st) {
+  if (slots_ub <= 0) return;
+  if (g.d == 1) launch_stage<1>(stage, g, b, slots_ub, st);
+  if (g.d == 2) launch_stage<2>(stage, g, b, slots_ub, st);
+  if (g.d == 3) launch_stage<3>(stage, g, b, slots_ub, st);
+}
+
 void launch_energy(const Geo& g, const Batch& b, int slots_ub, bool with_grad,
                    hipStream_t stream) {
-  if (slots_ub <= 0) return;
-  if (g.d == 1) launch_d<1>(g, b, slots_ub, with_grad, stream);
-  if (g.d == 2) launch_d<2>(g, b, slots_ub, with_grad, stream);
-  if (g.d == 3) launch_d<3>(g, b, slots_ub, with_grad, stream);
+  launch_energy_stage(0, g, b, slots_ub, stream);
+  launch_energy_stage(1, g, b, slots_ub, stream);
+  if (with_grad) launch_energy_stage(2, g, b, slots_ub, stream);
+}
+
+__global__ __launch_bounds__(256) void copy_kernel(const double4* __restrict__ src,
+                                                   double4* __restrict__ dst, size_t n) {
+  const size_t stride = (size_t)gridDim.x * blockDim.x;
+  for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
+void launch_copy(const void* src, void* dst, size_t bytes, hipStream_t st) {
+  const size_t n = bytes / sizeof(double4);
+  hipLaunchKernelGGL(copy_kernel, dim3(2048), dim3(256), 0, st, (const double4*)src, (double4*)dst,
+                     n);
 }
 
 }  // namespace cco

@@ -59,6 +59,7 @@ struct LbfgsLane {
   double f, last_step;
   long long it, n_evals, pass_evals, stag;
   int pass_idx, opt_count, mode, rec_count;
+  long long pairs;  // two-loop (s, y) pairs visited so far (traffic accounting)
   LsState ls;
 };
 
@@ -100,7 +101,11 @@ struct RelaxOutDev {
 };
 
 void launch_descent_advance(const RelaxDev& r, const DescentCfg& c, int slots_ub, hipStream_t st);
-void launch_lbfgs_advance(const RelaxDev& r, const LbfgsCfg& c, int slots_ub, hipStream_t st);
+// variant: 0 auto, 1 global-memory vectors (any size), 2 register-resident vectors
+// (nd <= 8192, one 1024-thread workgroup per lane)
+void launch_lbfgs_advance(const RelaxDev& r, const LbfgsCfg& c, int slots_ub, hipStream_t st,
+                          int variant = 0);
+int lbfgs_register_ept(int nd);  // elements per thread of the register variant, 0 if too large
 void launch_compact(const RelaxDev& r, hipStream_t st);
 void launch_finalize(const RelaxDev& r, int method, const LbfgsCfg& lc, const RelaxOutDev& o,
                      hipStream_t st);

@@ -303,8 +303,11 @@ __device__ void lbfgs_start_iteration(const RelaxDev& r, const LbfgsCfg& c, int 
   const double rs = c.rescales[L.pass_idx];
   const int count = L.opt_count;
   double gamma;
+  int slot_new = -1;
+  double rho_new = 0.0;
   if (count > 0) {
     const int slot = (count - 1) % m;
+    slot_new = slot;
     double* Ss = S + (size_t)slot * r.stride;
     double* Ys = Y + (size_t)slot * r.stride;
     double sy = 0.0, yy = 0.0;
@@ -318,7 +321,8 @@ __device__ void lbfgs_start_iteration(const RelaxDev& r, const LbfgsCfg& c, int 
     }
     sy = block_sum<kAdvBlock>(sy, red);
     yy = block_sum<kAdvBlock>(yy, red);
-    if (threadIdx.x == 0) rho[slot] = (sy == 0.0) ? 0.0 : 1.0 / sy;
+    rho_new = (sy == 0.0) ? 0.0 : 1.0 / sy;
+    if (threadIdx.x == 0) rho[slot] = rho_new;
     gamma = (yy > 0.0) ? sy / yy : 1.0;
   } else {
     double gg = 0.0;
@@ -339,7 +343,7 @@ __device__ void lbfgs_start_iteration(const RelaxDev& r, const LbfgsCfg& c, int 
     double dot = 0.0;
     for (int i = threadIdx.x; i < nd; i += blockDim.x) dot += Si[i] * u[i];
     dot = block_sum<kAdvBlock>(dot, red);
-    const double a = rho[sl] * dot;
+    const double a = (sl == slot_new ? rho_new : rho[sl]) * dot;
     if (threadIdx.x == 0) alphas[t] = a;
     for (int i = threadIdx.x; i < nd; i += blockDim.x) u[i] = u[i] + (-a) * Yi[i];
   }
@@ -352,7 +356,7 @@ __device__ void lbfgs_start_iteration(const RelaxDev& r, const LbfgsCfg& c, int 
     double dot = 0.0;
     for (int i = threadIdx.x; i < nd; i += blockDim.x) dot += Yi[i] * u[i];
     dot = block_sum<kAdvBlock>(dot, red);
-    const double bcoef = rho[sl] * dot;
+    const double bcoef = (sl == slot_new ? rho_new : rho[sl]) * dot;
     const double coef = alphas[t] - bcoef;
     for (int i = threadIdx.x; i < nd; i += blockDim.x) u[i] = u[i] + coef * Si[i];
   }
@@ -367,6 +371,7 @@ __device__ void lbfgs_start_iteration(const RelaxDev& r, const LbfgsCfg& c, int 
   }
   slope = block_sum<kAdvBlock>(slope, red);
   L.opt_count = count + 1;
+  L.pairs += nvalid;
   LsState& Q = L.ls;
   Q.f0 = L.f;
   Q.slope0 = slope;
@@ -519,9 +524,409 @@ __global__ __launch_bounds__(kAdvBlock) void lbfgs_advance_kernel(RelaxDev r, Lb
   }
 }
 
-void launch_lbfgs_advance(const RelaxDev& r, const LbfgsCfg& c, int slots_ub, hipStream_t st) {
+// ---------------------------------------------------------------------------
+// L-BFGS advance with register-resident vectors: one 1024-thread workgroup
+// per lane, element i of every vector owned by thread i % 1024.  The two-loop
+// recursion keeps q in registers, prefetches the next (s, y) pair before each
+// reduction, and pays one barrier per pair (double-buffered reduction slots).
+// Same arithmetic, in the same order per element, as lbfgs_advance_kernel.
+// ---------------------------------------------------------------------------
+constexpr int kRegBlock = 1024;
+constexpr int kRegWaves = kRegBlock / kWave;
+
+struct Red2 {
+  double* buf;  // [2][2 * kRegWaves]
+  int parity;
+};
+
+__device__ __forceinline__ double reg_sum(double v, Red2& R) {
+  v = wave_sum(v);
+  double* b = R.buf + R.parity * 2 * kRegWaves;
+  R.parity ^= 1;
+  if ((threadIdx.x % kWave) == 0) b[threadIdx.x / kWave] = v;
+  __syncthreads();
+  double t = 0.0;
+#pragma unroll
+  for (int i = 0; i < kRegWaves; ++i) t += b[i];
+  return t;
+}
+
+__device__ __forceinline__ void reg_sum2(double& v0, double& v1, Red2& R) {
+  v0 = wave_sum(v0);
+  v1 = wave_sum(v1);
+  double* b = R.buf + R.parity * 2 * kRegWaves;
+  R.parity ^= 1;
+  if ((threadIdx.x % kWave) == 0) {
+    b[2 * (threadIdx.x / kWave)] = v0;
+    b[2 * (threadIdx.x / kWave) + 1] = v1;
+  }
+  __syncthreads();
+  double t0 = 0.0, t1 = 0.0;
+#pragma unroll
+  for (int i = 0; i < kRegWaves; ++i) {
+    t0 += b[2 * i];
+    t1 += b[2 * i + 1];
+  }
+  v0 = t0;
+  v1 = t1;
+}
+
+template <int EPT>
+__device__ void lbfgs_start_reg(const RelaxDev& r, const LbfgsCfg& c, int lane, LbfgsLane& L,
+                                Red2& R, double* alphas) {
+  const int nd = r.nd;
+  const int tid = threadIdx.x;
+  const size_t off = (size_t)lane * r.stride;
+  const int m = c.memory;
+  const double* x = r.x + off;
+  const double* gx = r.g + off;
+  double* px = r.px + off;
+  double* pg = r.pg + off;
+  double* u = r.u + off;
+  double* S = r.S + (size_t)lane * m * r.stride;
+  double* Y = r.Y + (size_t)lane * m * r.stride;
+  double* rho = r.rho + (size_t)lane * m;
+  const double rs = c.rescales[L.pass_idx];
+  const int count = L.opt_count;
+  double q[EPT];
+  double gamma;
+  int slot_new = -1;
+  double rho_new = 0.0;
+  if (count > 0) {
+    const int slot = (count - 1) % m;
+    slot_new = slot;
+    double* Ss = S + (size_t)slot * r.stride;
+    double* Ys = Y + (size_t)slot * r.stride;
+    double sy = 0.0, yy = 0.0;
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      const int i = tid + e * kRegBlock;
+      q[e] = 0.0;
+      if (i < nd) {
+        const double gi = gx[i];
+        const double ds = x[i] - px[i];
+        const double dy = gi - pg[i];
+        Ss[i] = ds;
+        Ys[i] = dy;
+        sy += dy * ds;
+        yy += dy * dy;
+        q[e] = gi;
+      }
+    }
+    reg_sum2(sy, yy, R);
+    rho_new = (sy == 0.0) ? 0.0 : 1.0 / sy;
+    if (tid == 0) rho[slot] = rho_new;
+    gamma = (yy > 0.0) ? sy / yy : 1.0;
+  } else {
+    double gg = 0.0;
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      const int i = tid + e * kRegBlock;
+      q[e] = 0.0;
+      if (i < nd) {
+        q[e] = gx[i];
+        gg += q[e] * q[e];
+      }
+    }
+    gg = reg_sum(gg, R);
+    gamma = nanmin(1.0, 1.0 / sqrt(gg));
+  }
+  const int nvalid = count < m ? count : m;
+  const int midx = count % m;
+  double sn[EPT], yn[EPT];
+  auto load_pair = [&](int t) {
+    const int sl = (midx + t) % m;
+    const double* Si = S + (size_t)sl * r.stride;
+    const double* Yi = Y + (size_t)sl * r.stride;
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      const int i = tid + e * kRegBlock;
+      sn[e] = i < nd ? Si[i] : 0.0;
+      yn[e] = i < nd ? Yi[i] : 0.0;
+    }
+  };
+  // first loop: newest -> oldest
+  if (nvalid > 0) load_pair(m - 1);
+  for (int t = m - 1; t >= m - nvalid; --t) {
+    double sc[EPT], yc[EPT];
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      sc[e] = sn[e];
+      yc[e] = yn[e];
+    }
+    if (t - 1 >= m - nvalid) load_pair(t - 1);
+    double dot = 0.0;
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) dot += sc[e] * q[e];
+    dot = reg_sum(dot, R);
+    const int sl = (midx + t) % m;
+    const double a = (sl == slot_new ? rho_new : rho[sl]) * dot;
+    if (tid == 0) alphas[t] = a;
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) q[e] = q[e] + (-a) * yc[e];
+  }
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) q[e] = gamma * q[e];
+  // second loop: oldest -> newest
+  if (nvalid > 0) load_pair(m - nvalid);
+  for (int t = m - nvalid; t < m; ++t) {
+    double sc[EPT], yc[EPT];
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      sc[e] = sn[e];
+      yc[e] = yn[e];
+    }
+    if (t + 1 < m) load_pair(t + 1);
+    double dot = 0.0;
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) dot += yc[e] * q[e];
+    dot = reg_sum(dot, R);
+    const int sl = (midx + t) % m;
+    const double bcoef = (sl == slot_new ? rho_new : rho[sl]) * dot;
+    const double coef = alphas[t] - bcoef;
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) q[e] = q[e] + coef * sc[e];
+  }
+  double slope = 0.0;
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    const int i = tid + e * kRegBlock;
+    if (i < nd) {
+      const double ui = -q[e];
+      q[e] = ui;
+      const double gi = gx[i];
+      u[i] = ui;
+      px[i] = x[i];
+      pg[i] = gi;
+      slope += ui * gi;
+    }
+  }
+  slope = reg_sum(slope, R);
+  L.opt_count = count + 1;
+  L.pairs += nvalid;
+  LsState& Q = L.ls;
+  Q.f0 = L.f;
+  Q.slope0 = slope;
+  Q.count = 0;
+  Q.interval_found = 0;
+  Q.prev_a = 0.0; Q.prev_v = L.f; Q.prev_s = slope;
+  Q.low_a = 0.0; Q.low_v = L.f; Q.low_s = slope;
+  Q.high_a = 0.0; Q.high_v = L.f; Q.high_s = slope;
+  Q.cref_a = 0.0; Q.cref_v = L.f;
+  Q.safe_a = 0.0; Q.safe_v = L.f;
+  Q.last_dec = __longlong_as_double(0x7ff0000000000000LL);
+  ls_propose(Q);
+  const double a = Q.a;
+  double* se = r.s_eval + off;
+#pragma unroll
+  for (int e = 0; e < EPT; ++e) {
+    const int i = tid + e * kRegBlock;
+    if (i < nd) se[i] = (x[i] + a * q[e]) / rs;
+  }
+}
+
+template <int EPT>
+__global__ __launch_bounds__(kRegBlock) void lbfgs_advance_reg_kernel(RelaxDev r, LbfgsCfg c) {
+  __shared__ double redbuf[4 * kRegWaves];
+  __shared__ double alphas[1024];
+  Red2 R{redbuf, 0};
+  const int slot = blockIdx.x;
+  if (slot >= *r.n_active) return;
+  const int lane = r.active[slot];
+  const int nd = r.nd;
+  const int tid = threadIdx.x;
+  const size_t off = (size_t)lane * r.stride;
+  double* s = r.s + off;
+  double* x = r.x + off;
+  double* gx = r.g + off;
+  const double* u = r.u + off;
+  double* sg = r.safe_g + off;
+  const double* ge = r.g_eval + off;
+  LbfgsLane L = r.ll[lane];
+  const double fe = r.phi_eval[lane];
+  int code = ST_RUNNING;
+  bool start = false;
+
+  if (L.mode == MODE_INIT) {
+    const double r0 = c.rescales[0];
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      const int i = tid + e * kRegBlock;
+      if (i < nd) {
+        x[i] = s[i] * r0;
+        gx[i] = ge[i] / r0;
+      }
+    }
+    L.f = fe;
+    L.last_step = 0.0;
+    L.it = 0;
+    L.n_evals = 1;
+    L.pass_idx = 0;
+    L.pass_evals = 0;
+    L.stag = 0;
+    L.opt_count = 0;
+    L.mode = MODE_EVAL;
+    code = (fe < c.energy_tol) ? ST_ENERGY_TOL : ST_RUNNING;
+    if (c.record_every > 0 && c.record_cap > 0 && L.rec_count == 0) {
+      const size_t row = (size_t)lane * c.record_cap;
+#pragma unroll
+      for (int e = 0; e < EPT; ++e) {
+        const int i = tid + e * kRegBlock;
+        if (i < nd) r.tr_s[row * r.stride + i] = wrap01(s[i]);
+      }
+      if (tid == 0) {
+        r.tr_e[row] = L.f;
+        r.tr_step[row] = 0.0;
+        r.tr_it[row] = 0;
+      }
+      L.rec_count = 1;
+    }
+    start = code == ST_RUNNING;
+  } else {
+    const double rs = c.rescales[L.pass_idx];
+    double gt[EPT], uu[EPT];
+    double sa = 0.0;
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      const int i = tid + e * kRegBlock;
+      gt[e] = 0.0;
+      uu[e] = 0.0;
+      if (i < nd) {
+        gt[e] = ge[i] / rs;
+        uu[e] = u[i];
+        sa += gt[e] * uu[e];
+      }
+    }
+    sa = reg_sum(sa, R);
+    const int flags = ls_consume(L.ls, fe, sa, c.max_linesearch_steps);
+    if (flags & 1) {
+#pragma unroll
+      for (int e = 0; e < EPT; ++e) {
+        const int i = tid + e * kRegBlock;
+        if (i < nd) sg[i] = gt[e];
+      }
+    }
+    if (flags & 2) {
+      const bool use_safe = (flags & 4) && (L.ls.safe_a > 0.0);
+      const double alpha = use_safe ? L.ls.safe_a : L.ls.prev_a;
+      const double value = use_safe ? L.ls.safe_v : fe;
+      const double dec = use_safe ? 0.0 : L.ls.last_dec;
+      const bool fail = (dec > 0.0) || !isfinite(value);
+      const long long evals = L.ls.count > 1 ? L.ls.count : 1;
+      const double f_old = L.f;
+      double gg = 0.0;
+      double gv[EPT];
+#pragma unroll
+      for (int e = 0; e < EPT; ++e) {
+        const int i = tid + e * kRegBlock;
+        gv[e] = 0.0;
+        if (i < nd) {
+          if (!fail) {
+            const double xi = x[i] + alpha * uu[e];
+            x[i] = xi;
+            s[i] = xi / rs;
+            gv[e] = use_safe ? sg[i] : gt[e];
+            gx[i] = gv[e];
+          } else {
+            gv[e] = gx[i];
+          }
+          gg += gv[e] * gv[e];
+        }
+      }
+      if (!fail) L.f = value;
+      L.stag = (L.f < f_old) ? 0 : L.stag + 1;
+      L.pass_evals += evals;
+      gg = reg_sum(gg, R);
+      const double gnorm = sqrt(gg) * rs;
+      int glob = (gnorm < c.grad_tol) ? ST_GRAD_TOL : ST_RUNNING;
+      if (L.f < c.energy_tol) glob = ST_ENERGY_TOL;
+      int local = (L.pass_evals >= c.max_evals_per_pass) ? ST_BUDGET : ST_RUNNING;
+      if (L.stag >= c.stagnation_patience) local = ST_STAGNATED;
+      if (fail) local = ST_LINESEARCH_FAIL;
+      const bool last = L.pass_idx >= c.n_rescales - 1;
+      const bool pass_end = (local != ST_RUNNING) && (glob == ST_RUNNING);
+      const bool advance = pass_end && !last;
+      code = (glob != ST_RUNNING) ? glob : ST_RUNNING;
+      if (pass_end && last) code = local;
+      if (advance) {
+        const double rn = c.rescales[L.pass_idx + 1];
+#pragma unroll
+        for (int e = 0; e < EPT; ++e) {
+          const int i = tid + e * kRegBlock;
+          if (i < nd) {
+            x[i] = s[i] * rn;
+            gx[i] = gv[e] * rs / rn;
+          }
+        }
+        L.opt_count = 0;
+        L.pass_idx += 1;
+        L.pass_evals = 0;
+        L.stag = 0;
+      }
+      L.last_step = alpha;
+      L.it += 1;
+      L.n_evals += evals;
+      if (c.record_every > 0 && c.record_cap > 0 && (L.it % c.record_every) == 0 &&
+          L.rec_count < c.record_cap) {
+        const size_t row = (size_t)lane * c.record_cap + L.rec_count;
+#pragma unroll
+        for (int e = 0; e < EPT; ++e) {
+          const int i = tid + e * kRegBlock;
+          if (i < nd) r.tr_s[row * r.stride + i] = wrap01(s[i]);
+        }
+        if (tid == 0) {
+          r.tr_e[row] = L.f;
+          r.tr_step[row] = L.last_step;
+          r.tr_it[row] = L.it;
+        }
+        L.rec_count += 1;
+      }
+      start = code == ST_RUNNING;
+    } else {
+      const double a = L.ls.a;
+      double* se = r.s_eval + off;
+#pragma unroll
+      for (int e = 0; e < EPT; ++e) {
+        const int i = tid + e * kRegBlock;
+        if (i < nd) se[i] = (x[i] + a * uu[e]) / rs;
+      }
+    }
+  }
+  if (start) lbfgs_start_reg<EPT>(r, c, lane, L, R, alphas);
+  if (tid == 0) {
+    r.ll[lane] = L;
+    r.status[lane] = code;
+  }
+}
+
+int lbfgs_register_ept(int nd) {
+  for (int e : {1, 2, 4, 8})
+    if (nd <= e * kRegBlock) return e;
+  return 0;
+}
+
+void launch_lbfgs_advance(const RelaxDev& r, const LbfgsCfg& c, int slots_ub, hipStream_t st,
+                          int variant) {
   if (slots_ub <= 0) return;
-  hipLaunchKernelGGL(lbfgs_advance_kernel, dim3(slots_ub), dim3(kAdvBlock), 0, st, r, c);
+  const int ept = lbfgs_register_ept(r.nd);
+  if (variant == 1 || ept == 0) {
+    hipLaunchKernelGGL(lbfgs_advance_kernel, dim3(slots_ub), dim3(kAdvBlock), 0, st, r, c);
+    return;
+  }
+  switch (ept) {
+    case 1:
+      hipLaunchKernelGGL(lbfgs_advance_reg_kernel<1>, dim3(slots_ub), dim3(kRegBlock), 0, st, r, c);
+      break;
+    case 2:
+      hipLaunchKernelGGL(lbfgs_advance_reg_kernel<2>, dim3(slots_ub), dim3(kRegBlock), 0, st, r, c);
+      break;
+    case 4:
+      hipLaunchKernelGGL(lbfgs_advance_reg_kernel<4>, dim3(slots_ub), dim3(kRegBlock), 0, st, r, c);
+      break;
+    default:
+      hipLaunchKernelGGL(lbfgs_advance_reg_kernel<8>, dim3(slots_ub), dim3(kRegBlock), 0, st, r, c);
+      break;
+  }
 }
 
 // ---------------------------------------------------------------------------

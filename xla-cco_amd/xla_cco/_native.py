@@ -112,6 +112,12 @@ SIGNATURES = {
     "cco_relax_result_dev": (C.c_int, [_P, _P, _P, _P, _P, _P, _P]),
     "cco_relax_trajectory": (C.c_int, [_P, _PI64, _PI64, _PD, _PD, _PD]),
     "cco_relax_stats": (C.c_int, [_P, _PI64, _PI64]),
+    "cco_relax_counters": (C.c_int, [_P, _PI64]),
+    "cco_relax_set_option": (C.c_int, [_P, C.c_int, C.c_int64]),
+    "cco_relax_set_profiling": (C.c_int, [_P, C.c_int]),
+    "cco_relax_profile": (C.c_int, [_P, _PD, _PI64]),
+    "cco_energy_profile": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int, _PD]),
+    "cco_copy_dev": (C.c_int, [_P, _P, _P, C.c_size_t, _PD]),
 }
 
 _lib: Optional[C.CDLL] = None
@@ -200,3 +206,39 @@ def context(device: int = 0) -> Context:
             _contexts.setdefault(device, ctx)
             ctx = _contexts[device]
     return ctx
+
+
+class DeviceArray:
+    """A float64/int64 device buffer on a context (cco_dev_alloc), for the *_dev entry points."""
+
+    def __init__(self, ctx: Context, shape, dtype=np.float64) -> None:
+        self.ctx = ctx
+        self.shape = tuple(shape)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = int(np.prod(self.shape)) * self.dtype.itemsize
+        p = _P()
+        check(lib().cco_dev_alloc(ctx.handle, self.nbytes, C.byref(p)))
+        self.ptr = p
+
+    @classmethod
+    def from_host(cls, ctx: Context, a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = cls(ctx, a.shape, a.dtype)
+        check(lib().cco_memcpy_h2d(ctx.handle, d.ptr, a.ctypes.data_as(_P), d.nbytes))
+        return d
+
+    def to_host(self) -> np.ndarray:
+        out = np.empty(self.shape, self.dtype)
+        check(lib().cco_memcpy_d2h(self.ctx.handle, out.ctypes.data_as(_P), self.ptr, self.nbytes))
+        return out
+
+    def free(self) -> None:
+        if self.ptr is not None and _lib is not None:
+            _lib.cco_dev_free(self.ctx.handle, self.ptr)
+        self.ptr = None
+
+    def __del__(self) -> None:  # pragma: no cover
+        try:
+            self.free()
+        except Exception:
+            pass
