@@ -94,6 +94,13 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="skip per-kernel event timing")
+    ap.add_argument(
+        "--calibrate-copy",
+        type=int,
+        default=0,
+        metavar="BYTES",
+        help="first run the library's plain copy kernel over BYTES (HBM counter calibration)",
+    )
     args = ap.parse_args()
     if args.gpus != 1:
         print(
@@ -108,6 +115,17 @@ def main() -> int:
 
     L = _native.lib()
     ctx = _native.context(0)
+    if args.calibrate_copy > 0:
+        nbytes = (args.calibrate_copy // 32) * 32
+        src = _native.DeviceArray(ctx, (nbytes // 8,))
+        dst = _native.DeviceArray(ctx, (nbytes // 8,))
+        t = C.c_double(0.0)
+        for _ in range(2):
+            _native.check(L.cco_copy_dev(ctx.handle, dst.ptr, src.ptr, nbytes, C.byref(t)))
+        print(f"copy_kernel {nbytes} bytes: {t.value:.4f} ms, {2 * nbytes / t.value / 1e6:.1f} GB/s",
+              file=sys.stderr)
+        src.free()
+        dst.free()
     d, n, chi, B = args.d, args.n, args.chi, args.lanes
     box, ks, energy = build_workload(d, n, chi)
     M = int(ks.n.shape[0])
