@@ -177,8 +177,8 @@ def main() -> int:
         iters += cnt[1]
         pairs += cnt[2]
         steps_total += cnt[3]
-    ms = (C.c_double * 5)()
-    launches = (C.c_int64 * 5)()
+    ms = (C.c_double * 6)()
+    launches = (C.c_int64 * 6)()
     _native.check(L.cco_relax_profile(sess.handle, ms, launches))
 
     value = converged / elapsed

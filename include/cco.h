@@ -30,6 +30,8 @@ extern "C" {
 
 #define CCO_METHOD_DESCENT 0
 #define CCO_METHOD_LBFGS 1
+#define CCO_METHOD_FLOW 2
+#define CCO_METHOD_VERLET 3
 
 typedef struct cco_ctx cco_ctx;
 typedef struct cco_energy cco_energy;
@@ -142,12 +144,60 @@ int cco_relax_counters(cco_relax* r, int64_t* out4);
 #define CCO_OPT_ADVANCE_VARIANT 1
 int cco_relax_set_option(cco_relax* r, int key, int64_t value);
 
+/* Gradient flow dr/dt = -grad_r Phi in the Cartesian metric of basis
+ * (d x d host, rows = lattice vectors), Dormand-Prince 5(4) with error
+ * control on the RMS Cartesian step and on the clocks (t, arclength).
+ * Replaces gradient_flow (src/xla_cco/minimize/flow.py:258-323, DOPRI5,
+ * vmapped by the caller) and FlowSettings (:93-125; arclength = 1 selects
+ * parametrization "arclength").  e == NULL: external energy through
+ * cco_relax_pending / cco_relax_submit.  Run with cco_relax_run/_step. */
+typedef struct {
+  double rtol, atol, dt_init, dt_min, dt_max, t_max, energy_tol, grad_tol;
+  int64_t max_steps;
+  int32_t arclength;
+  int32_t record_every, max_records;
+} cco_flow_settings;
+
+int cco_flow_create(cco_ctx* ctx, cco_energy* e, int B, int N, int d,
+                    const cco_flow_settings* fs, const double* basis_host, cco_relax** out);
+/* s0: B x N x d; dt0: B first steps (NULL: dt_init), as gradient_flow's dt0. */
+int cco_flow_init(cco_relax* r, const double* s0_host, const double* dt0_host);
+/* FlowResult fields (flow.py:138-154): unwrapped s, energy, t, length, next
+ * dt, accepted steps, evaluations, status. */
+int cco_flow_result(cco_relax* r, double* s_host, double* energy_host, double* t_host,
+                    double* length_host, double* dt_host, int64_t* n_steps_host,
+                    int64_t* n_evals_host, int64_t* status_host);
+/* Recorded rows (count per lane), s (unwrapped), t, length, energy of
+ * B x max_records rows (the final state is appended by the caller). */
+int cco_flow_trajectory(cco_relax* r, int64_t* count_host, double* s_host, double* t_host,
+                        double* length_host, double* energy_host);
+
+/* NVE velocity Verlet, mass 1, Cartesian positions r (B x N x d), forces at
+ * s = r inv(B): replaces verlet_step (src/xla_cco/dynamics/integrators.py:83-96)
+ * with make_force_fn (dynamics/run.py:219-244) and the verlet branch of
+ * _step (run.py:415-442): kinetic energy, drift guard |Phi + KE - e0| <=
+ * drift_tol * d (N - 1) T / 2 (else status 7, lane frozen; drift_tol = inf
+ * disables it), running sums.  cco_md_init evaluates the initial forces and
+ * e0 = Phi + KE(v0) (init_state, run.py:300-381). */
+int cco_md_create(cco_ctx* ctx, cco_energy* e, int B, int N, int d, const double* basis_host,
+                  double dt, double drift_tol, cco_relax** out);
+int cco_md_init(cco_relax* r, const double* r0_host, const double* v0_host,
+                const double* temperature_host);
+/* n_steps Verlet steps for every running lane (host sync every sync_every). */
+int cco_md_run(cco_relax* r, int64_t n_steps, int sync_every, int* n_active);
+/* r, v, f (B x N x d, NULL to skip); scalars B x 9: phi, kinetic, e0,
+ * max_drift, sum_kinetic, sum_phi, sum_phi2, time, temperature; ints B x 3:
+ * step, n_sum, status. */
+int cco_md_state(cco_relax* r, double* r_host, double* v_host, double* f_host,
+                 double* scalars_host, int64_t* ints_host);
+
 /* Measurement (no reference counterpart; the reference times with
  * block_until_ready, benchmarks/bench_energy.py:104-115).  Per-stage device
  * time from HIP events recorded on the context stream around every launch:
- * stages rho, reduce, force, advance, compact (ms summed, launches counted). */
+ * stages rho, reduce, force, advance (or MD finish), compact, MD prepare
+ * (ms summed, launches counted; six entries each). */
 int cco_relax_set_profiling(cco_relax* r, int on);
-int cco_relax_profile(cco_relax* r, double* ms_sum5, int64_t* launches5);
+int cco_relax_profile(cco_relax* r, double* ms_sum6, int64_t* launches6);
 /* reps timed evaluations of B x N on device data: avg ms of the rho, reduce
  * and force kernels and of the whole evaluation (avg_ms[0..3]). */
 int cco_energy_profile(cco_energy* e, const double* s_dev, int B, int N, int reps,

@@ -651,3 +651,182 @@ def relax(
     if isinstance(settings, LBFGSSettings):
         return [lbfgs(vg, s0[i], settings) for i in range(s0.shape[0])]
     raise TypeError(f"unsupported settings type {type(settings).__name__}")
+
+
+# ============================================================
+# GRADIENT FLOW (minimize/flow.py:60-481, DOPRI5)
+# ============================================================
+
+_DP_A = (
+    (),
+    (1 / 5,),
+    (3 / 40, 9 / 40),
+    (44 / 45, -56 / 15, 32 / 9),
+    (19372 / 6561, -25360 / 2187, 64448 / 6561, -212 / 729),
+    (9017 / 3168, -355 / 33, 46732 / 5247, 49 / 176, -5103 / 18656),
+)
+_DP_B5 = (35 / 384, 0.0, 500 / 1113, 125 / 192, -2187 / 6784, 11 / 84, 0.0)
+_DP_B4 = (5179 / 57600, 0.0, 7571 / 16695, 393 / 640, -92097 / 339200, 187 / 2100, 1 / 40)
+_DP_E = tuple(b5 - b4 for b5, b4 in zip(_DP_B5, _DP_B4))
+
+
+@dataclasses.dataclass(frozen=True)
+class FlowSettings:
+    rtol: float = 1e-6
+    atol: float = 1e-14
+    parametrization: str = "time"
+    dt_init: float = 1e-3
+    dt_min: float = 1e-14
+    dt_max: float = math.inf
+    t_max: float = math.inf
+    max_steps: int = 200_000
+    energy_tol: float = 0.0
+    grad_tol: float = 0.0
+    record_every: int = 0
+    max_records: int = 0
+
+
+def gradient_flow(vg: ValueGrad, s0: np.ndarray, basis: np.ndarray, st: FlowSettings) -> Dict[str, object]:
+    """Dormand-Prince 5(4) on dr/dt = -grad_r Phi (flow.py:212-481), one configuration.
+
+    Returns a dict with s, energy, t, length, dt, n_steps, n_evals, status and,
+    when recording, the trajectory rows (s, t, length, energy).
+    """
+    arclength = st.parametrization == "arclength"
+    metric_inv = np.linalg.inv(basis @ basis.T)
+    inv_t = np.linalg.inv(basis).T
+
+    def velocity(s):
+        f, g = vg(s)
+        v = -g @ metric_inv
+        n = math.sqrt(float(np.sum((g @ inv_t) ** 2)))
+        if not arclength:
+            return f, g, v, np.array([1.0, n]), n
+        inv = 1.0 / n if n > 0.0 else 0.0
+        return f, g, v * inv, np.array([inv, 1.0]), n
+
+    def status(f, g, gn, t, dt, steps, tries):
+        code = STATUS_STEP_FLOOR if dt < st.dt_min else STATUS_RUNNING
+        if t >= st.t_max or steps >= st.max_steps or tries >= 4 * st.max_steps:
+            code = STATUS_BUDGET
+        if np.all(g == 0.0) or gn < st.grad_tol:
+            code = STATUS_GRAD_TOL
+        if f < st.energy_tol:
+            code = STATUS_ENERGY_TOL
+        return code
+
+    def scaled_error(err_s, step):
+        move = math.sqrt(float(np.mean((step @ basis) ** 2)))
+        tol = st.atol + st.rtol * move
+        return math.sqrt(float(np.mean((err_s @ basis) ** 2))) / tol
+
+    s = s0.copy()
+    ev = velocity(s)
+    t = length = 0.0
+    dt = min(st.dt_init, st.dt_max)
+    steps = tries = 0
+    n_evals = 1
+    code = status(ev[0], ev[1], ev[4], t, dt, steps, tries)
+    rows = [(s.copy(), 0.0, 0.0, ev[0])] if st.record_every > 0 else []
+    while code == STATUS_RUNNING:
+        h = dt if arclength else min(dt, st.t_max - t)
+        evs = [ev]
+        for i in range(1, 6):
+            incr = sum(a * e[2] for a, e in zip(_DP_A[i], evs))
+            evs.append(velocity(s + h * incr))
+        s_new = s + h * sum(b * e[2] for b, e in zip(_DP_B5[:6], evs))
+        tl_new = h * sum(b * e[3] for b, e in zip(_DP_B5[:6], evs))
+        ev_new = velocity(s_new)
+        err_s = h * sum(e * x[2] for e, x in zip(_DP_E, evs + [ev_new]))
+        err_tl = h * sum(e * x[3] for e, x in zip(_DP_E, evs + [ev_new]))
+        err = scaled_error(err_s, s_new - s)
+        tol_tl = st.atol + st.rtol * np.abs(tl_new)
+        with np.errstate(all="ignore"):
+            err = _nanmax(err, float(np.max(np.abs(err_tl) / tol_tl)))
+            factor = float(np.clip(0.9 * err ** (-1.0 / 5), 0.2, 5.0))
+        if not math.isfinite(factor):
+            factor = 0.2
+        accept = err <= 1.0
+        dt = min(h * factor, st.dt_max)
+        if accept:
+            s, ev = s_new, ev_new
+            t += tl_new[0]
+            length += tl_new[1]
+            steps += 1
+            if st.record_every > 0 and steps % st.record_every == 0 and len(rows) < st.max_records:
+                rows.append((s.copy(), t, length, ev[0]))
+        tries += 1
+        n_evals += 6
+        code = status(ev[0], ev[1], ev[4], t, dt, steps, tries)
+    out: Dict[str, object] = {
+        "s": s, "energy": ev[0], "t": t, "length": length, "dt": dt,
+        "n_steps": steps, "n_evals": n_evals, "status": code,
+    }
+    if st.record_every > 0:
+        last = rows[-1]
+        if not (last[1] == t and last[2] == length):
+            if len(rows) < st.max_records:
+                rows.append((s.copy(), t, length, ev[0]))
+            else:
+                rows[-1] = (s.copy(), t, length, ev[0])
+        out["traj"] = rows
+    return out
+
+
+# ============================================================
+# VELOCITY VERLET (dynamics/integrators.py:83-96, run.py:219-442)
+# ============================================================
+
+
+def verlet_nve(
+    vg: ValueGrad,
+    basis: np.ndarray,
+    r0: np.ndarray,
+    v0: np.ndarray,
+    dt: float,
+    n_steps: int,
+    temperature: float = 1.0,
+    drift_tol: float = math.inf,
+) -> Dict[str, object]:
+    """NVE Verlet with forces at s = r inv(B), F = -g inv(B)^T; drift guard and sums."""
+    inv = np.linalg.inv(basis)
+    inv_t = inv.T
+    n, d = r0.shape
+
+    def force(r):
+        f, g = vg(r @ inv)
+        return f, -g @ inv_t
+
+    r, v = r0.copy(), v0.copy()
+    phi, f = force(r)
+    e0 = phi + 0.5 * float(np.sum(v * v))
+    half = 0.5 * dt
+    status = STATUS_RUNNING
+    max_drift = 0.0
+    sums = [0.0, 0.0, 0.0]
+    step = 0
+    hist = []
+    for _ in range(n_steps):
+        if status != STATUS_RUNNING:
+            break
+        v_half = v + half * f
+        r = r + dt * v_half
+        phi, f = force(r)
+        v = v_half + half * f
+        ke = 0.5 * float(np.sum(v * v))
+        drift = abs(phi + ke - e0)
+        if math.isfinite(drift_tol):
+            limit = drift_tol * 0.5 * d * (n - 1) * temperature
+            if not drift <= limit:
+                status = STATUS_DRIFT
+        max_drift = _nanmax(max_drift, drift)
+        sums[0] += ke
+        sums[1] += phi
+        sums[2] += phi * phi
+        step += 1
+        hist.append((phi, ke))
+    return {
+        "r": r, "v": v, "f": f, "phi": phi, "e0": e0, "max_drift": max_drift,
+        "sum_kinetic": sums[0], "sum_phi": sums[1], "sum_phi2": sums[2], "step": step,
+        "status": status, "hist": hist,
+    }

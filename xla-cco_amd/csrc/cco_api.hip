@@ -270,7 +270,12 @@ int energy_batch(cco_energy* E, const double* s_dev, int B, int N, double* phi_d
 // ===========================================================================
 // relax session
 // ===========================================================================
-constexpr int kStages = 5;  // rho, reduce, force, advance, compact
+// Stages of one evaluation step; profiling accumulates per stage id.
+constexpr int kStages = 6;
+enum StageId { STG_RHO = 0, STG_REDUCE = 1, STG_FORCE = 2, STG_ADVANCE = 3, STG_COMPACT = 4,
+               STG_PREPARE = 5 };
+constexpr int METHOD_FLOW = CCO_METHOD_FLOW;
+constexpr int METHOD_MD = CCO_METHOD_VERLET;
 
 struct cco_relax {
   cco_ctx* ctx;
@@ -278,6 +283,10 @@ struct cco_relax {
   int B, N, d, method;
   DescentCfg dc;
   LbfgsCfg lc;
+  FlowCfg fc;
+  MdCfg mc;
+  bool md_need_init = false;  // the first MD step evaluates r0 without a drift
+  double* flow_dt0 = nullptr;  // device copy of per-lane first steps
   RelaxDev dev;
   std::vector<DevBuf> bufs;
   DevBuf part, cbox, out_s, out_scalars, pack, pack_idx;
@@ -290,8 +299,8 @@ struct cco_relax {
   bool profile = false;
   int advance_variant = 0;  // 0 auto, 1 global-memory vectors, 2 register-resident
   std::vector<hipEvent_t> evs;
-  double stage_ms[kStages] = {0, 0, 0, 0, 0};
-  int64_t stage_n[kStages] = {0, 0, 0, 0, 0};
+  double stage_ms[kStages] = {0, 0, 0, 0, 0, 0};
+  int64_t stage_n[kStages] = {0, 0, 0, 0, 0, 0};
 };
 
 namespace {
@@ -340,10 +349,21 @@ __global__ void relax_init_kernel(RelaxDev r, int method) {
     DescentLane L = {};
     L.mode = MODE_INIT;
     r.dl[lane] = L;
-  } else {
+  } else if (method == CCO_METHOD_LBFGS) {
     LbfgsLane L = {};
     L.mode = MODE_INIT;
     r.ll[lane] = L;
+  } else if (method == CCO_METHOD_FLOW) {
+    FlowLane L = {};
+    L.mode = MODE_INIT;
+    r.fl[lane] = L;
+  } else {
+    MdLane L = r.ml[lane];  // temperature set by the host before init
+    const double temp = L.temperature;
+    L = MdLane{};
+    L.temperature = temp;
+    L.mode = MODE_INIT;
+    r.ml[lane] = L;
   }
 }
 
@@ -368,21 +388,39 @@ Batch relax_batch(cco_relax* R) {
 
 void relax_stage(cco_relax* R, int stage) {
   hipStream_t st = R->ctx->stream;
-  if (stage <= 2) {
+  if (stage <= STG_FORCE) {
     launch_energy_stage(stage, R->E->geo, relax_batch(R), R->slots_ub, st);
-  } else if (stage == 3) {
+  } else if (stage == STG_ADVANCE) {
     if (R->method == CCO_METHOD_DESCENT)
       launch_descent_advance(R->dev, R->dc, R->slots_ub, st);
-    else
+    else if (R->method == CCO_METHOD_LBFGS)
       launch_lbfgs_advance(R->dev, R->lc, R->slots_ub, st, R->advance_variant);
-  } else {
+    else if (R->method == METHOD_FLOW)
+      launch_flow_advance(R->dev, R->fc, R->slots_ub, st);
+    else
+      launch_md_finish(R->dev, R->mc, R->slots_ub, st);
+  } else if (stage == STG_COMPACT) {
     launch_compact(R->dev, st);
+  } else {
+    launch_md_prepare(R->dev, R->mc, R->slots_ub, st);
   }
 }
 
+// Stage sequence of one evaluation step of the session's method.
+int step_stages(const cco_relax* R, int* out) {
+  int n = 0;
+  if (R->method == METHOD_MD && !R->md_need_init) out[n++] = STG_PREPARE;
+  out[n++] = STG_RHO;
+  out[n++] = STG_REDUCE;
+  out[n++] = STG_FORCE;
+  out[n++] = STG_ADVANCE;
+  out[n++] = STG_COMPACT;
+  return n;
+}
+
 int relax_advance(cco_relax* R) {
-  relax_stage(R, 3);
-  relax_stage(R, 4);
+  relax_stage(R, STG_ADVANCE);
+  relax_stage(R, STG_COMPACT);
   CCO_HIP(hipGetLastError());
   return CCO_OK;
 }
@@ -698,6 +736,7 @@ int cco_relax_destroy(cco_relax* R) {
 
 int cco_relax_init_dev(cco_relax* R, const double* s0_dev) {
   if (!R || !s0_dev) return fail(CCO_ERR_ARG, "NULL argument");
+  if (R->method > CCO_METHOD_LBFGS) return fail(CCO_ERR_STATE, "not a relax session");
   DevScope ds(R->ctx->device);
   hipStream_t st = R->ctx->stream;
   const size_t bytes = sizeof(double) * (size_t)R->B * R->dev.stride;
@@ -715,6 +754,7 @@ int cco_relax_init_dev(cco_relax* R, const double* s0_dev) {
 
 int cco_relax_init(cco_relax* R, const double* s0_host) {
   if (!R || !s0_host) return fail(CCO_ERR_ARG, "NULL argument");
+  if (R->method > CCO_METHOD_LBFGS) return fail(CCO_ERR_STATE, "not a relax session");
   DevScope ds(R->ctx->device);
   hipStream_t st = R->ctx->stream;
   const size_t bytes = sizeof(double) * (size_t)R->B * R->dev.stride;
@@ -733,38 +773,48 @@ int cco_relax_init(cco_relax* R, const double* s0_host) {
 
 int cco_relax_step(cco_relax* R, int64_t n_evals, int* n_active) {
   if (!R) return fail(CCO_ERR_ARG, "relax is NULL");
-  if (!R->ready) return fail(CCO_ERR_STATE, "relax not initialised");
-  if (!R->E) return fail(CCO_ERR_STATE, "external-energy relax: use pending/submit");
+  if (!R->ready) return fail(CCO_ERR_STATE, "session not initialised");
+  if (!R->E) return fail(CCO_ERR_STATE, "external-energy session: use pending/submit");
   DevScope ds(R->ctx->device);
   hipStream_t st = R->ctx->stream;
   int64_t done = 0;
-  const size_t need = R->profile ? (size_t)(kStages + 1) * (size_t)std::max<int64_t>(n_evals, 0) : 0;
+  constexpr int kEv = kStages + 1;
+  const size_t need = R->profile ? (size_t)kEv * (size_t)std::max<int64_t>(n_evals, 0) : 0;
   while (R->evs.size() < need) {
     hipEvent_t ev;
     CCO_HIP(hipEventCreate(&ev));
     R->evs.push_back(ev);
   }
+  std::vector<int> seq_ids;
   for (int64_t i = 0; i < n_evals && R->slots_ub > 0; ++i) {
-    for (int stage = 0; stage < kStages; ++stage) {
-      if (R->profile && stage == 0) CCO_HIP(hipEventRecord(R->evs[(kStages + 1) * i], st));
-      relax_stage(R, stage);
-      if (R->profile) CCO_HIP(hipEventRecord(R->evs[(kStages + 1) * i + stage + 1], st));
+    int seq[kStages];
+    const int ns = step_stages(R, seq);
+    for (int q = 0; q < ns; ++q) {
+      if (R->profile && q == 0) CCO_HIP(hipEventRecord(R->evs[kEv * i], st));
+      relax_stage(R, seq[q]);
+      if (R->profile) {
+        CCO_HIP(hipEventRecord(R->evs[kEv * i + q + 1], st));
+        seq_ids.push_back(seq[q]);
+      }
     }
+    if (R->profile) seq_ids.push_back(-1);
     CCO_HIP(hipGetLastError());
+    R->md_need_init = false;
     R->steps += 1;
     ++done;
   }
   int rc = relax_sync_count(R, n_active);
   if (rc) return rc;
   if (R->profile) {
+    size_t p = 0;
     for (int64_t i = 0; i < done; ++i) {
-      for (int stage = 0; stage < kStages; ++stage) {
+      for (int q = 0; p < seq_ids.size() && seq_ids[p] >= 0; ++q, ++p) {
         float ms = 0.0f;
-        CCO_HIP(hipEventElapsedTime(&ms, R->evs[(kStages + 1) * i + stage],
-                                    R->evs[(kStages + 1) * i + stage + 1]));
-        R->stage_ms[stage] += ms;
-        R->stage_n[stage] += 1;
+        CCO_HIP(hipEventElapsedTime(&ms, R->evs[kEv * i + q], R->evs[kEv * i + q + 1]));
+        R->stage_ms[seq_ids[p]] += ms;
+        R->stage_n[seq_ids[p]] += 1;
       }
+      ++p;  // step separator
     }
   }
   return CCO_OK;
@@ -787,7 +837,21 @@ int cco_relax_counters(cco_relax* R, int64_t* out4) {
   DevScope ds(R->ctx->device);
   const size_t B = R->B;
   int64_t ev = 0, it = 0, pairs = 0;
-  if (R->method == CCO_METHOD_DESCENT) {
+  if (R->method == METHOD_FLOW) {
+    std::vector<FlowLane> L(B);
+    CCO_HIP(hipMemcpy(L.data(), R->dev.fl, sizeof(FlowLane) * B, hipMemcpyDeviceToHost));
+    for (auto& l : L) {
+      ev += l.n_evals;
+      it += l.n_tries;
+    }
+  } else if (R->method == METHOD_MD) {
+    std::vector<MdLane> L(B);
+    CCO_HIP(hipMemcpy(L.data(), R->dev.ml, sizeof(MdLane) * B, hipMemcpyDeviceToHost));
+    for (auto& l : L) {
+      ev += l.step + 1;
+      it += l.step;
+    }
+  } else if (R->method == CCO_METHOD_DESCENT) {
     std::vector<DescentLane> L(B);
     CCO_HIP(hipMemcpy(L.data(), R->dev.dl, sizeof(DescentLane) * B, hipMemcpyDeviceToHost));
     for (auto& l : L) {
@@ -973,6 +1037,7 @@ int cco_relax_result_dev(cco_relax* R, double* s_dev, double* energy_dev, double
   if (!R || !s_dev || !energy_dev || !grad_norm_dev || !n_evals_dev || !n_iters_dev || !status_dev)
     return fail(CCO_ERR_ARG, "NULL argument");
   if (!R->ready) return fail(CCO_ERR_STATE, "relax not initialised");
+  if (R->method > CCO_METHOD_LBFGS) return fail(CCO_ERR_STATE, "not a relax session");
   DevScope ds(R->ctx->device);
   RelaxOutDev o;
   o.s = s_dev;
@@ -1015,6 +1080,7 @@ int cco_relax_result(cco_relax* R, double* s_host, double* energy_host, double* 
 int cco_relax_trajectory(cco_relax* R, int64_t* count_host, int64_t* iteration_host,
                          double* s_host, double* energy_host, double* step_host) {
   if (!R || !count_host) return fail(CCO_ERR_ARG, "NULL argument");
+  if (R->method > CCO_METHOD_LBFGS) return fail(CCO_ERR_STATE, "not a relax session");
   DevScope ds(R->ctx->device);
   const int cap = R->method == CCO_METHOD_DESCENT ? R->dc.record_cap : R->lc.record_cap;
   hipStream_t st = R->ctx->stream;
@@ -1051,6 +1117,352 @@ int cco_relax_stats(cco_relax* R, int64_t* steps, int64_t* lane_evals) {
   if (!R) return fail(CCO_ERR_ARG, "relax is NULL");
   if (steps) *steps = R->steps;
   if (lane_evals) *lane_evals = R->lane_evals;
+  return CCO_OK;
+}
+
+
+// ---------------------------------------------------------------------------
+// gradient flow and velocity Verlet sessions
+// ---------------------------------------------------------------------------
+static bool invert3(const double* a, int d, double* out) {
+  // Gauss-Jordan with partial pivoting on a d x d (d <= 3) row-major matrix
+  double m[3][6];
+  for (int i = 0; i < d; ++i)
+    for (int j = 0; j < 2 * d; ++j) m[i][j] = j < d ? a[i * d + j] : (j - d == i ? 1.0 : 0.0);
+  for (int c = 0; c < d; ++c) {
+    int p = c;
+    for (int r = c + 1; r < d; ++r)
+      if (std::fabs(m[r][c]) > std::fabs(m[p][c])) p = r;
+    if (!(std::fabs(m[p][c]) > 0.0)) return false;
+    if (p != c)
+      for (int j = 0; j < 2 * d; ++j) std::swap(m[p][j], m[c][j]);
+    const double piv = m[c][c];
+    for (int j = 0; j < 2 * d; ++j) m[c][j] /= piv;
+    for (int r = 0; r < d; ++r) {
+      if (r == c) continue;
+      const double f = m[r][c];
+      for (int j = 0; j < 2 * d; ++j) m[r][j] -= f * m[c][j];
+    }
+  }
+  for (int i = 0; i < d; ++i)
+    for (int j = 0; j < d; ++j) out[i * d + j] = m[i][d + j];
+  return true;
+}
+
+static int new_session(cco_ctx* ctx, cco_energy* E, int B, int N, int d, int method,
+                       cco_relax** out) {
+  if (!ctx || !out) return fail(CCO_ERR_ARG, "NULL argument");
+  if (B < 1 || N < 1) return fail(CCO_ERR_ARG, "need B >= 1 and N >= 1");
+  if (d < 1 || d > 3) return fail(CCO_ERR_ARG, "d must be 1, 2 or 3");
+  if (E && E->geo.d != d) return fail(CCO_ERR_ARG, "dimension mismatch");
+  cco_relax* R = new cco_relax();
+  R->ctx = ctx;
+  R->E = E;
+  R->B = B;
+  R->N = N;
+  R->d = d;
+  R->method = method;
+  std::memset(&R->dc, 0, sizeof(R->dc));
+  std::memset(&R->lc, 0, sizeof(R->lc));
+  std::memset(&R->fc, 0, sizeof(R->fc));
+  std::memset(&R->mc, 0, sizeof(R->mc));
+  R->dev.B = B;
+  R->dev.nd = N * d;
+  R->dev.stride = N * d;
+  *out = R;
+  return CCO_OK;
+}
+
+static int finish_session(cco_relax* R) {
+  int rc = 0;
+  if (R->E) {
+    rc = dev_reserve(R->part, sizeof(double2) * (size_t)R->B * n_splits_for(R->N) * R->E->geo.Mbox);
+    if (rc == 0) rc = dev_reserve(R->cbox, sizeof(double2) * (size_t)R->B * R->E->geo.Mbox);
+  }
+  if (rc == 0 && hipHostMalloc((void**)&R->host_count, sizeof(int)) != hipSuccess)
+    rc = fail(CCO_ERR_DEVICE, "hipHostMalloc failed");
+  return rc;
+}
+
+int cco_flow_create(cco_ctx* ctx, cco_energy* E, int B, int N, int d,
+                    const cco_flow_settings* fs, const double* basis, cco_relax** out) {
+  if (!fs || !basis) return fail(CCO_ERR_ARG, "NULL argument");
+  if (!(fs->atol > 0.0) || !(fs->rtol > 0.0) || !(fs->dt_init > 0.0) || fs->max_steps <= 0)
+    return fail(CCO_ERR_ARG, "atol, rtol, dt_init and max_steps must be positive");
+  if (fs->record_every < 0 || fs->max_records < 0)
+    return fail(CCO_ERR_ARG, "record_every and max_records must be >= 0");
+  if ((fs->record_every > 0) != (fs->max_records > 1))
+    return fail(CCO_ERR_ARG, "recording needs record_every > 0 and max_records >= 2");
+  if (fs->arclength && std::isfinite(fs->t_max))
+    return fail(CCO_ERR_ARG, "t_max is supported only with parametrization='time'");
+  double inv[9], bbt[9], minv[9];
+  if (!invert3(basis, d, inv)) return fail(CCO_ERR_ARG, "basis must be invertible");
+  for (int i = 0; i < d; ++i)
+    for (int j = 0; j < d; ++j) {
+      double acc = 0.0;
+      for (int k = 0; k < d; ++k) acc += basis[i * d + k] * basis[j * d + k];
+      bbt[i * d + j] = acc;
+    }
+  if (!invert3(bbt, d, minv)) return fail(CCO_ERR_ARG, "basis must be invertible");
+  DevScope dscope(ctx ? ctx->device : 0);
+  cco_relax* R = nullptr;
+  int rc = new_session(ctx, E, B, N, d, METHOD_FLOW, &R);
+  if (rc) return rc;
+  FlowCfg& c = R->fc;
+  c.rtol = fs->rtol;
+  c.atol = fs->atol;
+  c.dt_init = fs->dt_init;
+  c.dt_min = fs->dt_min;
+  c.dt_max = fs->dt_max;
+  c.t_max = fs->t_max;
+  c.energy_tol = fs->energy_tol;
+  c.grad_tol = fs->grad_tol;
+  c.max_steps = fs->max_steps;
+  for (int i = 0; i < d * d; ++i) {
+    c.metric_inv[i] = minv[i];
+    c.basis[i] = basis[i];
+  }
+  for (int i = 0; i < d; ++i)
+    for (int j = 0; j < d; ++j) c.inv_t[i * d + j] = inv[j * d + i];
+  c.d = d;
+  c.arclength = fs->arclength ? 1 : 0;
+  c.record_every = fs->record_every;
+  c.record_cap = fs->record_every > 0 ? fs->max_records : 0;
+  c.dt0 = nullptr;
+  RelaxDev& v = R->dev;
+  const size_t vec = (size_t)B * v.stride;
+  auto A = [&](auto*& p, size_t cnt) {
+    if (rc == 0) rc = alloc_into(R, p, cnt);
+  };
+  A(v.status, B);
+  A(v.active, B);
+  A(v.n_active, 1);
+  A(v.s_eval, vec);
+  A(v.g_eval, vec);
+  A(v.phi_eval, B);
+  A(v.s, vec);
+  A(v.k, vec * 7);
+  A(v.fl, B);
+  double* dt0 = nullptr;
+  A(dt0, B);
+  c.dt0 = nullptr;
+  R->flow_dt0 = dt0;
+  if (c.record_cap > 0) {
+    A(v.tr_s, vec * c.record_cap);
+    A(v.tr_e, (size_t)B * c.record_cap);
+    A(v.tr_step, (size_t)B * c.record_cap);
+    A(v.tr_len, (size_t)B * c.record_cap);
+    A(v.tr_it, (size_t)B * c.record_cap);
+  }
+  if (rc == 0) rc = finish_session(R);
+  if (rc) {
+    cco_relax_destroy(R);
+    return rc;
+  }
+  *out = R;
+  return CCO_OK;
+}
+
+int cco_flow_init(cco_relax* R, const double* s0_host, const double* dt0_host) {
+  if (!R || !s0_host) return fail(CCO_ERR_ARG, "NULL argument");
+  if (R->method != METHOD_FLOW) return fail(CCO_ERR_STATE, "not a flow session");
+  DevScope ds(R->ctx->device);
+  hipStream_t st = R->ctx->stream;
+  const size_t bytes = sizeof(double) * (size_t)R->B * R->dev.stride;
+  CCO_HIP(hipMemcpyAsync(R->dev.s, s0_host, bytes, hipMemcpyHostToDevice, st));
+  CCO_HIP(hipMemcpyAsync(R->dev.s_eval, R->dev.s, bytes, hipMemcpyDeviceToDevice, st));
+  if (dt0_host) {
+    CCO_HIP(hipMemcpyAsync(R->flow_dt0, dt0_host, sizeof(double) * R->B, hipMemcpyHostToDevice, st));
+    R->fc.dt0 = R->flow_dt0;
+  } else {
+    R->fc.dt0 = nullptr;
+  }
+  hipLaunchKernelGGL(relax_init_kernel, dim3((R->B + 255) / 256), dim3(256), 0, st, R->dev,
+                     R->method);
+  CCO_HIP(hipGetLastError());
+  CCO_HIP(hipStreamSynchronize(st));
+  R->slots_ub = R->B;
+  R->steps = 0;
+  R->ready = true;
+  return CCO_OK;
+}
+
+int cco_flow_result(cco_relax* R, double* s, double* energy, double* t, double* length,
+                    double* dt, int64_t* n_steps, int64_t* n_evals, int64_t* status) {
+  if (!R || !s || !energy || !t || !length || !dt || !n_steps || !n_evals || !status)
+    return fail(CCO_ERR_ARG, "NULL argument");
+  if (R->method != METHOD_FLOW) return fail(CCO_ERR_STATE, "not a flow session");
+  DevScope ds(R->ctx->device);
+  const size_t B = R->B;
+  std::vector<FlowLane> L(B);
+  std::vector<int> stv(B);
+  CCO_HIP(hipMemcpy(L.data(), R->dev.fl, sizeof(FlowLane) * B, hipMemcpyDeviceToHost));
+  CCO_HIP(hipMemcpy(stv.data(), R->dev.status, sizeof(int) * B, hipMemcpyDeviceToHost));
+  CCO_HIP(hipMemcpy(s, R->dev.s, sizeof(double) * B * R->dev.stride, hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < B; ++i) {
+    energy[i] = L[i].f;
+    t[i] = L[i].t;
+    length[i] = L[i].length;
+    dt[i] = L[i].dt;
+    n_steps[i] = L[i].n_steps;
+    n_evals[i] = L[i].n_evals;
+    status[i] = stv[i];
+  }
+  return CCO_OK;
+}
+
+int cco_flow_trajectory(cco_relax* R, int64_t* count, double* s, double* t, double* length,
+                        double* energy) {
+  if (!R || !count) return fail(CCO_ERR_ARG, "NULL argument");
+  if (R->method != METHOD_FLOW) return fail(CCO_ERR_STATE, "not a flow session");
+  DevScope ds(R->ctx->device);
+  const size_t B = R->B;
+  std::vector<FlowLane> L(B);
+  CCO_HIP(hipMemcpy(L.data(), R->dev.fl, sizeof(FlowLane) * B, hipMemcpyDeviceToHost));
+  for (size_t i = 0; i < B; ++i) count[i] = L[i].rec_count;
+  const size_t rows = B * (size_t)R->fc.record_cap;
+  if (rows == 0) return CCO_OK;
+  if (s) CCO_HIP(hipMemcpy(s, R->dev.tr_s, sizeof(double) * rows * R->dev.stride, hipMemcpyDeviceToHost));
+  if (t) CCO_HIP(hipMemcpy(t, R->dev.tr_step, sizeof(double) * rows, hipMemcpyDeviceToHost));
+  if (length) CCO_HIP(hipMemcpy(length, R->dev.tr_len, sizeof(double) * rows, hipMemcpyDeviceToHost));
+  if (energy) CCO_HIP(hipMemcpy(energy, R->dev.tr_e, sizeof(double) * rows, hipMemcpyDeviceToHost));
+  return CCO_OK;
+}
+
+int cco_md_create(cco_ctx* ctx, cco_energy* E, int B, int N, int d, const double* basis,
+                  double dt, double drift_tol, cco_relax** out) {
+  if (!E) return fail(CCO_ERR_ARG, "MD needs the built-in stealthy energy");
+  if (!basis) return fail(CCO_ERR_ARG, "NULL basis");
+  if (!(std::isfinite(dt) && dt > 0.0)) return fail(CCO_ERR_ARG, "dt must be finite and positive");
+  if (!(drift_tol > 0.0)) return fail(CCO_ERR_ARG, "drift_tol must be positive (inf disables)");
+  double inv[9];
+  if (!invert3(basis, d, inv)) return fail(CCO_ERR_ARG, "basis must be invertible");
+  DevScope dscope(ctx ? ctx->device : 0);
+  cco_relax* R = nullptr;
+  int rc = new_session(ctx, E, B, N, d, METHOD_MD, &R);
+  if (rc) return rc;
+  MdCfg& c = R->mc;
+  c.dt = dt;
+  c.half = 0.5 * dt;
+  c.drift_tol = drift_tol;
+  for (int i = 0; i < d; ++i)
+    for (int j = 0; j < d; ++j) {
+      c.inv[i * d + j] = inv[i * d + j];
+      c.inv_t[i * d + j] = inv[j * d + i];
+    }
+  c.d = d;
+  c.n = N;
+  RelaxDev& v = R->dev;
+  const size_t vec = (size_t)B * v.stride;
+  auto A = [&](auto*& p, size_t cnt) {
+    if (rc == 0) rc = alloc_into(R, p, cnt);
+  };
+  A(v.status, B);
+  A(v.active, B);
+  A(v.n_active, 1);
+  A(v.s_eval, vec);
+  A(v.g_eval, vec);
+  A(v.phi_eval, B);
+  A(v.r, vec);
+  A(v.v, vec);
+  A(v.f, vec);
+  A(v.ml, B);
+  if (rc == 0) rc = finish_session(R);
+  if (rc) {
+    cco_relax_destroy(R);
+    return rc;
+  }
+  *out = R;
+  return CCO_OK;
+}
+
+int cco_md_init(cco_relax* R, const double* r0, const double* v0, const double* temperature) {
+  if (!R || !r0 || !v0 || !temperature) return fail(CCO_ERR_ARG, "NULL argument");
+  if (R->method != METHOD_MD) return fail(CCO_ERR_STATE, "not an MD session");
+  DevScope ds(R->ctx->device);
+  hipStream_t st = R->ctx->stream;
+  const int d = R->d;
+  const size_t nd = (size_t)R->dev.stride;
+  const size_t B = R->B;
+  // s = r0 inv(B) for the first force evaluation (same sum order as md_prepare_kernel)
+  std::vector<double> s(B * nd);
+  for (size_t b = 0; b < B; ++b)
+    for (int j = 0; j < R->N; ++j)
+      for (int a = 0; a < d; ++a) {
+        double acc = 0.0;
+        for (int q = 0; q < d; ++q) acc = acc + r0[b * nd + (size_t)j * d + q] * R->mc.inv[q * d + a];
+        s[b * nd + (size_t)j * d + a] = acc;
+      }
+  std::vector<MdLane> L(B);
+  for (size_t b = 0; b < B; ++b) {
+    L[b] = MdLane{};
+    L[b].temperature = temperature[b];
+  }
+  CCO_HIP(hipMemcpyAsync(R->dev.r, r0, sizeof(double) * B * nd, hipMemcpyHostToDevice, st));
+  CCO_HIP(hipMemcpyAsync(R->dev.v, v0, sizeof(double) * B * nd, hipMemcpyHostToDevice, st));
+  CCO_HIP(hipMemcpyAsync(R->dev.s_eval, s.data(), sizeof(double) * B * nd, hipMemcpyHostToDevice, st));
+  CCO_HIP(hipMemcpyAsync(R->dev.ml, L.data(), sizeof(MdLane) * B, hipMemcpyHostToDevice, st));
+  hipLaunchKernelGGL(relax_init_kernel, dim3((R->B + 255) / 256), dim3(256), 0, st, R->dev,
+                     R->method);
+  CCO_HIP(hipGetLastError());
+  CCO_HIP(hipStreamSynchronize(st));
+  R->slots_ub = R->B;
+  R->steps = 0;
+  R->ready = true;
+  R->md_need_init = true;
+  // the initial force evaluation (no drift): e0 = Phi + KE(v0)
+  int na = 0;
+  return cco_relax_step(R, 1, &na);
+}
+
+int cco_md_run(cco_relax* R, int64_t n_steps, int sync_every, int* n_active) {
+  if (!R) return fail(CCO_ERR_ARG, "NULL argument");
+  if (R->method != METHOD_MD) return fail(CCO_ERR_STATE, "not an MD session");
+  if (n_steps < 0 || sync_every < 1) return fail(CCO_ERR_ARG, "need n_steps >= 0, sync_every >= 1");
+  int na = R->slots_ub;
+  int64_t done = 0;
+  while (done < n_steps && na > 0) {
+    const int64_t chunk = std::min<int64_t>(sync_every, n_steps - done);
+    int rc = cco_relax_step(R, chunk, &na);
+    if (rc) return rc;
+    done += chunk;
+  }
+  if (n_active) *n_active = na;
+  return CCO_OK;
+}
+
+int cco_md_state(cco_relax* R, double* r, double* v, double* f, double* scalars, int64_t* ints) {
+  if (!R) return fail(CCO_ERR_ARG, "NULL argument");
+  if (R->method != METHOD_MD) return fail(CCO_ERR_STATE, "not an MD session");
+  DevScope ds(R->ctx->device);
+  const size_t B = R->B;
+  const size_t bytes = sizeof(double) * B * R->dev.stride;
+  if (r) CCO_HIP(hipMemcpy(r, R->dev.r, bytes, hipMemcpyDeviceToHost));
+  if (v) CCO_HIP(hipMemcpy(v, R->dev.v, bytes, hipMemcpyDeviceToHost));
+  if (f) CCO_HIP(hipMemcpy(f, R->dev.f, bytes, hipMemcpyDeviceToHost));
+  std::vector<MdLane> L(B);
+  std::vector<int> stv(B);
+  CCO_HIP(hipMemcpy(L.data(), R->dev.ml, sizeof(MdLane) * B, hipMemcpyDeviceToHost));
+  CCO_HIP(hipMemcpy(stv.data(), R->dev.status, sizeof(int) * B, hipMemcpyDeviceToHost));
+  for (size_t b = 0; b < B; ++b) {
+    if (scalars) {
+      double* o = scalars + b * 9;
+      o[0] = L[b].phi;
+      o[1] = L[b].kinetic;
+      o[2] = L[b].e0;
+      o[3] = L[b].max_drift;
+      o[4] = L[b].sum_kinetic;
+      o[5] = L[b].sum_phi;
+      o[6] = L[b].sum_phi2;
+      o[7] = L[b].time;
+      o[8] = L[b].temperature;
+    }
+    if (ints) {
+      ints[b * 3 + 0] = L[b].step;
+      ints[b * 3 + 1] = L[b].n_sum;
+      ints[b * 3 + 2] = stv[b];
+    }
+  }
   return CCO_OK;
 }
 

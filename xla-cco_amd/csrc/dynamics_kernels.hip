@@ -1,0 +1,383 @@
+// Device-side batched dynamics: velocity-Verlet NVE steps and the
+// Dormand-Prince 5(4) gradient flow, sharing the energy kernels.
+//
+// Restated algorithms:
+//   Verlet  : src/xla_cco/dynamics/integrators.py:83-96 (verlet_step),
+//             src/xla_cco/dynamics/run.py:219-244 (force at s = r inv(B),
+//             F = -g inv(B)^T), :415-442 (_step: KE, drift guard, sums),
+//             src/xla_cco/dynamics/thermal.py:100-106 (KE = sum v^2 / 2)
+//   flow    : src/xla_cco/minimize/flow.py:212-251 (status, velocity),
+//             :337-364 (scaled error, DOPRI5 proposal), :401-451 (step)
+// One workgroup per active lane; threads own whole particles (d components)
+// because the metric transforms mix the components of one particle.
+// Contraction is off so the update arithmetic rounds like its NumPy
+// statement.
+#pragma clang fp contract(off)
+
+#include "minimize_kernels.h"
+
+namespace cco {
+
+constexpr int kDynBlock = 256;
+
+__device__ __forceinline__ double nanmax_d(double a, double b) {
+  if (a != a || b != b) return __longlong_as_double(0x7ff8000000000000LL);
+  return a > b ? a : b;
+}
+
+// ---------------------------------------------------------------------------
+// velocity Verlet
+// ---------------------------------------------------------------------------
+// Half kick and drift: v <- v + dt/2 f, r <- r + dt v, s_eval = r inv(B).
+__global__ __launch_bounds__(kDynBlock) void md_prepare_kernel(RelaxDev R, MdCfg c) {
+  const int slot = blockIdx.x;
+  if (slot >= *R.n_active) return;
+  const int lane = R.active[slot];
+  const size_t off = (size_t)lane * R.stride;
+  double* r = R.r + off;
+  double* v = R.v + off;
+  const double* f = R.f + off;
+  double* se = R.s_eval + off;
+  const int d = c.d;
+  for (int j = threadIdx.x; j < c.n; j += blockDim.x) {
+    double rn[3];
+    for (int a = 0; a < d; ++a) {
+      const double vh = v[j * d + a] + c.half * f[j * d + a];
+      v[j * d + a] = vh;
+      rn[a] = r[j * d + a] + c.dt * vh;
+      r[j * d + a] = rn[a];
+    }
+    for (int a = 0; a < d; ++a) {
+      double sa = 0.0;
+      for (int b = 0; b < d; ++b) sa = sa + rn[b] * c.inv[b * d + a];
+      se[j * d + a] = sa;
+    }
+  }
+}
+
+// Second half kick with the new forces, kinetic energy, drift guard and sums.
+__global__ __launch_bounds__(kDynBlock) void md_finish_kernel(RelaxDev R, MdCfg c) {
+  __shared__ double red[16];
+  const int slot = blockIdx.x;
+  if (slot >= *R.n_active) return;
+  const int lane = R.active[slot];
+  const size_t off = (size_t)lane * R.stride;
+  double* v = R.v + off;
+  double* f = R.f + off;
+  const double* g = R.g_eval + off;
+  MdLane L = R.ml[lane];
+  const int d = c.d;
+  const bool init = L.mode == MODE_INIT;
+  double ke = 0.0;
+  for (int j = threadIdx.x; j < c.n; j += blockDim.x) {
+    for (int a = 0; a < d; ++a) {
+      double acc = 0.0;
+      for (int b = 0; b < d; ++b) acc = acc + g[j * d + b] * c.inv_t[b * d + a];
+      const double fa = -acc;
+      f[j * d + a] = fa;
+      double va = v[j * d + a];
+      if (!init) {
+        va = va + c.half * fa;
+        v[j * d + a] = va;
+      }
+      ke += va * va;
+    }
+  }
+  ke = 0.5 * block_sum<kDynBlock>(ke, red);
+  const double phi = R.phi_eval[lane];
+  int code = ST_RUNNING;
+  if (init) {
+    L.phi = phi;
+    L.kinetic = ke;
+    L.e0 = phi + ke;
+    L.mode = MODE_EVAL;
+  } else {
+    const double drift = fabs(phi + ke - L.e0);
+    if (isfinite(c.drift_tol)) {
+      const double limit = c.drift_tol * 0.5 * (double)d * (double)(c.n - 1) * L.temperature;
+      if (!(drift <= limit)) code = ST_DRIFT;
+    }
+    L.phi = phi;
+    L.kinetic = ke;
+    L.step += 1;
+    L.time = L.time + c.dt;
+    L.max_drift = nanmax_d(L.max_drift, drift);
+    L.sum_kinetic = L.sum_kinetic + ke;
+    L.sum_phi = L.sum_phi + phi;
+    L.sum_phi2 = L.sum_phi2 + phi * phi;
+    L.n_sum += 1;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    R.ml[lane] = L;
+    R.status[lane] = code;
+  }
+}
+
+void launch_md_prepare(const RelaxDev& r, const MdCfg& c, int slots_ub, hipStream_t st) {
+  if (slots_ub <= 0) return;
+  hipLaunchKernelGGL(md_prepare_kernel, dim3(slots_ub), dim3(kDynBlock), 0, st, r, c);
+}
+
+void launch_md_finish(const RelaxDev& r, const MdCfg& c, int slots_ub, hipStream_t st) {
+  if (slots_ub <= 0) return;
+  hipLaunchKernelGGL(md_finish_kernel, dim3(slots_ub), dim3(kDynBlock), 0, st, r, c);
+}
+
+// ---------------------------------------------------------------------------
+// Dormand-Prince 5(4) gradient flow
+// ---------------------------------------------------------------------------
+__constant__ double kA[6][5] = {
+    {0, 0, 0, 0, 0},
+    {1.0 / 5, 0, 0, 0, 0},
+    {3.0 / 40, 9.0 / 40, 0, 0, 0},
+    {44.0 / 45, -56.0 / 15, 32.0 / 9, 0, 0},
+    {19372.0 / 6561, -25360.0 / 2187, 64448.0 / 6561, -212.0 / 729, 0},
+    {9017.0 / 3168, -355.0 / 33, 46732.0 / 5247, 49.0 / 176, -5103.0 / 18656},
+};
+__constant__ double kB5[7] = {35.0 / 384, 0.0, 500.0 / 1113, 125.0 / 192, -2187.0 / 6784,
+                              11.0 / 84, 0.0};
+__constant__ double kB4[7] = {5179.0 / 57600, 0.0, 7571.0 / 16695, 393.0 / 640,
+                              -92097.0 / 339200, 187.0 / 2100, 1.0 / 40};
+
+// Velocity of the chosen parametrization from the gradient g (flow.py:230-250):
+// writes k = v into kout, returns (gnorm, all-zero flag) via references.
+__device__ void flow_velocity(const double* g, double* kout, const FlowCfg& c, int n,
+                              double* red, double& gnorm, bool& gzero) {
+  const int d = c.d;
+  double gg = 0.0, nz = 0.0;
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    for (int a = 0; a < d; ++a) {
+      double acc = 0.0, cart = 0.0;
+      for (int b = 0; b < d; ++b) {
+        acc = acc + (-g[j * d + b]) * c.metric_inv[b * d + a];
+        cart = cart + g[j * d + b] * c.inv_t[b * d + a];
+      }
+      kout[j * d + a] = acc;
+      gg += cart * cart;
+      nz += (g[j * d + a] != 0.0) ? 1.0 : 0.0;
+    }
+  }
+  gg = block_sum<kDynBlock>(gg, red);
+  nz = block_sum<kDynBlock>(nz, red);
+  gnorm = sqrt(gg);
+  gzero = nz == 0.0;
+}
+
+__device__ void flow_rate(double gnorm, bool arclength, double rate[2], double& scale) {
+  if (!arclength) {
+    rate[0] = 1.0;
+    rate[1] = gnorm;
+    scale = 1.0;
+  } else {
+    const double safe = gnorm > 0.0 ? gnorm : 1.0;
+    const double inv = gnorm > 0.0 ? 1.0 / safe : 0.0;
+    rate[0] = inv;
+    rate[1] = 1.0;
+    scale = inv;
+  }
+}
+
+__device__ int flow_status(double f, bool gzero, double gnorm, double t, double dt, long long steps,
+                           long long tries, const FlowCfg& c) {
+  int code = dt < c.dt_min ? ST_STEP_FLOOR : ST_RUNNING;
+  const bool over = (t >= c.t_max) || (steps >= c.max_steps);
+  if (over || tries >= 4 * c.max_steps) code = ST_BUDGET;
+  if (gzero || gnorm < c.grad_tol) code = ST_GRAD_TOL;
+  if (f < c.energy_tol) code = ST_ENERGY_TOL;
+  return code;
+}
+
+// s_eval = s + dt * sum_{j < i} A[i][j] k_j   (stage i = 1..5)
+__device__ void flow_stage_point(const double* s, const double* k, int stride, int nd, int i,
+                                 double dt, double* se) {
+  for (int e = threadIdx.x; e < nd; e += blockDim.x) {
+    double incr = kA[i][0] * k[e];
+    for (int j = 1; j < i; ++j) incr = incr + kA[i][j] * k[(size_t)j * stride + e];
+    se[e] = s[e] + dt * incr;
+  }
+}
+
+__device__ void flow_record(const RelaxDev& R, const FlowCfg& c, int lane, FlowLane& L,
+                            const double* s, int nd) {
+  const size_t row = (size_t)lane * c.record_cap + L.rec_count;
+  for (int e = threadIdx.x; e < nd; e += blockDim.x) R.tr_s[row * R.stride + e] = s[e];
+  if (threadIdx.x == 0) {
+    R.tr_e[row] = L.f;
+    R.tr_step[row] = L.t;
+    R.tr_len[row] = L.length;
+    R.tr_it[row] = L.n_steps;
+  }
+  L.rec_count += 1;
+}
+
+__global__ __launch_bounds__(kDynBlock) void flow_advance_kernel(RelaxDev R, FlowCfg c) {
+  __shared__ double red[16];
+  const int slot = blockIdx.x;
+  if (slot >= *R.n_active) return;
+  const int lane = R.active[slot];
+  const int nd = R.nd;
+  const int n = nd / c.d;
+  const size_t off = (size_t)lane * R.stride;
+  double* s = R.s + off;
+  double* se = R.s_eval + off;
+  const double* ge = R.g_eval + off;
+  double* k = R.k + (size_t)lane * 7 * R.stride;
+  FlowLane L = R.fl[lane];
+  const double fe = R.phi_eval[lane];
+  int code = ST_RUNNING;
+  bool start_attempt = false;
+
+  if (L.mode == MODE_INIT) {
+    double gn;
+    bool gz;
+    flow_velocity(ge, k, c, n, red, gn, gz);
+    double rate[2], scale;
+    flow_rate(gn, c.arclength, rate, scale);
+    if (c.arclength)
+      for (int e = threadIdx.x; e < nd; e += blockDim.x) k[e] = k[e] * scale;
+    L.f = fe;
+    L.gnorm = gn;
+    L.gzero = gz;
+    L.rate[0] = rate[0];
+    L.rate[1] = rate[1];
+    L.var = 0.0;
+    L.t = 0.0;
+    L.length = 0.0;
+    L.dt = fmin(c.dt0 != nullptr ? c.dt0[lane] : c.dt_init, c.dt_max);
+    L.n_steps = 0;
+    L.n_tries = 0;
+    L.n_evals = 1;
+    L.mode = MODE_EVAL;
+    code = flow_status(L.f, gz, gn, 0.0, L.dt, 0, 0, c);
+    if (c.record_every > 0 && c.record_cap > 0 && L.rec_count == 0) flow_record(R, c, lane, L, s, nd);
+    start_attempt = code == ST_RUNNING;
+  } else if (L.stage >= 1 && L.stage <= 5) {
+    // velocity at stage point i; next stage point (or the candidate after stage 5)
+    const int i = L.stage;
+    double* ki = k + (size_t)i * R.stride;
+    double gn;
+    bool gz;
+    flow_velocity(ge, ki, c, n, red, gn, gz);
+    double rate[2], scale;
+    flow_rate(gn, c.arclength, rate, scale);
+    if (c.arclength)
+      for (int e = threadIdx.x; e < nd; e += blockDim.x) ki[e] = ki[e] * scale;
+    L.srate[i][0] = rate[0];
+    L.srate[i][1] = rate[1];
+    __syncthreads();
+    if (i < 5) {
+      flow_stage_point(s, k, R.stride, nd, i + 1, L.dt_try, se);
+    } else {
+      for (int e = threadIdx.x; e < nd; e += blockDim.x) {
+        double acc = kB5[0] * k[e];
+        for (int j = 1; j < 6; ++j) acc = acc + kB5[j] * k[(size_t)j * R.stride + e];
+        se[e] = s[e] + L.dt_try * acc;
+      }
+    }
+    L.stage = i + 1;
+  } else {
+    // stage 6: the candidate s_new = s_eval was evaluated
+    double* k6 = k + (size_t)6 * R.stride;
+    double gn;
+    bool gz;
+    flow_velocity(ge, k6, c, n, red, gn, gz);
+    double rate[2], scale;
+    flow_rate(gn, c.arclength, rate, scale);
+    if (c.arclength)
+      for (int e = threadIdx.x; e < nd; e += blockDim.x) k6[e] = k6[e] * scale;
+    __syncthreads();
+    L.srate[6][0] = rate[0];
+    L.srate[6][1] = rate[1];
+    const double dt = L.dt_try;
+    double tl[2], etl[2];
+    for (int q = 0; q < 2; ++q) {
+      double acc = kB5[0] * L.srate[0][q];
+      for (int j = 1; j < 6; ++j) acc = acc + kB5[j] * L.srate[j][q];
+      tl[q] = dt * acc;
+      double ea = (kB5[0] - kB4[0]) * L.srate[0][q];
+      for (int j = 1; j < 7; ++j) ea = ea + (kB5[j] - kB4[j]) * L.srate[j][q];
+      etl[q] = dt * ea;
+    }
+    // RMS Cartesian step and error (flow.py:337-343)
+    const int d = c.d;
+    double mv = 0.0, me = 0.0;
+    for (int j = threadIdx.x; j < n; j += blockDim.x) {
+      double st[3], er[3];
+      for (int a = 0; a < d; ++a) {
+        const int e = j * d + a;
+        st[a] = se[e] - s[e];
+        double ea = (kB5[0] - kB4[0]) * k[e];
+        for (int q = 1; q < 7; ++q) ea = ea + (kB5[q] - kB4[q]) * k[(size_t)q * R.stride + e];
+        er[a] = dt * ea;
+      }
+      for (int a = 0; a < d; ++a) {
+        double cs = 0.0, ce = 0.0;
+        for (int b = 0; b < d; ++b) {
+          cs = cs + st[b] * c.basis[b * d + a];
+          ce = ce + er[b] * c.basis[b * d + a];
+        }
+        mv += cs * cs;
+        me += ce * ce;
+      }
+    }
+    mv = block_sum<kDynBlock>(mv, red);
+    me = block_sum<kDynBlock>(me, red);
+    const double move = sqrt(mv / (double)nd);
+    const double tol = c.atol + c.rtol * move;
+    double err = sqrt(me / (double)nd) / tol;
+    double etl_max = fabs(etl[0]) / (c.atol + c.rtol * fabs(tl[0]));
+    etl_max = nanmax_d(etl_max, fabs(etl[1]) / (c.atol + c.rtol * fabs(tl[1])));
+    err = nanmax_d(err, etl_max);
+    const bool accept = err <= 1.0;
+    double factor = 0.9 * pow(err, -1.0 / 5.0);
+    factor = fmin(fmax(factor, 0.2), 5.0);
+    if (!isfinite(factor) || factor != factor) factor = 0.2;
+    const double dt_next = fmin(dt * factor, c.dt_max);
+    if (accept) {
+      for (int e = threadIdx.x; e < nd; e += blockDim.x) {
+        s[e] = se[e];
+        k[e] = k6[e];
+      }
+      L.f = fe;
+      L.gnorm = gn;
+      L.gzero = gz;
+      L.rate[0] = rate[0];
+      L.rate[1] = rate[1];
+      L.t = L.t + tl[0];
+      L.length = L.length + tl[1];
+      L.var = L.var + dt;
+      L.n_steps += 1;
+    }
+    L.n_tries += 1;
+    L.n_evals += 6;
+    L.dt = dt_next;
+    code = flow_status(L.f, L.gzero, L.gnorm, L.t, dt_next, L.n_steps, L.n_tries, c);
+    if (accept && c.record_every > 0 && (L.n_steps % c.record_every) == 0 &&
+        L.rec_count < c.record_cap)
+      flow_record(R, c, lane, L, s, nd);
+    start_attempt = code == ST_RUNNING;
+  }
+  if (start_attempt) {
+    __syncthreads();
+    double dt = L.dt;
+    if (!c.arclength) dt = fmin(dt, c.t_max - L.t);
+    L.dt_try = dt;
+    L.srate[0][0] = L.rate[0];
+    L.srate[0][1] = L.rate[1];
+    flow_stage_point(s, k, R.stride, nd, 1, dt, se);
+    L.stage = 1;
+  }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    R.fl[lane] = L;
+    R.status[lane] = code;
+  }
+}
+
+void launch_flow_advance(const RelaxDev& r, const FlowCfg& c, int slots_ub, hipStream_t st) {
+  if (slots_ub <= 0) return;
+  hipLaunchKernelGGL(flow_advance_kernel, dim3(slots_ub), dim3(kDynBlock), 0, st, r, c);
+}
+
+}  // namespace cco

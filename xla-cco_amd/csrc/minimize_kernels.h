@@ -63,6 +63,45 @@ struct LbfgsLane {
   LsState ls;
 };
 
+// Velocity-Verlet NVE settings (src/xla_cco/dynamics/state.py:57-88, verlet scheme).
+struct MdCfg {
+  double dt, half;
+  double drift_tol;   // +inf disables the drift guard
+  double inv[9];      // inv(B), row-major d x d (s = r inv(B))
+  double inv_t[9];    // inv(B)^T (F = -g inv(B)^T)
+  int d, n;
+};
+
+struct MdLane {
+  double phi, e0, max_drift, sum_kinetic, sum_phi, sum_phi2, time, temperature, kinetic;
+  long long step, n_sum;
+  int mode, pad;
+};
+
+// Dormand-Prince 5(4) gradient flow settings (src/xla_cco/minimize/flow.py:93-125).
+struct FlowCfg {
+  double rtol, atol, dt_init, dt_min, dt_max, t_max, energy_tol, grad_tol;
+  long long max_steps;
+  double metric_inv[9];  // inv(B B^T): v = -g metric_inv
+  double basis[9];       // B (Cartesian error norms)
+  double inv_t[9];       // inv(B)^T (|grad_r Phi|)
+  int d, arclength;
+  int record_every, record_cap;
+  const double* dt0;     // optional per-lane first step (nullptr: dt_init)
+};
+
+struct FlowLane {
+  double f, gnorm;           // at the current point
+  double rate[2];            // d(t, length)/d(var) at the current point
+  double var, t, length, dt, dt_try;
+  double srate[7][2];        // stage rates of the attempted step
+  double f_new, gnorm_new;   // at the candidate point (stage 6)
+  long long n_steps, n_tries, n_evals;
+  int stage, mode, rec_count;
+  int gzero, gzero_new;      // gradient exactly zero at the current / candidate point
+  int pad;
+};
+
 // Session arrays (device).  nd = N * d, stride = lane stride in doubles.
 struct RelaxDev {
   int B, nd, stride;
@@ -84,11 +123,20 @@ struct RelaxDev {
   double* rho;    // [B][m]
   DescentLane* dl;
   LbfgsLane* ll;
+  // MD: r = positions (Cartesian), v = velocities, f = forces
+  double* r;
+  double* v;
+  double* f;
+  MdLane* ml;
+  // flow: k = 7 stage velocities [B][7][stride], g = gradient at s
+  double* k;
+  FlowLane* fl;
   // trajectory [B][cap][stride], [B][cap]
   double* tr_s;
   double* tr_e;
-  double* tr_step;
+  double* tr_step;     // descent / L-BFGS: step; flow: flow time t
   long long* tr_it;
+  double* tr_len;      // flow: arclength
 };
 
 struct RelaxOutDev {
@@ -107,6 +155,9 @@ void launch_lbfgs_advance(const RelaxDev& r, const LbfgsCfg& c, int slots_ub, hi
                           int variant = 0);
 int lbfgs_register_ept(int nd);  // elements per thread of the register variant, 0 if too large
 void launch_compact(const RelaxDev& r, hipStream_t st);
+void launch_md_prepare(const RelaxDev& r, const MdCfg& c, int slots_ub, hipStream_t st);
+void launch_md_finish(const RelaxDev& r, const MdCfg& c, int slots_ub, hipStream_t st);
+void launch_flow_advance(const RelaxDev& r, const FlowCfg& c, int slots_ub, hipStream_t st);
 void launch_finalize(const RelaxDev& r, int method, const LbfgsCfg& lc, const RelaxOutDev& o,
                      hipStream_t st);
 

@@ -32,6 +32,23 @@ class NativeLibraryError(ImportError):
     """The HIP library is missing or does not export the expected ABI."""
 
 
+class FlowSettingsC(C.Structure):
+    _fields_ = [
+        ("rtol", C.c_double),
+        ("atol", C.c_double),
+        ("dt_init", C.c_double),
+        ("dt_min", C.c_double),
+        ("dt_max", C.c_double),
+        ("t_max", C.c_double),
+        ("energy_tol", C.c_double),
+        ("grad_tol", C.c_double),
+        ("max_steps", C.c_int64),
+        ("arclength", C.c_int32),
+        ("record_every", C.c_int32),
+        ("max_records", C.c_int32),
+    ]
+
+
 class DescentSettingsC(C.Structure):
     _fields_ = [
         ("step_init", C.c_double),
@@ -114,6 +131,16 @@ SIGNATURES = {
     "cco_relax_stats": (C.c_int, [_P, _PI64, _PI64]),
     "cco_relax_counters": (C.c_int, [_P, _PI64]),
     "cco_relax_set_option": (C.c_int, [_P, C.c_int, C.c_int64]),
+    "cco_flow_create": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int, _P, _PD, C.POINTER(_P)]),
+    "cco_flow_init": (C.c_int, [_P, _PD, _PD]),
+    "cco_flow_result": (C.c_int, [_P, _PD, _PD, _PD, _PD, _PD, _PI64, _PI64, _PI64]),
+    "cco_flow_trajectory": (C.c_int, [_P, _PI64, _PD, _PD, _PD, _PD]),
+    "cco_md_create": (
+        C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int, _PD, C.c_double, C.c_double, C.POINTER(_P)]
+    ),
+    "cco_md_init": (C.c_int, [_P, _PD, _PD, _PD]),
+    "cco_md_run": (C.c_int, [_P, C.c_int64, C.c_int, _PI]),
+    "cco_md_state": (C.c_int, [_P, _PD, _PD, _PD, _PD, _PI64]),
     "cco_relax_set_profiling": (C.c_int, [_P, C.c_int]),
     "cco_relax_profile": (C.c_int, [_P, _PD, _PI64]),
     "cco_energy_profile": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int, _PD]),
