@@ -191,6 +191,25 @@ int cco_md_run(cco_relax* r, int64_t n_steps, int sync_every, int* n_active);
 int cco_md_state(cco_relax* r, double* r_host, double* v_host, double* f_host,
                  double* scalars_host, int64_t* ints_host);
 
+/* Isotropic structure-factor moments: for the D directions n_c (D x d
+ * int64) with L_c multiples each, k = m n_c (m = 1..L_c, directions in
+ * order, multiples contiguous), tot[k] = sum over the B configurations of
+ * S(k) = |rho(k)|^2 / N and tot2[k] = sum of S(k)^2.  Replaces the device
+ * part of isotropic_structure_factor (src/xla_cco/analysis/structure.py:
+ * 55-173: _sk_sums / _sk_moments); the binning stays on the host. */
+int cco_sk_moments(cco_ctx* ctx, const double* s_host, int B, int N, int d,
+                   const int64_t* dir_n_host, const int32_t* dir_L_host, int D,
+                   double* tot_host, double* tot2_host);
+/* Minimum-image pair statistics over the 3^d neighbouring images: the
+ * pair-distance histogram summed over the B configurations (each unordered
+ * pair once; np.histogram bins, 2..1024 edges) and, if nn_host is not NULL,
+ * every particle's nearest-neighbour distance (B x N).  Replaces
+ * pair_distance_histogram and nearest_neighbor_distances
+ * (src/xla_cco/analysis/pairs.py:42-104). */
+int cco_pair_stats(cco_ctx* ctx, const double* s_host, int B, int N, int d,
+                   const double* basis_host, const double* edges_host, int n_edges,
+                   int64_t* counts_host, double* nn_host);
+
 /* Measurement (no reference counterpart; the reference times with
  * block_until_ready, benchmarks/bench_energy.py:104-115).  Per-stage device
  * time from HIP events recorded on the context stream around every launch:

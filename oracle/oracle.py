@@ -830,3 +830,61 @@ def verlet_nve(
         "sum_kinetic": sums[0], "sum_phi": sums[1], "sum_phi2": sums[2], "step": step,
         "status": status, "hist": hist,
     }
+
+
+# ============================================================
+# ANALYSIS (tests/oracle.py:158-209, analysis/structure.py:111-201)
+# ============================================================
+
+
+def structure_factor(s: np.ndarray, n_vectors: np.ndarray) -> np.ndarray:
+    """S(n) = |rho(n)|^2 / N by the trig double loop (vectorised over particles)."""
+    out = np.zeros(n_vectors.shape[0])
+    for kk in range(n_vectors.shape[0]):
+        ph = TWO_PI * (s @ np.asarray(n_vectors[kk], dtype=np.float64))
+        out[kk] = (np.cos(ph).sum() ** 2 + np.sin(ph).sum() ** 2) / s.shape[0]
+    return out
+
+
+def min_image_distances(s: np.ndarray, basis: np.ndarray) -> Tuple[np.ndarray, np.ndarray]:
+    """(pair distances i < j, nearest-neighbour distance per particle) over 3^d images."""
+    n, dim = s.shape
+    shifts = np.array(list(itertools.product((-1.0, 0.0, 1.0), repeat=dim)))
+    ds = s[None, :, :] - s[:, None, :]
+    ds = ds - np.round(ds)
+    best = np.full((n, n), np.inf)
+    for sh in shifts:
+        r = (ds + sh) @ basis
+        r2 = np.sum(r * r, axis=-1)
+        if not np.any(sh):
+            np.fill_diagonal(r2, np.inf)
+        best = np.minimum(best, r2)
+    dist = np.sqrt(best)
+    iu = np.triu_indices(n, 1)
+    return dist[iu], dist.min(axis=1)
+
+
+def isotropic_kset(basis: np.ndarray, k_dir_max: float, k_max: float) -> Tuple[np.ndarray, np.ndarray]:
+    """Integer vectors (M, d) and |k|^2 of the isotropic S(k) set (structure.py:111-144)."""
+    if basis.shape[0] == 1:
+        b = TWO_PI / basis[0, 0]
+        n_prim, k_prim = np.ones((1, 1), dtype=np.int64), np.array([[b]])
+    else:
+        n_all, _ = halfspace_kvectors(basis, 0.0, k_dir_max)
+        recip = TWO_PI * np.linalg.inv(basis).T
+        prim = np.gcd.reduce(np.abs(n_all), axis=1) == 1
+        n_prim = n_all[prim]
+        k_prim = np.array([[sum(float(n[i]) * recip[i, j] for i in range(len(n))) for j in range(basis.shape[0])]
+                           for n in n_prim])
+    rows, ksq = [], []
+    for n_row, k_row in zip(n_prim, k_prim):
+        m = 1
+        while True:
+            kk = m * k_row
+            q = modulus2(kk)
+            if q > k_max * k_max:
+                break
+            rows.append(m * n_row)
+            ksq.append(q)
+            m += 1
+    return np.array(rows, dtype=np.int64), np.array(ksq)

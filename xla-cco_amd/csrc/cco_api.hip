@@ -1466,4 +1466,109 @@ int cco_md_state(cco_relax* R, double* r, double* v, double* f, double* scalars,
   return CCO_OK;
 }
 
+
+// ---------------------------------------------------------------------------
+// ensemble analysis
+// ---------------------------------------------------------------------------
+int cco_sk_moments(cco_ctx* ctx, const double* s_host, int B, int N, int d, const int64_t* dir_n,
+                   const int32_t* dir_L, int D, double* tot_host, double* tot2_host) {
+  if (!ctx || !s_host || !dir_n || !dir_L || !tot_host || !tot2_host)
+    return fail(CCO_ERR_ARG, "NULL argument");
+  if (B < 1 || N < 1 || D < 1 || d < 1 || d > 3) return fail(CCO_ERR_ARG, "bad sizes");
+  DevScope ds(ctx->device);
+  std::vector<int> koff(D);
+  int64_t Mk = 0;
+  int max_L = 0;
+  for (int c = 0; c < D; ++c) {
+    if (dir_L[c] < 1) return fail(CCO_ERR_ARG, "every direction needs at least one multiple");
+    koff[c] = (int)Mk;
+    Mk += dir_L[c];
+    max_L = std::max(max_L, (int)dir_L[c]);
+  }
+  if (Mk > (int64_t)1 << 30) return fail(CCO_ERR_ARG, "k-set too large");
+  // directions sorted by their number of multiples (uniform work per tile)
+  std::vector<int> order(D);
+  for (int c = 0; c < D; ++c) order[c] = c;
+  std::stable_sort(order.begin(), order.end(), [&](int a, int b) { return dir_L[a] > dir_L[b]; });
+  std::vector<int4> dirs(D);
+  std::vector<int> koff_sorted(D);
+  for (int q = 0; q < D; ++q) {
+    const int c = order[q];
+    dirs[q] = make_int4((int)dir_n[(size_t)c * d], d >= 2 ? (int)dir_n[(size_t)c * d + 1] : 0,
+                        d >= 3 ? (int)dir_n[(size_t)c * d + 2] : 0, dir_L[c]);
+    koff_sorted[q] = koff[c];
+  }
+  const int chunk = std::max(1, std::min(B, (int)std::max<int64_t>(1, ((int64_t)1 << 27) / Mk)));
+  DevBuf d_s, d_dirs, d_koff, d_out, d_tot, d_tot2;
+  int rc = 0;
+  hipStream_t st = ctx->stream;
+  auto cleanup = [&]() {
+    for (DevBuf* b : {&d_s, &d_dirs, &d_koff, &d_out, &d_tot, &d_tot2}) dev_release(*b);
+  };
+  if ((rc = dev_reserve(d_s, sizeof(double) * (size_t)B * N * d)) ||
+      (rc = dev_reserve(d_dirs, sizeof(int4) * D)) || (rc = dev_reserve(d_koff, sizeof(int) * D)) ||
+      (rc = dev_reserve(d_out, sizeof(double) * (size_t)chunk * Mk)) ||
+      (rc = dev_reserve(d_tot, sizeof(double) * Mk)) || (rc = dev_reserve(d_tot2, sizeof(double) * Mk))) {
+    cleanup();
+    return rc;
+  }
+  hipError_t e = hipMemcpyAsync(d_s.p, s_host, sizeof(double) * (size_t)B * N * d,
+                                hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(d_dirs.p, dirs.data(), sizeof(int4) * D, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(d_koff.p, koff_sorted.data(), sizeof(int) * D, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemsetAsync(d_tot.p, 0, sizeof(double) * Mk, st);
+  if (e == hipSuccess) e = hipMemsetAsync(d_tot2.p, 0, sizeof(double) * Mk, st);
+  for (int c0 = 0; e == hipSuccess && c0 < B; c0 += chunk) {
+    const int nc = std::min(chunk, B - c0);
+    launch_sk((const double*)d_s.p + (size_t)c0 * N * d, nc, N, d, (const int4*)d_dirs.p,
+              (const int*)d_koff.p, D, (int)Mk, max_L, (double*)d_out.p, st);
+    launch_sk_sum((const double*)d_out.p, nc, (int)Mk, (double*)d_tot.p, (double*)d_tot2.p, st);
+    e = hipGetLastError();
+  }
+  if (e == hipSuccess) e = hipMemcpyAsync(tot_host, d_tot.p, sizeof(double) * Mk, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(tot2_host, d_tot2.p, sizeof(double) * Mk, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  cleanup();
+  if (e != hipSuccess) return fail(CCO_ERR_DEVICE, std::string("cco_sk_moments: ") + hipGetErrorString(e));
+  return CCO_OK;
+}
+
+int cco_pair_stats(cco_ctx* ctx, const double* s_host, int B, int N, int d, const double* basis,
+                   const double* edges, int n_edges, int64_t* counts_host, double* nn_host) {
+  if (!ctx || !s_host || !basis || !edges || !counts_host) return fail(CCO_ERR_ARG, "NULL argument");
+  if (B < 1 || N < 1 || d < 1 || d > 3) return fail(CCO_ERR_ARG, "bad sizes");
+  if (n_edges < 2 || n_edges > 1024) return fail(CCO_ERR_ARG, "need 2 to 1024 bin edges");
+  DevScope ds(ctx->device);
+  DevBuf d_s, d_b, d_e, d_c, d_nn;
+  int rc = 0;
+  hipStream_t st = ctx->stream;
+  auto cleanup = [&]() {
+    for (DevBuf* b : {&d_s, &d_b, &d_e, &d_c, &d_nn}) dev_release(*b);
+  };
+  if ((rc = dev_reserve(d_s, sizeof(double) * (size_t)B * N * d)) ||
+      (rc = dev_reserve(d_b, sizeof(double) * d * d)) || (rc = dev_reserve(d_e, sizeof(double) * n_edges)) ||
+      (rc = dev_reserve(d_c, sizeof(unsigned long long) * (n_edges - 1))) ||
+      (nn_host && (rc = dev_reserve(d_nn, sizeof(double) * (size_t)B * N)))) {
+    cleanup();
+    return rc;
+  }
+  hipError_t e = hipMemcpyAsync(d_s.p, s_host, sizeof(double) * (size_t)B * N * d, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(d_b.p, basis, sizeof(double) * d * d, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemcpyAsync(d_e.p, edges, sizeof(double) * n_edges, hipMemcpyHostToDevice, st);
+  if (e == hipSuccess) e = hipMemsetAsync(d_c.p, 0, sizeof(unsigned long long) * (n_edges - 1), st);
+  if (e == hipSuccess) {
+    launch_pair((const double*)d_s.p, B, N, d, (const double*)d_b.p, (const double*)d_e.p, n_edges,
+                (unsigned long long*)d_c.p, nn_host ? (double*)d_nn.p : nullptr, st);
+    e = hipGetLastError();
+  }
+  std::vector<unsigned long long> cnt(n_edges - 1);
+  if (e == hipSuccess) e = hipMemcpyAsync(cnt.data(), d_c.p, sizeof(unsigned long long) * (n_edges - 1), hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess && nn_host) e = hipMemcpyAsync(nn_host, d_nn.p, sizeof(double) * (size_t)B * N, hipMemcpyDeviceToHost, st);
+  if (e == hipSuccess) e = hipStreamSynchronize(st);
+  cleanup();
+  if (e != hipSuccess) return fail(CCO_ERR_DEVICE, std::string("cco_pair_stats: ") + hipGetErrorString(e));
+  for (int b = 0; b < n_edges - 1; ++b) counts_host[b] = (int64_t)cnt[b];
+  return CCO_OK;
+}
+
 }  // extern "C"

@@ -69,6 +69,11 @@ void launch_energy(const Geo& g, const Batch& b, int slots_ub, bool with_grad,
                    hipStream_t stream);
 void launch_energy_stage(int stage, const Geo& g, const Batch& b, int slots_ub, hipStream_t st);
 void launch_copy(const void* src, void* dst, size_t bytes, hipStream_t st);
+void launch_sk(const double* s, int n_cfg, int N, int d, const int4* dirs, const int* koff, int D,
+               int Mk, int max_L, double* out, hipStream_t st);
+void launch_sk_sum(const double* sk, int n_cfg, int Mk, double* tot, double* tot2, hipStream_t st);
+void launch_pair(const double* s, int B, int N, int d, const double* basis, const double* edges,
+                 int n_edges, unsigned long long* counts, double* nn, hipStream_t st);
 size_t rho_lds_bytes(const Geo& g);
 
 // ---------------- small helpers ----------------

@@ -141,6 +141,12 @@ SIGNATURES = {
     "cco_md_init": (C.c_int, [_P, _PD, _PD, _PD]),
     "cco_md_run": (C.c_int, [_P, C.c_int64, C.c_int, _PI]),
     "cco_md_state": (C.c_int, [_P, _PD, _PD, _PD, _PD, _PI64]),
+    "cco_sk_moments": (
+        C.c_int, [_P, _PD, C.c_int, C.c_int, C.c_int, _PI64, _PI32, C.c_int, _PD, _PD]
+    ),
+    "cco_pair_stats": (
+        C.c_int, [_P, _PD, C.c_int, C.c_int, C.c_int, _PD, _PD, C.c_int, _PI64, _PD]
+    ),
     "cco_relax_set_profiling": (C.c_int, [_P, C.c_int]),
     "cco_relax_profile": (C.c_int, [_P, _PD, _PI64]),
     "cco_energy_profile": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int, _PD]),
