@@ -145,17 +145,23 @@ int build_geometry(cco_energy* E, const int64_t* n, const double* w, int M, int 
   }
   g.Mbox = off;
   g.Cn = (int)cols.size();
-  g.CT = std::min(256, ((g.Cn + kWave - 1) / kWave) * kWave);
-  g.n_coltiles = (g.Cn + g.CT - 1) / g.CT;
-  g.n_segs = (P1 + 1 + kRowSeg - 1) / kRowSeg;
-  std::vector<int4> col_tab(cols.size());
-  for (size_t c = 0; c < cols.size(); ++c)
-    col_tab[c] = make_int4(cols[c].off, cols[c].L, cols[c].n2, cols[c].n3);
-  std::vector<int> tile_lmax(g.n_coltiles, 0);
+  // rho units: runs of <= kRowSeg rows of one column, longest first (stable)
+  std::vector<int4> unit_tab;
+  g.Q = 1;
   for (size_t c = 0; c < cols.size(); ++c) {
-    int t = (int)c / g.CT;
-    tile_lmax[t] = std::max(tile_lmax[t], cols[c].L);
+    const int nq = (cols[c].L + kRowSeg - 1) / kRowSeg;
+    g.Q = std::max(g.Q, nq);
+    for (int q = 0; q < nq; ++q) {
+      const int nr = std::min(kRowSeg, cols[c].L - q * kRowSeg);
+      unit_tab.push_back(make_int4(cols[c].off + q * kRowSeg, nr | (q << 8), cols[c].n2, cols[c].n3));
+    }
   }
+  std::stable_sort(unit_tab.begin(), unit_tab.end(),
+                   [](const int4& a, const int4& b) { return (a.y & 0xff) > (b.y & 0xff); });
+  g.n_units = (int)unit_tab.size();
+  g.n_utiles = (g.n_units + 255) / 256;
+  const int per_tile = (g.n_units + g.n_utiles - 1) / g.n_utiles;
+  g.UT = ((per_tile + kWave - 1) / kWave) * kWave;
   std::vector<int> kbox(M);
   std::vector<double> wbox(g.Mbox, 0.0);
   for (int k = 0; k < M; ++k) {
@@ -192,20 +198,20 @@ int build_geometry(cco_energy* E, const int64_t* n, const double* w, int M, int 
     }
   }
   g.n_u = P2 + 1;
+  g.rho_chunk = kRhoChunkMax;
+  while (g.rho_chunk > 16 && rho_lds_bytes(g) > 64 * 1024) g.rho_chunk /= 2;
   if (rho_lds_bytes(g) > 64 * 1024)
     return fail(CCO_ERR_ARG, "k-set too wide for the rho kernel's LDS tables (max |n| too large)");
   // one device allocation holding every table
   auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
-  const size_t o_col = 0;
-  const size_t o_tl = align(o_col + sizeof(int4) * col_tab.size());
-  const size_t o_kb = align(o_tl + sizeof(int) * tile_lmax.size());
+  const size_t o_un = 0;
+  const size_t o_kb = align(o_un + sizeof(int4) * unit_tab.size());
   const size_t o_wb = align(o_kb + sizeof(int) * kbox.size());
   const size_t o_us = align(o_wb + sizeof(double) * wbox.size());
   const size_t o_gm = align(o_us + sizeof(int2) * ustart.size());
   const size_t total = align(o_gm + sizeof(int4) * std::max<size_t>(gmem.size(), 1));
   std::vector<char> host(total, 0);
-  std::memcpy(host.data() + o_col, col_tab.data(), sizeof(int4) * col_tab.size());
-  std::memcpy(host.data() + o_tl, tile_lmax.data(), sizeof(int) * tile_lmax.size());
+  std::memcpy(host.data() + o_un, unit_tab.data(), sizeof(int4) * unit_tab.size());
   std::memcpy(host.data() + o_kb, kbox.data(), sizeof(int) * kbox.size());
   std::memcpy(host.data() + o_wb, wbox.data(), sizeof(double) * wbox.size());
   std::memcpy(host.data() + o_us, ustart.data(), sizeof(int2) * ustart.size());
@@ -214,8 +220,7 @@ int build_geometry(cco_energy* E, const int64_t* n, const double* w, int M, int 
   if (rc) return rc;
   CCO_HIP(hipMemcpy(E->tables.p, host.data(), total, hipMemcpyHostToDevice));
   char* base = (char*)E->tables.p;
-  g.col = (const int4*)(base + o_col);
-  g.tile_lmax = (const int*)(base + o_tl);
+  g.unit = (const int4*)(base + o_un);
   g.kbox = (const int*)(base + o_kb);
   g.wbox = (const double*)(base + o_wb);
   g.ustart = (const int2*)(base + o_us);

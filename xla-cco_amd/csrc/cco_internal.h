@@ -10,6 +10,9 @@
 //   rho[c][r] = sum_j z1_j^r * B_c(j),   B_c(j) = z2_j^{n2(c)} z3_j^{n3(c)},
 //   z_a = exp(2 pi i s_a),
 // built by phase recurrence along r, so no trig is evaluated per (k, j).
+// The rho kernel's work unit is a run of at most kRowSeg consecutive rows of
+// one column (a "unit"); units are sorted by length so the lanes of a wave
+// run the same number of rows.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -21,8 +24,8 @@
 namespace cco {
 
 constexpr int kWave = 64;
-constexpr int kRowSeg = 32;      // rows of one rho accumulator segment
-constexpr int kRhoChunk = 64;    // particles staged in LDS per rho chunk
+constexpr int kRowSeg = 8;       // rows of one rho unit (accumulated in registers)
+constexpr int kRhoChunkMax = 64; // particles staged in LDS per rho chunk (upper bound)
 constexpr int kForceTile = 64;   // particles per force workgroup (one wave)
 
 // Geometry of one k-set, as the kernels read it.  Pointers are device memory.
@@ -33,14 +36,15 @@ struct Geo {
   int Mbox;     // entries in the column box
   int Cn;       // number of columns
   int P1, P2, P3;  // max |n_a| per axis (P2 = P3 = 0 when absent)
-  int CT;       // rho column tile width (threads per rho workgroup)
-  int n_coltiles;
-  int n_segs;   // row segments of kRowSeg
+  int n_units;  // rho units (column, row run) over the box
+  int UT;       // rho unit tile width (threads per rho workgroup)
+  int n_utiles; // rho workgroups per (lane, split)
+  int Q;        // row-run seeds per particle: z1^(q kRowSeg), q < Q
+  int rho_chunk;  // particles staged in LDS per rho chunk (16, 32 or 64)
   int n_u;      // force traversal: u = 0 .. n_u-1 (u = |n2|)
   double volume;
   double pref;  // 2 * (2 pi) / V
-  const int4* col;       // rho order: {box offset, L, n2, n3}
-  const int* tile_lmax;  // per column tile: max L
+  const int4* unit;      // rho units: {box offset of the first row, nrows | q << 8, n2, n3}
   const int* kbox;       // k -> box index (M); ~i marks a conjugated row
   const double* wbox;    // summed weight of the k rows on each box entry (Mbox)
   const int2* ustart;    // per u: {first group, number of v values}

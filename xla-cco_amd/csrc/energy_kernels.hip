@@ -11,9 +11,10 @@
 // (k, j) pair costs one complex multiply and one complex add (rho) or one
 // complex multiply and three FMAs (forces); no trig runs per pair.
 //
-// rho_kernel : one thread per column c; loops over a split of particles,
-//              accumulating kRowSeg rows of rho in registers; particle phase
-//              tables for a chunk of kRhoChunk particles sit in LDS.
+// rho_kernel : one thread per unit (a run of <= kRowSeg rows of one column,
+//              seeded by z1^(q kRowSeg) z2^n2 z3^n3 from LDS tables); loops
+//              over a split of particles, accumulating the run in registers;
+//              the phase tables of a chunk of particles sit in LDS.
 // reduce_kernel : per lane, sums the particle splits in a fixed order,
 //              forms Phi = sum_k w_k |rho_k|^2 / V, C = w rho on the box,
 //              and writes Re/Im rho in k order (padding rows: N, 0).
@@ -29,44 +30,42 @@ namespace cco {
 template <int D>
 __global__ __launch_bounds__(256) void rho_kernel(Geo g, Batch b) {
   extern __shared__ double2 lds[];
-  const int tiles_per_lane = g.n_coltiles * g.n_segs * b.n_splits;
+  const int tiles_per_lane = g.n_utiles * b.n_splits;
   const int slot = blockIdx.x / tiles_per_lane;
   if (slot >= *b.n_active) return;
-  int t = blockIdx.x % tiles_per_lane;
+  const int t = blockIdx.x % tiles_per_lane;
   const int split = t % b.n_splits;
-  t /= b.n_splits;
-  const int seg = t % g.n_segs;
-  const int ctile = t / g.n_segs;
+  const int utile = t / b.n_splits;
   const int lane = b.active[slot];
-  const int r0 = seg * kRowSeg;
-  const int lmax = g.tile_lmax[ctile];
-  if (r0 >= lmax) return;  // whole workgroup leaves before any barrier
-  const int nrows = min(kRowSeg, lmax - r0);
-  const int c = ctile * g.CT + (int)threadIdx.x;
-  const bool cvalid = ((int)threadIdx.x < g.CT) && (c < g.Cn);
-  const int4 col = cvalid ? g.col[c] : make_int4(0, 0, 0, 0);
+  const int u = utile * g.UT + (int)threadIdx.x;
+  const bool uvalid = u < g.n_units;
+  const int4 un = uvalid ? g.unit[u] : make_int4(0, 0, 0, 0);
+  const int nrows = un.y & 0xff;
+  const int q = un.y >> 8;
   const int j0 = split * b.split_len;
   const int j1 = min(b.N, j0 + b.split_len);
   const double* s = b.s + (size_t)lane * b.lane_stride;
 
+  const int CH = g.rho_chunk;
   const int w2 = g.P2 + 1;
   const int w3 = g.P3 + 1;
-  double2* Lz1r0 = lds;
-  double2* Lz1 = lds + kRhoChunk;
-  double2* LT2 = lds + 2 * kRhoChunk;
-  double2* LT3 = LT2 + kRhoChunk * w2;
+  // LDS per chunk particle jj: z1, z1^(q kRowSeg) for q < Q, z2^p (p <= P2), z3^p (p <= P3)
+  double2* Lz1 = lds;
+  double2* LZR = lds + CH;
+  double2* LT2 = LZR + CH * g.Q;
+  double2* LT3 = LT2 + CH * w2;
 
   double2 acc[kRowSeg];
 #pragma unroll
   for (int r = 0; r < kRowSeg; ++r) acc[r] = make_double2(0.0, 0.0);
 
-  const int a2 = abs(col.z);
-  const int a3 = abs(col.w);
-  const double sg2 = col.z < 0 ? -1.0 : 1.0;
-  const double sg3 = col.w < 0 ? -1.0 : 1.0;
+  const int a2 = abs(un.z);
+  const int a3 = abs(un.w);
+  const double sg2 = un.z < 0 ? -1.0 : 1.0;
+  const double sg3 = un.w < 0 ? -1.0 : 1.0;
 
-  for (int jc = j0; jc < j1; jc += kRhoChunk) {
-    const int nj = min(kRhoChunk, j1 - jc);
+  for (int jc = j0; jc < j1; jc += CH) {
+    const int nj = min(CH, j1 - jc);
     for (int idx = threadIdx.x; idx < nj * D; idx += blockDim.x) {
       const int jj = idx % nj;
       const int ax = idx / nj;
@@ -76,44 +75,42 @@ __global__ __launch_bounds__(256) void rho_kernel(Geo g, Batch b) {
       const double2 z = make_double2(cs, sn);
       if (ax == 0) {
         Lz1[jj] = z;
-        if (r0 == 0) {
-          Lz1r0[jj] = make_double2(1.0, 0.0);
-        } else {
+        LZR[jj * g.Q] = make_double2(1.0, 0.0);
+        for (int qq = 1; qq < g.Q; ++qq) {
           double s0, c0;
-          sincospi(2.0 * (double)r0 * sv, &s0, &c0);
-          Lz1r0[jj] = make_double2(c0, s0);
+          sincospi(2.0 * (double)(qq * kRowSeg) * sv, &s0, &c0);
+          LZR[jj * g.Q + qq] = make_double2(c0, s0);
         }
       } else {
         double2* T = (ax == 1) ? (LT2 + jj * w2) : (LT3 + jj * w3);
         const int P = (ax == 1) ? g.P2 : g.P3;
         double2 p = make_double2(1.0, 0.0);
         T[0] = p;
-        for (int q = 1; q <= P; ++q) {
+        for (int k = 1; k <= P; ++k) {
           p = cmul(p, z);
-          T[q] = p;
+          T[k] = p;
         }
       }
     }
     __syncthreads();
-    if (cvalid) {
+    if (uvalid) {
       int jj = 0;
       // two particles per pass: two independent recurrence chains
       for (; jj + 1 < nj; jj += 2) {
-        double2 B0 = make_double2(1.0, 0.0), B1 = make_double2(1.0, 0.0);
+        double2 e0 = LZR[jj * g.Q + q];
+        double2 e1 = LZR[(jj + 1) * g.Q + q];
         if (D >= 2) {
           const double2 t0 = LT2[jj * w2 + a2];
           const double2 t1 = LT2[(jj + 1) * w2 + a2];
-          B0 = make_double2(t0.x, sg2 * t0.y);
-          B1 = make_double2(t1.x, sg2 * t1.y);
+          e0 = cmul(e0, make_double2(t0.x, sg2 * t0.y));
+          e1 = cmul(e1, make_double2(t1.x, sg2 * t1.y));
         }
         if (D >= 3) {
           const double2 t0 = LT3[jj * w3 + a3];
           const double2 t1 = LT3[(jj + 1) * w3 + a3];
-          B0 = cmul(B0, make_double2(t0.x, sg3 * t0.y));
-          B1 = cmul(B1, make_double2(t1.x, sg3 * t1.y));
+          e0 = cmul(e0, make_double2(t0.x, sg3 * t0.y));
+          e1 = cmul(e1, make_double2(t1.x, sg3 * t1.y));
         }
-        double2 e0 = cmul(B0, Lz1r0[jj]);
-        double2 e1 = cmul(B1, Lz1r0[jj + 1]);
         const double2 z0 = Lz1[jj];
         const double2 z1 = Lz1[jj + 1];
 #pragma unroll
@@ -127,16 +124,15 @@ __global__ __launch_bounds__(256) void rho_kernel(Geo g, Batch b) {
         }
       }
       if (jj < nj) {
-        double2 B0 = make_double2(1.0, 0.0);
+        double2 e0 = LZR[jj * g.Q + q];
         if (D >= 2) {
           const double2 t0 = LT2[jj * w2 + a2];
-          B0 = make_double2(t0.x, sg2 * t0.y);
+          e0 = cmul(e0, make_double2(t0.x, sg2 * t0.y));
         }
         if (D >= 3) {
           const double2 t0 = LT3[jj * w3 + a3];
-          B0 = cmul(B0, make_double2(t0.x, sg3 * t0.y));
+          e0 = cmul(e0, make_double2(t0.x, sg3 * t0.y));
         }
-        double2 e0 = cmul(B0, Lz1r0[jj]);
         const double2 z0 = Lz1[jj];
 #pragma unroll
         for (int r = 0; r < kRowSeg; ++r) {
@@ -150,12 +146,11 @@ __global__ __launch_bounds__(256) void rho_kernel(Geo g, Batch b) {
     }
     __syncthreads();
   }
-  if (cvalid) {
-    const int L = col.y;
-    double2* out = b.part + ((size_t)lane * b.n_splits + split) * g.Mbox + col.x;
+  if (uvalid) {
+    double2* out = b.part + ((size_t)lane * b.n_splits + split) * g.Mbox + un.x;
 #pragma unroll
     for (int r = 0; r < kRowSeg; ++r) {
-      if (r < nrows && r0 + r < L) out[r0 + r] = acc[r];
+      if (r < nrows) out[r] = acc[r];
     }
   }
 }
@@ -268,15 +263,15 @@ __global__ __launch_bounds__(kForceTile) void force_kernel(Geo g, Batch b) {
 }
 
 size_t rho_lds_bytes(const Geo& g) {
-  return sizeof(double2) * (size_t)kRhoChunk * (2 + (g.P2 + 1) + (g.P3 + 1));
+  return sizeof(double2) * (size_t)g.rho_chunk * (1 + g.Q + (g.P2 + 1) + (g.P3 + 1));
 }
 
 // Launch one stage (0 = rho, 1 = reduce, 2 = force) of an evaluation.
 template <int D>
 static void launch_stage(int stage, const Geo& g, const Batch& b, int slots_ub, hipStream_t st) {
   if (stage == 0) {
-    const int rho_tiles = g.n_coltiles * g.n_segs * b.n_splits;
-    hipLaunchKernelGGL(rho_kernel<D>, dim3(slots_ub * rho_tiles), dim3(g.CT), rho_lds_bytes(g),
+    const int rho_tiles = g.n_utiles * b.n_splits;
+    hipLaunchKernelGGL(rho_kernel<D>, dim3(slots_ub * rho_tiles), dim3(g.UT), rho_lds_bytes(g),
                        st, g, b);
   } else if (stage == 1) {
     hipLaunchKernelGGL(reduce_kernel, dim3(slots_ub), dim3(256), 0, st, g, b);
