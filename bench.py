@@ -34,7 +34,8 @@ sys.path.insert(0, str(ROOT))
 PEAK_FP64_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
 PEAK_HBM_GBS = 8000.0
 METRIC = "converged configs/s (E<1e-20), mixed vs FP64 on 1 GPU; energy+force evals/s"
-STAGES = ("rho_kernel", "reduce_kernel", "force_kernel", "lbfgs_advance_kernel", "compact_kernel")
+STAGES = ("rho_kernel", "reduce_kernel", "force_kernel", "lbfgs_advance_kernel", "compact_kernel",
+          "md_prepare_kernel", "lbfgs_gram_dot_kernel", "lbfgs_gram_solve_kernel", "lbfgs_gram_comb_kernel")
 
 
 def build_workload(d: int, n: int, chi: float):
@@ -95,6 +96,12 @@ def main() -> int:
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="skip per-kernel event timing")
     ap.add_argument(
+        "--advance-variant",
+        type=int,
+        default=0,
+        help="L-BFGS advance: 0 auto, 1 global-memory two-loop, 2 register two-loop, 3 Gram form",
+    )
+    ap.add_argument(
         "--calibrate-copy",
         type=int,
         default=0,
@@ -132,6 +139,7 @@ def main() -> int:
     tol = 1e-20
     settings = LBFGSSettings(energy_tol=tol)
     sess = RelaxSession(energy, B, n, d, settings)
+    _native.check(L.cco_relax_set_option(sess.handle, 1, args.advance_variant))
     bound = 1 + 3 * (settings.max_evals_per_pass + settings.max_linesearch_steps + 1)
 
     inputs = [
@@ -177,8 +185,11 @@ def main() -> int:
         iters += cnt[1]
         pairs += cnt[2]
         steps_total += cnt[3]
-    ms = (C.c_double * 6)()
-    launches = (C.c_int64 * 6)()
+    opt = C.c_int64(0)
+    _native.check(L.cco_relax_get_option(sess.handle, 1, C.byref(opt)))
+    variant_run = int(opt.value)
+    ms = (C.c_double * len(STAGES))()
+    launches = (C.c_int64 * len(STAGES))()
     _native.check(L.cco_relax_profile(sess.handle, ms, launches))
 
     value = converged / elapsed
@@ -186,16 +197,25 @@ def main() -> int:
     nd = n * d
     # algorithmic work per launch of each kernel
     lane_evals_per_launch = lane_evals / max(steps_total, 1)
+    gram = variant_run == 3
+    per_step = 1.0 / max(steps_total, 1)
     work = {
         "rho_kernel": ("mfma", 8.0 * M * n * lane_evals_per_launch),
         "force_kernel": ("mfma", 8.0 * M * n * d * lane_evals_per_launch),
-        "lbfgs_advance_kernel": ("hbm", (32.0 * nd * pairs + 8.0 * nd * (3 * lane_evals + 12 * iters)) / max(steps_total, 1)),
+        # fused two-loop: s_i, y_i read twice per visited pair; Gram form: line search only
+        "lbfgs_advance_kernel": (
+            "hbm",
+            ((0.0 if gram else 32.0 * nd * pairs) + 8.0 * nd * (3 * lane_evals + 12 * iters)) * per_step,
+        ),
+        # Gram form: one read of s_i, y_i per pair in each of the dot and direction passes
+        "lbfgs_gram_dot_kernel": ("hbm", (16.0 * nd * pairs + 24.0 * nd * iters) * per_step),
+        "lbfgs_gram_comb_kernel": ("hbm", (16.0 * nd * pairs + 48.0 * nd * iters) * per_step),
     }
     roofline = None
     stage_avg = {}
     if not args.no_profile and launches[0] > 0:
-        stage_avg = {STAGES[i]: ms[i] / max(launches[i], 1) for i in range(5)}
-        dominant = max(range(5), key=lambda i: ms[i])
+        stage_avg = {STAGES[i]: ms[i] / launches[i] for i in range(len(STAGES)) if launches[i] > 0}
+        dominant = max((i for i in range(len(STAGES)) if launches[i] > 0), key=lambda i: ms[i])
         name = STAGES[dominant]
         avg_ms = stage_avg[name]
         bound_kind, per_launch = work.get(name, ("hbm", 0.0))
@@ -251,6 +271,8 @@ def main() -> int:
             "M": M,
             "lanes": B,
             "method": "lbfgs (memory 100, rescales 1e4/1/1e-2, zoom line search)",
+            "advance": {1: "two-loop, global memory", 2: "two-loop, register-resident",
+                        3: "Gram form (dot / recursion / direction kernels)"}.get(variant_run),
             "energy_tol": tol,
             "precision": "fp64",
         },

@@ -113,8 +113,9 @@ int cco_relax_destroy(cco_relax* r);
 int cco_relax_init(cco_relax* r, const double* s0_host);
 int cco_relax_init_dev(cco_relax* r, const double* s0_dev);
 /* Built-in energy: run until no lane is running, syncing with the host every
- * sync_every evaluations; returns CCO_ERR_BOUND if max_evals evaluations
- * pass first (the reference's round bound). */
+ * sync_every evaluations (at most 64 once <= 1/4 of the lanes run, 16 once
+ * <= 1/32 do); returns CCO_ERR_BOUND if max_evals evaluations pass first
+ * (the reference's round bound). */
 int cco_relax_run(cco_relax* r, int64_t max_evals, int sync_every);
 /* Same, stopping after at most n_evals evaluations; *n_active = lanes left. */
 int cco_relax_step(cco_relax* r, int64_t n_evals, int* n_active);
@@ -140,9 +141,13 @@ int cco_relax_stats(cco_relax* r, int64_t* steps, int64_t* lane_evals);
  * two-loop (s, y) pairs visited (L-BFGS), evaluation steps of the session. */
 int cco_relax_counters(cco_relax* r, int64_t* out4);
 /* Tuning knobs (no reference counterpart).  CCO_OPT_ADVANCE_VARIANT: 0 auto,
- * 1 L-BFGS vectors in global memory, 2 register-resident (N*d <= 8192). */
+ * 1 L-BFGS vectors in global memory, 2 register-resident (N*d <= 8192),
+ * 3 Gram form (memory <= 128).  Takes effect at the next cco_relax_init. */
 #define CCO_OPT_ADVANCE_VARIANT 1
 int cco_relax_set_option(cco_relax* r, int key, int64_t value);
+/* The value in force: for CCO_OPT_ADVANCE_VARIANT the variant latched by the
+ * last cco_relax_init (1, 2 or 3; 0 before the first init). */
+int cco_relax_get_option(cco_relax* r, int key, int64_t* value);
 
 /* Gradient flow dr/dt = -grad_r Phi in the Cartesian metric of basis
  * (d x d host, rows = lattice vectors), Dormand-Prince 5(4) with error
@@ -213,10 +218,13 @@ int cco_pair_stats(cco_ctx* ctx, const double* s_host, int B, int N, int d,
 /* Measurement (no reference counterpart; the reference times with
  * block_until_ready, benchmarks/bench_energy.py:104-115).  Per-stage device
  * time from HIP events recorded on the context stream around every launch:
- * stages rho, reduce, force, advance (or MD finish), compact, MD prepare
- * (ms summed, launches counted; six entries each). */
+ * stages rho, reduce, force, advance (or MD finish; the line-search kernel of
+ * the Gram form), compact, MD prepare, Gram dot products, Gram recursion,
+ * Gram direction (ms summed, launches counted; CCO_PROFILE_STAGES entries
+ * each). */
 int cco_relax_set_profiling(cco_relax* r, int on);
-int cco_relax_profile(cco_relax* r, double* ms_sum6, int64_t* launches6);
+#define CCO_PROFILE_STAGES 9
+int cco_relax_profile(cco_relax* r, double* ms_sum, int64_t* launches);
 /* reps timed evaluations of B x N on device data: avg ms of the rho, reduce
  * and force kernels and of the whole evaluation (avg_ms[0..3]). */
 int cco_energy_profile(cco_energy* e, const double* s_dev, int B, int N, int reps,

@@ -289,7 +289,7 @@ def test_relax_d3_batch_to_ground_state(gpu_ctx) -> None:
 
 @pytest.mark.parametrize("n,d", [(64, 2), (700, 3)])
 def test_lbfgs_advance_variants_agree(gpu_ctx, n: int, d: int) -> None:
-    """Global-memory and register-resident L-BFGS advance kernels: same path to ~1e-9."""
+    """Global-memory, register-resident and Gram-form L-BFGS advance: same path to ~1e-9."""
     from xla_cco import _native
     from xla_cco.minimize import RelaxSession
 
@@ -297,19 +297,47 @@ def test_lbfgs_advance_variants_agree(gpu_ctx, n: int, d: int) -> None:
     s0 = np.random.default_rng(11).uniform(size=(3, n, d))
     st = LBFGSSettings(rescales=(1.0,), max_evals_per_pass=60, record_every=1, max_records=12)
     outs = []
-    for variant in (1, 2):
+    for variant in (1, 2, 3):
         sess = RelaxSession(e, 3, n, d, st)
         _native.check(_native.lib().cco_relax_set_option(sess.handle, 1, variant))
         sess.init(s0)
         sess.run(10_000, 64)
         outs.append(sess.result())
         sess.close()
-    a, b = outs
-    assert np.array_equal(a.trajectory.count, b.trajectory.count)
-    assert np.allclose(a.trajectory.energy, b.trajectory.energy, rtol=1e-9, atol=1e-15)
-    assert np.allclose(a.trajectory.step, b.trajectory.step, rtol=1e-6)
+    a, b, c = outs
+    for other in (b, c):
+        assert np.array_equal(a.trajectory.count, other.trajectory.count)
+        assert np.allclose(a.trajectory.energy, other.trajectory.energy, rtol=1e-9, atol=1e-15)
+        assert np.allclose(a.trajectory.step, other.trajectory.step, rtol=1e-6)
     for i in range(3):
         ref = O.lbfgs(vg, s0[i], O.LBFGSSettings(rescales=(1.0,), max_evals_per_pass=60,
                                                   record_every=1, max_records=12))
         k = int(b.trajectory.count[i])
         assert np.allclose(b.trajectory.energy[i, :k], ref.traj["energy"][:k], rtol=1e-8)
+
+
+def test_lbfgs_advance_variant_selection(gpu_ctx) -> None:
+    """Auto runs the register two-loop; the Gram form is opt-in and needs memory <= 128."""
+    import ctypes as C
+
+    from xla_cco import _native
+    from xla_cco.minimize import RelaxSession
+
+    e, _, _ = _poisson(16, 2, 0.25)
+    s0 = np.random.default_rng(12).uniform(size=(2, 16, 2))
+    L = _native.lib()
+    got = {}
+    for mem in (100, 150):
+        sess = RelaxSession(e, 2, 16, 2, LBFGSSettings(memory=mem))
+        if mem > 128:
+            with pytest.raises(ValueError):
+                _native.check(L.cco_relax_set_option(sess.handle, 1, 3))
+        sess.init(s0)
+        v = C.c_int64(0)
+        _native.check(L.cco_relax_get_option(sess.handle, 1, C.byref(v)))
+        got[mem] = int(v.value)
+        sess.run(10_000, 64)
+        res = sess.result()
+        assert np.all(res.energy < 1e-20)
+        sess.close()
+    assert got == {100: 2, 150: 2}

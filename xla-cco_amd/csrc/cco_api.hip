@@ -66,7 +66,7 @@ struct cco_energy {
   int nmax[3];
   DevBuf tables;
   // batch workspace for cco_value_grad_density*
-  DevBuf s, phi, grad, re, im, part, cbox, active, n_active;
+  DevBuf s, phi, grad, re, im, part, cbox, active, n_active, rphi;
   int ident_B = 0;
 };
 
@@ -76,6 +76,11 @@ constexpr int kSplitLen = 128;  // particles per rho split (fixed per lane shape
 
 int n_splits_for(int N) { return (N + kSplitLen - 1) / kSplitLen; }
 int n_ftiles_for(int N) { return (N + kForceTile - 1) / kForceTile; }
+
+// Energy partials of the tiled reduce.
+int reserve_reduce(DevBuf& rphi, int B, const Geo& g) {
+  return dev_reserve(rphi, sizeof(double) * (size_t)B * g.n_rtiles);
+}
 
 struct ColKey {
   int n2, n3;
@@ -198,6 +203,30 @@ int build_geometry(cco_energy* E, const int64_t* n, const double* w, int M, int 
     }
   }
   g.n_u = P2 + 1;
+  g.n_rtiles = (std::max(g.Mbox, g.M_pad) + kReduceTile - 1) / kReduceTile;
+  // force parts: contiguous u ranges with about equal row counts
+  {
+    std::vector<long> rows_u(g.n_u, 0);
+    long total = 0;
+    for (int u = 0; u < g.n_u; ++u) {
+      for (int v = 0; v < ustart[u].y; ++v)
+        for (int q = 0; q < 4; ++q) rows_u[u] += gmem[4 * (size_t)(ustart[u].x + v) + q].y;
+      total += rows_u[u];
+    }
+    g.fparts = std::max(1, std::min(kForceParts, g.n_u));
+    int u = 0;
+    long acc = 0;
+    for (int p = 0; p < kForceParts; ++p) {
+      const int begin = u;
+      if (p < g.fparts) {
+        const long target = (total * (p + 1) + g.fparts - 1) / g.fparts;
+        // leave at least one u for each later part
+        while (u < g.n_u - (g.fparts - 1 - p) && (acc < target || u == begin)) acc += rows_u[u++];
+        if (p == g.fparts - 1) u = g.n_u;
+      }
+      g.fpart[p] = make_int2(begin, u);
+    }
+  }
   g.rho_chunk = kRhoChunkMax;
   while (g.rho_chunk > 16 && rho_lds_bytes(g) > 64 * 1024) g.rho_chunk /= 2;
   if (rho_lds_bytes(g) > 64 * 1024)
@@ -244,6 +273,7 @@ int energy_batch(cco_energy* E, const double* s_dev, int B, int N, double* phi_d
   if ((rc = dev_reserve(E->active, sizeof(int) * (size_t)B))) return rc;
   if ((rc = dev_reserve(E->n_active, sizeof(int)))) return rc;
   hipStream_t st = E->ctx->stream;
+  if ((rc = reserve_reduce(E->rphi, B, g))) return rc;
   if (E->ident_B != B) {
     hipLaunchKernelGGL(iota_kernel, dim3((B + 255) / 256), dim3(256), 0, st, (int*)E->active.p, B,
                        (int*)E->n_active.p);
@@ -265,6 +295,7 @@ int energy_batch(cco_energy* E, const double* s_dev, int B, int N, double* phi_d
   b.im = im_dev;
   b.part = (double2*)E->part.p;
   b.cbox = (double2*)E->cbox.p;
+  b.phi_part = (double*)E->rphi.p;
   launch_energy(g, b, B, grad_dev != nullptr, st);
   CCO_HIP(hipGetLastError());
   return CCO_OK;
@@ -276,9 +307,9 @@ int energy_batch(cco_energy* E, const double* s_dev, int B, int N, double* phi_d
 // relax session
 // ===========================================================================
 // Stages of one evaluation step; profiling accumulates per stage id.
-constexpr int kStages = 6;
+constexpr int kStages = CCO_PROFILE_STAGES;
 enum StageId { STG_RHO = 0, STG_REDUCE = 1, STG_FORCE = 2, STG_ADVANCE = 3, STG_COMPACT = 4,
-               STG_PREPARE = 5 };
+               STG_PREPARE = 5, STG_GRAM_DOT = 6, STG_GRAM_SOLVE = 7, STG_GRAM_COMB = 8 };
 constexpr int METHOD_FLOW = CCO_METHOD_FLOW;
 constexpr int METHOD_MD = CCO_METHOD_VERLET;
 
@@ -294,7 +325,7 @@ struct cco_relax {
   double* flow_dt0 = nullptr;  // device copy of per-lane first steps
   RelaxDev dev;
   std::vector<DevBuf> bufs;
-  DevBuf part, cbox, out_s, out_scalars, pack, pack_idx;
+  DevBuf part, cbox, rphi, out_s, out_scalars, pack, pack_idx;
   int slots_ub = 0;
   int64_t steps = 0;
   int64_t lane_evals = 0;
@@ -302,10 +333,11 @@ struct cco_relax {
   int* host_count = nullptr;  // pinned
   // optional per-stage timing with events on the context stream
   bool profile = false;
-  int advance_variant = 0;  // 0 auto, 1 global-memory vectors, 2 register-resident
+  int advance_variant = 0;  // requested: 0 auto, 1 global-memory, 2 register-resident, 3 Gram
+  int variant_run = 0;      // the variant latched by the last init (never 0)
   std::vector<hipEvent_t> evs;
-  double stage_ms[kStages] = {0, 0, 0, 0, 0, 0};
-  int64_t stage_n[kStages] = {0, 0, 0, 0, 0, 0};
+  double stage_ms[kStages] = {};
+  int64_t stage_n[kStages] = {};
 };
 
 namespace {
@@ -388,6 +420,7 @@ Batch relax_batch(cco_relax* R) {
   b.im = nullptr;
   b.part = (double2*)R->part.p;
   b.cbox = (double2*)R->cbox.p;
+  b.phi_part = (double*)R->rphi.p;
   return b;
 }
 
@@ -399,13 +432,15 @@ void relax_stage(cco_relax* R, int stage) {
     if (R->method == CCO_METHOD_DESCENT)
       launch_descent_advance(R->dev, R->dc, R->slots_ub, st);
     else if (R->method == CCO_METHOD_LBFGS)
-      launch_lbfgs_advance(R->dev, R->lc, R->slots_ub, st, R->advance_variant);
+      launch_lbfgs_advance(R->dev, R->lc, R->slots_ub, st, R->variant_run);
     else if (R->method == METHOD_FLOW)
       launch_flow_advance(R->dev, R->fc, R->slots_ub, st);
     else
       launch_md_finish(R->dev, R->mc, R->slots_ub, st);
   } else if (stage == STG_COMPACT) {
     launch_compact(R->dev, st);
+  } else if (stage >= STG_GRAM_DOT) {
+    launch_lbfgs_gram(R->dev, R->lc, R->slots_ub, stage - STG_GRAM_DOT + 1, st);
   } else {
     launch_md_prepare(R->dev, R->mc, R->slots_ub, st);
   }
@@ -419,12 +454,31 @@ int step_stages(const cco_relax* R, int* out) {
   out[n++] = STG_REDUCE;
   out[n++] = STG_FORCE;
   out[n++] = STG_ADVANCE;
+  if (R->method == CCO_METHOD_LBFGS && R->variant_run == 3) {
+    out[n++] = STG_GRAM_DOT;
+    out[n++] = STG_GRAM_SOLVE;
+    out[n++] = STG_GRAM_COMB;
+  }
   out[n++] = STG_COMPACT;
   return n;
 }
 
+// The L-BFGS advance variant a session runs: the requested one, or for 0 the
+// register-resident two-loop when N*d fits it, else the global-memory one.
+// The Gram form (3) has the shortest per-lane latency but streams the
+// history at a lower rate when every CU holds a lane, so it is opt-in.
+int resolve_variant(const cco_relax* R) {
+  if (R->advance_variant != 0) return R->advance_variant;
+  return lbfgs_register_ept(R->dev.nd) ? 2 : 1;
+}
+
 int relax_advance(cco_relax* R) {
   relax_stage(R, STG_ADVANCE);
+  if (R->method == CCO_METHOD_LBFGS && R->variant_run == 3) {
+    relax_stage(R, STG_GRAM_DOT);
+    relax_stage(R, STG_GRAM_SOLVE);
+    relax_stage(R, STG_GRAM_COMB);
+  }
   relax_stage(R, STG_COMPACT);
   CCO_HIP(hipGetLastError());
   return CCO_OK;
@@ -550,7 +604,7 @@ int cco_energy_destroy(cco_energy* E) {
   DevScope ds(E->ctx->device);
   (void)hipStreamSynchronize(E->ctx->stream);
   for (DevBuf* b : {&E->tables, &E->s, &E->phi, &E->grad, &E->re, &E->im, &E->part, &E->cbox,
-                    &E->active, &E->n_active})
+                    &E->active, &E->n_active, &E->rphi})
     dev_release(*b);
   delete E;
   return CCO_OK;
@@ -705,6 +759,11 @@ int cco_relax_create(cco_ctx* ctx, cco_energy* E, int B, int N, int d, int metho
     A(v.S, vec * m);
     A(v.Y, vec * m);
     A(v.rho, (size_t)B * m);
+    if (m <= (size_t)kGramMaxMem) {
+      v.gram_stride = gram_stride_for((int)m);
+      A(v.dir, B);
+      A(v.gram, (size_t)B * v.gram_stride);
+    }
   }
   if (cap > 0) {
     A(v.tr_s, vec * cap);
@@ -715,6 +774,7 @@ int cco_relax_create(cco_ctx* ctx, cco_energy* E, int B, int N, int d, int metho
   if (rc == 0 && E) {
     rc = dev_reserve(R->part, sizeof(double2) * (size_t)B * n_splits_for(N) * E->geo.Mbox);
     if (rc == 0) rc = dev_reserve(R->cbox, sizeof(double2) * (size_t)B * E->geo.Mbox);
+    if (rc == 0) rc = reserve_reduce(R->rphi, B, E->geo);
   }
   if (rc == 0 && hipHostMalloc((void**)&R->host_count, sizeof(int)) != hipSuccess)
     rc = fail(CCO_ERR_DEVICE, "hipHostMalloc failed");
@@ -731,7 +791,8 @@ int cco_relax_destroy(cco_relax* R) {
   DevScope ds(R->ctx->device);
   (void)hipStreamSynchronize(R->ctx->stream);
   for (DevBuf& b : R->bufs) dev_release(b);
-  for (DevBuf* b : {&R->part, &R->cbox, &R->out_s, &R->out_scalars, &R->pack, &R->pack_idx})
+  for (DevBuf* b : {&R->part, &R->cbox, &R->rphi, &R->out_s, &R->out_scalars, &R->pack,
+                    &R->pack_idx})
     dev_release(*b);
   if (R->host_count) (void)hipHostFree(R->host_count);
   for (hipEvent_t e : R->evs) (void)hipEventDestroy(e);
@@ -747,6 +808,7 @@ int cco_relax_init_dev(cco_relax* R, const double* s0_dev) {
   const size_t bytes = sizeof(double) * (size_t)R->B * R->dev.stride;
   CCO_HIP(hipMemcpyAsync(R->dev.s, s0_dev, bytes, hipMemcpyDeviceToDevice, st));
   CCO_HIP(hipMemcpyAsync(R->dev.s_eval, s0_dev, bytes, hipMemcpyDeviceToDevice, st));
+  R->variant_run = resolve_variant(R);
   hipLaunchKernelGGL(relax_init_kernel, dim3((R->B + 255) / 256), dim3(256), 0, st, R->dev,
                      R->method);
   CCO_HIP(hipGetLastError());
@@ -765,6 +827,7 @@ int cco_relax_init(cco_relax* R, const double* s0_host) {
   const size_t bytes = sizeof(double) * (size_t)R->B * R->dev.stride;
   CCO_HIP(hipMemcpyAsync(R->dev.s, s0_host, bytes, hipMemcpyHostToDevice, st));
   CCO_HIP(hipMemcpyAsync(R->dev.s_eval, R->dev.s, bytes, hipMemcpyDeviceToDevice, st));
+  R->variant_run = resolve_variant(R);
   hipLaunchKernelGGL(relax_init_kernel, dim3((R->B + 255) / 256), dim3(256), 0, st, R->dev,
                      R->method);
   CCO_HIP(hipGetLastError());
@@ -825,12 +888,23 @@ int cco_relax_step(cco_relax* R, int64_t n_evals, int* n_active) {
   return CCO_OK;
 }
 
+int cco_relax_get_option(cco_relax* R, int key, int64_t* value) {
+  if (!R || !value) return fail(CCO_ERR_ARG, "NULL argument");
+  if (key == CCO_OPT_ADVANCE_VARIANT) {
+    *value = R->variant_run;
+    return CCO_OK;
+  }
+  return fail(CCO_ERR_ARG, "unknown option key");
+}
+
 int cco_relax_set_option(cco_relax* R, int key, int64_t value) {
   if (!R) return fail(CCO_ERR_ARG, "relax is NULL");
   if (key == CCO_OPT_ADVANCE_VARIANT) {
-    if (value < 0 || value > 2) return fail(CCO_ERR_ARG, "advance variant must be 0, 1 or 2");
+    if (value < 0 || value > 3) return fail(CCO_ERR_ARG, "advance variant must be 0, 1, 2 or 3");
     if (value == 2 && lbfgs_register_ept(R->dev.nd) == 0)
       return fail(CCO_ERR_ARG, "register-resident advance needs N*d <= 8192");
+    if (value == 3 && R->dev.gram == nullptr)
+      return fail(CCO_ERR_ARG, "Gram-form advance needs an L-BFGS session with memory <= 128");
     R->advance_variant = (int)value;
     return CCO_OK;
   }
@@ -926,6 +1000,7 @@ int cco_energy_profile(cco_energy* E, const double* s_dev, int B, int N, int rep
   b.im = nullptr;
   b.part = (double2*)E->part.p;
   b.cbox = (double2*)E->cbox.p;
+  b.phi_part = (double*)E->rphi.p;
   hipStream_t st = E->ctx->stream;
   std::vector<hipEvent_t> ev(4 * (size_t)reps);
   for (auto& e : ev) CCO_HIP(hipEventCreate(&e));
@@ -981,7 +1056,12 @@ int cco_relax_run(cco_relax* R, int64_t max_evals, int sync_every) {
   while (n_active > 0) {
     if (R->steps >= max_evals)
       return fail(CCO_ERR_BOUND, "relax did not finish within the round bound");
-    const int64_t chunk = std::min<int64_t>(sync_every, max_evals - R->steps);
+    // shorter rounds once most lanes have stopped: steps launched after the
+    // last lane stops are wasted, while a host sync costs a few steps' time
+    int64_t every = sync_every;
+    if (n_active * 4 <= R->B) every = std::min<int64_t>(every, 64);
+    if (n_active * 32 <= R->B) every = std::min<int64_t>(every, 16);
+    const int64_t chunk = std::min<int64_t>(every, max_evals - R->steps);
     int rc = cco_relax_step(R, chunk, &n_active);
     if (rc) return rc;
   }
@@ -1183,6 +1263,7 @@ static int finish_session(cco_relax* R) {
   if (R->E) {
     rc = dev_reserve(R->part, sizeof(double2) * (size_t)R->B * n_splits_for(R->N) * R->E->geo.Mbox);
     if (rc == 0) rc = dev_reserve(R->cbox, sizeof(double2) * (size_t)R->B * R->E->geo.Mbox);
+    if (rc == 0) rc = reserve_reduce(R->rphi, R->B, R->E->geo);
   }
   if (rc == 0 && hipHostMalloc((void**)&R->host_count, sizeof(int)) != hipSuccess)
     rc = fail(CCO_ERR_DEVICE, "hipHostMalloc failed");
@@ -1282,6 +1363,7 @@ int cco_flow_init(cco_relax* R, const double* s0_host, const double* dt0_host) {
   } else {
     R->fc.dt0 = nullptr;
   }
+  R->variant_run = resolve_variant(R);
   hipLaunchKernelGGL(relax_init_kernel, dim3((R->B + 255) / 256), dim3(256), 0, st, R->dev,
                      R->method);
   CCO_HIP(hipGetLastError());
@@ -1407,6 +1489,7 @@ int cco_md_init(cco_relax* R, const double* r0, const double* v0, const double* 
   CCO_HIP(hipMemcpyAsync(R->dev.v, v0, sizeof(double) * B * nd, hipMemcpyHostToDevice, st));
   CCO_HIP(hipMemcpyAsync(R->dev.s_eval, s.data(), sizeof(double) * B * nd, hipMemcpyHostToDevice, st));
   CCO_HIP(hipMemcpyAsync(R->dev.ml, L.data(), sizeof(MdLane) * B, hipMemcpyHostToDevice, st));
+  R->variant_run = resolve_variant(R);
   hipLaunchKernelGGL(relax_init_kernel, dim3((R->B + 255) / 256), dim3(256), 0, st, R->dev,
                      R->method);
   CCO_HIP(hipGetLastError());

@@ -26,7 +26,9 @@ namespace cco {
 constexpr int kWave = 64;
 constexpr int kRowSeg = 8;       // rows of one rho unit (accumulated in registers)
 constexpr int kRhoChunkMax = 64; // particles staged in LDS per rho chunk (upper bound)
-constexpr int kForceTile = 64;   // particles per force workgroup (one wave)
+constexpr int kForceTile = 64;   // particles per force workgroup (one wave per k part)
+constexpr int kForceParts = 4;   // k parts per force workgroup (waves sharing 64 particles)
+constexpr int kReduceTile = 256; // box entries (and k rows) per reduce workgroup
 
 // Geometry of one k-set, as the kernels read it.  Pointers are device memory.
 struct Geo {
@@ -41,7 +43,10 @@ struct Geo {
   int n_utiles; // rho workgroups per (lane, split)
   int Q;        // row-run seeds per particle: z1^(q kRowSeg), q < Q
   int rho_chunk;  // particles staged in LDS per rho chunk (16, 32 or 64)
+  int n_rtiles;   // reduce workgroups per lane (kReduceTile box / k entries each)
   int n_u;      // force traversal: u = 0 .. n_u-1 (u = |n2|)
+  int fparts;   // force k parts in use (1 .. kForceParts)
+  int2 fpart[kForceParts];  // per part: u range [x, y), balanced by rows
   double volume;
   double pref;  // 2 * (2 pi) / V
   const int4* unit;      // rho units: {box offset of the first row, nrows | q << 8, n2, n3}
@@ -67,6 +72,7 @@ struct Batch {
   double* im;            // Im rho out
   double2* part;         // rho partials [lane][split][Mbox]
   double2* cbox;         // w * rho per box entry [lane][Mbox]
+  double* phi_part;      // energy partials [lane][n_rtiles]
 };
 
 void launch_energy(const Geo& g, const Batch& b, int slots_ub, bool with_grad,

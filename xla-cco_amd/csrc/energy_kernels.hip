@@ -15,13 +15,15 @@
 //              seeded by z1^(q kRowSeg) z2^n2 z3^n3 from LDS tables); loops
 //              over a split of particles, accumulating the run in registers;
 //              the phase tables of a chunk of particles sit in LDS.
-// reduce_kernel : per lane, sums the particle splits in a fixed order,
-//              forms Phi = sum_k w_k |rho_k|^2 / V, C = w rho on the box,
-//              and writes Re/Im rho in k order (padding rows: N, 0).
-// force_kernel : one thread per particle, one wave per 64 particles; the
-//              columns are walked by (|n2|, |n3|) groups whose four sign
-//              members share one z2^u z3^v product; C is read with uniform
-//              (scalar) loads.  grad_j = (4 pi / V) sum_k w_k n_k Im(rho_k
+// reduce_kernel : per (lane, tile of box entries), sums the particle splits
+//              in a fixed order, forms C = w rho on the box and the tile's
+//              part of Phi = sum_k w_k |rho_k|^2 / V (summed over tiles in
+//              tile order by the lane's last workgroup), and writes Re/Im rho
+//              in k order (padding rows: N, 0).
+// force_kernel : one thread per (particle, k part), four waves per 64
+//              particles, each wave walking a u = |n2| range of the
+//              (|n2|, |n3|) groups whose four sign members share one
+//              z2^u z3^v product; C is read with uniform (scalar) loads.  grad_j = (4 pi / V) sum_k w_k n_k Im(rho_k
 //              conj e_kj).
 #include "cco_internal.h"
 
@@ -155,28 +157,35 @@ __global__ __launch_bounds__(256) void rho_kernel(Geo g, Batch b) {
   }
 }
 
-__global__ __launch_bounds__(256) void reduce_kernel(Geo g, Batch b) {
-  __shared__ double red[16];
-  const int slot = blockIdx.x;
+// grid: slots_ub x n_rtiles.  Workgroup t of a lane owns box entries and k
+// rows [t kReduceTile, (t+1) kReduceTile) and writes its part of the energy;
+// the force kernel (or phi_kernel) sums the parts in tile order.
+__global__ __launch_bounds__(kReduceTile) void reduce_kernel(Geo g, Batch b) {
+  __shared__ double red[kReduceTile / kWave];
+  const int slot = blockIdx.x / g.n_rtiles;
   if (slot >= *b.n_active) return;
+  const int tile = blockIdx.x % g.n_rtiles;
   const int lane = b.active[slot];
   const double2* part = b.part + (size_t)lane * b.n_splits * g.Mbox;
   double2* cbox = b.cbox + (size_t)lane * g.Mbox;
+  const int idx = tile * kReduceTile + (int)threadIdx.x;
   double acc = 0.0;
-  for (int i = threadIdx.x; i < g.Mbox; i += blockDim.x) {
-    double2 rho = part[i];
+  if (idx < g.Mbox) {
+    double2 rho = part[idx];
     for (int sp = 1; sp < b.n_splits; ++sp) {
-      const double2 v = part[(size_t)sp * g.Mbox + i];
+      const double2 v = part[(size_t)sp * g.Mbox + idx];
       rho.x += v.x;
       rho.y += v.y;
     }
-    const double w = g.wbox[i];
-    cbox[i] = make_double2(w * rho.x, w * rho.y);
-    acc += w * (rho.x * rho.x + rho.y * rho.y);
+    const double w = g.wbox[idx];
+    cbox[idx] = make_double2(w * rho.x, w * rho.y);
+    acc = w * (rho.x * rho.x + rho.y * rho.y);
   }
-  if (b.re != nullptr) {
-    for (int k = threadIdx.x; k < g.M; k += blockDim.x) {
-      const int code = g.kbox[k];
+  if (b.re != nullptr && idx < g.M_pad) {
+    double re = (double)b.N;
+    double im = 0.0;
+    if (idx < g.M) {
+      const int code = g.kbox[idx];
       const int i = code < 0 ? ~code : code;
       double2 rho = part[i];
       for (int sp = 1; sp < b.n_splits; ++sp) {
@@ -184,62 +193,94 @@ __global__ __launch_bounds__(256) void reduce_kernel(Geo g, Batch b) {
         rho.x += v.x;
         rho.y += v.y;
       }
-      b.re[(size_t)lane * g.M_pad + k] = rho.x;
-      b.im[(size_t)lane * g.M_pad + k] = code < 0 ? -rho.y : rho.y;
+      re = rho.x;
+      im = code < 0 ? -rho.y : rho.y;
     }
-    for (int k = g.M + threadIdx.x; k < g.M_pad; k += blockDim.x) {
-      b.re[(size_t)lane * g.M_pad + k] = (double)b.N;
-      b.im[(size_t)lane * g.M_pad + k] = 0.0;
-    }
+    b.re[(size_t)lane * g.M_pad + idx] = re;
+    b.im[(size_t)lane * g.M_pad + idx] = im;
   }
-  const double tot = block_sum<256>(acc, red);
-  if (threadIdx.x == 0) b.phi[lane] = tot / g.volume;
+  const double tot = block_sum<kReduceTile>(acc, red);
+  if (threadIdx.x == 0) b.phi_part[(size_t)lane * g.n_rtiles + tile] = tot;
 }
 
+// Phi of a lane: its reduce tiles' partials summed in tile order.
+__device__ __forceinline__ double phi_from_parts(const Geo& g, const Batch& b, int lane) {
+  const double* pp = b.phi_part + (size_t)lane * g.n_rtiles;
+  double phi = 0.0;
+  for (int t = 0; t < g.n_rtiles; ++t) phi += pp[t];
+  return phi / g.volume;
+}
+
+// Energy-only evaluations: one thread per lane finishes Phi.
+__global__ void phi_kernel(Geo g, Batch b, int slots_ub) {
+  const int slot = blockIdx.x * blockDim.x + threadIdx.x;
+  if (slot >= slots_ub || slot >= *b.n_active) return;
+  const int lane = b.active[slot];
+  b.phi[lane] = phi_from_parts(g, b, lane);
+}
+
+// One column (member) of a force group: with u_r = Im(conj(C_r) e_r) along the
+// rows r = 0 .. L-1, g1 gains sum_r r u_r = L P - Q (P = sum_r u_r, Q = sum_r of
+// the running P), and g2, g3 gain n2 P, n3 P.
 template <int D>
 __device__ __forceinline__ void force_member(const int4 m, const double2 B, const double2 z1,
                                              const double2* __restrict__ C, double& g1,
                                              double& g2, double& g3) {
   if (m.y == 0) return;
   double2 e = B;
-  double us = 0.0;
-  double rr = 0.0;
+  double P = 0.0;
+  double Q = 0.0;
   const double2* Cm = C + m.x;
 #pragma unroll 4
   for (int r = 0; r < m.y; ++r) {
     const double2 c = Cm[r];
     const double uu = c.y * e.x - c.x * e.y;
-    g1 += rr * uu;
-    us += uu;
-    rr += 1.0;
+    P += uu;
+    Q += P;
     e = cmul(e, z1);
   }
-  if (D >= 2) g2 += (double)m.z * us;
-  if (D >= 3) g3 += (double)m.w * us;
+  g1 += (double)m.y * P - Q;
+  if (D >= 2) g2 += (double)m.z * P;
+  if (D >= 3) g3 += (double)m.w * P;
 }
 
+// grid: slots_ub x n_ftiles; block: kForceTile particles x g.fparts k parts.
+// Wave p walks the u range g.fpart[p]; the parts' partial gradients are summed
+// in part order through LDS.
 template <int D>
-__global__ __launch_bounds__(kForceTile) void force_kernel(Geo g, Batch b) {
+__global__ __launch_bounds__(kForceTile * kForceParts) void force_kernel(Geo g, Batch b) {
+  __shared__ double red[kForceParts > 1 ? kForceParts - 1 : 1][3][kForceTile];
   const int slot = blockIdx.x / b.n_ftiles;
   if (slot >= *b.n_active) return;
   const int tile = blockIdx.x % b.n_ftiles;
   const int lane = b.active[slot];
-  const int j = tile * kForceTile + (int)threadIdx.x;
+  const int jl = (int)threadIdx.x % kForceTile;
+  // wave-uniform: keeps the k walk (and the loads of C) on the scalar unit
+  const int part = __builtin_amdgcn_readfirstlane((int)threadIdx.x / kForceTile);
+  const int j = tile * kForceTile + jl;
   const bool valid = j < b.N;
   const double* sp = b.s + (size_t)lane * b.lane_stride + (size_t)(valid ? j : 0) * D;
   double2 z[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) z[a] = make_double2(1.0, 0.0);
+  double sv[3] = {0.0, 0.0, 0.0};
 #pragma unroll
   for (int a = 0; a < D; ++a) {
+    sv[a] = sp[a];
     double sn, cs;
-    sincospi(2.0 * sp[a], &sn, &cs);
+    sincospi(2.0 * sv[a], &sn, &cs);
     z[a] = make_double2(cs, sn);
   }
   const double2* __restrict__ C = b.cbox + (size_t)lane * g.Mbox;
   double g1 = 0.0, g2 = 0.0, g3 = 0.0;
+  const int2 ur = g.fpart[part];
   double2 zu = make_double2(1.0, 0.0);
-  for (int u = 0; u < g.n_u; ++u) {
+  if (D >= 2 && ur.x > 0) {
+    double sn, cs;
+    sincospi(2.0 * (double)ur.x * sv[1], &sn, &cs);
+    zu = make_double2(cs, sn);
+  }
+  for (int u = ur.x; u < ur.y; ++u) {
     const int2 us = g.ustart[u];
     double2 zv = make_double2(1.0, 0.0);
     for (int v = 0; v < us.y; ++v) {
@@ -254,6 +295,21 @@ __global__ __launch_bounds__(kForceTile) void force_kernel(Geo g, Batch b) {
     }
     zu = cmul(zu, z[1]);
   }
+  if (g.fparts > 1) {
+    if (part > 0) {
+      red[part - 1][0][jl] = g1;
+      red[part - 1][1][jl] = g2;
+      red[part - 1][2][jl] = g3;
+    }
+    __syncthreads();
+    if (part > 0) return;
+    for (int p = 1; p < g.fparts; ++p) {
+      g1 += red[p - 1][0][jl];
+      g2 += red[p - 1][1][jl];
+      g3 += red[p - 1][2][jl];
+    }
+  }
+  if (tile == 0 && threadIdx.x == 0) b.phi[lane] = phi_from_parts(g, b, lane);
   if (valid && b.grad != nullptr) {
     double* out = b.grad + (size_t)lane * b.lane_stride + (size_t)j * D;
     out[0] = g.pref * g1;
@@ -274,10 +330,10 @@ static void launch_stage(int stage, const Geo& g, const Batch& b, int slots_ub, 
     hipLaunchKernelGGL(rho_kernel<D>, dim3(slots_ub * rho_tiles), dim3(g.UT), rho_lds_bytes(g),
                        st, g, b);
   } else if (stage == 1) {
-    hipLaunchKernelGGL(reduce_kernel, dim3(slots_ub), dim3(256), 0, st, g, b);
+    hipLaunchKernelGGL(reduce_kernel, dim3(slots_ub * g.n_rtiles), dim3(kReduceTile), 0, st, g, b);
   } else {
-    hipLaunchKernelGGL(force_kernel<D>, dim3(slots_ub * b.n_ftiles), dim3(kForceTile), 0, st, g,
-                       b);
+    hipLaunchKernelGGL(force_kernel<D>, dim3(slots_ub * b.n_ftiles), dim3(kForceTile * g.fparts),
+                       0, st, g, b);
   }
 }
 
@@ -292,7 +348,12 @@ void launch_energy(const Geo& g, const Batch& b, int slots_ub, bool with_grad,
                    hipStream_t stream) {
   launch_energy_stage(0, g, b, slots_ub, stream);
   launch_energy_stage(1, g, b, slots_ub, stream);
-  if (with_grad) launch_energy_stage(2, g, b, slots_ub, stream);
+  if (with_grad) {
+    launch_energy_stage(2, g, b, slots_ub, stream);
+  } else if (slots_ub > 0) {
+    hipLaunchKernelGGL(phi_kernel, dim3((slots_ub + 255) / 256), dim3(256), 0, stream, g, b,
+                       slots_ub);
+  }
 }
 
 __global__ __launch_bounds__(256) void copy_kernel(const double4* __restrict__ src,

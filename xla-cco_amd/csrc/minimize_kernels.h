@@ -121,6 +121,10 @@ struct RelaxDev {
   double* S;      // [B][m][stride]
   double* Y;      // [B][m][stride]
   double* rho;    // [B][m]
+  // L-BFGS in Gram form: per-lane start flag {1, count} and dot-product tables
+  int2* dir;       // [B]
+  double* gram;    // [B][gram_stride]: SY (m x m), YY (m x m), Sg, Yg, cs, cy (m), misc (8)
+  int gram_stride;
   DescentLane* dl;
   LbfgsLane* ll;
   // MD: r = positions (Cartesian), v = velocities, f = forces
@@ -149,10 +153,15 @@ struct RelaxOutDev {
 };
 
 void launch_descent_advance(const RelaxDev& r, const DescentCfg& c, int slots_ub, hipStream_t st);
-// variant: 0 auto, 1 global-memory vectors (any size), 2 register-resident vectors
-// (nd <= 8192, one 1024-thread workgroup per lane)
+// variant: 1 global-memory vectors (any size), 2 register-resident vectors
+// (nd <= 8192, one 1024-thread workgroup per lane), 3 Gram form (first kernel of four)
 void launch_lbfgs_advance(const RelaxDev& r, const LbfgsCfg& c, int slots_ub, hipStream_t st,
-                          int variant = 0);
+                          int variant);
+// Gram form, kernels 2..4: part 1 dot products, 2 recursion, 3 direction and trial point
+constexpr int kGramMaxMem = 128;
+inline int gram_stride_for(int m) { return 2 * m * m + 4 * m + 8; }
+void launch_lbfgs_gram(const RelaxDev& r, const LbfgsCfg& c, int slots_ub, int part,
+                       hipStream_t st);
 int lbfgs_register_ept(int nd);  // elements per thread of the register variant, 0 if too large
 void launch_compact(const RelaxDev& r, hipStream_t st);
 void launch_md_prepare(const RelaxDev& r, const MdCfg& c, int slots_ub, hipStream_t st);

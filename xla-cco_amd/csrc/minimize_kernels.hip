@@ -389,7 +389,40 @@ __device__ void lbfgs_start_iteration(const RelaxDev& r, const LbfgsCfg& c, int 
   for (int i = threadIdx.x; i < nd; i += blockDim.x) se[i] = (x[i] + a * u[i]) / rs;
 }
 
-__global__ __launch_bounds__(kAdvBlock) void lbfgs_advance_kernel(RelaxDev r, LbfgsCfg c) {
+// Gram form, first kernel: when a lane starts an iteration, store the new
+// (s, y) pair (x - px, gx - pg, as lbfgs_start_iteration does) and flag the
+// lane for the dot-product, recursion and direction kernels.
+__device__ void lbfgs_gram_begin(const RelaxDev& r, const LbfgsCfg& c, int lane,
+                                 const LbfgsLane& L) {
+  const int count = L.opt_count;
+  if (count <= 0) return;
+  const int m = c.memory;
+  const int slot = (count - 1) % m;
+  const size_t off = (size_t)lane * r.stride;
+  const double* x = r.x + off;
+  const double* gx = r.g + off;
+  const double* px = r.px + off;
+  const double* pg = r.pg + off;
+  double* Ss = r.S + ((size_t)lane * m + slot) * r.stride;
+  double* Ys = r.Y + ((size_t)lane * m + slot) * r.stride;
+  for (int i = threadIdx.x; i < r.nd; i += blockDim.x) {
+    Ss[i] = x[i] - px[i];
+    Ys[i] = gx[i] - pg[i];
+  }
+}
+
+// The first slope along a new direction arrives with the first trial
+// evaluation (the Gram form's recursion kernel does not reduce over x).
+__device__ __forceinline__ void ls_set_slope0(LsState& Q, double slope) {
+  Q.slope0 = slope;
+  Q.prev_s = slope;
+  Q.low_s = slope;
+  Q.high_s = slope;
+  Q.pad0 = 0;
+}
+
+template <int BLOCK, bool GRAM>
+__global__ __launch_bounds__(BLOCK) void lbfgs_advance_kernel(RelaxDev r, LbfgsCfg c) {
   __shared__ double red[16];
   const int slot = blockIdx.x;
   if (slot >= *r.n_active) return;
@@ -405,6 +438,7 @@ __global__ __launch_bounds__(kAdvBlock) void lbfgs_advance_kernel(RelaxDev r, Lb
   LbfgsLane L = r.ll[lane];
   const double fe = r.phi_eval[lane];
   int code = ST_RUNNING;
+  bool started = false;
 
   if (L.mode == MODE_INIT) {
     // init_one (lbfgs.py:163-197)
@@ -433,17 +467,28 @@ __global__ __launch_bounds__(kAdvBlock) void lbfgs_advance_kernel(RelaxDev r, Lb
       L.rec_count = 1;
     }
     __syncthreads();
-    if (code == ST_RUNNING) lbfgs_start_iteration(r, c, lane, L, red);
+    if (code == ST_RUNNING) {
+      if constexpr (GRAM) {
+        lbfgs_gram_begin(r, c, lane, L);
+        started = true;
+      } else {
+        lbfgs_start_iteration(r, c, lane, L, red);
+      }
+    }
   } else {
     const double rs = c.rescales[L.pass_idx];
     // gradient w.r.t. x at the trial point, and its slope along u
     double sa = 0.0;
+    double s0 = 0.0;
+    const bool need_slope0 = GRAM && L.ls.pad0;
     for (int i = threadIdx.x; i < nd; i += blockDim.x) {
       const double gi = ge[i] / rs;
       ge[i] = gi;
       sa += gi * u[i];
+      if (need_slope0) s0 += u[i] * gx[i];
     }
-    sa = block_sum<kAdvBlock>(sa, red);
+    sa = block_sum<BLOCK>(sa, red);
+    if (need_slope0) ls_set_slope0(L.ls, block_sum<BLOCK>(s0, red));
     // every thread runs the same scalar update, so L stays uniform
     const int flags = ls_consume(L.ls, fe, sa, c.max_linesearch_steps);
     if (flags & 1) {
@@ -472,7 +517,7 @@ __global__ __launch_bounds__(kAdvBlock) void lbfgs_advance_kernel(RelaxDev r, Lb
       L.pass_evals += evals;
       double gg = 0.0;
       for (int i = threadIdx.x; i < nd; i += blockDim.x) gg += gx[i] * gx[i];
-      gg = block_sum<kAdvBlock>(gg, red);
+      gg = block_sum<BLOCK>(gg, red);
       const double gnorm = sqrt(gg) * rs;
       int glob = (gnorm < c.grad_tol) ? ST_GRAD_TOL : ST_RUNNING;
       if (L.f < c.energy_tol) glob = ST_ENERGY_TOL;
@@ -510,7 +555,14 @@ __global__ __launch_bounds__(kAdvBlock) void lbfgs_advance_kernel(RelaxDev r, Lb
         L.rec_count += 1;
       }
       __syncthreads();
-      if (code == ST_RUNNING) lbfgs_start_iteration(r, c, lane, L, red);
+      if (code == ST_RUNNING) {
+        if constexpr (GRAM) {
+          lbfgs_gram_begin(r, c, lane, L);
+          started = true;
+        } else {
+          lbfgs_start_iteration(r, c, lane, L, red);
+        }
+      }
     } else {
       const double a = L.ls.a;
       double* se = r.s_eval + off;
@@ -521,14 +573,17 @@ __global__ __launch_bounds__(kAdvBlock) void lbfgs_advance_kernel(RelaxDev r, Lb
   if (threadIdx.x == 0) {
     r.ll[lane] = L;
     r.status[lane] = code;
+    if constexpr (GRAM) r.dir[lane] = make_int2(started ? 1 : 0, L.opt_count);
   }
 }
 
 // ---------------------------------------------------------------------------
 // L-BFGS advance with register-resident vectors: one 1024-thread workgroup
 // per lane, element i of every vector owned by thread i % 1024.  The two-loop
-// recursion keeps q in registers, prefetches the next (s, y) pair before each
-// reduction, and pays one barrier per pair (double-buffered reduction slots).
+// recursion keeps q in registers, keeps the next 1-3 (s, y) pairs in flight
+// while one is reduced (enough bytes in flight per CU to stream at HBM rate
+// with one lane per CU), and pays one barrier per pair (double-buffered
+// reduction slots).
 // Same arithmetic, in the same order per element, as lbfgs_advance_kernel.
 // ---------------------------------------------------------------------------
 constexpr int kRegBlock = 1024;
@@ -573,7 +628,7 @@ __device__ __forceinline__ void reg_sum2(double& v0, double& v1, Red2& R) {
 
 template <int EPT>
 __device__ void lbfgs_start_reg(const RelaxDev& r, const LbfgsCfg& c, int lane, LbfgsLane& L,
-                                Red2& R, double* alphas) {
+                                Red2& R, double* alphas, double* rhos) {
   const int nd = r.nd;
   const int tid = threadIdx.x;
   const size_t off = (size_t)lane * r.stride;
@@ -633,59 +688,83 @@ __device__ void lbfgs_start_reg(const RelaxDev& r, const LbfgsCfg& c, int lane, 
   }
   const int nvalid = count < m ? count : m;
   const int midx = count % m;
-  double sn[EPT], yn[EPT];
-  auto load_pair = [&](int t) {
+  // rho of every valid pair by age, staged once in LDS (off the critical path)
+  for (int t = m - nvalid + tid; t < m; t += kRegBlock) {
+    const int sl = (midx + t) % m;
+    rhos[t] = (sl == slot_new) ? rho_new : rho[sl];
+  }
+  __syncthreads();
+  // the next PF pairs are in flight while one is reduced
+  constexpr int PF = EPT <= 2 ? 3 : (EPT == 4 ? 2 : 1);
+  double bs[PF][EPT], by[PF][EPT];
+  auto load_into = [&](double (&ds)[EPT], double (&dy)[EPT], int t) {
     const int sl = (midx + t) % m;
     const double* Si = S + (size_t)sl * r.stride;
     const double* Yi = Y + (size_t)sl * r.stride;
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
       const int i = tid + e * kRegBlock;
-      sn[e] = i < nd ? Si[i] : 0.0;
-      yn[e] = i < nd ? Yi[i] : 0.0;
+      ds[e] = i < nd ? Si[i] : 0.0;
+      dy[e] = i < nd ? Yi[i] : 0.0;
     }
   };
-  // first loop: newest -> oldest
-  if (nvalid > 0) load_pair(m - 1);
-  for (int t = m - 1; t >= m - nvalid; --t) {
-    double sc[EPT], yc[EPT];
+  // first loop: newest -> oldest (k = 0 is age m - 1)
 #pragma unroll
-    for (int e = 0; e < EPT; ++e) {
-      sc[e] = sn[e];
-      yc[e] = yn[e];
+  for (int p = 0; p < PF; ++p)
+    if (p < nvalid) load_into(bs[p], by[p], m - 1 - p);
+  for (int k0 = 0; k0 < nvalid; k0 += PF) {
+#pragma unroll
+    for (int p = 0; p < PF; ++p) {
+      const int k = k0 + p;
+      if (k < nvalid) {
+        const int t = m - 1 - k;
+        double sc[EPT], yc[EPT];
+#pragma unroll
+        for (int e = 0; e < EPT; ++e) {
+          sc[e] = bs[p][e];
+          yc[e] = by[p][e];
+        }
+        if (k + PF < nvalid) load_into(bs[p], by[p], t - PF);
+        double dot = 0.0;
+#pragma unroll
+        for (int e = 0; e < EPT; ++e) dot += sc[e] * q[e];
+        dot = reg_sum(dot, R);
+        const double a = rhos[t] * dot;
+        if (tid == 0) alphas[t] = a;
+#pragma unroll
+        for (int e = 0; e < EPT; ++e) q[e] = q[e] + (-a) * yc[e];
+      }
     }
-    if (t - 1 >= m - nvalid) load_pair(t - 1);
-    double dot = 0.0;
-#pragma unroll
-    for (int e = 0; e < EPT; ++e) dot += sc[e] * q[e];
-    dot = reg_sum(dot, R);
-    const int sl = (midx + t) % m;
-    const double a = (sl == slot_new ? rho_new : rho[sl]) * dot;
-    if (tid == 0) alphas[t] = a;
-#pragma unroll
-    for (int e = 0; e < EPT; ++e) q[e] = q[e] + (-a) * yc[e];
   }
 #pragma unroll
   for (int e = 0; e < EPT; ++e) q[e] = gamma * q[e];
-  // second loop: oldest -> newest
-  if (nvalid > 0) load_pair(m - nvalid);
-  for (int t = m - nvalid; t < m; ++t) {
-    double sc[EPT], yc[EPT];
+  // second loop: oldest -> newest (k = 0 is age m - nvalid)
 #pragma unroll
-    for (int e = 0; e < EPT; ++e) {
-      sc[e] = sn[e];
-      yc[e] = yn[e];
+  for (int p = 0; p < PF; ++p)
+    if (p < nvalid) load_into(bs[p], by[p], m - nvalid + p);
+  for (int k0 = 0; k0 < nvalid; k0 += PF) {
+#pragma unroll
+    for (int p = 0; p < PF; ++p) {
+      const int k = k0 + p;
+      if (k < nvalid) {
+        const int t = m - nvalid + k;
+        double sc[EPT], yc[EPT];
+#pragma unroll
+        for (int e = 0; e < EPT; ++e) {
+          sc[e] = bs[p][e];
+          yc[e] = by[p][e];
+        }
+        if (k + PF < nvalid) load_into(bs[p], by[p], t + PF);
+        double dot = 0.0;
+#pragma unroll
+        for (int e = 0; e < EPT; ++e) dot += yc[e] * q[e];
+        dot = reg_sum(dot, R);
+        const double bcoef = rhos[t] * dot;
+        const double coef = alphas[t] - bcoef;
+#pragma unroll
+        for (int e = 0; e < EPT; ++e) q[e] = q[e] + coef * sc[e];
+      }
     }
-    if (t + 1 < m) load_pair(t + 1);
-    double dot = 0.0;
-#pragma unroll
-    for (int e = 0; e < EPT; ++e) dot += yc[e] * q[e];
-    dot = reg_sum(dot, R);
-    const int sl = (midx + t) % m;
-    const double bcoef = (sl == slot_new ? rho_new : rho[sl]) * dot;
-    const double coef = alphas[t] - bcoef;
-#pragma unroll
-    for (int e = 0; e < EPT; ++e) q[e] = q[e] + coef * sc[e];
   }
   double slope = 0.0;
 #pragma unroll
@@ -729,6 +808,7 @@ template <int EPT>
 __global__ __launch_bounds__(kRegBlock) void lbfgs_advance_reg_kernel(RelaxDev r, LbfgsCfg c) {
   __shared__ double redbuf[4 * kRegWaves];
   __shared__ double alphas[1024];
+  __shared__ double rhos[1024];
   Red2 R{redbuf, 0};
   const int slot = blockIdx.x;
   if (slot >= *r.n_active) return;
@@ -892,7 +972,7 @@ __global__ __launch_bounds__(kRegBlock) void lbfgs_advance_reg_kernel(RelaxDev r
       }
     }
   }
-  if (start) lbfgs_start_reg<EPT>(r, c, lane, L, R, alphas);
+  if (start) lbfgs_start_reg<EPT>(r, c, lane, L, R, alphas, rhos);
   if (tid == 0) {
     r.ll[lane] = L;
     r.status[lane] = code;
@@ -908,9 +988,15 @@ int lbfgs_register_ept(int nd) {
 void launch_lbfgs_advance(const RelaxDev& r, const LbfgsCfg& c, int slots_ub, hipStream_t st,
                           int variant) {
   if (slots_ub <= 0) return;
+  if (variant == 3) {
+    hipLaunchKernelGGL((lbfgs_advance_kernel<kRegBlock, true>), dim3(slots_ub), dim3(kRegBlock), 0,
+                       st, r, c);
+    return;
+  }
   const int ept = lbfgs_register_ept(r.nd);
   if (variant == 1 || ept == 0) {
-    hipLaunchKernelGGL(lbfgs_advance_kernel, dim3(slots_ub), dim3(kAdvBlock), 0, st, r, c);
+    hipLaunchKernelGGL((lbfgs_advance_kernel<kAdvBlock, false>), dim3(slots_ub), dim3(kAdvBlock), 0,
+                       st, r, c);
     return;
   }
   switch (ept) {
@@ -926,6 +1012,416 @@ void launch_lbfgs_advance(const RelaxDev& r, const LbfgsCfg& c, int slots_ub, hi
     default:
       hipLaunchKernelGGL(lbfgs_advance_reg_kernel<8>, dim3(slots_ub), dim3(kRegBlock), 0, st, r, c);
       break;
+  }
+}
+
+// ---------------------------------------------------------------------------
+// L-BFGS in Gram form.  The two-loop recursion only ever forms dot products of
+// s_i, y_i and g, so it is restated on the tables S^T Y, Y^T Y, S^T g, Y^T g
+// (the vector-free two-loop of Chen, Wang and Zhou, "Large-scale L-BFGS using
+// MapReduce", NIPS 2014):
+//   alpha_t = rho_t (s_t.g - sum_{j newer} alpha_j s_t.y_j)          (newest first)
+//   w_t     = gamma (y_t.g - sum_i alpha_i y_t.y_i)
+//   c_t     = alpha_t - rho_t (w_t + sum_{j older} c_j s_j.y_t)       (oldest first)
+//   u       = -(gamma g - gamma sum_i alpha_i y_i + sum_t c_t s_t)
+// The same direction as lbfgs_start_iteration up to rounding.  Per iteration:
+//   dot kernel    one pass over the history (5 dot products per slot, and
+//                 g.g), spread over ceil(m / kGramSG) workgroups per lane;
+//   solve kernel  the recursion on LDS-resident S^T Y, one workgroup per lane;
+//   comb kernel   one streaming pass forming u and the first trial point,
+//                 kCombBlock elements per workgroup.
+// Every reduction has a fixed shape per lane size, so results do not depend
+// on which other lanes are active.
+// ---------------------------------------------------------------------------
+constexpr int kGramSG = 4;       // history slots per dot workgroup
+constexpr int kGramBlock = 256;
+constexpr int kSolveBlock = 256;
+constexpr int kCombBlock = 256;
+constexpr int kGramVals = kGramSG * 5 + 1;
+
+struct GramView {
+  double* sy;    // [m][m]  sy[a][b] = s_a . y_b (physical slots)
+  double* yy;    // [m][m]  symmetric
+  double* sg;    // [m]     s_a . g
+  double* yg;    // [m]     y_a . g
+  double* cs;    // [m]     coefficient of s_a in q
+  double* cy;    // [m]     coefficient of y_a in q
+  double* misc;  // [8]     0: g.g, 1: gamma
+};
+
+__device__ __forceinline__ GramView gram_view(const RelaxDev& r, int m, int lane) {
+  double* b = r.gram + (size_t)lane * r.gram_stride;
+  GramView v;
+  v.sy = b;
+  v.yy = b + (size_t)m * m;
+  v.sg = b + 2 * (size_t)m * m;
+  v.yg = v.sg + m;
+  v.cs = v.yg + m;
+  v.cy = v.cs + m;
+  v.misc = v.cy + m;
+  return v;
+}
+
+__device__ __forceinline__ double readlane_d(double v, int l) {
+  const long long bits = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(bits & 0xffffffffLL), l);
+  const int hi = __builtin_amdgcn_readlane((int)(bits >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// grid: slots_ub x ceil(m / kGramSG); workgroup (lane, group) owns slots
+// group*kGramSG .. +kGramSG-1 and reduces over all of x.
+__global__ __launch_bounds__(kGramBlock) void lbfgs_gram_dot_kernel(RelaxDev r, LbfgsCfg c) {
+  __shared__ double red[kGramBlock / kWave][kGramVals];
+  const int m = c.memory;
+  const int n_groups = (m + kGramSG - 1) / kGramSG;
+  const int slot = blockIdx.x / n_groups;
+  const int grp = blockIdx.x % n_groups;
+  if (slot >= *r.n_active) return;
+  const int lane = r.active[slot];
+  const int2 di = r.dir[lane];
+  if (di.x == 0) return;
+  const int count = di.y;
+  const int nvalid = count < m ? count : m;
+  const int midx = count % m;
+  const bool newpair = count > 0;
+  const int sn = newpair ? (count - 1) % m : 0;
+  int sl[kGramSG];
+  bool val[kGramSG];
+  bool any = grp == 0;
+#pragma unroll
+  for (int k = 0; k < kGramSG; ++k) {
+    sl[k] = grp * kGramSG + k;
+    const int age = (sl[k] - midx + m) % m;
+    val[k] = sl[k] < m && age >= m - nvalid;
+    any = any || val[k];
+  }
+  if (!any) return;  // uniform per workgroup, before any barrier
+  const int nd = r.nd;
+  const size_t stride = r.stride;
+  const double* g = r.g + (size_t)lane * stride;
+  const double* S = r.S + (size_t)lane * m * stride;
+  const double* Y = r.Y + (size_t)lane * m * stride;
+  const double* Sn = S + (size_t)sn * stride;
+  const double* Yn = Y + (size_t)sn * stride;
+  double acc[kGramSG][5];
+#pragma unroll
+  for (int k = 0; k < kGramSG; ++k)
+#pragma unroll
+    for (int q = 0; q < 5; ++q) acc[k][q] = 0.0;
+  double gg = 0.0;
+  // two elements per pass (i, i + kGramBlock): twice the loads in flight
+  for (int i0 = threadIdx.x; i0 < nd; i0 += 2 * kGramBlock) {
+    double gi[2], sni[2], yni[2], sv[kGramSG][2], yv[kGramSG][2];
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      const int i = i0 + h * kGramBlock;
+      const bool ok = i < nd;
+      gi[h] = ok ? g[i] : 0.0;
+      sni[h] = (ok && newpair) ? Sn[i] : 0.0;
+      yni[h] = (ok && newpair) ? Yn[i] : 0.0;
+#pragma unroll
+      for (int k = 0; k < kGramSG; ++k) {
+        sv[k][h] = (ok && val[k]) ? S[(size_t)sl[k] * stride + i] : 0.0;
+        yv[k][h] = (ok && val[k]) ? Y[(size_t)sl[k] * stride + i] : 0.0;
+      }
+    }
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+      gg += gi[h] * gi[h];
+#pragma unroll
+      for (int k = 0; k < kGramSG; ++k) {
+        acc[k][0] += sv[k][h] * gi[h];
+        acc[k][1] += yv[k][h] * gi[h];
+        acc[k][2] += sni[h] * yv[k][h];
+        acc[k][3] += sv[k][h] * yni[h];
+        acc[k][4] += yni[h] * yv[k][h];
+      }
+    }
+  }
+  const int wid = threadIdx.x / kWave;
+  const int lid = threadIdx.x % kWave;
+#pragma unroll
+  for (int k = 0; k < kGramSG; ++k) {
+#pragma unroll
+    for (int q = 0; q < 5; ++q) {
+      const double v = wave_sum(acc[k][q]);
+      if (lid == 0) red[wid][k * 5 + q] = v;
+    }
+  }
+  {
+    const double v = wave_sum(gg);
+    if (lid == 0) red[wid][kGramSG * 5] = v;
+  }
+  __syncthreads();
+  const int t = threadIdx.x;
+  if (t >= kGramVals) return;
+  double tot = 0.0;
+#pragma unroll
+  for (int w = 0; w < kGramBlock / kWave; ++w) tot += red[w][t];
+  GramView G = gram_view(r, m, lane);
+  if (t == kGramSG * 5) {
+    if (grp == 0) G.misc[0] = tot;
+    return;
+  }
+  const int k = t / 5;
+  const int q = t % 5;
+  int slk = grp * kGramSG;
+  bool vk = false;
+#pragma unroll
+  for (int kk = 0; kk < kGramSG; ++kk)
+    if (kk == k) {
+      slk = sl[kk];
+      vk = val[kk];
+    }
+  if (!vk) return;
+  if (q == 0) G.sg[slk] = tot;
+  if (q == 1) G.yg[slk] = tot;
+  if (!newpair) return;
+  if (q == 2) G.sy[(size_t)sn * m + slk] = tot;  // s_new . y_slot
+  if (q == 3) G.sy[(size_t)slk * m + sn] = tot;  // s_slot . y_new
+  if (q == 4) {
+    G.yy[(size_t)sn * m + slk] = tot;
+    G.yy[(size_t)slk * m + sn] = tot;
+  }
+}
+
+// grid: slots_ub; the sequential parts run on wave 0, lane l owning relative
+// ages l and l + 64 (age 0 = oldest valid pair).  S^T Y sits in LDS in
+// physical slot order with an odd row stride.
+constexpr int kSolveBatch = 16;  // global loads in flight per thread while staging
+
+__global__ __launch_bounds__(kSolveBlock) void lbfgs_gram_solve_kernel(RelaxDev r, LbfgsCfg c) {
+  __shared__ double Lsy[kGramMaxMem * (kGramMaxMem + 1)];
+  __shared__ double La[kGramMaxMem];
+  __shared__ double Lw[kGramMaxMem];
+  __shared__ double Lr[kGramMaxMem];
+  const int slot = blockIdx.x;
+  if (slot >= *r.n_active) return;
+  const int lane = r.active[slot];
+  const int2 di = r.dir[lane];
+  if (di.x == 0) return;
+  const int m = c.memory;
+  const int count = di.y;
+  const int nvalid = count < m ? count : m;
+  const int first = (count % m + m - nvalid) % m;  // physical slot of relative age 0
+  const int tid = threadIdx.x;
+  GramView G = gram_view(r, m, lane);
+  double* rho = r.rho + (size_t)lane * m;
+  const int sn = count > 0 ? (count - 1) % m : -1;
+  double gamma;
+  double rho_new = 0.0;
+  if (count > 0) {
+    const double sy = G.sy[(size_t)sn * m + sn];
+    const double yy = G.yy[(size_t)sn * m + sn];
+    rho_new = (sy == 0.0) ? 0.0 : 1.0 / sy;
+    gamma = (yy > 0.0) ? sy / yy : 1.0;
+  } else {
+    gamma = nanmin(1.0, 1.0 / sqrt(G.misc[0]));
+  }
+  // waves 2 and 3: relative age a = tid - 128 prefetches y_b.y_a for every b
+  // (overlapping the staging and the first loop)
+  const int aw = tid - 2 * kWave;
+  const bool wok = aw >= 0 && aw < nvalid;
+  const int saw = wok ? (first + aw) % m : 0;
+  double yyv[kGramMaxMem];
+#pragma unroll
+  for (int b = 0; b < kGramMaxMem; ++b)
+    yyv[b] = (wok && b < nvalid) ? G.yy[(size_t)((first + b) % m) * m + saw] : 0.0;
+  const int ld = m | 1;
+  if (nvalid > 0) {
+    const int total = m * m;
+    for (int base = tid; base < total; base += kSolveBlock * kSolveBatch) {
+      double v[kSolveBatch];
+#pragma unroll
+      for (int q = 0; q < kSolveBatch; ++q) {
+        const int idx = base + q * kSolveBlock;
+        v[q] = idx < total ? G.sy[idx] : 0.0;
+      }
+#pragma unroll
+      for (int q = 0; q < kSolveBatch; ++q) {
+        const int idx = base + q * kSolveBlock;
+        if (idx < total) {
+          const int row = idx / m;
+          Lsy[row * ld + (idx - row * m)] = v[q];
+        }
+      }
+    }
+  }
+  for (int a = tid; a < nvalid; a += kSolveBlock) {
+    const int sa = (first + a) % m;
+    Lr[a] = (sa == sn) ? rho_new : rho[sa];
+  }
+  __syncthreads();
+  const int a0 = tid;
+  const int a1 = tid + kWave;
+  const bool v0ok = a0 < nvalid;
+  const bool v1ok = a1 < nvalid;
+  const int s0 = (first + a0) % m;
+  const int s1 = (first + a1) % m;
+  if (tid < kWave) {
+    // first loop, newest first: alpha_j = rho_j (s_j.g - sum_{t newer} alpha_t s_j.y_t)
+    double v0 = v0ok ? G.sg[s0] : 0.0;
+    double v1 = v1ok ? G.sg[s1] : 0.0;
+    const double r0 = v0ok ? Lr[a0] : 0.0;
+    const double r1 = v1ok ? Lr[a1] : 0.0;
+    const double* row0 = Lsy + s0 * ld;
+    const double* row1 = Lsy + s1 * ld;
+    int sj = (first + nvalid - 1) % m;
+    for (int j = nvalid - 1; j >= 0; --j) {
+      const double l0 = row0[sj];
+      const double l1 = row1[sj];
+      sj = (sj == 0) ? m - 1 : sj - 1;
+      double alpha;
+      if (j >= kWave) {
+        alpha = readlane_d(r1 * v1, j - kWave);
+      } else {
+        alpha = readlane_d(r0 * v0, j);
+      }
+      if (a0 < j) v0 = v0 - alpha * l0;
+      if (a1 < j) v1 = v1 - alpha * l1;
+      if (a0 == j) La[a0] = alpha;
+      if (a1 == j) La[a1] = alpha;
+    }
+  }
+  __syncthreads();
+  // w_t = gamma (y_t.g - sum_i alpha_i y_t.y_i), oldest i first
+  if (wok) {
+    double w = G.yg[saw];
+#pragma unroll
+    for (int b = 0; b < kGramMaxMem; ++b)
+      if (b < nvalid) w = w - La[b] * yyv[b];
+    Lw[aw] = gamma * w;
+  }
+  __syncthreads();
+  if (tid < kWave) {
+    // second loop, oldest first: c_j = alpha_j - rho_j (w_j + sum_{t older} c_t s_t.y_j)
+    const double r0 = v0ok ? Lr[a0] : 0.0;
+    const double r1 = v1ok ? Lr[a1] : 0.0;
+    const double al0 = v0ok ? La[a0] : 0.0;
+    const double al1 = v1ok ? La[a1] : 0.0;
+    const double w0 = v0ok ? Lw[a0] : 0.0;
+    const double w1 = v1ok ? Lw[a1] : 0.0;
+    double acc0 = 0.0, acc1 = 0.0, c0 = 0.0, c1 = 0.0;
+    int sj = first;
+    for (int j = 0; j < nvalid; ++j) {
+      const double l0 = Lsy[sj * ld + s0];
+      const double l1 = Lsy[sj * ld + s1];
+      sj = (sj == m - 1) ? 0 : sj + 1;
+      double cj;
+      if (j >= kWave) {
+        cj = readlane_d(al1 - r1 * (w1 + acc1), j - kWave);
+      } else {
+        cj = readlane_d(al0 - r0 * (w0 + acc0), j);
+      }
+      if (a0 == j) c0 = cj;
+      if (a1 == j) c1 = cj;
+      if (a0 > j) acc0 = acc0 + cj * l0;
+      if (a1 > j) acc1 = acc1 + cj * l1;
+    }
+    if (v0ok) {
+      G.cs[s0] = c0;
+      G.cy[s0] = -gamma * al0;
+    }
+    if (v1ok) {
+      G.cs[s1] = c1;
+      G.cy[s1] = -gamma * al1;
+    }
+  }
+  if (tid == 0) {
+    if (count > 0) rho[sn] = rho_new;
+    G.misc[1] = gamma;
+    LbfgsLane L = r.ll[lane];
+    L.opt_count = count + 1;
+    L.pairs += nvalid;
+    LsState& Q = L.ls;
+    const double nan = __longlong_as_double(0x7ff8000000000000LL);
+    Q.f0 = L.f;
+    Q.slope0 = nan;
+    Q.count = 0;
+    Q.interval_found = 0;
+    Q.prev_a = 0.0; Q.prev_v = L.f; Q.prev_s = nan;
+    Q.low_a = 0.0; Q.low_v = L.f; Q.low_s = nan;
+    Q.high_a = 0.0; Q.high_v = L.f; Q.high_s = nan;
+    Q.cref_a = 0.0; Q.cref_v = L.f;
+    Q.safe_a = 0.0; Q.safe_v = L.f;
+    Q.last_dec = __longlong_as_double(0x7ff0000000000000LL);
+    Q.pad0 = 1;  // slope0 = u.gx is reduced with the first trial evaluation
+    ls_propose(Q);
+    r.ll[lane] = L;
+  }
+}
+
+// grid: slots_ub x ceil(nd / (2 kCombBlock)); two elements per thread.
+__global__ __launch_bounds__(kCombBlock) void lbfgs_gram_comb_kernel(RelaxDev r, LbfgsCfg c,
+                                                                   int n_ctiles) {
+  const int slot = blockIdx.x / n_ctiles;
+  const int tile = blockIdx.x % n_ctiles;
+  if (slot >= *r.n_active) return;
+  const int lane = r.active[slot];
+  const int2 di = r.dir[lane];
+  if (di.x == 0) return;
+  const int i0 = tile * 2 * kCombBlock + (int)threadIdx.x;
+  if (i0 >= r.nd) return;
+  const bool two = i0 + kCombBlock < r.nd;
+  const int m = c.memory;
+  const int count = di.y;
+  const int nvalid = count < m ? count : m;
+  const int first = (count % m + m - nvalid) % m;
+  const GramView G = gram_view(r, m, lane);
+  const double a = r.ll[lane].ls.a;
+  const double rs = c.rescales[r.ll[lane].pass_idx];
+  const size_t stride = r.stride;
+  const size_t off = (size_t)lane * stride;
+  const double* S = r.S + (size_t)lane * m * stride + i0;
+  const double* Y = r.Y + (size_t)lane * m * stride + i0;
+  const double gamma = G.misc[1];
+  const double g0 = r.g[off + i0];
+  const double g1 = two ? r.g[off + i0 + kCombBlock] : 0.0;
+  double q0 = gamma * g0;
+  double q1 = gamma * g1;
+#pragma unroll 4
+  for (int t = 0; t < nvalid; ++t) {
+    const int sl = (first + t) % m;
+    const double cy = G.cy[sl];
+    const double cs = G.cs[sl];
+    const double* Ys = Y + (size_t)sl * stride;
+    const double* Ss = S + (size_t)sl * stride;
+    const double y0 = Ys[0];
+    const double s0 = Ss[0];
+    const double y1 = two ? Ys[kCombBlock] : 0.0;
+    const double s1 = two ? Ss[kCombBlock] : 0.0;
+    q0 = q0 + cy * y0;
+    q0 = q0 + cs * s0;
+    q1 = q1 + cy * y1;
+    q1 = q1 + cs * s1;
+  }
+#pragma unroll
+  for (int h = 0; h < 2; ++h) {
+    if (h == 1 && !two) break;
+    const size_t i = off + i0 + h * kCombBlock;
+    const double ui = -(h == 0 ? q0 : q1);
+    const double xi = r.x[i];
+    r.u[i] = ui;
+    r.px[i] = xi;
+    r.pg[i] = h == 0 ? g0 : g1;
+    r.s_eval[i] = (xi + a * ui) / rs;
+  }
+}
+
+void launch_lbfgs_gram(const RelaxDev& r, const LbfgsCfg& c, int slots_ub, int part,
+                       hipStream_t st) {
+  if (slots_ub <= 0) return;
+  if (part == 1) {
+    const int n_groups = (c.memory + kGramSG - 1) / kGramSG;
+    hipLaunchKernelGGL(lbfgs_gram_dot_kernel, dim3(slots_ub * n_groups), dim3(kGramBlock), 0, st,
+                       r, c);
+  } else if (part == 2) {
+    hipLaunchKernelGGL(lbfgs_gram_solve_kernel, dim3(slots_ub), dim3(kSolveBlock), 0, st, r, c);
+  } else {
+    const int n_ctiles = (r.nd + 2 * kCombBlock - 1) / (2 * kCombBlock);
+    hipLaunchKernelGGL(lbfgs_gram_comb_kernel, dim3(slots_ub * n_ctiles), dim3(kCombBlock), 0, st,
+                       r, c, n_ctiles);
   }
 }
 

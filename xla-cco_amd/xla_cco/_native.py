@@ -131,6 +131,7 @@ SIGNATURES = {
     "cco_relax_stats": (C.c_int, [_P, _PI64, _PI64]),
     "cco_relax_counters": (C.c_int, [_P, _PI64]),
     "cco_relax_set_option": (C.c_int, [_P, C.c_int, C.c_int64]),
+    "cco_relax_get_option": (C.c_int, [_P, C.c_int, _PI64]),
     "cco_flow_create": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int, _P, _PD, C.POINTER(_P)]),
     "cco_flow_init": (C.c_int, [_P, _PD, _PD]),
     "cco_flow_result": (C.c_int, [_P, _PD, _PD, _PD, _PD, _PD, _PI64, _PI64, _PI64]),
