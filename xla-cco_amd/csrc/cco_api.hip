@@ -164,9 +164,10 @@ int build_geometry(cco_energy* E, const int64_t* n, const double* w, int M, int 
   std::stable_sort(unit_tab.begin(), unit_tab.end(),
                    [](const int4& a, const int4& b) { return (a.y & 0xff) > (b.y & 0xff); });
   g.n_units = (int)unit_tab.size();
-  g.n_utiles = (g.n_units + 255) / 256;
+  g.n_utiles = (g.n_units + kRhoUnitsMax - 1) / kRhoUnitsMax;
   const int per_tile = (g.n_units + g.n_utiles - 1) / g.n_utiles;
   g.UT = ((per_tile + kWave - 1) / kWave) * kWave;
+  g.rho_halves = g.n_utiles == 1 ? 2 : 1;
   std::vector<int> kbox(M);
   std::vector<double> wbox(g.Mbox, 0.0);
   for (int k = 0; k < M; ++k) {

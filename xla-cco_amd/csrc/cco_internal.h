@@ -26,6 +26,7 @@ namespace cco {
 constexpr int kWave = 64;
 constexpr int kRowSeg = 8;       // rows of one rho unit (accumulated in registers)
 constexpr int kRhoChunkMax = 64; // particles staged in LDS per rho chunk (upper bound)
+constexpr int kRhoUnitsMax = 256;  // rho units per workgroup (two threads per unit)
 constexpr int kForceTile = 64;   // particles per force workgroup (one wave per k part)
 constexpr int kForceParts = 4;   // k parts per force workgroup (waves sharing 64 particles)
 constexpr int kReduceTile = 256; // box entries (and k rows) per reduce workgroup
@@ -44,6 +45,7 @@ struct Geo {
   int Q;        // row-run seeds per particle: z1^(q kRowSeg), q < Q
   int rho_chunk;  // particles staged in LDS per rho chunk (16, 32 or 64)
   int n_rtiles;   // reduce workgroups per lane (kReduceTile box / k entries each)
+  int rho_halves; // threads per rho unit (1 or 2)
   int n_u;      // force traversal: u = 0 .. n_u-1 (u = |n2|)
   int fparts;   // force k parts in use (1 .. kForceParts)
   int2 fpart[kForceParts];  // per part: u range [x, y), balanced by rows

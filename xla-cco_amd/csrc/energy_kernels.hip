@@ -11,10 +11,12 @@
 // (k, j) pair costs one complex multiply and one complex add (rho) or one
 // complex multiply and three FMAs (forces); no trig runs per pair.
 //
-// rho_kernel : one thread per unit (a run of <= kRowSeg rows of one column,
-//              seeded by z1^(q kRowSeg) z2^n2 z3^n3 from LDS tables); loops
-//              over a split of particles, accumulating the run in registers;
-//              the phase tables of a chunk of particles sit in LDS.
+// rho_kernel : one or two threads per unit (a run of <= kRowSeg rows of one
+//              column, seeded by z1^(q kRowSeg) z2^n2 z3^n3 from LDS tables),
+//              each taking its share of the particle pairs of a split and
+//              accumulating the run in registers; the phase tables of a chunk
+//              of particles sit in LDS.  Two threads per unit when all of a
+//              lane's units fit one workgroup (more waves per LDS table).
 // reduce_kernel : per (lane, tile of box entries), sums the particle splits
 //              in a fixed order, forms C = w rho on the box and the tile's
 //              part of Phi = sum_k w_k |rho_k|^2 / V (summed over tiles in
@@ -29,8 +31,11 @@
 
 namespace cco {
 
-template <int D>
-__global__ __launch_bounds__(256) void rho_kernel(Geo g, Batch b) {
+// grid: slots_ub x n_utiles x n_splits; block: H UT threads.  Thread
+// (h, u) runs unit u over the particle pairs {2Hi + 2h, 2Hi + 2h + 1} of each
+// LDS chunk; with H = 2, half 1 hands its sums to half 0 through LDS.
+template <int D, int H>
+__global__ __launch_bounds__(H * kRhoUnitsMax) void rho_kernel(Geo g, Batch b) {
   extern __shared__ double2 lds[];
   const int tiles_per_lane = g.n_utiles * b.n_splits;
   const int slot = blockIdx.x / tiles_per_lane;
@@ -39,7 +44,9 @@ __global__ __launch_bounds__(256) void rho_kernel(Geo g, Batch b) {
   const int split = t % b.n_splits;
   const int utile = t / b.n_splits;
   const int lane = b.active[slot];
-  const int u = utile * g.UT + (int)threadIdx.x;
+  const int h = H == 1 ? 0 : (int)threadIdx.x / g.UT;
+  const int ul = (int)threadIdx.x - h * g.UT;
+  const int u = utile * g.UT + ul;
   const bool uvalid = u < g.n_units;
   const int4 un = uvalid ? g.unit[u] : make_int4(0, 0, 0, 0);
   const int nrows = un.y & 0xff;
@@ -96,25 +103,27 @@ __global__ __launch_bounds__(256) void rho_kernel(Geo g, Batch b) {
     }
     __syncthreads();
     if (uvalid) {
-      int jj = 0;
-      // two particles per pass: two independent recurrence chains
-      for (; jj + 1 < nj; jj += 2) {
+      for (int jb = 2 * h; jb < nj; jb += 2 * H) {
+        const bool two = jb + 1 < nj;
+        const int jj = jb;
+        const int jk = two ? jb + 1 : jb;
         double2 e0 = LZR[jj * g.Q + q];
-        double2 e1 = LZR[(jj + 1) * g.Q + q];
+        double2 e1 = LZR[jk * g.Q + q];
         if (D >= 2) {
           const double2 t0 = LT2[jj * w2 + a2];
-          const double2 t1 = LT2[(jj + 1) * w2 + a2];
+          const double2 t1 = LT2[jk * w2 + a2];
           e0 = cmul(e0, make_double2(t0.x, sg2 * t0.y));
           e1 = cmul(e1, make_double2(t1.x, sg2 * t1.y));
         }
         if (D >= 3) {
           const double2 t0 = LT3[jj * w3 + a3];
-          const double2 t1 = LT3[(jj + 1) * w3 + a3];
+          const double2 t1 = LT3[jk * w3 + a3];
           e0 = cmul(e0, make_double2(t0.x, sg3 * t0.y));
           e1 = cmul(e1, make_double2(t1.x, sg3 * t1.y));
         }
+        if (!two) e1 = make_double2(0.0, 0.0);
         const double2 z0 = Lz1[jj];
-        const double2 z1 = Lz1[jj + 1];
+        const double2 z1 = Lz1[jk];
 #pragma unroll
         for (int r = 0; r < kRowSeg; ++r) {
           if (r < nrows) {
@@ -125,34 +134,29 @@ __global__ __launch_bounds__(256) void rho_kernel(Geo g, Batch b) {
           }
         }
       }
-      if (jj < nj) {
-        double2 e0 = LZR[jj * g.Q + q];
-        if (D >= 2) {
-          const double2 t0 = LT2[jj * w2 + a2];
-          e0 = cmul(e0, make_double2(t0.x, sg2 * t0.y));
-        }
-        if (D >= 3) {
-          const double2 t0 = LT3[jj * w3 + a3];
-          e0 = cmul(e0, make_double2(t0.x, sg3 * t0.y));
-        }
-        const double2 z0 = Lz1[jj];
-#pragma unroll
-        for (int r = 0; r < kRowSeg; ++r) {
-          if (r < nrows) {
-            acc[r].x += e0.x;
-            acc[r].y += e0.y;
-            e0 = cmul(e0, z0);
-          }
-        }
-      }
     }
     __syncthreads();
   }
-  if (uvalid) {
+  // half 1 -> LDS -> half 0 (the tables are no longer read)
+  double2* hand = lds;
+  if (H == 2 && h == 1 && uvalid) {
+#pragma unroll
+    for (int r = 0; r < kRowSeg; ++r)
+      if (r < nrows) hand[r * g.UT + ul] = acc[r];
+  }
+  if (H == 2) __syncthreads();
+  if (h == 0 && uvalid) {
     double2* out = b.part + ((size_t)lane * b.n_splits + split) * g.Mbox + un.x;
 #pragma unroll
     for (int r = 0; r < kRowSeg; ++r) {
-      if (r < nrows) out[r] = acc[r];
+      if (r < nrows) {
+        if (H == 2) {
+          const double2 o = hand[r * g.UT + ul];
+          out[r] = make_double2(acc[r].x + o.x, acc[r].y + o.y);
+        } else {
+          out[r] = acc[r];
+        }
+      }
     }
   }
 }
@@ -319,7 +323,9 @@ __global__ __launch_bounds__(kForceTile * kForceParts) void force_kernel(Geo g, 
 }
 
 size_t rho_lds_bytes(const Geo& g) {
-  return sizeof(double2) * (size_t)g.rho_chunk * (1 + g.Q + (g.P2 + 1) + (g.P3 + 1));
+  const size_t tables = (size_t)g.rho_chunk * (1 + g.Q + (g.P2 + 1) + (g.P3 + 1));
+  const size_t hand = g.rho_halves == 2 ? (size_t)kRowSeg * g.UT : 0;  // half-1 sums
+  return sizeof(double2) * (tables > hand ? tables : hand);
 }
 
 // Launch one stage (0 = rho, 1 = reduce, 2 = force) of an evaluation.
@@ -327,8 +333,12 @@ template <int D>
 static void launch_stage(int stage, const Geo& g, const Batch& b, int slots_ub, hipStream_t st) {
   if (stage == 0) {
     const int rho_tiles = g.n_utiles * b.n_splits;
-    hipLaunchKernelGGL(rho_kernel<D>, dim3(slots_ub * rho_tiles), dim3(g.UT), rho_lds_bytes(g),
-                       st, g, b);
+    if (g.rho_halves == 2)
+      hipLaunchKernelGGL((rho_kernel<D, 2>), dim3(slots_ub * rho_tiles), dim3(2 * g.UT),
+                         rho_lds_bytes(g), st, g, b);
+    else
+      hipLaunchKernelGGL((rho_kernel<D, 1>), dim3(slots_ub * rho_tiles), dim3(g.UT),
+                         rho_lds_bytes(g), st, g, b);
   } else if (stage == 1) {
     hipLaunchKernelGGL(reduce_kernel, dim3(slots_ub * g.n_rtiles), dim3(kReduceTile), 0, st, g, b);
   } else {
