@@ -111,11 +111,39 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
 }
 __device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
 
-// Deterministic sum over a workgroup of up to 1024 threads (fixed tree order).
-__device__ __forceinline__ double wave_sum(double v) {
-  for (int off = kWave / 2; off > 0; off >>= 1) v += __shfl_xor(v, off, kWave);
-  return v;
+// One DPP move of a double (two 32-bit halves); lanes whose source is outside
+// the row, or whose row is masked off, read 0.
+template <int CTRL, int ROW_MASK, bool BOUND_ZERO>
+__device__ __forceinline__ double dpp_d(double v) {
+  const long long bits = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(bits & 0xffffffffLL), CTRL, ROW_MASK, 0xf,
+                                             BOUND_ZERO);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(bits >> 32), CTRL, ROW_MASK, 0xf,
+                                             BOUND_ZERO);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
 }
+
+__device__ __forceinline__ double readlane_d(double v, int l) {
+  const long long bits = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_readlane((int)(bits & 0xffffffffLL), l);
+  const int hi = __builtin_amdgcn_readlane((int)(bits >> 32), l);
+  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
+}
+
+// Sum over a full wave (every lane active), returned in every lane.  A scan
+// within each row of 16 lanes (row_shr 1, 2, 4, 8), then the row totals
+// carried by row_bcast15 / row_bcast31 into lane 63; fixed order.
+__device__ __forceinline__ double wave_sum(double v) {
+  v += dpp_d<0x111, 0xf, true>(v);   // row_shr:1
+  v += dpp_d<0x112, 0xf, true>(v);   // row_shr:2
+  v += dpp_d<0x114, 0xf, true>(v);   // row_shr:4
+  v += dpp_d<0x118, 0xf, true>(v);   // row_shr:8
+  v += dpp_d<0x142, 0xa, false>(v);  // row_bcast:15 into rows 1 and 3
+  v += dpp_d<0x143, 0xc, false>(v);  // row_bcast:31 into rows 2 and 3
+  return readlane_d(v, kWave - 1);
+}
+
+// Deterministic sum over a workgroup of up to 1024 threads (fixed tree order).
 
 template <int BLOCK>
 __device__ __forceinline__ double block_sum(double v, double* lds) {
@@ -126,14 +154,14 @@ __device__ __forceinline__ double block_sum(double v, double* lds) {
   __syncthreads();
   if (lid == 0) lds[wid] = v;
   __syncthreads();
-  double t = 0.0;
-  if (nw == 1) {
-    t = lds[0];
-  } else {
+  double t[nw];
 #pragma unroll
-    for (int i = 0; i < nw; ++i) t += lds[i];
-  }
-  return t;
+  for (int i = 0; i < nw; ++i) t[i] = lds[i];
+#pragma unroll
+  for (int w = 1; w < nw; w *= 2)
+#pragma unroll
+    for (int i = 0; i + w < nw; i += 2 * w) t[i] += t[i + w];
+  return t[0];
 }
 
 }  // namespace cco

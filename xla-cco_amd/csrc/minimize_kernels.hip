@@ -600,10 +600,14 @@ __device__ __forceinline__ double reg_sum(double v, Red2& R) {
   R.parity ^= 1;
   if ((threadIdx.x % kWave) == 0) b[threadIdx.x / kWave] = v;
   __syncthreads();
-  double t = 0.0;
+  double t[kRegWaves];
 #pragma unroll
-  for (int i = 0; i < kRegWaves; ++i) t += b[i];
-  return t;
+  for (int i = 0; i < kRegWaves; ++i) t[i] = b[i];
+#pragma unroll
+  for (int w = 1; w < kRegWaves; w *= 2)
+#pragma unroll
+    for (int i = 0; i + w < kRegWaves; i += 2 * w) t[i] += t[i + w];
+  return t[0];
 }
 
 __device__ __forceinline__ void reg_sum2(double& v0, double& v1, Red2& R) {
@@ -616,14 +620,21 @@ __device__ __forceinline__ void reg_sum2(double& v0, double& v1, Red2& R) {
     b[2 * (threadIdx.x / kWave) + 1] = v1;
   }
   __syncthreads();
-  double t0 = 0.0, t1 = 0.0;
+  double t0[kRegWaves], t1[kRegWaves];
 #pragma unroll
   for (int i = 0; i < kRegWaves; ++i) {
-    t0 += b[2 * i];
-    t1 += b[2 * i + 1];
+    t0[i] = b[2 * i];
+    t1[i] = b[2 * i + 1];
   }
-  v0 = t0;
-  v1 = t1;
+#pragma unroll
+  for (int w = 1; w < kRegWaves; w *= 2)
+#pragma unroll
+    for (int i = 0; i + w < kRegWaves; i += 2 * w) {
+      t0[i] += t0[i + w];
+      t1[i] += t1[i + w];
+    }
+  v0 = t0[0];
+  v1 = t1[0];
 }
 
 template <int EPT>
@@ -1062,12 +1073,6 @@ __device__ __forceinline__ GramView gram_view(const RelaxDev& r, int m, int lane
   return v;
 }
 
-__device__ __forceinline__ double readlane_d(double v, int l) {
-  const long long bits = __double_as_longlong(v);
-  const int lo = __builtin_amdgcn_readlane((int)(bits & 0xffffffffLL), l);
-  const int hi = __builtin_amdgcn_readlane((int)(bits >> 32), l);
-  return __longlong_as_double(((long long)hi << 32) | (unsigned int)lo);
-}
 
 // grid: slots_ub x ceil(m / kGramSG); workgroup (lane, group) owns slots
 // group*kGramSG .. +kGramSG-1 and reduces over all of x.
