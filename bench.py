@@ -95,6 +95,9 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="skip per-kernel event timing")
+    ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64",
+                    help="mixed: split-precision MFMA evaluations while E > --switch, FP64 below")
+    ap.add_argument("--switch", type=float, default=1e-8, help="energy below which mixed lanes turn FP64")
     ap.add_argument(
         "--advance-variant",
         type=int,
@@ -112,7 +115,7 @@ def main() -> int:
     if args.gpus != 1:
         print(
             f"bench.py runs on one GPU only (asked for {args.gpus}): lanes are independent, "
-            "so a multi-GPU run is G copies of this one; see DESIGN.md for the sharding plan.",
+            "so a multi-GPU run is G copies of this one with the lanes split across ranks.",
             file=sys.stderr,
         )
         return 2
@@ -137,7 +140,7 @@ def main() -> int:
     box, ks, energy = build_workload(d, n, chi)
     M = int(ks.n.shape[0])
     tol = 1e-20
-    settings = LBFGSSettings(energy_tol=tol)
+    settings = LBFGSSettings(energy_tol=tol, precision=args.precision, precision_switch=args.switch)
     sess = RelaxSession(energy, B, n, d, settings)
     _native.check(L.cco_relax_set_option(sess.handle, 1, args.advance_variant))
     bound = 1 + 3 * (settings.max_evals_per_pass + settings.max_linesearch_steps + 1)
@@ -274,7 +277,7 @@ def main() -> int:
             "advance": {1: "two-loop, global memory", 2: "two-loop, register-resident",
                         3: "Gram form (dot / recursion / direction kernels)"}.get(variant_run),
             "energy_tol": tol,
-            "precision": "fp64",
+            "precision": args.precision if args.precision == "fp64" else f"mixed (FP64 below E={args.switch:g})",
         },
         "converged": f"{converged}/{B * args.steps}",
         "energy_force_evals_per_s": lane_evals / elapsed,

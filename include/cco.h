@@ -148,6 +148,22 @@ int cco_relax_set_option(cco_relax* r, int key, int64_t value);
 /* The value in force: for CCO_OPT_ADVANCE_VARIANT the variant latched by the
  * last cco_relax_init (1, 2 or 3; 0 before the first init). */
 int cco_relax_get_option(cco_relax* r, int key, int64_t* value);
+/* Split-precision evaluations (no reference counterpart; the precision row of
+ * the target design).  mode 1: rho and the forces run as complex GEMMs on the
+ * FP16 matrix cores with every operand split hi + lo (about 22 significant
+ * bits) and FP32 accumulation; the density partials, Phi and every minimizer
+ * and integrator update stay FP64.  L-BFGS lanes start split and turn FP64 for
+ * good at the first iteration whose energy is <= switch_phi (1e-8 by default;
+ * 0 keeps them split); MD lanes stay split.  mode 0: FP64 (the default).
+ * Takes effect at the next init; a k-set that does not fit the MFMA kernels
+ * (more than 48 rows per column) runs FP64. */
+int cco_relax_set_precision(cco_relax* r, int mode, double switch_phi);
+/* Whether the last init enabled split precision, and how many lanes are
+ * still split now (synchronises). */
+int cco_relax_precision_state(cco_relax* r, int* active_mode, int* lanes_split);
+/* Split-precision cco_value_grad_density* calls on this energy (mode 1) or
+ * FP64 (0, default); *applied reports whether the k-set allows it. */
+int cco_energy_set_precision(cco_energy* e, int mode, int* applied);
 
 /* Gradient flow dr/dt = -grad_r Phi in the Cartesian metric of basis
  * (d x d host, rows = lattice vectors), Dormand-Prince 5(4) with error

@@ -1,0 +1,132 @@
+"""Split-precision (FP16 hi/lo MFMA, FP32 accumulation) energy, forces and relax.
+
+The precision row of the target design: the density and force contractions run
+on the matrix cores while a lane's energy is above the switch, FP64 below.
+These tests state the error of every result against the FP64 path and the
+oracle, and check that a mixed relax ends with the FP64 path's success set.
+"""
+
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from xla_cco import kvectors as K
+from xla_cco.box import cubic_box
+from xla_cco.energy import make_stealthy_energy
+from xla_cco.minimize import LBFGSSettings, RelaxSession, relax
+
+pytestmark = pytest.mark.gpu
+
+# Error bounds of the split-precision evaluations (operands carry ~22
+# significant bits, products and sums are FP32): relative energy error, force
+# error relative to max |grad|, density error relative to sqrt(N).
+MIXED_RTOL_PHI = 2e-5
+MIXED_RTOL_GRAD = 2e-4
+MIXED_RTOL_RHO = 2e-5
+
+
+def _system(n: int, d: int, chi: float):
+    box = cubic_box(n, d)
+    ks = K.enumerate_kvectors(box.basis, 0.0, K.k_outer_from_ka(K.ka_from_chi(chi, 0.05, d), 0.05, d))
+    return box, ks
+
+
+def _pair(n: int, d: int, chi: float):
+    box, ks = _system(n, d, chi)
+    e64 = make_stealthy_energy(box, ks)
+    emp = make_stealthy_energy(box, ks)
+    assert emp.set_precision("mixed")
+    return e64, emp, ks, box
+
+
+@pytest.mark.parametrize(
+    "n,d,chi",
+    [(64, 1, 0.3), (256, 2, 0.25), (1000, 2, 0.45), (100, 3, 0.45), (700, 3, 0.3), (257, 3, 0.2)],
+)
+def test_mixed_energy_forces_density_errors(gpu_ctx, n: int, d: int, chi: float) -> None:
+    e64, emp, ks, box = _pair(n, d, chi)
+    s = np.random.default_rng(n + d).uniform(size=(3, n, d))
+    p64, g64, re64, im64 = e64.value_grad_density_batch(s)
+    pmp, gmp, remp, immp = emp.value_grad_density_batch(s)
+    assert np.all(np.abs(pmp - p64) <= MIXED_RTOL_PHI * p64)
+    for i in range(3):
+        assert np.max(np.abs(gmp[i] - g64[i])) <= MIXED_RTOL_GRAD * np.max(np.abs(g64[i]))
+    scale = np.sqrt(n)
+    assert np.max(np.abs(remp - re64)) <= MIXED_RTOL_RHO * scale
+    assert np.max(np.abs(immp - im64)) <= MIXED_RTOL_RHO * scale
+    # and against the oracle directly (first configuration)
+    vg = O.make_value_grad(ks.n, ks.weights, box.volume)
+    phi_o, g_o = vg(s[0])
+    assert abs(pmp[0] - phi_o) <= MIXED_RTOL_PHI * phi_o
+    assert np.max(np.abs(gmp[0] - g_o)) <= MIXED_RTOL_GRAD * np.max(np.abs(g_o))
+
+
+def test_mixed_error_is_absolute_near_ground_state(gpu_ctx) -> None:
+    """The split-precision density error is absolute (|d rho_k| <~ 1e-6 sqrt(N) at
+    any configuration), so close to a ground state the mixed path stops being
+    accurate and lanes switch to FP64.  The forces stay within the bound that
+    error implies, |d g_j| <= pref sum_k w_k |n_k| |d rho_k|: the per-lane
+    power-of-two scale of C keeps the fp16 operands in range however small rho
+    is (a wrong scale would overflow or flush to zero far beyond this bound)."""
+    e64, emp, ks, box = _pair(64, 2, 0.2)
+    s0 = np.random.default_rng(3).uniform(size=(1, 64, 2))
+    res = relax(e64, s0, LBFGSSettings(energy_tol=1e-14))
+    s = res.s
+    p64, g64, re64, im64 = e64.value_grad_density_batch(s)
+    pmp, gmp, remp, immp = emp.value_grad_density_batch(s)
+    assert 0.0 < p64[0] < 1e-12
+    m = ks.n.shape[0]
+    drho = np.hypot(remp - re64, immp - im64)[0, :m]
+    assert np.max(drho) <= 1e-6 * np.sqrt(64)
+    w = ks.weights
+    pref = 4.0 * np.pi / box.volume
+    bound = pref * np.sum(w * np.abs(ks.n).max(axis=1) * drho) * np.sqrt(2.0)
+    assert np.max(np.abs(gmp - g64)) <= 2.0 * bound + 1e-14
+    rho = np.hypot(re64, im64)[0, :m]
+    phi_bound = np.sum(w * (2.0 * rho * drho + drho**2)) / box.volume
+    assert abs(pmp[0] - p64[0]) <= 2.0 * phi_bound + 1e-18
+
+
+def test_mixed_relax_same_success_as_fp64(gpu_ctx) -> None:
+    box, ks = _system(300, 2, 0.3)
+    e = make_stealthy_energy(box, ks)
+    s0 = np.random.default_rng(7).uniform(size=(12, 300, 2))
+    tol = 9.99999968e-21
+    r64 = relax(e, s0, LBFGSSettings(energy_tol=tol))
+    rmp = relax(e, s0, LBFGSSettings(energy_tol=tol, precision="mixed"))
+    assert np.array_equal(r64.energy < tol, rmp.energy < tol)
+    assert np.all(rmp.energy < tol)
+    # converged lanes were verified in FP64 (they switched below 1e-8)
+    assert np.all(e.energy_batch(rmp.s) < 1e-19)
+
+
+def test_mixed_session_switches_every_lane(gpu_ctx) -> None:
+    box, ks = _system(200, 3, 0.2)
+    e = make_stealthy_energy(box, ks)
+    s0 = np.random.default_rng(8).uniform(size=(4, 200, 3))
+    sess = RelaxSession(e, 4, 200, 3, LBFGSSettings(energy_tol=1e-20, precision="mixed"))
+    sess.init(s0)
+    on, split = sess.precision_state()
+    assert on and split == 4
+    sess.run(10_000_000, 64)
+    on, split = sess.precision_state()
+    assert on and split == 0
+    assert np.all(sess.result().energy < 1e-20)
+    sess.close()
+    # switch 0: lanes stay split for the whole run
+    sess = RelaxSession(e, 4, 200, 3, LBFGSSettings(energy_tol=1e-6, precision="mixed", precision_switch=0.0))
+    sess.init(s0)
+    sess.run(10_000_000, 64)
+    assert sess.precision_state() == (True, 4)
+    sess.close()
+
+
+def test_mixed_rejects_bad_settings(gpu_ctx) -> None:
+    box, ks = _system(16, 2, 0.2)
+    e = make_stealthy_energy(box, ks)
+    with pytest.raises(ValueError):
+        e.set_precision("fp32")
+    with pytest.raises(ValueError):
+        RelaxSession(e, 1, 16, 2, LBFGSSettings(precision="half"))

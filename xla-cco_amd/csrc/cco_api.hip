@@ -66,8 +66,10 @@ struct cco_energy {
   int nmax[3];
   DevBuf tables;
   // batch workspace for cco_value_grad_density*
-  DevBuf s, phi, grad, re, im, part, cbox, active, n_active, rphi;
+  DevBuf s, phi, grad, re, im, part, cbox, active, n_active, rphi, prec;
   int ident_B = 0;
+  int precision = 0;  // 1: split-precision MFMA evaluations (when the k-set allows)
+  int prec_B = 0;     // lanes of prec filled with 1
 };
 
 namespace {
@@ -232,16 +234,33 @@ int build_geometry(cco_energy* E, const int64_t* n, const double* w, int M, int 
   while (g.rho_chunk > 16 && rho_lds_bytes(g) > 64 * 1024) g.rho_chunk /= 2;
   if (rho_lds_bytes(g) > 64 * 1024)
     return fail(CCO_ERR_ARG, "k-set too wide for the rho kernel's LDS tables (max |n| too large)");
+  // split-precision path: per-column table, padded rows, weight range
+  // padded to whole 64-column tiles with empty columns (L = 0)
+  std::vector<int4> gcol_tab(((cols.size() + 63) / 64) * 64, make_int4(0, 0, 0, 0));
+  for (size_t c = 0; c < cols.size(); ++c)
+    gcol_tab[c] = make_int4(cols[c].off, cols[c].L, cols[c].n2, cols[c].n3);
+  g.RP = ((P1 + 1 + 15) / 16) * 16;
+  g.n_mp_ctiles = (g.Cn + 63) / 64;
+  g.wmax = 0.0;
+  g.wmin = 0.0;
+  for (double wv : wbox) {
+    if (wv > g.wmax) g.wmax = wv;
+    if (wv > 0.0 && (g.wmin == 0.0 || wv < g.wmin)) g.wmin = wv;
+  }
+  g.mp_ok = (g.RP <= 48 && g.wmin > 0.0 && mp_force_batch_for(g) >= 1 &&
+              mp_lds_bytes(g) <= 64 * 1024) ? 1 : 0;
   // one device allocation holding every table
   auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
   const size_t o_un = 0;
-  const size_t o_kb = align(o_un + sizeof(int4) * unit_tab.size());
+  const size_t o_gc = align(o_un + sizeof(int4) * unit_tab.size());
+  const size_t o_kb = align(o_gc + sizeof(int4) * gcol_tab.size());
   const size_t o_wb = align(o_kb + sizeof(int) * kbox.size());
   const size_t o_us = align(o_wb + sizeof(double) * wbox.size());
   const size_t o_gm = align(o_us + sizeof(int2) * ustart.size());
   const size_t total = align(o_gm + sizeof(int4) * std::max<size_t>(gmem.size(), 1));
   std::vector<char> host(total, 0);
   std::memcpy(host.data() + o_un, unit_tab.data(), sizeof(int4) * unit_tab.size());
+  std::memcpy(host.data() + o_gc, gcol_tab.data(), sizeof(int4) * gcol_tab.size());
   std::memcpy(host.data() + o_kb, kbox.data(), sizeof(int) * kbox.size());
   std::memcpy(host.data() + o_wb, wbox.data(), sizeof(double) * wbox.size());
   std::memcpy(host.data() + o_us, ustart.data(), sizeof(int2) * ustart.size());
@@ -251,11 +270,17 @@ int build_geometry(cco_energy* E, const int64_t* n, const double* w, int M, int 
   CCO_HIP(hipMemcpy(E->tables.p, host.data(), total, hipMemcpyHostToDevice));
   char* base = (char*)E->tables.p;
   g.unit = (const int4*)(base + o_un);
+  g.gcol = (const int4*)(base + o_gc);
   g.kbox = (const int*)(base + o_kb);
   g.wbox = (const double*)(base + o_wb);
   g.ustart = (const int2*)(base + o_us);
   g.gmem = (const int4*)(base + o_gm);
   return CCO_OK;
+}
+
+__global__ void fill_int_kernel(int* a, int n, int v) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) a[i] = v;
 }
 
 __global__ void iota_kernel(int* a, int n, int* count) {
@@ -297,6 +322,16 @@ int energy_batch(cco_energy* E, const double* s_dev, int B, int N, double* phi_d
   b.part = (double2*)E->part.p;
   b.cbox = (double2*)E->cbox.p;
   b.phi_part = (double*)E->rphi.p;
+  b.prec = nullptr;
+  if (E->precision == 1 && g.mp_ok) {
+    if ((rc = dev_reserve(E->prec, sizeof(int) * (size_t)B))) return rc;
+    if (E->prec_B < B) {
+      hipLaunchKernelGGL(fill_int_kernel, dim3((B + 255) / 256), dim3(256), 0, st, (int*)E->prec.p,
+                         B, 1);
+      E->prec_B = B;
+    }
+    b.prec = (const int*)E->prec.p;
+  }
   launch_energy(g, b, B, grad_dev != nullptr, st);
   CCO_HIP(hipGetLastError());
   return CCO_OK;
@@ -336,6 +371,8 @@ struct cco_relax {
   bool profile = false;
   int advance_variant = 0;  // requested: 0 auto, 1 global-memory, 2 register-resident, 3 Gram
   int variant_run = 0;      // the variant latched by the last init (never 0)
+  int precision = 0;        // requested: 1 = split-precision evaluations
+  int* prec_store = nullptr;  // [B] per-lane precision flags
   std::vector<hipEvent_t> evs;
   double stage_ms[kStages] = {};
   int64_t stage_n[kStages] = {};
@@ -380,6 +417,7 @@ __global__ void scatter_eval_kernel(const double* __restrict__ phi_p,
 __global__ void relax_init_kernel(RelaxDev r, int method) {
   const int lane = blockIdx.x * blockDim.x + threadIdx.x;
   if (lane >= r.B) return;
+  if (r.prec != nullptr) r.prec[lane] = 1;
   r.status[lane] = ST_RUNNING;
   r.active[lane] = lane;
   if (lane == 0) *r.n_active = r.B;
@@ -422,6 +460,7 @@ Batch relax_batch(cco_relax* R) {
   b.part = (double2*)R->part.p;
   b.cbox = (double2*)R->cbox.p;
   b.phi_part = (double*)R->rphi.p;
+  b.prec = R->dev.prec;
   return b;
 }
 
@@ -471,6 +510,14 @@ int step_stages(const cco_relax* R, int* out) {
 int resolve_variant(const cco_relax* R) {
   if (R->advance_variant != 0) return R->advance_variant;
   return lbfgs_register_ept(R->dev.nd) ? 2 : 1;
+}
+
+// Per-lane precision flags for the run being initialised: split precision
+// when asked for, the k-set fits the MFMA kernels and the method uses it.
+int* resolve_prec(const cco_relax* R) {
+  if (R->precision != 1 || R->E == nullptr || !R->E->geo.mp_ok) return nullptr;
+  if (R->method != CCO_METHOD_LBFGS && R->method != METHOD_MD) return nullptr;
+  return R->prec_store;
 }
 
 int relax_advance(cco_relax* R) {
@@ -605,7 +652,7 @@ int cco_energy_destroy(cco_energy* E) {
   DevScope ds(E->ctx->device);
   (void)hipStreamSynchronize(E->ctx->stream);
   for (DevBuf* b : {&E->tables, &E->s, &E->phi, &E->grad, &E->re, &E->im, &E->part, &E->cbox,
-                    &E->active, &E->n_active, &E->rphi})
+                    &E->active, &E->n_active, &E->rphi, &E->prec})
     dev_release(*b);
   delete E;
   return CCO_OK;
@@ -728,6 +775,7 @@ int cco_relax_create(cco_ctx* ctx, cco_energy* E, int B, int N, int d, int metho
     R->lc.max_linesearch_steps = s.max_linesearch_steps;
     R->lc.record_every = s.record_every;
     R->lc.record_cap = (s.record_every > 0) ? s.max_records : 0;
+    R->lc.mixed_switch = 1e-8;
     cap = R->lc.record_cap;
   }
   RelaxDev& v = R->dev;
@@ -740,6 +788,7 @@ int cco_relax_create(cco_ctx* ctx, cco_energy* E, int B, int N, int d, int metho
     if (rc == 0) rc = alloc_into(R, p, cnt);
   };
   A(v.status, B);
+  A(R->prec_store, B);
   A(v.active, B);
   A(v.n_active, 1);
   A(v.s_eval, vec);
@@ -810,6 +859,7 @@ int cco_relax_init_dev(cco_relax* R, const double* s0_dev) {
   CCO_HIP(hipMemcpyAsync(R->dev.s, s0_dev, bytes, hipMemcpyDeviceToDevice, st));
   CCO_HIP(hipMemcpyAsync(R->dev.s_eval, s0_dev, bytes, hipMemcpyDeviceToDevice, st));
   R->variant_run = resolve_variant(R);
+  R->dev.prec = resolve_prec(R);
   hipLaunchKernelGGL(relax_init_kernel, dim3((R->B + 255) / 256), dim3(256), 0, st, R->dev,
                      R->method);
   CCO_HIP(hipGetLastError());
@@ -829,6 +879,7 @@ int cco_relax_init(cco_relax* R, const double* s0_host) {
   CCO_HIP(hipMemcpyAsync(R->dev.s, s0_host, bytes, hipMemcpyHostToDevice, st));
   CCO_HIP(hipMemcpyAsync(R->dev.s_eval, R->dev.s, bytes, hipMemcpyDeviceToDevice, st));
   R->variant_run = resolve_variant(R);
+  R->dev.prec = resolve_prec(R);
   hipLaunchKernelGGL(relax_init_kernel, dim3((R->B + 255) / 256), dim3(256), 0, st, R->dev,
                      R->method);
   CCO_HIP(hipGetLastError());
@@ -886,6 +937,39 @@ int cco_relax_step(cco_relax* R, int64_t n_evals, int* n_active) {
       ++p;  // step separator
     }
   }
+  return CCO_OK;
+}
+
+int cco_relax_set_precision(cco_relax* R, int mode, double switch_phi) {
+  if (!R) return fail(CCO_ERR_ARG, "relax is NULL");
+  if (mode != 0 && mode != 1) return fail(CCO_ERR_ARG, "precision mode must be 0 (FP64) or 1 (split)");
+  if (mode == 1 && R->method != CCO_METHOD_LBFGS && R->method != METHOD_MD)
+    return fail(CCO_ERR_ARG, "split precision is available for L-BFGS and MD sessions");
+  if (!(switch_phi >= 0.0)) return fail(CCO_ERR_ARG, "switch_phi must be >= 0");
+  R->precision = mode;
+  R->lc.mixed_switch = switch_phi;
+  return CCO_OK;
+}
+
+int cco_relax_precision_state(cco_relax* R, int* active_mode, int* lanes_split) {
+  if (!R || !active_mode || !lanes_split) return fail(CCO_ERR_ARG, "NULL argument");
+  DevScope ds(R->ctx->device);
+  *active_mode = R->dev.prec != nullptr ? 1 : 0;
+  *lanes_split = 0;
+  if (R->dev.prec == nullptr) return CCO_OK;
+  std::vector<int> h(R->B);
+  CCO_HIP(hipMemcpyAsync(h.data(), R->dev.prec, sizeof(int) * R->B, hipMemcpyDeviceToHost,
+                         R->ctx->stream));
+  CCO_HIP(hipStreamSynchronize(R->ctx->stream));
+  for (int v : h) *lanes_split += v != 0;
+  return CCO_OK;
+}
+
+int cco_energy_set_precision(cco_energy* E, int mode, int* applied) {
+  if (!E) return fail(CCO_ERR_ARG, "energy is NULL");
+  if (mode != 0 && mode != 1) return fail(CCO_ERR_ARG, "precision mode must be 0 (FP64) or 1 (split)");
+  E->precision = mode;
+  if (applied) *applied = (mode == 1 && E->geo.mp_ok) ? 1 : 0;
   return CCO_OK;
 }
 
@@ -1002,6 +1086,7 @@ int cco_energy_profile(cco_energy* E, const double* s_dev, int B, int N, int rep
   b.part = (double2*)E->part.p;
   b.cbox = (double2*)E->cbox.p;
   b.phi_part = (double*)E->rphi.p;
+  b.prec = (E->precision == 1 && g.mp_ok) ? (const int*)E->prec.p : nullptr;
   hipStream_t st = E->ctx->stream;
   std::vector<hipEvent_t> ev(4 * (size_t)reps);
   for (auto& e : ev) CCO_HIP(hipEventCreate(&e));
@@ -1322,6 +1407,7 @@ int cco_flow_create(cco_ctx* ctx, cco_energy* E, int B, int N, int d,
     if (rc == 0) rc = alloc_into(R, p, cnt);
   };
   A(v.status, B);
+  A(R->prec_store, B);
   A(v.active, B);
   A(v.n_active, 1);
   A(v.s_eval, vec);
@@ -1365,6 +1451,7 @@ int cco_flow_init(cco_relax* R, const double* s0_host, const double* dt0_host) {
     R->fc.dt0 = nullptr;
   }
   R->variant_run = resolve_variant(R);
+  R->dev.prec = resolve_prec(R);
   hipLaunchKernelGGL(relax_init_kernel, dim3((R->B + 255) / 256), dim3(256), 0, st, R->dev,
                      R->method);
   CCO_HIP(hipGetLastError());
@@ -1446,6 +1533,7 @@ int cco_md_create(cco_ctx* ctx, cco_energy* E, int B, int N, int d, const double
     if (rc == 0) rc = alloc_into(R, p, cnt);
   };
   A(v.status, B);
+  A(R->prec_store, B);
   A(v.active, B);
   A(v.n_active, 1);
   A(v.s_eval, vec);
@@ -1491,6 +1579,7 @@ int cco_md_init(cco_relax* R, const double* r0, const double* v0, const double* 
   CCO_HIP(hipMemcpyAsync(R->dev.s_eval, s.data(), sizeof(double) * B * nd, hipMemcpyHostToDevice, st));
   CCO_HIP(hipMemcpyAsync(R->dev.ml, L.data(), sizeof(MdLane) * B, hipMemcpyHostToDevice, st));
   R->variant_run = resolve_variant(R);
+  R->dev.prec = resolve_prec(R);
   hipLaunchKernelGGL(relax_init_kernel, dim3((R->B + 255) / 256), dim3(256), 0, st, R->dev,
                      R->method);
   CCO_HIP(hipGetLastError());

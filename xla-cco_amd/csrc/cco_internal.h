@@ -30,6 +30,7 @@ constexpr int kRhoUnitsMax = 256;  // rho units per workgroup (two threads per u
 constexpr int kForceTile = 64;   // particles per force workgroup (one wave per k part)
 constexpr int kForceParts = 4;   // k parts per force workgroup (waves sharing 64 particles)
 constexpr int kReduceTile = 256; // box entries (and k rows) per reduce workgroup
+constexpr int kMpChunk = 64;     // particles per LDS chunk of the split-precision kernels
 
 // Geometry of one k-set, as the kernels read it.  Pointers are device memory.
 struct Geo {
@@ -46,6 +47,12 @@ struct Geo {
   int rho_chunk;  // particles staged in LDS per rho chunk (16, 32 or 64)
   int n_rtiles;   // reduce workgroups per lane (kReduceTile box / k entries each)
   int rho_halves; // threads per rho unit (1 or 2)
+  // split-precision (MFMA) path
+  int mp_ok;      // the k-set fits the MFMA kernels (rows <= 48, LDS <= 64 KB)
+  int RP;         // rows padded to a multiple of 16
+  int n_mp_ctiles;  // 64-column workgroup tiles of the rho MFMA kernel
+  double wmax, wmin;  // largest and smallest positive box weight (C scaling)
+  const int4* gcol;   // per column (rho order): {box offset, L, n2, n3}
   int n_u;      // force traversal: u = 0 .. n_u-1 (u = |n2|)
   int fparts;   // force k parts in use (1 .. kForceParts)
   int2 fpart[kForceParts];  // per part: u range [x, y), balanced by rows
@@ -75,6 +82,7 @@ struct Batch {
   double2* part;         // rho partials [lane][split][Mbox]
   double2* cbox;         // w * rho per box entry [lane][Mbox]
   double* phi_part;      // energy partials [lane][n_rtiles]
+  const int* prec;       // [lane] 1 = split-precision MFMA path (null: every lane FP64)
 };
 
 void launch_energy(const Geo& g, const Batch& b, int slots_ub, bool with_grad,
@@ -87,6 +95,9 @@ void launch_sk_sum(const double* sk, int n_cfg, int Mk, double* tot, double* tot
 void launch_pair(const double* s, int B, int N, int d, const double* basis, const double* edges,
                  int n_edges, unsigned long long* counts, double* nn, hipStream_t st);
 size_t rho_lds_bytes(const Geo& g);
+size_t mp_lds_bytes(const Geo& g);
+int mp_force_batch_for(const Geo& g);  // 16-column tiles per force batch (0: does not fit)
+void launch_energy_mixed(int stage, const Geo& g, const Batch& b, int slots_ub, hipStream_t st);
 
 // ---------------- small helpers ----------------
 

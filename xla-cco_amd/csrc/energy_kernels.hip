@@ -44,6 +44,7 @@ __global__ __launch_bounds__(H * kRhoUnitsMax) void rho_kernel(Geo g, Batch b) {
   const int split = t % b.n_splits;
   const int utile = t / b.n_splits;
   const int lane = b.active[slot];
+  if (b.prec != nullptr && b.prec[lane]) return;  // split-precision lane
   const int h = H == 1 ? 0 : (int)threadIdx.x / g.UT;
   const int ul = (int)threadIdx.x - h * g.UT;
   const int u = utile * g.UT + ul;
@@ -258,6 +259,7 @@ __global__ __launch_bounds__(kForceTile * kForceParts) void force_kernel(Geo g, 
   if (slot >= *b.n_active) return;
   const int tile = blockIdx.x % b.n_ftiles;
   const int lane = b.active[slot];
+  if (b.prec != nullptr && b.prec[lane]) return;  // split-precision lane
   const int jl = (int)threadIdx.x % kForceTile;
   // wave-uniform: keeps the k walk (and the loads of C) on the scalar unit
   const int part = __builtin_amdgcn_readfirstlane((int)threadIdx.x / kForceTile);
@@ -352,6 +354,7 @@ void launch_energy_stage(int stage, const Geo& g, const Batch& b, int slots_ub, 
   if (g.d == 1) launch_stage<1>(stage, g, b, slots_ub, st);
   if (g.d == 2) launch_stage<2>(stage, g, b, slots_ub, st);
   if (g.d == 3) launch_stage<3>(stage, g, b, slots_ub, st);
+  if (stage != 1) launch_energy_mixed(stage, g, b, slots_ub, st);
 }
 
 void launch_energy(const Geo& g, const Batch& b, int slots_ub, bool with_grad,

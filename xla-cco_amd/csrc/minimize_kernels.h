@@ -35,6 +35,7 @@ struct LbfgsCfg {
   double energy_tol, grad_tol;
   int stagnation_patience, max_linesearch_steps;
   int record_every, record_cap;
+  double mixed_switch;  // split-precision lanes turn FP64 once f <= this (<= 0: never)
 };
 
 // Zoom line-search scalars of one lane (optax 0.2.8 state, restated).
@@ -106,6 +107,7 @@ struct FlowLane {
 struct RelaxDev {
   int B, nd, stride;
   int* status;    // [B]
+  int* prec;      // [B] 1 = split-precision evaluations (null: all FP64)
   int* active;    // [B]
   int* n_active;  // [1]
   double* s_eval; // [B][stride] trial point (energy input, s coordinates)

@@ -574,6 +574,8 @@ __global__ __launch_bounds__(BLOCK) void lbfgs_advance_kernel(RelaxDev r, LbfgsC
     r.ll[lane] = L;
     r.status[lane] = code;
     if constexpr (GRAM) r.dir[lane] = make_int2(started ? 1 : 0, L.opt_count);
+    if (code == ST_RUNNING && r.prec != nullptr && c.mixed_switch > 0.0 && L.f <= c.mixed_switch)
+      r.prec[lane] = 0;
   }
 }
 
@@ -987,6 +989,8 @@ __global__ __launch_bounds__(kRegBlock) void lbfgs_advance_reg_kernel(RelaxDev r
   if (tid == 0) {
     r.ll[lane] = L;
     r.status[lane] = code;
+    if (code == ST_RUNNING && r.prec != nullptr && c.mixed_switch > 0.0 && L.f <= c.mixed_switch)
+      r.prec[lane] = 0;
   }
 }
 

@@ -127,6 +127,24 @@ class StealthyEnergy(AbstractEnergy):
         except Exception:
             pass
 
+    def set_precision(self, precision: str) -> bool:
+        """Select FP64 ("fp64", default) or split-precision ("mixed") evaluations.
+
+        "mixed" runs rho and the forces as complex GEMMs on the FP16 matrix
+        cores with hi/lo operand splits and FP32 accumulation (Phi, the density
+        sums and the outputs stay FP64).  Returns whether the k-set allows it;
+        when it does not, evaluations stay FP64.
+        """
+        if precision not in ("fp64", "mixed"):
+            raise ValueError(f"precision must be 'fp64' or 'mixed', got {precision!r}")
+        applied = C.c_int(0)
+        _native.check(
+            _native.lib().cco_energy_set_precision(
+                self.handle(), 1 if precision == "mixed" else 0, C.byref(applied)
+            )
+        )
+        return bool(applied.value)
+
     # ----- evaluation -----
     def _check(self, s: np.ndarray, batched: bool) -> np.ndarray:
         require_float64(s, "s")

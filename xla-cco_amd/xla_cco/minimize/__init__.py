@@ -18,7 +18,7 @@ minimizer.
 from __future__ import annotations
 
 import ctypes as C
-from typing import Union
+from typing import Tuple, Union
 
 import numpy as np
 
@@ -137,6 +137,23 @@ class RelaxSession:
             )
         )
         self.handle = h
+        if isinstance(settings, LBFGSSettings):
+            if settings.precision not in ("fp64", "mixed"):
+                raise ValueError(f"precision must be 'fp64' or 'mixed', got {settings.precision!r}")
+            if not settings.precision_switch >= 0.0:
+                raise ValueError("precision_switch must be >= 0")
+            if settings.precision == "mixed":
+                _native.check(
+                    _native.lib().cco_relax_set_precision(h, 1, float(settings.precision_switch))
+                )
+
+    def precision_state(self) -> Tuple[bool, int]:
+        """(split precision enabled by the last init, lanes still split now)."""
+        mode, lanes = C.c_int(0), C.c_int(0)
+        _native.check(
+            _native.lib().cco_relax_precision_state(self.handle, C.byref(mode), C.byref(lanes))
+        )
+        return bool(mode.value), int(lanes.value)
 
     def close(self) -> None:
         if self.handle is not None and _native._lib is not None:

@@ -35,6 +35,10 @@ class LBFGSSettings:
     max_linesearch_steps: int = 30
     record_every: int = 0
     max_records: int = 0
+    # split-precision evaluations while the energy is above precision_switch
+    # (an extension: "fp64" keeps the reference's arithmetic)
+    precision: str = "fp64"
+    precision_switch: float = 1e-8
 
 
 def update_stagnation(f_new: float, f_old: float, stag: int) -> int:
