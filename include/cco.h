@@ -108,6 +108,19 @@ typedef struct {
 int cco_relax_create(cco_ctx* ctx, cco_energy* e, int B, int N, int d, int method,
                      const cco_descent_settings* ds, const cco_lbfgs_settings* ls,
                      cco_relax** out);
+/* Uneven batch (the batching row of the target design; no reference
+ * counterpart: the reference relaxes one (N, d, k-set) at a time and its CLI
+ * loops over them, cli/relax.py:339-358).  Lane i relaxes N = lane_n[i]
+ * particles under energies[lane_energy[i]] (its own d, box volume and
+ * k-set); every lane shares the settings.  Positions are packed lane after
+ * lane (N_i d_i doubles each) in cco_relax_init, cco_relax_result and the
+ * trajectory (record_cap rows of N_i d_i per lane).  One launch per kernel
+ * covers every lane; descent and L-BFGS (two-loop variants) only; the
+ * energies must outlive the session. */
+int cco_relax_uneven_create(cco_ctx* ctx, int n_energies, cco_energy* const* energies, int B,
+                            const int32_t* lane_energy, const int32_t* lane_n, int method,
+                            const cco_descent_settings* ds, const cco_lbfgs_settings* ls,
+                            cco_relax** out);
 int cco_relax_destroy(cco_relax* r);
 /* Load starting points (B x N x d) and mark every lane for its first evaluation. */
 int cco_relax_init(cco_relax* r, const double* s0_host);

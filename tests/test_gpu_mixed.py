@@ -130,3 +130,30 @@ def test_mixed_rejects_bad_settings(gpu_ctx) -> None:
         e.set_precision("fp32")
     with pytest.raises(ValueError):
         RelaxSession(e, 1, 16, 2, LBFGSSettings(precision="half"))
+
+
+def test_mixed_md_energy_drift_vs_fp64(gpu_ctx) -> None:
+    """NVE with split-precision forces from a relaxed state at T = 1e-4: the run
+    stays stable and its energy drift is reported against the FP64 run."""
+    import math
+
+    from xla_cco.dynamics import MDSettings, init_state, simulate
+
+    box, ks = _system(400, 3, 0.3)
+    e = make_stealthy_energy(box, ks)
+    s0 = np.random.default_rng(21).uniform(size=(2, 400, 3))
+    res = relax(e, s0, LBFGSSettings(energy_tol=1e-20))
+    r0 = res.s @ box.basis
+    v0 = np.random.default_rng(22).normal(scale=math.sqrt(1e-4), size=r0.shape)
+    out = {}
+    for prec in ("fp64", "mixed"):
+        st = MDSettings(dt=0.01, remove_com=False, drift_tol=math.inf, precision=prec)
+        out[prec] = simulate(init_state(e, box.basis, r0, v0, 1e-4, st), 400, st).state
+    a, m = out["fp64"], out["mixed"]
+    assert np.all(m.step == 400) and np.all(m.status == 0)
+    kin = 0.5 * np.sum(v0**2, axis=(1, 2))
+    # FP64 Verlet drift is O(dt^2); the split-precision forces add a bounded error
+    assert np.all(a.max_drift < 1e-3 * kin)
+    assert np.all(m.max_drift < 1e-2 * kin)
+    # the trajectories stay close over 400 steps
+    assert np.max(np.abs(m.r - a.r)) < 1e-3

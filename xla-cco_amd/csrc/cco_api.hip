@@ -373,6 +373,11 @@ struct cco_relax {
   int variant_run = 0;      // the variant latched by the last init (never 0)
   int precision = 0;        // requested: 1 = split-precision evaluations
   int* prec_store = nullptr;  // [B] per-lane precision flags
+  // uneven batch (per-lane energy and N): layouts and geometry copies on device
+  bool uneven = false;
+  size_t vec_total = 0;     // doubles of one position array (B N d when uniform)
+  DevBuf un_lay, un_geo;
+  int tiles_rho = 0, tiles_reduce = 0, tiles_force = 0, rho_lds = 0, dmask = 0;
   std::vector<hipEvent_t> evs;
   double stage_ms[kStages] = {};
   int64_t stage_n[kStages] = {};
@@ -461,6 +466,15 @@ Batch relax_batch(cco_relax* R) {
   b.cbox = (double2*)R->cbox.p;
   b.phi_part = (double*)R->rphi.p;
   b.prec = R->dev.prec;
+  if (R->uneven) {
+    b.lay = (const LaneLayout*)R->un_lay.p;
+    b.geos = (const Geo*)R->un_geo.p;
+    b.tiles_rho = R->tiles_rho;
+    b.tiles_reduce = R->tiles_reduce;
+    b.tiles_force = R->tiles_force;
+    b.rho_lds = R->rho_lds;
+    b.dmask = R->dmask;
+  }
   return b;
 }
 
@@ -515,7 +529,7 @@ int resolve_variant(const cco_relax* R) {
 // Per-lane precision flags for the run being initialised: split precision
 // when asked for, the k-set fits the MFMA kernels and the method uses it.
 int* resolve_prec(const cco_relax* R) {
-  if (R->precision != 1 || R->E == nullptr || !R->E->geo.mp_ok) return nullptr;
+  if (R->precision != 1 || R->E == nullptr || !R->E->geo.mp_ok || R->uneven) return nullptr;
   if (R->method != CCO_METHOD_LBFGS && R->method != METHOD_MD) return nullptr;
   return R->prec_store;
 }
@@ -720,9 +734,50 @@ int cco_value_grad_density(cco_energy* E, const double* s_host, int B, int N, do
   return CCO_OK;
 }
 
+namespace {
+// Per-lane energies and sizes of an uneven batch.
+struct UnevenSpec {
+  std::vector<cco_energy*> E;
+  std::vector<int> N;
+};
+}  // namespace
+
+static int relax_create_impl(cco_ctx* ctx, cco_energy* E, int B, int N, int d, int method,
+                             const cco_descent_settings* ds_, const cco_lbfgs_settings* ls_,
+                             const UnevenSpec* un, cco_relax** out);
+
 int cco_relax_create(cco_ctx* ctx, cco_energy* E, int B, int N, int d, int method,
                      const cco_descent_settings* ds_, const cco_lbfgs_settings* ls_,
                      cco_relax** out) {
+  return relax_create_impl(ctx, E, B, N, d, method, ds_, ls_, nullptr, out);
+}
+
+int cco_relax_uneven_create(cco_ctx* ctx, int n_energies, cco_energy* const* energies, int B,
+                            const int32_t* lane_energy, const int32_t* lane_n, int method,
+                            const cco_descent_settings* ds_, const cco_lbfgs_settings* ls_,
+                            cco_relax** out) {
+  if (!ctx || !out || !energies || !lane_energy || !lane_n) return fail(CCO_ERR_ARG, "NULL argument");
+  if (n_energies < 1 || B < 1) return fail(CCO_ERR_ARG, "need n_energies >= 1 and B >= 1");
+  for (int e = 0; e < n_energies; ++e) {
+    if (!energies[e]) return fail(CCO_ERR_ARG, "energy is NULL");
+    if (energies[e]->ctx != ctx) return fail(CCO_ERR_ARG, "energies must live on the context");
+  }
+  UnevenSpec un;
+  un.E.resize(B);
+  un.N.resize(B);
+  for (int i = 0; i < B; ++i) {
+    if (lane_energy[i] < 0 || lane_energy[i] >= n_energies)
+      return fail(CCO_ERR_ARG, "lane energy index out of range");
+    if (lane_n[i] < 1) return fail(CCO_ERR_ARG, "every lane needs N >= 1");
+    un.E[i] = energies[lane_energy[i]];
+    un.N[i] = lane_n[i];
+  }
+  return relax_create_impl(ctx, un.E[0], B, un.N[0], un.E[0]->geo.d, method, ds_, ls_, &un, out);
+}
+
+static int relax_create_impl(cco_ctx* ctx, cco_energy* E, int B, int N, int d, int method,
+                             const cco_descent_settings* ds_, const cco_lbfgs_settings* ls_,
+                             const UnevenSpec* un, cco_relax** out) {
   if (!ctx || !out) return fail(CCO_ERR_ARG, "NULL argument");
   if (B < 1 || N < 1) return fail(CCO_ERR_ARG, "need B >= 1 and N >= 1");
   if (d < 1 || d > 3) return fail(CCO_ERR_ARG, "d must be 1, 2 or 3");
@@ -782,7 +837,57 @@ int cco_relax_create(cco_ctx* ctx, cco_energy* E, int B, int N, int d, int metho
   v.B = B;
   v.nd = N * d;
   v.stride = N * d;
-  const size_t vec = (size_t)B * v.stride;
+  size_t vec = (size_t)B * v.stride;
+  // uneven batch: per-lane layouts (vectors packed lane after lane)
+  std::vector<long long> h_loff, h_lhoff;
+  std::vector<int> h_lnd;
+  std::vector<LaneLayout> h_lay;
+  std::vector<Geo> h_geo;
+  size_t part_total = 0, cbox_total = 0, rphi_total = 0;
+  if (un) {
+    R->uneven = true;
+    const int m = method == CCO_METHOD_LBFGS ? ls_->memory : 0;
+    h_loff.resize(B);
+    h_lhoff.resize(B);
+    h_lnd.resize(B);
+    h_lay.resize(B);
+    h_geo.resize(B);
+    long long vt = 0, ht = 0;
+    int maxnd = 0;
+    for (int i = 0; i < B; ++i) {
+      Geo g = un->E[i]->geo;
+      g.UT = kWave;  // uneven launches: 64 units per tile, two threads per unit
+      g.n_utiles = (g.n_units + kWave - 1) / kWave;
+      g.rho_halves = 2;
+      const int nd = un->N[i] * g.d;
+      const int ns = n_splits_for(un->N[i]);
+      h_loff[i] = vt;
+      h_lhoff[i] = ht;
+      h_lnd[i] = nd;
+      h_lay[i].off = vt;
+      h_lay[i].part_off = (long long)part_total;
+      h_lay[i].cbox_off = (long long)cbox_total;
+      h_lay[i].rt_off = (long long)rphi_total;
+      h_lay[i].N = un->N[i];
+      h_lay[i].pad = 0;
+      vt += nd;
+      ht += (long long)m * nd;
+      part_total += (size_t)ns * g.Mbox;
+      cbox_total += (size_t)g.Mbox;
+      rphi_total += (size_t)g.n_rtiles;
+      R->tiles_rho = std::max(R->tiles_rho, g.n_utiles * ns);
+      R->tiles_reduce = std::max(R->tiles_reduce, g.n_rtiles);
+      R->tiles_force = std::max(R->tiles_force, n_ftiles_for(un->N[i]));
+      R->rho_lds = std::max(R->rho_lds, (int)rho_lds_bytes(g));
+      R->dmask |= 1 << g.d;
+      maxnd = std::max(maxnd, nd);
+      h_geo[i] = g;
+    }
+    v.nd = maxnd;
+    v.stride = maxnd;
+    vec = (size_t)vt;
+  }
+  R->vec_total = vec;
   int rc = 0;
   auto A = [&](auto*& p, size_t cnt) {
     if (rc == 0) rc = alloc_into(R, p, cnt);
@@ -821,10 +926,41 @@ int cco_relax_create(cco_ctx* ctx, cco_energy* E, int B, int N, int d, int metho
     A(v.tr_step, (size_t)B * cap);
     A(v.tr_it, (size_t)B * cap);
   }
-  if (rc == 0 && E) {
+  if (rc == 0 && E && !un) {
     rc = dev_reserve(R->part, sizeof(double2) * (size_t)B * n_splits_for(N) * E->geo.Mbox);
     if (rc == 0) rc = dev_reserve(R->cbox, sizeof(double2) * (size_t)B * E->geo.Mbox);
     if (rc == 0) rc = reserve_reduce(R->rphi, B, E->geo);
+  }
+  if (rc == 0 && un) {
+    long long* d_loff = nullptr;
+    long long* d_lhoff = nullptr;
+    int* d_lnd = nullptr;
+    A(d_loff, B);
+    A(d_lhoff, B);
+    A(d_lnd, B);
+    if (rc == 0) rc = dev_reserve(R->part, sizeof(double2) * part_total);
+    if (rc == 0) rc = dev_reserve(R->cbox, sizeof(double2) * cbox_total);
+    if (rc == 0) rc = dev_reserve(R->rphi, sizeof(double) * rphi_total);
+    if (rc == 0) rc = dev_reserve(R->un_lay, sizeof(LaneLayout) * (size_t)B);
+    if (rc == 0) rc = dev_reserve(R->un_geo, sizeof(Geo) * (size_t)B);
+    if (rc == 0) {
+      hipStream_t st = ctx->stream;
+      if (hipMemcpyAsync(d_loff, h_loff.data(), sizeof(long long) * B, hipMemcpyHostToDevice, st) !=
+              hipSuccess ||
+          hipMemcpyAsync(d_lhoff, h_lhoff.data(), sizeof(long long) * B, hipMemcpyHostToDevice,
+                         st) != hipSuccess ||
+          hipMemcpyAsync(d_lnd, h_lnd.data(), sizeof(int) * B, hipMemcpyHostToDevice, st) !=
+              hipSuccess ||
+          hipMemcpyAsync(R->un_lay.p, h_lay.data(), sizeof(LaneLayout) * B, hipMemcpyHostToDevice,
+                         st) != hipSuccess ||
+          hipMemcpyAsync(R->un_geo.p, h_geo.data(), sizeof(Geo) * B, hipMemcpyHostToDevice, st) !=
+              hipSuccess ||
+          hipStreamSynchronize(st) != hipSuccess)
+        rc = fail(CCO_ERR_DEVICE, "uploading the uneven layout failed");
+    }
+    v.loff = d_loff;
+    v.lhoff = d_lhoff;
+    v.lnd = d_lnd;
   }
   if (rc == 0 && hipHostMalloc((void**)&R->host_count, sizeof(int)) != hipSuccess)
     rc = fail(CCO_ERR_DEVICE, "hipHostMalloc failed");
@@ -842,7 +978,7 @@ int cco_relax_destroy(cco_relax* R) {
   (void)hipStreamSynchronize(R->ctx->stream);
   for (DevBuf& b : R->bufs) dev_release(b);
   for (DevBuf* b : {&R->part, &R->cbox, &R->rphi, &R->out_s, &R->out_scalars, &R->pack,
-                    &R->pack_idx})
+                    &R->pack_idx, &R->un_lay, &R->un_geo})
     dev_release(*b);
   if (R->host_count) (void)hipHostFree(R->host_count);
   for (hipEvent_t e : R->evs) (void)hipEventDestroy(e);
@@ -855,7 +991,7 @@ int cco_relax_init_dev(cco_relax* R, const double* s0_dev) {
   if (R->method > CCO_METHOD_LBFGS) return fail(CCO_ERR_STATE, "not a relax session");
   DevScope ds(R->ctx->device);
   hipStream_t st = R->ctx->stream;
-  const size_t bytes = sizeof(double) * (size_t)R->B * R->dev.stride;
+  const size_t bytes = sizeof(double) * R->vec_total;
   CCO_HIP(hipMemcpyAsync(R->dev.s, s0_dev, bytes, hipMemcpyDeviceToDevice, st));
   CCO_HIP(hipMemcpyAsync(R->dev.s_eval, s0_dev, bytes, hipMemcpyDeviceToDevice, st));
   R->variant_run = resolve_variant(R);
@@ -875,7 +1011,7 @@ int cco_relax_init(cco_relax* R, const double* s0_host) {
   if (R->method > CCO_METHOD_LBFGS) return fail(CCO_ERR_STATE, "not a relax session");
   DevScope ds(R->ctx->device);
   hipStream_t st = R->ctx->stream;
-  const size_t bytes = sizeof(double) * (size_t)R->B * R->dev.stride;
+  const size_t bytes = sizeof(double) * R->vec_total;
   CCO_HIP(hipMemcpyAsync(R->dev.s, s0_host, bytes, hipMemcpyHostToDevice, st));
   CCO_HIP(hipMemcpyAsync(R->dev.s_eval, R->dev.s, bytes, hipMemcpyDeviceToDevice, st));
   R->variant_run = resolve_variant(R);
@@ -988,8 +1124,9 @@ int cco_relax_set_option(cco_relax* R, int key, int64_t value) {
     if (value < 0 || value > 3) return fail(CCO_ERR_ARG, "advance variant must be 0, 1, 2 or 3");
     if (value == 2 && lbfgs_register_ept(R->dev.nd) == 0)
       return fail(CCO_ERR_ARG, "register-resident advance needs N*d <= 8192");
-    if (value == 3 && R->dev.gram == nullptr)
-      return fail(CCO_ERR_ARG, "Gram-form advance needs an L-BFGS session with memory <= 128");
+    if (value == 3 && (R->dev.gram == nullptr || R->uneven))
+      return fail(CCO_ERR_ARG,
+                  "Gram-form advance needs a uniform L-BFGS session with memory <= 128");
     R->advance_variant = (int)value;
     return CCO_OK;
   }
@@ -1157,6 +1294,7 @@ int cco_relax_run(cco_relax* R, int64_t max_evals, int sync_every) {
 int cco_relax_pending(cco_relax* R, int* n_active, int32_t* lanes_host, double* s_trial_host) {
   if (!R || !n_active) return fail(CCO_ERR_ARG, "NULL argument");
   if (!R->ready) return fail(CCO_ERR_STATE, "relax not initialised");
+  if (R->uneven) return fail(CCO_ERR_STATE, "uneven batches use the built-in energies");
   DevScope ds(R->ctx->device);
   int rc = relax_sync_count(R, n_active);
   if (rc) return rc;
@@ -1228,7 +1366,7 @@ int cco_relax_result(cco_relax* R, double* s_host, double* energy_host, double* 
       !status_host)
     return fail(CCO_ERR_ARG, "NULL argument");
   DevScope ds(R->ctx->device);
-  const size_t vec = (size_t)R->B * R->dev.stride;
+  const size_t vec = R->vec_total;
   int rc;
   if ((rc = dev_reserve(R->out_s, sizeof(double) * vec))) return rc;
   if ((rc = dev_reserve(R->out_scalars, sizeof(double) * 6 * (size_t)R->B))) return rc;
@@ -1273,7 +1411,7 @@ int cco_relax_trajectory(cco_relax* R, int64_t* count_host, int64_t* iteration_h
     CCO_HIP(hipMemcpyAsync(iteration_host, R->dev.tr_it, sizeof(int64_t) * rows,
                            hipMemcpyDeviceToHost, st));
   if (s_host)
-    CCO_HIP(hipMemcpyAsync(s_host, R->dev.tr_s, sizeof(double) * rows * R->dev.stride,
+    CCO_HIP(hipMemcpyAsync(s_host, R->dev.tr_s, sizeof(double) * cap * R->vec_total,
                            hipMemcpyDeviceToHost, st));
   if (energy_host)
     CCO_HIP(hipMemcpyAsync(energy_host, R->dev.tr_e, sizeof(double) * rows, hipMemcpyDeviceToHost, st));
@@ -1340,6 +1478,7 @@ static int new_session(cco_ctx* ctx, cco_energy* E, int B, int N, int d, int met
   R->dev.B = B;
   R->dev.nd = N * d;
   R->dev.stride = N * d;
+  R->vec_total = (size_t)B * N * d;
   *out = R;
   return CCO_OK;
 }
@@ -1441,7 +1580,7 @@ int cco_flow_init(cco_relax* R, const double* s0_host, const double* dt0_host) {
   if (R->method != METHOD_FLOW) return fail(CCO_ERR_STATE, "not a flow session");
   DevScope ds(R->ctx->device);
   hipStream_t st = R->ctx->stream;
-  const size_t bytes = sizeof(double) * (size_t)R->B * R->dev.stride;
+  const size_t bytes = sizeof(double) * R->vec_total;
   CCO_HIP(hipMemcpyAsync(R->dev.s, s0_host, bytes, hipMemcpyHostToDevice, st));
   CCO_HIP(hipMemcpyAsync(R->dev.s_eval, R->dev.s, bytes, hipMemcpyDeviceToDevice, st));
   if (dt0_host) {

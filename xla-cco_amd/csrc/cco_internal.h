@@ -65,6 +65,16 @@ struct Geo {
   const int4* gmem;      // per group: 4 members {box offset, L, n2, n3}
 };
 
+// Uneven batches: where one lane's data lives (offsets in elements).
+struct LaneLayout {
+  long long off;       // positions / gradients (doubles)
+  long long part_off;  // rho partials (double2)
+  long long cbox_off;  // C on the box (double2)
+  long long rt_off;    // energy partials (doubles)
+  int N;               // particles of the lane
+  int pad;
+};
+
 // Per-launch batch description for the energy kernels.
 struct Batch {
   const int* active;     // slot -> lane
@@ -82,8 +92,57 @@ struct Batch {
   double2* part;         // rho partials [lane][split][Mbox]
   double2* cbox;         // w * rho per box entry [lane][Mbox]
   double* phi_part;      // energy partials [lane][n_rtiles]
-  const int* prec;       // [lane] 1 = split-precision MFMA path (null: every lane FP64)
+  const int* prec = nullptr;  // [lane] 1 = split-precision MFMA path (null: every lane FP64)
+  // uneven batches (null: every lane uses the launch's Geo and the strides above)
+  const LaneLayout* lay = nullptr;  // [lane]
+  const Geo* geos = nullptr;        // [lane] geometry (rho: UT = 64 units, two threads per unit)
+  int tiles_rho = 0, tiles_reduce = 0, tiles_force = 0;  // workgroups per lane (max)
+  int rho_lds = 0;                  // dynamic LDS of the rho kernel (bytes, max over lanes)
+  int dmask = 0;                    // bit d set when some lane has dimension d
 };
+
+// One lane's geometry and buffers inside a launch.
+struct LaneIO {
+  const double* s;
+  double* grad;
+  double2* part;
+  double2* cbox;
+  double* phi_part;
+  double* re;
+  double* im;
+  int N, n_splits, n_ftiles;
+};
+
+// MODE 0: decided by b.lay at run time; 1: uniform batch; 2: uneven batch.
+template <int MODE = 0>
+__device__ __forceinline__ LaneIO lane_io(const Geo& g, const Batch& b, int lane) {
+  LaneIO io;
+  if (MODE == 1 || (MODE == 0 && b.lay == nullptr)) {
+    io.N = b.N;
+    io.n_splits = b.n_splits;
+    io.n_ftiles = b.n_ftiles;
+    io.s = b.s + (size_t)lane * b.lane_stride;
+    io.grad = b.grad != nullptr ? b.grad + (size_t)lane * b.lane_stride : nullptr;
+    io.part = b.part + (size_t)lane * b.n_splits * g.Mbox;
+    io.cbox = b.cbox + (size_t)lane * g.Mbox;
+    io.phi_part = b.phi_part + (size_t)lane * g.n_rtiles;
+    io.re = b.re != nullptr ? b.re + (size_t)lane * g.M_pad : nullptr;
+    io.im = b.im != nullptr ? b.im + (size_t)lane * g.M_pad : nullptr;
+  } else {
+    const LaneLayout L = b.lay[lane];
+    io.N = L.N;
+    io.n_splits = (L.N + b.split_len - 1) / b.split_len;
+    io.n_ftiles = (L.N + kForceTile - 1) / kForceTile;
+    io.s = b.s + L.off;
+    io.grad = b.grad != nullptr ? b.grad + L.off : nullptr;
+    io.part = b.part + L.part_off;
+    io.cbox = b.cbox + L.cbox_off;
+    io.phi_part = b.phi_part + L.rt_off;
+    io.re = nullptr;
+    io.im = nullptr;
+  }
+  return io;
+}
 
 void launch_energy(const Geo& g, const Batch& b, int slots_ub, bool with_grad,
                    hipStream_t stream);

@@ -35,16 +35,20 @@ namespace cco {
 // (h, u) runs unit u over the particle pairs {2Hi + 2h, 2Hi + 2h + 1} of each
 // LDS chunk; with H = 2, half 1 hands its sums to half 0 through LDS.
 template <int D, int H>
-__global__ __launch_bounds__(H * kRhoUnitsMax) void rho_kernel(Geo g, Batch b) {
+__global__ __launch_bounds__(H * kRhoUnitsMax) void rho_kernel(Geo g0, Batch b) {
   extern __shared__ double2 lds[];
-  const int tiles_per_lane = g.n_utiles * b.n_splits;
+  const int tiles_per_lane = b.lay != nullptr ? b.tiles_rho : g0.n_utiles * b.n_splits;
   const int slot = blockIdx.x / tiles_per_lane;
   if (slot >= *b.n_active) return;
   const int t = blockIdx.x % tiles_per_lane;
-  const int split = t % b.n_splits;
-  const int utile = t / b.n_splits;
   const int lane = b.active[slot];
   if (b.prec != nullptr && b.prec[lane]) return;  // split-precision lane
+  const Geo g = b.lay != nullptr ? b.geos[lane] : g0;
+  if (g.d != D) return;  // uneven batches launch once per dimension
+  const LaneIO io = lane_io(g, b, lane);
+  const int split = t % io.n_splits;
+  const int utile = t / io.n_splits;
+  if (utile >= g.n_utiles) return;  // uniform per workgroup, before any barrier
   const int h = H == 1 ? 0 : (int)threadIdx.x / g.UT;
   const int ul = (int)threadIdx.x - h * g.UT;
   const int u = utile * g.UT + ul;
@@ -53,8 +57,8 @@ __global__ __launch_bounds__(H * kRhoUnitsMax) void rho_kernel(Geo g, Batch b) {
   const int nrows = un.y & 0xff;
   const int q = un.y >> 8;
   const int j0 = split * b.split_len;
-  const int j1 = min(b.N, j0 + b.split_len);
-  const double* s = b.s + (size_t)lane * b.lane_stride;
+  const int j1 = min(io.N, j0 + b.split_len);
+  const double* s = io.s;
 
   const int CH = g.rho_chunk;
   const int w2 = g.P2 + 1;
@@ -147,7 +151,7 @@ __global__ __launch_bounds__(H * kRhoUnitsMax) void rho_kernel(Geo g, Batch b) {
   }
   if (H == 2) __syncthreads();
   if (h == 0 && uvalid) {
-    double2* out = b.part + ((size_t)lane * b.n_splits + split) * g.Mbox + un.x;
+    double2* out = io.part + (size_t)split * g.Mbox + un.x;
 #pragma unroll
     for (int r = 0; r < kRowSeg; ++r) {
       if (r < nrows) {
@@ -165,19 +169,23 @@ __global__ __launch_bounds__(H * kRhoUnitsMax) void rho_kernel(Geo g, Batch b) {
 // grid: slots_ub x n_rtiles.  Workgroup t of a lane owns box entries and k
 // rows [t kReduceTile, (t+1) kReduceTile) and writes its part of the energy;
 // the force kernel (or phi_kernel) sums the parts in tile order.
-__global__ __launch_bounds__(kReduceTile) void reduce_kernel(Geo g, Batch b) {
+__global__ __launch_bounds__(kReduceTile) void reduce_kernel(Geo g0, Batch b) {
   __shared__ double red[kReduceTile / kWave];
-  const int slot = blockIdx.x / g.n_rtiles;
+  const int tiles_per_lane = b.lay != nullptr ? b.tiles_reduce : g0.n_rtiles;
+  const int slot = blockIdx.x / tiles_per_lane;
   if (slot >= *b.n_active) return;
-  const int tile = blockIdx.x % g.n_rtiles;
+  const int tile = blockIdx.x % tiles_per_lane;
   const int lane = b.active[slot];
-  const double2* part = b.part + (size_t)lane * b.n_splits * g.Mbox;
-  double2* cbox = b.cbox + (size_t)lane * g.Mbox;
+  const Geo g = b.lay != nullptr ? b.geos[lane] : g0;
+  if (tile >= g.n_rtiles) return;  // uniform per workgroup, before any barrier
+  const LaneIO io = lane_io(g, b, lane);
+  const double2* part = io.part;
+  double2* cbox = io.cbox;
   const int idx = tile * kReduceTile + (int)threadIdx.x;
   double acc = 0.0;
   if (idx < g.Mbox) {
     double2 rho = part[idx];
-    for (int sp = 1; sp < b.n_splits; ++sp) {
+    for (int sp = 1; sp < io.n_splits; ++sp) {
       const double2 v = part[(size_t)sp * g.Mbox + idx];
       rho.x += v.x;
       rho.y += v.y;
@@ -186,14 +194,14 @@ __global__ __launch_bounds__(kReduceTile) void reduce_kernel(Geo g, Batch b) {
     cbox[idx] = make_double2(w * rho.x, w * rho.y);
     acc = w * (rho.x * rho.x + rho.y * rho.y);
   }
-  if (b.re != nullptr && idx < g.M_pad) {
-    double re = (double)b.N;
+  if (io.re != nullptr && idx < g.M_pad) {
+    double re = (double)io.N;
     double im = 0.0;
     if (idx < g.M) {
       const int code = g.kbox[idx];
       const int i = code < 0 ? ~code : code;
       double2 rho = part[i];
-      for (int sp = 1; sp < b.n_splits; ++sp) {
+      for (int sp = 1; sp < io.n_splits; ++sp) {
         const double2 v = part[(size_t)sp * g.Mbox + i];
         rho.x += v.x;
         rho.y += v.y;
@@ -201,27 +209,27 @@ __global__ __launch_bounds__(kReduceTile) void reduce_kernel(Geo g, Batch b) {
       re = rho.x;
       im = code < 0 ? -rho.y : rho.y;
     }
-    b.re[(size_t)lane * g.M_pad + idx] = re;
-    b.im[(size_t)lane * g.M_pad + idx] = im;
+    io.re[idx] = re;
+    io.im[idx] = im;
   }
   const double tot = block_sum<kReduceTile>(acc, red);
-  if (threadIdx.x == 0) b.phi_part[(size_t)lane * g.n_rtiles + tile] = tot;
+  if (threadIdx.x == 0) io.phi_part[tile] = tot;
 }
 
 // Phi of a lane: its reduce tiles' partials summed in tile order.
-__device__ __forceinline__ double phi_from_parts(const Geo& g, const Batch& b, int lane) {
-  const double* pp = b.phi_part + (size_t)lane * g.n_rtiles;
+__device__ __forceinline__ double phi_from_parts(const Geo& g, const LaneIO& io) {
   double phi = 0.0;
-  for (int t = 0; t < g.n_rtiles; ++t) phi += pp[t];
+  for (int t = 0; t < g.n_rtiles; ++t) phi += io.phi_part[t];
   return phi / g.volume;
 }
 
 // Energy-only evaluations: one thread per lane finishes Phi.
-__global__ void phi_kernel(Geo g, Batch b, int slots_ub) {
+__global__ void phi_kernel(Geo g0, Batch b, int slots_ub) {
   const int slot = blockIdx.x * blockDim.x + threadIdx.x;
   if (slot >= slots_ub || slot >= *b.n_active) return;
   const int lane = b.active[slot];
-  b.phi[lane] = phi_from_parts(g, b, lane);
+  const Geo g = b.lay != nullptr ? b.geos[lane] : g0;
+  b.phi[lane] = phi_from_parts(g, lane_io(g, b, lane));
 }
 
 // One column (member) of a force group: with u_r = Im(conj(C_r) e_r) along the
@@ -249,23 +257,18 @@ __device__ __forceinline__ void force_member(const int4 m, const double2 B, cons
   if (D >= 3) g3 += (double)m.w * P;
 }
 
-// grid: slots_ub x n_ftiles; block: kForceTile particles x g.fparts k parts.
-// Wave p walks the u range g.fpart[p]; the parts' partial gradients are summed
-// in part order through LDS.
+// The force scan of one workgroup: particles [64 tile, +64) of `lane`, k part
+// `part` walking the u range ur; the parts' partial gradients are summed in
+// part order through LDS.
 template <int D>
-__global__ __launch_bounds__(kForceTile * kForceParts) void force_kernel(Geo g, Batch b) {
-  __shared__ double red[kForceParts > 1 ? kForceParts - 1 : 1][3][kForceTile];
-  const int slot = blockIdx.x / b.n_ftiles;
-  if (slot >= *b.n_active) return;
-  const int tile = blockIdx.x % b.n_ftiles;
-  const int lane = b.active[slot];
-  if (b.prec != nullptr && b.prec[lane]) return;  // split-precision lane
+__device__ __forceinline__ void force_tile(const Geo& g, const LaneIO& io, const Batch& b,
+                                           int lane, int tile, int part, int2 ur,
+                                           double (*red)[3][kForceTile]) {
+  const int nparts = (int)blockDim.x / kForceTile;
   const int jl = (int)threadIdx.x % kForceTile;
-  // wave-uniform: keeps the k walk (and the loads of C) on the scalar unit
-  const int part = __builtin_amdgcn_readfirstlane((int)threadIdx.x / kForceTile);
   const int j = tile * kForceTile + jl;
-  const bool valid = j < b.N;
-  const double* sp = b.s + (size_t)lane * b.lane_stride + (size_t)(valid ? j : 0) * D;
+  const bool valid = j < io.N;
+  const double* sp = io.s + (size_t)(valid ? j : 0) * D;
   double2 z[3];
 #pragma unroll
   for (int a = 0; a < 3; ++a) z[a] = make_double2(1.0, 0.0);
@@ -277,9 +280,8 @@ __global__ __launch_bounds__(kForceTile * kForceParts) void force_kernel(Geo g, 
     sincospi(2.0 * sv[a], &sn, &cs);
     z[a] = make_double2(cs, sn);
   }
-  const double2* __restrict__ C = b.cbox + (size_t)lane * g.Mbox;
+  const double2* __restrict__ C = io.cbox;
   double g1 = 0.0, g2 = 0.0, g3 = 0.0;
-  const int2 ur = g.fpart[part];
   double2 zu = make_double2(1.0, 0.0);
   if (D >= 2 && ur.x > 0) {
     double sn, cs;
@@ -301,7 +303,7 @@ __global__ __launch_bounds__(kForceTile * kForceParts) void force_kernel(Geo g, 
     }
     zu = cmul(zu, z[1]);
   }
-  if (g.fparts > 1) {
+  if (nparts > 1) {
     if (part > 0) {
       red[part - 1][0][jl] = g1;
       red[part - 1][1][jl] = g2;
@@ -309,18 +311,46 @@ __global__ __launch_bounds__(kForceTile * kForceParts) void force_kernel(Geo g, 
     }
     __syncthreads();
     if (part > 0) return;
-    for (int p = 1; p < g.fparts; ++p) {
+    for (int p = 1; p < nparts; ++p) {
       g1 += red[p - 1][0][jl];
       g2 += red[p - 1][1][jl];
       g3 += red[p - 1][2][jl];
     }
   }
-  if (tile == 0 && threadIdx.x == 0) b.phi[lane] = phi_from_parts(g, b, lane);
-  if (valid && b.grad != nullptr) {
-    double* out = b.grad + (size_t)lane * b.lane_stride + (size_t)j * D;
+  if (tile == 0 && threadIdx.x == 0) b.phi[lane] = phi_from_parts(g, io);
+  if (valid && io.grad != nullptr) {
+    double* out = io.grad + (size_t)j * D;
     out[0] = g.pref * g1;
     if (D >= 2) out[1] = g.pref * g2;
     if (D >= 3) out[2] = g.pref * g3;
+  }
+}
+
+// grid: slots_ub x n_ftiles; block: kForceTile particles x k parts (g.fparts
+// for a uniform batch, kForceParts for an uneven one).  UNEVEN reads each
+// lane's geometry from b.geos; the uniform instance reads the argument (the
+// group tables then stay on the scalar unit).
+template <int D, bool UNEVEN>
+__global__ __launch_bounds__(kForceTile * kForceParts) void force_kernel(Geo g0, Batch b) {
+  __shared__ double red[kForceParts > 1 ? kForceParts - 1 : 1][3][kForceTile];
+  const int tiles_per_lane = UNEVEN ? b.tiles_force : b.n_ftiles;
+  const int slot = blockIdx.x / tiles_per_lane;
+  if (slot >= *b.n_active) return;
+  const int tile = blockIdx.x % tiles_per_lane;
+  const int lane = b.active[slot];
+  if (b.prec != nullptr && b.prec[lane]) return;  // split-precision lane
+  // wave-uniform: keeps the k walk (and the loads of C) on the scalar unit
+  const int part = __builtin_amdgcn_readfirstlane((int)threadIdx.x / kForceTile);
+  if constexpr (UNEVEN) {
+    const Geo* gp = b.geos + lane;
+    if (gp->d != D) return;  // uneven batches launch once per dimension
+    const Geo g = *gp;
+    const LaneIO io = lane_io<2>(g, b, lane);
+    if (tile >= io.n_ftiles) return;  // uniform per workgroup, before any barrier
+    force_tile<D>(g, io, b, lane, tile, part, gp->fpart[part], red);
+  } else {
+    const LaneIO io = lane_io<1>(g0, b, lane);
+    force_tile<D>(g0, io, b, lane, tile, part, g0.fpart[part], red);
   }
 }
 
@@ -333,6 +363,18 @@ size_t rho_lds_bytes(const Geo& g) {
 // Launch one stage (0 = rho, 1 = reduce, 2 = force) of an evaluation.
 template <int D>
 static void launch_stage(int stage, const Geo& g, const Batch& b, int slots_ub, hipStream_t st) {
+  if (b.lay != nullptr) {  // uneven batch: per-lane geometry, launch-wide maxima
+    if (stage == 0)
+      hipLaunchKernelGGL((rho_kernel<D, 2>), dim3(slots_ub * b.tiles_rho), dim3(2 * kWave),
+                         b.rho_lds, st, g, b);
+    else if (stage == 1)
+      hipLaunchKernelGGL(reduce_kernel, dim3(slots_ub * b.tiles_reduce), dim3(kReduceTile), 0, st,
+                         g, b);
+    else
+      hipLaunchKernelGGL((force_kernel<D, true>), dim3(slots_ub * b.tiles_force),
+                         dim3(kForceTile * kForceParts), 0, st, g, b);
+    return;
+  }
   if (stage == 0) {
     const int rho_tiles = g.n_utiles * b.n_splits;
     if (g.rho_halves == 2)
@@ -344,13 +386,23 @@ static void launch_stage(int stage, const Geo& g, const Batch& b, int slots_ub, 
   } else if (stage == 1) {
     hipLaunchKernelGGL(reduce_kernel, dim3(slots_ub * g.n_rtiles), dim3(kReduceTile), 0, st, g, b);
   } else {
-    hipLaunchKernelGGL(force_kernel<D>, dim3(slots_ub * b.n_ftiles), dim3(kForceTile * g.fparts),
-                       0, st, g, b);
+    hipLaunchKernelGGL((force_kernel<D, false>), dim3(slots_ub * b.n_ftiles),
+                       dim3(kForceTile * g.fparts), 0, st, g, b);
   }
 }
 
 void launch_energy_stage(int stage, const Geo& g, const Batch& b, int slots_ub, hipStream_t st) {
   if (slots_ub <= 0) return;
+  if (b.lay != nullptr) {  // uneven: reduce once, rho and forces once per dimension present
+    if (stage == 1) {
+      launch_stage<1>(stage, g, b, slots_ub, st);
+      return;
+    }
+    if (b.dmask & 2) launch_stage<1>(stage, g, b, slots_ub, st);
+    if (b.dmask & 4) launch_stage<2>(stage, g, b, slots_ub, st);
+    if (b.dmask & 8) launch_stage<3>(stage, g, b, slots_ub, st);
+    return;
+  }
   if (g.d == 1) launch_stage<1>(stage, g, b, slots_ub, st);
   if (g.d == 2) launch_stage<2>(stage, g, b, slots_ub, st);
   if (g.d == 3) launch_stage<3>(stage, g, b, slots_ub, st);

@@ -106,6 +106,11 @@ struct FlowLane {
 // Session arrays (device).  nd = N * d, stride = lane stride in doubles.
 struct RelaxDev {
   int B, nd, stride;
+  // uneven batches (null: every lane has nd = N d and stride `stride`): per
+  // lane vector offset and length, and offset of its (s, y) history
+  const long long* loff;
+  const int* lnd;
+  const long long* lhoff;
   int* status;    // [B]
   int* prec;      // [B] 1 = split-precision evaluations (null: all FP64)
   int* active;    // [B]
@@ -144,6 +149,21 @@ struct RelaxDev {
   long long* tr_it;
   double* tr_len;      // flow: arclength
 };
+
+// Per-lane views of the session arrays (uniform or uneven batches).
+__device__ __forceinline__ size_t rv_off(const RelaxDev& r, int lane) {
+  return r.loff != nullptr ? (size_t)r.loff[lane] : (size_t)lane * r.stride;
+}
+__device__ __forceinline__ int rv_nd(const RelaxDev& r, int lane) {
+  return r.lnd != nullptr ? r.lnd[lane] : r.nd;
+}
+__device__ __forceinline__ size_t rv_hist(const RelaxDev& r, int lane, int m) {
+  return r.lhoff != nullptr ? (size_t)r.lhoff[lane] : (size_t)lane * m * r.stride;
+}
+// trajectory row `row` of a lane: rows of nd doubles, record_cap rows per lane
+__device__ __forceinline__ size_t rv_trow(const RelaxDev& r, int lane, int cap, int row) {
+  return (size_t)cap * rv_off(r, lane) + (size_t)row * rv_nd(r, lane);
+}
 
 struct RelaxOutDev {
   double* s;          // [B][stride] wrapped

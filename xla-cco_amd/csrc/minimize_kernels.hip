@@ -84,8 +84,8 @@ __global__ __launch_bounds__(kAdvBlock) void descent_advance_kernel(RelaxDev r, 
   const int slot = blockIdx.x;
   if (slot >= *r.n_active) return;
   const int lane = r.active[slot];
-  const int nd = r.nd;
-  const size_t off = (size_t)lane * r.stride;
+  const int nd = rv_nd(r, lane);
+  const size_t off = rv_off(r, lane);
   double* s = r.s + off;
   double* g = r.g + off;
   double* se = r.s_eval + off;
@@ -136,7 +136,8 @@ __global__ __launch_bounds__(kAdvBlock) void descent_advance_kernel(RelaxDev r, 
                                   L.it > 0);
     if (due) {
       const size_t row = (size_t)lane * c.record_cap + L.rec_count;
-      for (int i = threadIdx.x; i < nd; i += blockDim.x) r.tr_s[row * r.stride + i] = wrap01(s[i]);
+      const size_t trow = rv_trow(r, lane, c.record_cap, L.rec_count);
+      for (int i = threadIdx.x; i < nd; i += blockDim.x) r.tr_s[trow + i] = wrap01(s[i]);
       if (threadIdx.x == 0) {
         r.tr_e[row] = L.f;
         r.tr_step[row] = L.step;
@@ -289,16 +290,16 @@ __device__ int ls_consume(LsState& L, double va, double sa, int max_steps) {
 // init, first trial point x + 1 * u.  All threads call it.
 __device__ void lbfgs_start_iteration(const RelaxDev& r, const LbfgsCfg& c, int lane,
                                       LbfgsLane& L, double* red) {
-  const int nd = r.nd;
-  const size_t off = (size_t)lane * r.stride;
+  const int nd = rv_nd(r, lane);
+  const size_t off = rv_off(r, lane);
   const int m = c.memory;
   const double* x = r.x + off;
   const double* gx = r.g + off;
   double* px = r.px + off;
   double* pg = r.pg + off;
   double* u = r.u + off;
-  double* S = r.S + (size_t)lane * m * r.stride;
-  double* Y = r.Y + (size_t)lane * m * r.stride;
+  double* S = r.S + rv_hist(r, lane, m);
+  double* Y = r.Y + rv_hist(r, lane, m);
   double* rho = r.rho + (size_t)lane * m;
   const double rs = c.rescales[L.pass_idx];
   const int count = L.opt_count;
@@ -308,8 +309,8 @@ __device__ void lbfgs_start_iteration(const RelaxDev& r, const LbfgsCfg& c, int 
   if (count > 0) {
     const int slot = (count - 1) % m;
     slot_new = slot;
-    double* Ss = S + (size_t)slot * r.stride;
-    double* Ys = Y + (size_t)slot * r.stride;
+    double* Ss = S + (size_t)slot * nd;
+    double* Ys = Y + (size_t)slot * nd;
     double sy = 0.0, yy = 0.0;
     for (int i = threadIdx.x; i < nd; i += blockDim.x) {
       const double ds = x[i] - px[i];
@@ -338,8 +339,8 @@ __device__ void lbfgs_start_iteration(const RelaxDev& r, const LbfgsCfg& c, int 
   for (int i = threadIdx.x; i < nd; i += blockDim.x) u[i] = gx[i];
   for (int t = m - 1; t >= m - nvalid; --t) {
     const int sl = (midx + t) % m;
-    const double* Si = S + (size_t)sl * r.stride;
-    const double* Yi = Y + (size_t)sl * r.stride;
+    const double* Si = S + (size_t)sl * nd;
+    const double* Yi = Y + (size_t)sl * nd;
     double dot = 0.0;
     for (int i = threadIdx.x; i < nd; i += blockDim.x) dot += Si[i] * u[i];
     dot = block_sum<kAdvBlock>(dot, red);
@@ -351,8 +352,8 @@ __device__ void lbfgs_start_iteration(const RelaxDev& r, const LbfgsCfg& c, int 
   __syncthreads();
   for (int t = m - nvalid; t < m; ++t) {
     const int sl = (midx + t) % m;
-    const double* Si = S + (size_t)sl * r.stride;
-    const double* Yi = Y + (size_t)sl * r.stride;
+    const double* Si = S + (size_t)sl * nd;
+    const double* Yi = Y + (size_t)sl * nd;
     double dot = 0.0;
     for (int i = threadIdx.x; i < nd; i += blockDim.x) dot += Yi[i] * u[i];
     dot = block_sum<kAdvBlock>(dot, red);
@@ -398,14 +399,15 @@ __device__ void lbfgs_gram_begin(const RelaxDev& r, const LbfgsCfg& c, int lane,
   if (count <= 0) return;
   const int m = c.memory;
   const int slot = (count - 1) % m;
-  const size_t off = (size_t)lane * r.stride;
+  const size_t off = rv_off(r, lane);
+  const int nd = rv_nd(r, lane);
   const double* x = r.x + off;
   const double* gx = r.g + off;
   const double* px = r.px + off;
   const double* pg = r.pg + off;
-  double* Ss = r.S + ((size_t)lane * m + slot) * r.stride;
-  double* Ys = r.Y + ((size_t)lane * m + slot) * r.stride;
-  for (int i = threadIdx.x; i < r.nd; i += blockDim.x) {
+  double* Ss = r.S + rv_hist(r, lane, m) + (size_t)slot * nd;
+  double* Ys = r.Y + rv_hist(r, lane, m) + (size_t)slot * nd;
+  for (int i = threadIdx.x; i < nd; i += blockDim.x) {
     Ss[i] = x[i] - px[i];
     Ys[i] = gx[i] - pg[i];
   }
@@ -427,8 +429,8 @@ __global__ __launch_bounds__(BLOCK) void lbfgs_advance_kernel(RelaxDev r, LbfgsC
   const int slot = blockIdx.x;
   if (slot >= *r.n_active) return;
   const int lane = r.active[slot];
-  const int nd = r.nd;
-  const size_t off = (size_t)lane * r.stride;
+  const int nd = rv_nd(r, lane);
+  const size_t off = rv_off(r, lane);
   double* s = r.s + off;
   double* x = r.x + off;
   double* gx = r.g + off;
@@ -458,7 +460,7 @@ __global__ __launch_bounds__(BLOCK) void lbfgs_advance_kernel(RelaxDev r, LbfgsC
     L.mode = MODE_EVAL;
     code = (fe < c.energy_tol) ? ST_ENERGY_TOL : ST_RUNNING;
     if (c.record_every > 0 && c.record_cap > 0 && L.rec_count == 0) {
-      for (int i = threadIdx.x; i < nd; i += blockDim.x) r.tr_s[(size_t)lane * c.record_cap * r.stride + i] = wrap01(s[i]);
+      for (int i = threadIdx.x; i < nd; i += blockDim.x) r.tr_s[rv_trow(r, lane, c.record_cap, 0) + i] = wrap01(s[i]);
       if (threadIdx.x == 0) {
         r.tr_e[(size_t)lane * c.record_cap] = L.f;
         r.tr_step[(size_t)lane * c.record_cap] = 0.0;
@@ -546,7 +548,8 @@ __global__ __launch_bounds__(BLOCK) void lbfgs_advance_kernel(RelaxDev r, LbfgsC
       if (c.record_every > 0 && c.record_cap > 0 && (L.it % c.record_every) == 0 &&
           L.rec_count < c.record_cap) {
         const size_t row = (size_t)lane * c.record_cap + L.rec_count;
-        for (int i = threadIdx.x; i < nd; i += blockDim.x) r.tr_s[row * r.stride + i] = wrap01(s[i]);
+        const size_t trow = rv_trow(r, lane, c.record_cap, L.rec_count);
+        for (int i = threadIdx.x; i < nd; i += blockDim.x) r.tr_s[trow + i] = wrap01(s[i]);
         if (threadIdx.x == 0) {
           r.tr_e[row] = L.f;
           r.tr_step[row] = L.last_step;
@@ -642,17 +645,17 @@ __device__ __forceinline__ void reg_sum2(double& v0, double& v1, Red2& R) {
 template <int EPT>
 __device__ void lbfgs_start_reg(const RelaxDev& r, const LbfgsCfg& c, int lane, LbfgsLane& L,
                                 Red2& R, double* alphas, double* rhos) {
-  const int nd = r.nd;
+  const int nd = rv_nd(r, lane);
   const int tid = threadIdx.x;
-  const size_t off = (size_t)lane * r.stride;
+  const size_t off = rv_off(r, lane);
   const int m = c.memory;
   const double* x = r.x + off;
   const double* gx = r.g + off;
   double* px = r.px + off;
   double* pg = r.pg + off;
   double* u = r.u + off;
-  double* S = r.S + (size_t)lane * m * r.stride;
-  double* Y = r.Y + (size_t)lane * m * r.stride;
+  double* S = r.S + rv_hist(r, lane, m);
+  double* Y = r.Y + rv_hist(r, lane, m);
   double* rho = r.rho + (size_t)lane * m;
   const double rs = c.rescales[L.pass_idx];
   const int count = L.opt_count;
@@ -663,8 +666,8 @@ __device__ void lbfgs_start_reg(const RelaxDev& r, const LbfgsCfg& c, int lane, 
   if (count > 0) {
     const int slot = (count - 1) % m;
     slot_new = slot;
-    double* Ss = S + (size_t)slot * r.stride;
-    double* Ys = Y + (size_t)slot * r.stride;
+    double* Ss = S + (size_t)slot * nd;
+    double* Ys = Y + (size_t)slot * nd;
     double sy = 0.0, yy = 0.0;
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
@@ -712,8 +715,8 @@ __device__ void lbfgs_start_reg(const RelaxDev& r, const LbfgsCfg& c, int lane, 
   double bs[PF][EPT], by[PF][EPT];
   auto load_into = [&](double (&ds)[EPT], double (&dy)[EPT], int t) {
     const int sl = (midx + t) % m;
-    const double* Si = S + (size_t)sl * r.stride;
-    const double* Yi = Y + (size_t)sl * r.stride;
+    const double* Si = S + (size_t)sl * nd;
+    const double* Yi = Y + (size_t)sl * nd;
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
       const int i = tid + e * kRegBlock;
@@ -826,9 +829,9 @@ __global__ __launch_bounds__(kRegBlock) void lbfgs_advance_reg_kernel(RelaxDev r
   const int slot = blockIdx.x;
   if (slot >= *r.n_active) return;
   const int lane = r.active[slot];
-  const int nd = r.nd;
+  const int nd = rv_nd(r, lane);
   const int tid = threadIdx.x;
-  const size_t off = (size_t)lane * r.stride;
+  const size_t off = rv_off(r, lane);
   double* s = r.s + off;
   double* x = r.x + off;
   double* gx = r.g + off;
@@ -862,10 +865,11 @@ __global__ __launch_bounds__(kRegBlock) void lbfgs_advance_reg_kernel(RelaxDev r
     code = (fe < c.energy_tol) ? ST_ENERGY_TOL : ST_RUNNING;
     if (c.record_every > 0 && c.record_cap > 0 && L.rec_count == 0) {
       const size_t row = (size_t)lane * c.record_cap;
+      const size_t trow = rv_trow(r, lane, c.record_cap, 0);
 #pragma unroll
       for (int e = 0; e < EPT; ++e) {
         const int i = tid + e * kRegBlock;
-        if (i < nd) r.tr_s[row * r.stride + i] = wrap01(s[i]);
+        if (i < nd) r.tr_s[trow + i] = wrap01(s[i]);
       }
       if (tid == 0) {
         r.tr_e[row] = L.f;
@@ -962,10 +966,11 @@ __global__ __launch_bounds__(kRegBlock) void lbfgs_advance_reg_kernel(RelaxDev r
       if (c.record_every > 0 && c.record_cap > 0 && (L.it % c.record_every) == 0 &&
           L.rec_count < c.record_cap) {
         const size_t row = (size_t)lane * c.record_cap + L.rec_count;
+        const size_t trow = rv_trow(r, lane, c.record_cap, L.rec_count);
 #pragma unroll
         for (int e = 0; e < EPT; ++e) {
           const int i = tid + e * kRegBlock;
-          if (i < nd) r.tr_s[row * r.stride + i] = wrap01(s[i]);
+          if (i < nd) r.tr_s[trow + i] = wrap01(s[i]);
         }
         if (tid == 0) {
           r.tr_e[row] = L.f;
@@ -1441,11 +1446,12 @@ __global__ __launch_bounds__(kAdvBlock) void finalize_kernel(RelaxDev r, int met
                                                              RelaxOutDev o) {
   __shared__ double red[16];
   const int lane = blockIdx.x;
-  const size_t off = (size_t)lane * r.stride;
+  const size_t off = rv_off(r, lane);
+  const int nd = rv_nd(r, lane);
   const double* s = r.s + off;
   const double* g = r.g + off;
   double gg = 0.0;
-  for (int i = threadIdx.x; i < r.nd; i += blockDim.x) {
+  for (int i = threadIdx.x; i < nd; i += blockDim.x) {
     o.s[off + i] = wrap01(s[i]);
     gg += g[i] * g[i];
   }

@@ -21,7 +21,7 @@ from xla_cco.kvectors import (
     ka_from_chi,
     unit_ball_volume,
 )
-from xla_cco.minimize import relax
+from xla_cco.minimize import relax, relax_uneven
 from xla_cco.minimize.descent import DescentSettings, adaptive_descent
 from xla_cco.minimize.lbfgs import LBFGSSettings, lbfgs
 from xla_cco.minimize.result import RelaxResult, Trajectory
@@ -42,6 +42,7 @@ __all__ = [
     "ka_from_chi",
     "unit_ball_volume",
     "relax",
+    "relax_uneven",
     "DescentSettings",
     "adaptive_descent",
     "LBFGSSettings",

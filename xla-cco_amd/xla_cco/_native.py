@@ -132,6 +132,11 @@ SIGNATURES = {
     "cco_relax_counters": (C.c_int, [_P, _PI64]),
     "cco_relax_set_option": (C.c_int, [_P, C.c_int, C.c_int64]),
     "cco_relax_get_option": (C.c_int, [_P, C.c_int, _PI64]),
+    "cco_relax_uneven_create": (
+        C.c_int,
+        [_P, C.c_int, C.POINTER(_P), C.c_int, C.POINTER(C.c_int32), C.POINTER(C.c_int32), C.c_int,
+         C.POINTER(DescentSettingsC), C.POINTER(LBFGSSettingsC), C.POINTER(_P)],
+    ),
     "cco_relax_set_precision": (C.c_int, [_P, C.c_int, C.c_double]),
     "cco_relax_precision_state": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "cco_energy_set_precision": (C.c_int, [_P, C.c_int, C.POINTER(C.c_int)]),

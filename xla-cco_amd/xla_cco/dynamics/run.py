@@ -36,6 +36,9 @@ class MDSettings:
     remove_com: bool = True
     chunk_steps: int = 10_000
     drift_tol: float = 1e-3
+    # "mixed": forces from the split-precision MFMA kernels (positions and
+    # velocities stay FP64); an extension, "fp64" is the reference's arithmetic
+    precision: str = "fp64"
 
 
 @dataclasses.dataclass
@@ -92,6 +95,10 @@ class _MDSession:
         _native.check(_native.lib().cco_md_create(ctx.handle, energy.handle(), b, n, d, _native.dptr(bas),
                                                   float(settings.dt), tol, C.byref(h)))
         self.handle = h
+        if settings.precision not in ("fp64", "mixed"):
+            raise ValueError(f"precision must be 'fp64' or 'mixed', got {settings.precision!r}")
+        if settings.precision == "mixed":
+            _native.check(_native.lib().cco_relax_set_precision(h, 1, 0.0))
         self.b, self.n, self.d = b, n, d
 
     def close(self) -> None:

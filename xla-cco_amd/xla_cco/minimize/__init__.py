@@ -18,7 +18,7 @@ minimizer.
 from __future__ import annotations
 
 import ctypes as C
-from typing import Tuple, Union
+from typing import List, Sequence, Tuple, Union
 
 import numpy as np
 
@@ -43,6 +43,8 @@ from xla_cco.minimize.result import (
 
 __all__ = [
     "relax",
+    "relax_uneven",
+    "UnevenSession",
     "DescentSettings",
     "LBFGSSettings",
     "RelaxResult",
@@ -274,6 +276,183 @@ def relax(
     try:
         sess.init(s0)
         sess.run(_eval_bound(settings, chunk_iters), sync_every=max(1, min(chunk_iters, 256)))
+        return sess.result()
+    finally:
+        sess.close()
+
+
+class UnevenSession:
+    """Device session over configurations of different N, d and k-set.
+
+    ``energies[i]`` and ``shapes[i] = (n_i, d_i)`` describe lane i.  Positions
+    are packed lane after lane (``pack`` / ``unpack``); ``init``, ``run`` and
+    ``result`` mirror :class:`RelaxSession`.
+    """
+
+    def __init__(self, energies: Sequence[StealthyEnergy], shapes: Sequence[Tuple[int, int]],
+                 settings: Settings) -> None:
+        if len(energies) != len(shapes) or len(shapes) == 0:
+            raise ValueError("need one energy per configuration and at least one configuration")
+        if not isinstance(settings, (DescentSettings, LBFGSSettings)):
+            raise TypeError(f"unsupported settings type {type(settings).__name__}")
+        if isinstance(settings, LBFGSSettings) and settings.precision != "fp64":
+            raise ValueError("uneven batches run FP64")
+        uniq: List[StealthyEnergy] = []
+        index = {}
+        self.b = len(shapes)
+        self.lane_e = np.empty(self.b, dtype=np.int32)
+        self.lane_n = np.empty(self.b, dtype=np.int32)
+        self.shapes = [(int(n), int(d)) for n, d in shapes]
+        for i, (e, (n, d)) in enumerate(zip(energies, self.shapes)):
+            if not isinstance(e, StealthyEnergy):
+                raise TypeError("uneven batches need StealthyEnergy instances")
+            if d != e.dim or n < 1:
+                raise ValueError(f"configuration {i}: shape ({n}, {d}) does not fit its energy (d={e.dim})")
+            if id(e) not in index:
+                index[id(e)] = len(uniq)
+                uniq.append(e)
+            self.lane_e[i] = index[id(e)]
+            self.lane_n[i] = n
+        if len({e.device for e in uniq}) != 1:
+            raise ValueError("all energies must live on one device")
+        self.settings = settings
+        self.energies = uniq
+        self.size = int(sum(n * d for n, d in self.shapes))
+        method, ds, ls = _c_settings(settings)
+        ctx = _native.context(uniq[0].device)
+        handles = (C.c_void_p * len(uniq))(*[e.handle().value for e in uniq])
+        h = C.c_void_p()
+        _native.check(
+            _native.lib().cco_relax_uneven_create(
+                ctx.handle,
+                len(uniq),
+                C.cast(handles, C.POINTER(C.c_void_p)),
+                self.b,
+                self.lane_e.ctypes.data_as(C.POINTER(C.c_int32)),
+                self.lane_n.ctypes.data_as(C.POINTER(C.c_int32)),
+                method,
+                C.byref(ds) if ds is not None else None,
+                C.byref(ls) if ls is not None else None,
+                C.byref(h),
+            )
+        )
+        self.handle = h
+
+    def pack(self, s0: Sequence[np.ndarray]) -> np.ndarray:
+        if len(s0) != self.b:
+            raise ValueError("need one configuration per lane")
+        parts = []
+        for i, s in enumerate(s0):
+            require_float64(s, "s0")
+            s = np.asarray(s)
+            if s.shape != self.shapes[i]:
+                raise ValueError(f"s0[{i}] must have shape {self.shapes[i]}")
+            parts.append(np.ascontiguousarray(s).ravel())
+        return np.ascontiguousarray(np.concatenate(parts))
+
+    def unpack(self, flat: np.ndarray) -> List[np.ndarray]:
+        out, off = [], 0
+        for n, d in self.shapes:
+            out.append(flat[off: off + n * d].reshape(n, d).copy())
+            off += n * d
+        return out
+
+    def init(self, s0: Sequence[np.ndarray]) -> None:
+        packed = self.pack(s0)
+        _native.check(_native.lib().cco_relax_init(self.handle, _native.dptr(packed)))
+
+    def run(self, max_evals: int, sync_every: int) -> None:
+        _native.check(_native.lib().cco_relax_run(self.handle, int(max_evals), int(sync_every)))
+
+    def result(self) -> List[RelaxResult]:
+        b = self.b
+        L = _native.lib()
+        s = np.empty(self.size)
+        e = np.empty(b)
+        gn = np.empty(b)
+        ne = np.empty(b, dtype=np.int64)
+        ni = np.empty(b, dtype=np.int64)
+        st = np.empty(b, dtype=np.int64)
+        _native.check(
+            L.cco_relax_result(
+                self.handle, _native.dptr(s), _native.dptr(e), _native.dptr(gn), _native.i64ptr(ne),
+                _native.i64ptr(ni), _native.i64ptr(st),
+            )
+        )
+        every, cap = self.settings.record_every, self.settings.max_records
+        tr = None
+        if every > 0 and cap > 0:
+            cnt = np.empty(b, dtype=np.int64)
+            it = np.zeros((b, cap), dtype=np.int64)
+            ts = np.zeros(cap * self.size)
+            te = np.zeros((b, cap))
+            tt = np.zeros((b, cap))
+            _native.check(
+                L.cco_relax_trajectory(
+                    self.handle, _native.i64ptr(cnt), _native.i64ptr(it), _native.dptr(ts),
+                    _native.dptr(te), _native.dptr(tt),
+                )
+            )
+            tr = (cnt, it, ts, te, tt)
+        out: List[RelaxResult] = []
+        off = 0
+        for i, (n_i, d_i) in enumerate(self.shapes):
+            nd = n_i * d_i
+            traj = None
+            if tr is not None:
+                cnt, it, ts, te, tt = tr
+                rows = ts[cap * off: cap * (off + nd)].reshape(cap, n_i, d_i)
+                traj = Trajectory(s=rows, energy=te[i], step=tt[i], iteration=it[i], count=cnt[i])
+            out.append(
+                RelaxResult(
+                    s=s[off: off + nd].reshape(n_i, d_i).copy(), energy=e[i], grad_norm=gn[i],
+                    n_evals=ne[i], n_iters=ni[i], status=st[i], trajectory=traj,
+                )
+            )
+            off += nd
+        return out
+
+    def close(self) -> None:
+        if self.handle is not None and _native._lib is not None:
+            _native._lib.cco_relax_destroy(self.handle)
+        self.handle = None
+
+    def __del__(self) -> None:  # pragma: no cover
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def relax_uneven(
+    energies: Sequence[StealthyEnergy],
+    s0: Sequence[np.ndarray],
+    settings: Settings,
+    chunk_iters: int = 2_000,
+) -> List[RelaxResult]:
+    """Relax configurations of different size, dimension and k-set in one batch.
+
+    ``s0[i]`` (shape (n_i, d_i)) relaxes under ``energies[i]`` (the same
+    energy object may serve several configurations).  Every configuration is
+    a lane of one device session, so each kernel launch covers all of them;
+    per-lane freezing and compaction work as in :func:`relax`.  Returns one
+    :class:`RelaxResult` per configuration (leaves without a lane axis).
+
+    The reference relaxes one (N, d, k-set) per call (its CLI loops over
+    them, cli/relax.py:339-358); the per-configuration arithmetic is the same
+    as :func:`relax` with a single lane, up to summation order.
+    """
+    if len(energies) != len(s0):
+        raise ValueError("need one energy per configuration")
+    for i, s in enumerate(s0):
+        require_float64(s, "s0")
+        if np.ndim(s) != 2:
+            raise ValueError(f"s0[{i}] must have shape (n, d)")
+    assert chunk_iters >= 1, "chunk_iters must be positive"
+    sess = UnevenSession(energies, [np.shape(s) for s in s0], settings)
+    try:
+        sess.init(s0)
+        sess.run(_eval_bound(settings, chunk_iters), max(1, min(chunk_iters, 256)))
         return sess.result()
     finally:
         sess.close()
