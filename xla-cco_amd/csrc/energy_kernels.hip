@@ -34,6 +34,84 @@ namespace cco {
 // grid: slots_ub x n_utiles x n_splits; block: H UT threads.  Thread
 // (h, u) runs unit u over the particle pairs {2Hi + 2h, 2Hi + 2h + 1} of each
 // LDS chunk; with H = 2, half 1 hands its sums to half 0 through LDS.
+// Entries per table row (one power, every particle of the chunk): the chunk
+// plus room for the pair loop's prefetch past the last pair (read, never
+// used), rounded to 1 mod 16 entries so that rows 16 bytes x that length
+// apart start 4 banks apart.
+__host__ __device__ inline int rho_table_len(int chunk) { return ((chunk + 8 + 15) / 16) * 16 + 1; }
+
+static_assert(kRowSeg == 8, "rho_kernel dispatches row counts 1..8");
+
+// Rows 0 .. NR-1 of one unit for the particle pairs jb, jb + 1 (jb = jfirst,
+// jfirst + 2H, ... < nj; an odd nj has zero table entries at index nj).  zr,
+// t2, t3 point at the table rows of the unit's powers (z1^(8q), z2^|n2|,
+// z3^|n3|); each pair's values are loaded one pair ahead.
+template <int D, int NR, int H>
+__device__ __forceinline__ void rho_unit_rows(double2 (&acc)[kRowSeg], const double2* zr,
+                                              const double2* t2, const double2* t3,
+                                              const double2* z1t, int jb, int nj, double sg2,
+                                              double sg3) {
+  if (jb >= nj) return;
+  const double2* pa = zr + jb;
+  const double2* pb = t2 + jb;
+  const double2* pc = t3 + jb;
+  const double2* py = z1t + jb;
+  double2 a0 = pa[0], a1 = pa[1];
+  double2 b0 = make_double2(1.0, 0.0), b1 = b0, c0 = b0, c1 = b0;
+  if (D >= 2) {
+    b0 = pb[0];
+    b1 = pb[1];
+  }
+  if (D >= 3) {
+    c0 = pc[0];
+    c1 = pc[1];
+  }
+  double2 y0 = py[0], y1 = py[1];
+  for (; jb < nj; jb += 2 * H) {
+    pa += 2 * H;
+    pb += 2 * H;
+    pc += 2 * H;
+    py += 2 * H;
+    const double2 na0 = pa[0], na1 = pa[1];
+    double2 nb0 = b0, nb1 = b1, nc0 = c0, nc1 = c1;
+    if (D >= 2) {
+      nb0 = pb[0];
+      nb1 = pb[1];
+    }
+    if (D >= 3) {
+      nc0 = pc[0];
+      nc1 = pc[1];
+    }
+    const double2 ny0 = py[0], ny1 = py[1];
+    double2 e0 = a0, e1 = a1;
+    if (D >= 2) {
+      e0 = cmul(e0, make_double2(b0.x, sg2 * b0.y));
+      e1 = cmul(e1, make_double2(b1.x, sg2 * b1.y));
+    }
+    if (D >= 3) {
+      e0 = cmul(e0, make_double2(c0.x, sg3 * c0.y));
+      e1 = cmul(e1, make_double2(c1.x, sg3 * c1.y));
+    }
+    acc[0].x += e0.x + e1.x;
+    acc[0].y += e0.y + e1.y;
+#pragma unroll
+    for (int r = 1; r < NR; ++r) {
+      e0 = cmul(e0, y0);
+      e1 = cmul(e1, y1);
+      acc[r].x += e0.x + e1.x;
+      acc[r].y += e0.y + e1.y;
+    }
+    a0 = na0;
+    a1 = na1;
+    b0 = nb0;
+    b1 = nb1;
+    c0 = nc0;
+    c1 = nc1;
+    y0 = ny0;
+    y1 = ny1;
+  }
+}
+
 template <int D, int H>
 __global__ __launch_bounds__(H * kRhoUnitsMax) void rho_kernel(Geo g0, Batch b) {
   extern __shared__ double2 lds[];
@@ -60,14 +138,24 @@ __global__ __launch_bounds__(H * kRhoUnitsMax) void rho_kernel(Geo g0, Batch b) 
   const int j1 = min(io.N, j0 + b.split_len);
   const double* s = io.s;
 
+  // One row count per wave (its longest unit): shorter units also compute the
+  // rows past their end, which are never stored, so a wave never loops over
+  // the particles once per distinct length.
+  int wave_rows = uvalid ? nrows : 0;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) wave_rows = max(wave_rows, __shfl_xor(wave_rows, off));
+  wave_rows = __builtin_amdgcn_readfirstlane(wave_rows);
+
   const int CH = g.rho_chunk;
   const int w2 = g.P2 + 1;
   const int w3 = g.P3 + 1;
-  // LDS per chunk particle jj: z1, z1^(q kRowSeg) for q < Q, z2^p (p <= P2), z3^p (p <= P3)
+  // LDS tables, one row of TL entries (the chunk's particles jj) per power:
+  // z1, z1^(q kRowSeg) for q < Q, z2^p (p <= P2), z3^p (p <= P3)
+  const int TL = rho_table_len(CH);
   double2* Lz1 = lds;
-  double2* LZR = lds + CH;
-  double2* LT2 = LZR + CH * g.Q;
-  double2* LT3 = LT2 + CH * w2;
+  double2* LZR = lds + TL;
+  double2* LT2 = LZR + TL * g.Q;
+  double2* LT3 = LT2 + TL * w2;
 
   double2 acc[kRowSeg];
 #pragma unroll
@@ -80,64 +168,75 @@ __global__ __launch_bounds__(H * kRhoUnitsMax) void rho_kernel(Geo g0, Batch b) 
 
   for (int jc = j0; jc < j1; jc += CH) {
     const int nj = min(CH, j1 - jc);
-    for (int idx = threadIdx.x; idx < nj * D; idx += blockDim.x) {
-      const int jj = idx % nj;
-      const int ax = idx / nj;
-      const double sv = s[(size_t)(jc + jj) * D + ax];
-      double sn, cs;
-      sincospi(2.0 * sv, &sn, &cs);
-      const double2 z = make_double2(cs, sn);
-      if (ax == 0) {
-        Lz1[jj] = z;
-        LZR[jj * g.Q] = make_double2(1.0, 0.0);
-        for (int qq = 1; qq < g.Q; ++qq) {
-          double s0, c0;
-          sincospi(2.0 * (double)(qq * kRowSeg) * sv, &s0, &c0);
-          LZR[jj * g.Q + qq] = make_double2(c0, s0);
-        }
-      } else {
-        double2* T = (ax == 1) ? (LT2 + jj * w2) : (LT3 + jj * w3);
+    const int nje = (nj + 1) & ~1;  // an odd chunk gets a zero particle (CH is even)
+    // Work items per particle: the power tables of axes 1 .. D-1 (serial
+    // recurrences, listed first), then the Q row-run seeds z1^(q kRowSeg)
+    // (item q = 0 also stores z1).
+    const int n_items = D - 1 + g.Q;
+    for (int idx = threadIdx.x; idx < nje * n_items; idx += blockDim.x) {
+      const int jj = idx % nje;
+      const int it = idx / nje;
+      const bool real = jj < nj;
+      if (it < D - 1) {
+        const int ax = it + 1;
+        double2* T = (ax == 1 ? LT2 : LT3) + jj;
         const int P = (ax == 1) ? g.P2 : g.P3;
-        double2 p = make_double2(1.0, 0.0);
+        double2 z = make_double2(0.0, 0.0);
+        double2 p = make_double2(0.0, 0.0);
+        if (real) {
+          double sn, cs;
+          sincospi(2.0 * s[(size_t)(jc + jj) * D + ax], &sn, &cs);
+          z = make_double2(cs, sn);
+          p = make_double2(1.0, 0.0);
+        }
         T[0] = p;
         for (int k = 1; k <= P; ++k) {
           p = cmul(p, z);
-          T[k] = p;
+          T[k * TL] = p;
         }
+      } else {
+        const int qq = it - (D - 1);
+        double2 zq = make_double2(0.0, 0.0);
+        if (real) {
+          const double sv = s[(size_t)(jc + jj) * D];
+          if (qq == 0) {
+            double sn, cs;
+            sincospi(2.0 * sv, &sn, &cs);
+            Lz1[jj] = make_double2(cs, sn);
+            zq = make_double2(1.0, 0.0);
+          } else {
+            double sn, cs;
+            sincospi(2.0 * (double)(qq * kRowSeg) * sv, &sn, &cs);
+            zq = make_double2(cs, sn);
+          }
+        } else if (qq == 0) {
+          Lz1[jj] = make_double2(0.0, 0.0);
+        }
+        LZR[qq * TL + jj] = zq;
       }
     }
     __syncthreads();
     if (uvalid) {
-      for (int jb = 2 * h; jb < nj; jb += 2 * H) {
-        const bool two = jb + 1 < nj;
-        const int jj = jb;
-        const int jk = two ? jb + 1 : jb;
-        double2 e0 = LZR[jj * g.Q + q];
-        double2 e1 = LZR[jk * g.Q + q];
-        if (D >= 2) {
-          const double2 t0 = LT2[jj * w2 + a2];
-          const double2 t1 = LT2[jk * w2 + a2];
-          e0 = cmul(e0, make_double2(t0.x, sg2 * t0.y));
-          e1 = cmul(e1, make_double2(t1.x, sg2 * t1.y));
-        }
-        if (D >= 3) {
-          const double2 t0 = LT3[jj * w3 + a3];
-          const double2 t1 = LT3[jk * w3 + a3];
-          e0 = cmul(e0, make_double2(t0.x, sg3 * t0.y));
-          e1 = cmul(e1, make_double2(t1.x, sg3 * t1.y));
-        }
-        if (!two) e1 = make_double2(0.0, 0.0);
-        const double2 z0 = Lz1[jj];
-        const double2 z1 = Lz1[jk];
-#pragma unroll
-        for (int r = 0; r < kRowSeg; ++r) {
-          if (r < nrows) {
-            acc[r].x += e0.x + e1.x;
-            acc[r].y += e0.y + e1.y;
-            e0 = cmul(e0, z0);
-            e1 = cmul(e1, z1);
-          }
-        }
+      const double2* zr = LZR + q * TL;
+      const double2* t2 = LT2 + a2 * TL;
+      const double2* t3 = LT3 + a3 * TL;
+      const int jb = 2 * h;
+      switch (wave_rows) {
+#define CCO_RHO_ROWS(NR)                                                                    \
+  case NR:                                                                                  \
+    rho_unit_rows<D, NR, H>(acc, zr, t2, t3, Lz1, jb, nj, sg2, sg3);              \
+    break;
+        CCO_RHO_ROWS(1)
+        CCO_RHO_ROWS(2)
+        CCO_RHO_ROWS(3)
+        CCO_RHO_ROWS(4)
+        CCO_RHO_ROWS(5)
+        CCO_RHO_ROWS(6)
+        CCO_RHO_ROWS(7)
+        CCO_RHO_ROWS(8)
+#undef CCO_RHO_ROWS
+        default:
+          break;
       }
     }
     __syncthreads();
@@ -355,7 +454,7 @@ __global__ __launch_bounds__(kForceTile * kForceParts) void force_kernel(Geo g0,
 }
 
 size_t rho_lds_bytes(const Geo& g) {
-  const size_t tables = (size_t)g.rho_chunk * (1 + g.Q + (g.P2 + 1) + (g.P3 + 1));
+  const size_t tables = (size_t)rho_table_len(g.rho_chunk) * (1 + g.Q + (g.P2 + 1) + (g.P3 + 1));
   const size_t hand = g.rho_halves == 2 ? (size_t)kRowSeg * g.UT : 0;  // half-1 sums
   return sizeof(double2) * (tables > hand ? tables : hand);
 }
