@@ -9,8 +9,9 @@ time, inputs already resident in device memory when the timed region starts.
     python bench.py [--gpus 1] [--steps 2] [--warmup 1]
 
 Prints one JSON line.  Per-kernel device times come from HIP events recorded
-on the library's stream around every launch inside the timed region; the
-roofline object describes the kernel with the largest total time.  The CPU
+on the library's streams around the launches of every 4th evaluation step
+(--profile-every) inside the timed region; the roofline object describes the
+kernel with the largest total time.  The CPU
 baseline times the oracle's C restatement (OpenMP, one configuration per
 thread) on a bounded sample of the same workload.
 """
@@ -105,6 +106,7 @@ def bench_uneven(args) -> int:
         m_real.append(cache[(n, d, chi)][1])
     settings = LBFGSSettings(energy_tol=tol)
     sess = UnevenSession(energies, [(n, d) for n, d, _ in specs], settings)
+    _native.check(L.cco_relax_set_option(sess.handle, 2, args.shards))
     bound = 1 + 3 * (settings.max_evals_per_pass + settings.max_linesearch_steps + 1)
     inputs = [
         [np.random.default_rng(5000 + 97 * k + i).uniform(size=(n, d)) for i, (n, d, _) in enumerate(specs)]
@@ -113,7 +115,7 @@ def bench_uneven(args) -> int:
     for w in range(args.warmup):
         sess.init(inputs[w])
         sess.run(bound, 256)
-    _native.check(L.cco_relax_set_profiling(sess.handle, 0 if args.no_profile else 1))
+    _native.check(L.cco_relax_set_profiling(sess.handle, 0 if args.no_profile else args.profile_every))
     elapsed, converged, lane_evals = 0.0, 0, 0
     for k in range(args.steps):
         sess.init(inputs[args.warmup + k])  # host upload outside the timed region
@@ -175,6 +177,13 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-profile", action="store_true", help="skip per-kernel event timing")
+    ap.add_argument(
+        "--profile-every",
+        type=int,
+        default=4,
+        help="time the kernels of every k-th evaluation step with HIP events (events between "
+        "launches cost device time; 1 times every step)",
+    )
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64",
                     help="mixed: split-precision MFMA evaluations while E > --switch, FP64 below")
     ap.add_argument("--switch", type=float, default=1e-8, help="energy below which mixed lanes turn FP64")
@@ -183,6 +192,12 @@ def main() -> int:
         type=int,
         default=0,
         help="L-BFGS advance: 0 auto, 1 global-memory two-loop, 2 register two-loop, 3 Gram form",
+    )
+    ap.add_argument(
+        "--shards",
+        type=int,
+        default=0,
+        help="lane shards, one stream each (0 auto: 2 at this batch size; 1 = a single stream)",
     )
     ap.add_argument(
         "--uneven",
@@ -232,6 +247,7 @@ def main() -> int:
     settings = LBFGSSettings(energy_tol=tol, precision=args.precision, precision_switch=args.switch)
     sess = RelaxSession(energy, B, n, d, settings)
     _native.check(L.cco_relax_set_option(sess.handle, 1, args.advance_variant))
+    _native.check(L.cco_relax_set_option(sess.handle, 2, args.shards))
     bound = 1 + 3 * (settings.max_evals_per_pass + settings.max_linesearch_steps + 1)
 
     inputs = [
@@ -256,7 +272,7 @@ def main() -> int:
         one_step(inputs[w])
     ctx.synchronize()
 
-    _native.check(L.cco_relax_set_profiling(sess.handle, 0 if args.no_profile else 1))
+    _native.check(L.cco_relax_set_profiling(sess.handle, 0 if args.no_profile else args.profile_every))
     converged = 0
     lane_evals = 0
     iters = 0
@@ -280,6 +296,8 @@ def main() -> int:
     opt = C.c_int64(0)
     _native.check(L.cco_relax_get_option(sess.handle, 1, C.byref(opt)))
     variant_run = int(opt.value)
+    _native.check(L.cco_relax_get_option(sess.handle, 2, C.byref(opt)))
+    shards_run = int(opt.value)
     ms = (C.c_double * len(STAGES))()
     launches = (C.c_int64 * len(STAGES))()
     _native.check(L.cco_relax_profile(sess.handle, ms, launches))
@@ -322,7 +340,7 @@ def main() -> int:
         if tfile.exists():
             try:
                 tdat = json.loads(tfile.read_text()).get(name)
-                if tdat and tdat.get("workload") == f"d{d}_n{n}_chi{chi}_b{B}":
+                if tdat and tdat.get("workload") == f"d{d}_n{n}_chi{chi}_b{B}_s{shards_run}":
                     traffic = tdat.get("bytes_per_launch")
             except (ValueError, AttributeError):
                 traffic = None
@@ -336,6 +354,9 @@ def main() -> int:
             "traffic": traffic,
             "avg_launch_ms": avg_ms,
             "work_per_launch": per_launch,
+            # lane shards run on separate streams: a launch covers 1/streams of the
+            # lanes and its duration includes co-running with the other shards' kernels
+            "streams": shards_run,
         }
 
     cpu = None
@@ -367,6 +388,7 @@ def main() -> int:
                         3: "Gram form (dot / recursion / direction kernels)"}.get(variant_run),
             "energy_tol": tol,
             "precision": args.precision if args.precision == "fp64" else f"mixed (FP64 below E={args.switch:g})",
+            "streams": shards_run,
         },
         "converged": f"{converged}/{B * args.steps}",
         "energy_force_evals_per_s": lane_evals / elapsed,

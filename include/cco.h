@@ -126,9 +126,10 @@ int cco_relax_destroy(cco_relax* r);
 int cco_relax_init(cco_relax* r, const double* s0_host);
 int cco_relax_init_dev(cco_relax* r, const double* s0_dev);
 /* Built-in energy: run until no lane is running, syncing with the host every
- * sync_every evaluations (at most 64 once <= 1/4 of the lanes run, 16 once
- * <= 1/32 do); returns CCO_ERR_BOUND if max_evals evaluations pass first
- * (the reference's round bound). */
+ * sync_every evaluations; returns CCO_ERR_BOUND if max_evals evaluations
+ * pass first (the reference's round bound).  Steps are queued at most 16
+ * ahead of the device, which reports its progress through mapped host
+ * memory, so no step is queued once every lane has stopped. */
 int cco_relax_run(cco_relax* r, int64_t max_evals, int sync_every);
 /* Same, stopping after at most n_evals evaluations; *n_active = lanes left. */
 int cco_relax_step(cco_relax* r, int64_t n_evals, int* n_active);
@@ -151,15 +152,23 @@ int cco_relax_trajectory(cco_relax* r, int64_t* count_host, int64_t* iteration_h
 /* Total energy evaluations issued so far (sum over lanes of evaluation steps). */
 int cco_relax_stats(cco_relax* r, int64_t* steps, int64_t* lane_evals);
 /* Sums over lanes: energy evaluations (descent: iterations + 1), iterations,
- * two-loop (s, y) pairs visited (L-BFGS), evaluation steps of the session. */
+ * two-loop (s, y) pairs visited (L-BFGS); then the evaluation steps queued
+ * over all shards since the last init (the launches of each step kernel). */
 int cco_relax_counters(cco_relax* r, int64_t* out4);
 /* Tuning knobs (no reference counterpart).  CCO_OPT_ADVANCE_VARIANT: 0 auto,
  * 1 L-BFGS vectors in global memory, 2 register-resident (N*d <= 8192),
  * 3 Gram form (memory <= 128).  Takes effect at the next cco_relax_init. */
 #define CCO_OPT_ADVANCE_VARIANT 1
+/* CCO_OPT_SHARDS: lanes split into this many shards (0 auto: 2 for
+ * built-in-energy descent and L-BFGS sessions of 64 or more lanes, else 1;
+ * at most 4), each run on its own stream so that one shard's HBM-bound
+ * L-BFGS update overlaps another's energy evaluation.  A lane's results do
+ * not depend on the shard count.  Takes effect at the next init. */
+#define CCO_OPT_SHARDS 2
 int cco_relax_set_option(cco_relax* r, int key, int64_t value);
 /* The value in force: for CCO_OPT_ADVANCE_VARIANT the variant latched by the
- * last cco_relax_init (1, 2 or 3; 0 before the first init). */
+ * last cco_relax_init (1, 2 or 3; 0 before the first init); for
+ * CCO_OPT_SHARDS the shards of the last init. */
 int cco_relax_get_option(cco_relax* r, int key, int64_t* value);
 /* Split-precision evaluations (no reference counterpart; the precision row of
  * the target design).  mode 1: rho and the forces run as complex GEMMs on the
@@ -246,12 +255,14 @@ int cco_pair_stats(cco_ctx* ctx, const double* s_host, int B, int N, int d,
 
 /* Measurement (no reference counterpart; the reference times with
  * block_until_ready, benchmarks/bench_energy.py:104-115).  Per-stage device
- * time from HIP events recorded on the context stream around every launch:
- * stages rho, reduce, force, advance (or MD finish; the line-search kernel of
- * the Gram form), compact, MD prepare, Gram dot products, Gram recursion,
- * Gram direction (ms summed, launches counted; CCO_PROFILE_STAGES entries
- * each). */
-int cco_relax_set_profiling(cco_relax* r, int on);
+ * time from HIP events recorded on a shard's stream around the launches of
+ * every `every`-th evaluation step of that shard (0 turns it off; events
+ * between launches cost device time, so a period above 1 keeps them from
+ * slowing the run they measure): stages rho, reduce, force, advance (or MD
+ * finish; the line-search kernel of the Gram form), compact, MD prepare,
+ * Gram dot products, Gram recursion, Gram direction (ms summed, timed
+ * launches counted; CCO_PROFILE_STAGES entries each). */
+int cco_relax_set_profiling(cco_relax* r, int every);
 #define CCO_PROFILE_STAGES 9
 int cco_relax_profile(cco_relax* r, double* ms_sum, int64_t* launches);
 /* reps timed evaluations of B x N on device data: avg ms of the rho, reduce

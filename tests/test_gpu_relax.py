@@ -341,3 +341,47 @@ def test_lbfgs_advance_variant_selection(gpu_ctx) -> None:
         assert np.all(res.energy < 1e-20)
         sess.close()
     assert got == {100: 2, 150: 2}
+
+
+@pytest.mark.parametrize("method", ["lbfgs", "descent"])
+def test_relax_shards_bitwise(gpu_ctx, method: str) -> None:
+    """Lanes split over 1, 2 or 4 streams (shards) give bit-identical results and trajectories."""
+    import ctypes as C
+
+    from xla_cco import _native
+    from xla_cco.minimize import RelaxSession
+
+    n, d, b = 48, 2, 70
+    e, _, _ = _poisson(n, d, 0.25)
+    s0 = np.random.default_rng(21).uniform(size=(b, n, d))
+    if method == "lbfgs":
+        st = LBFGSSettings(record_every=3, max_records=40)
+    else:
+        st = DescentSettings(grow=2.0, max_evals=4000, energy_tol=1e-20, record_every=50, max_records=40)
+    L = _native.lib()
+    outs, used = [], []
+    for shards in (1, 2, 4):
+        sess = RelaxSession(e, b, n, d, st)
+        _native.check(L.cco_relax_set_option(sess.handle, 2, shards))
+        sess.init(s0)
+        v = C.c_int64(0)
+        _native.check(L.cco_relax_get_option(sess.handle, 2, C.byref(v)))
+        used.append(int(v.value))
+        sess.run(1_000_000, 256)
+        outs.append(sess.result())
+        sess.close()
+    assert used == [1, 2, 4]
+    a = outs[0]
+    for o in outs[1:]:
+        for f in ("s", "energy", "grad_norm", "n_evals", "n_iters", "status"):
+            assert np.array_equal(getattr(a, f), getattr(o, f)), f
+        assert np.array_equal(a.trajectory.count, o.trajectory.count)
+        assert np.array_equal(a.trajectory.energy, o.trajectory.energy)
+        assert np.array_equal(a.trajectory.s, o.trajectory.s)
+    # the default splits a batch this size in two
+    sess = RelaxSession(e, b, n, d, st)
+    sess.init(s0)
+    v = C.c_int64(0)
+    _native.check(L.cco_relax_get_option(sess.handle, 2, C.byref(v)))
+    assert int(v.value) == 2
+    sess.close()

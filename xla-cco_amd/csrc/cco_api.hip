@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <thread>
 #include <tuple>
 #include <vector>
 
@@ -349,6 +350,23 @@ enum StageId { STG_RHO = 0, STG_REDUCE = 1, STG_FORCE = 2, STG_ADVANCE = 3, STG_
 constexpr int METHOD_FLOW = CCO_METHOD_FLOW;
 constexpr int METHOD_MD = CCO_METHOD_VERLET;
 
+// A run splits a session's lanes into shards [lo, lo + count), each with its
+// own stream, active list, lane count and progress word, so that while one
+// shard streams its L-BFGS history through HBM another evaluates energies on
+// the vector units.  Every kernel indexes lanes, so a lane's arithmetic does
+// not depend on its shard.
+constexpr int kMaxShards = 4;
+constexpr int kShardMinLanes = 32;  // two shards by default from 2 x this many lanes
+
+struct Shard {
+  hipStream_t stream = nullptr;
+  int lo = 0, count = 0;
+  RelaxDev dev;  // the session's arrays with this shard's active list, count, progress
+  int slots_ub = 0;
+  int64_t steps = 0;  // evaluation steps queued since the last init
+  unsigned long long* progress_host = nullptr;
+};
+
 struct cco_relax {
   cco_ctx* ctx;
   cco_energy* E;
@@ -366,9 +384,18 @@ struct cco_relax {
   int64_t steps = 0;
   int64_t lane_evals = 0;
   bool ready = false;
-  int* host_count = nullptr;  // pinned
-  // optional per-stage timing with events on the context stream
-  bool profile = false;
+  int* host_count = nullptr;  // pinned, kMaxShards
+  unsigned long long* progress_host = nullptr;  // pinned, mapped, kMaxShards: compact_kernel
+  unsigned long long* progress_dev = nullptr;   // device address of progress_host
+  int* shard_counts = nullptr;                  // [kMaxShards] running lanes per shard
+  std::vector<Shard> shards;
+  hipStream_t extra_streams[kMaxShards] = {};   // streams of shards 1 ..
+  hipEvent_t fork_ev = nullptr;
+  int shard_req = 0;        // requested shards: 0 auto
+  int64_t launches = 0;     // evaluation steps queued over all shards since the last init
+  // optional per-stage timing: events around the launches of every
+  // profile_every-th step of each shard (0: off)
+  int profile_every = 0;
   int advance_variant = 0;  // requested: 0 auto, 1 global-memory, 2 register-resident, 3 Gram
   int variant_run = 0;      // the variant latched by the last init (never 0)
   int precision = 0;        // requested: 1 = split-precision evaluations
@@ -448,10 +475,10 @@ __global__ void relax_init_kernel(RelaxDev r, int method) {
   }
 }
 
-Batch relax_batch(cco_relax* R) {
+Batch relax_batch(cco_relax* R, const RelaxDev& dv) {
   Batch b;
-  b.active = R->dev.active;
-  b.n_active = R->dev.n_active;
+  b.active = dv.active;
+  b.n_active = dv.n_active;
   b.N = R->N;
   b.lane_stride = R->dev.stride;
   b.n_splits = n_splits_for(R->N);
@@ -478,26 +505,42 @@ Batch relax_batch(cco_relax* R) {
   return b;
 }
 
-void relax_stage(cco_relax* R, int stage) {
-  hipStream_t st = R->ctx->stream;
+void relax_stage(cco_relax* R, const Shard& sh, int stage) {
+  hipStream_t st = sh.stream;
+  const RelaxDev& dv = sh.dev;
+  const int ub = sh.slots_ub;
   if (stage <= STG_FORCE) {
-    launch_energy_stage(stage, R->E->geo, relax_batch(R), R->slots_ub, st);
+    launch_energy_stage(stage, R->E->geo, relax_batch(R, dv), ub, st);
   } else if (stage == STG_ADVANCE) {
     if (R->method == CCO_METHOD_DESCENT)
-      launch_descent_advance(R->dev, R->dc, R->slots_ub, st);
+      launch_descent_advance(dv, R->dc, ub, st);
     else if (R->method == CCO_METHOD_LBFGS)
-      launch_lbfgs_advance(R->dev, R->lc, R->slots_ub, st, R->variant_run);
+      launch_lbfgs_advance(dv, R->lc, ub, st, R->variant_run);
     else if (R->method == METHOD_FLOW)
-      launch_flow_advance(R->dev, R->fc, R->slots_ub, st);
+      launch_flow_advance(dv, R->fc, ub, st);
     else
-      launch_md_finish(R->dev, R->mc, R->slots_ub, st);
+      launch_md_finish(dv, R->mc, ub, st);
   } else if (stage == STG_COMPACT) {
-    launch_compact(R->dev, st);
+    launch_compact(dv, st, sh.lo, sh.count, (unsigned)(sh.steps + 1));
   } else if (stage >= STG_GRAM_DOT) {
-    launch_lbfgs_gram(R->dev, R->lc, R->slots_ub, stage - STG_GRAM_DOT + 1, st);
+    launch_lbfgs_gram(dv, R->lc, ub, stage - STG_GRAM_DOT + 1, st);
   } else {
-    launch_md_prepare(R->dev, R->mc, R->slots_ub, st);
+    launch_md_prepare(dv, R->mc, ub, st);
   }
+}
+
+// The whole session as one shard on the context stream, for the
+// external-energy path (no progress word: that path syncs every step).
+Shard whole_view(const cco_relax* R) {
+  Shard sh;
+  sh.stream = R->ctx->stream;
+  sh.lo = 0;
+  sh.count = R->B;
+  sh.dev = R->dev;
+  sh.dev.progress = nullptr;
+  sh.slots_ub = R->slots_ub;
+  sh.steps = R->steps;
+  return sh;
 }
 
 // Stage sequence of one evaluation step of the session's method.
@@ -535,16 +578,93 @@ int* resolve_prec(const cco_relax* R) {
 }
 
 int relax_advance(cco_relax* R) {
-  relax_stage(R, STG_ADVANCE);
+  const Shard sh = whole_view(R);
+  relax_stage(R, sh, STG_ADVANCE);
   if (R->method == CCO_METHOD_LBFGS && R->variant_run == 3) {
-    relax_stage(R, STG_GRAM_DOT);
-    relax_stage(R, STG_GRAM_SOLVE);
-    relax_stage(R, STG_GRAM_COMB);
+    relax_stage(R, sh, STG_GRAM_DOT);
+    relax_stage(R, sh, STG_GRAM_SOLVE);
+    relax_stage(R, sh, STG_GRAM_COMB);
   }
-  relax_stage(R, STG_COMPACT);
+  relax_stage(R, sh, STG_COMPACT);
   CCO_HIP(hipGetLastError());
   return CCO_OK;
 }
+
+// Pinned host words of a session: the lane counts read back at a sync, and
+// the progress words the compact kernel stores to (coherent, mapped, so the
+// host sees them while later steps are queued); the per-shard lane counts on
+// the device.
+int alloc_host_words(cco_relax* R) {
+  if (hipHostMalloc((void**)&R->host_count, sizeof(int) * kMaxShards) != hipSuccess)
+    return fail(CCO_ERR_DEVICE, "hipHostMalloc failed");
+  if (hipHostMalloc((void**)&R->progress_host, sizeof(unsigned long long) * kMaxShards,
+                    hipHostMallocMapped | hipHostMallocCoherent) != hipSuccess)
+    return fail(CCO_ERR_DEVICE, "hipHostMalloc (mapped) failed");
+  for (int k = 0; k < kMaxShards; ++k) R->progress_host[k] = ~0ull;
+  void* dp = nullptr;
+  if (hipHostGetDevicePointer(&dp, R->progress_host, 0) != hipSuccess)
+    return fail(CCO_ERR_DEVICE, "hipHostGetDevicePointer failed");
+  R->progress_dev = (unsigned long long*)dp;
+  R->dev.progress = nullptr;
+  return alloc_into(R, R->shard_counts, kMaxShards);
+}
+
+// Split the lanes into shards for the run being initialised and queue each
+// shard's first compaction (its active list and count) on the context stream.
+// Built-in-energy descent and L-BFGS sessions use two shards from
+// 2 kShardMinLanes lanes on (or the requested count); the others one.
+int setup_shards(cco_relax* R) {
+  int n = 1;
+  if (R->E != nullptr && (R->method == CCO_METHOD_DESCENT || R->method == CCO_METHOD_LBFGS))
+    n = R->shard_req > 0 ? R->shard_req : (R->B >= 2 * kShardMinLanes ? 2 : 1);
+  n = std::max(1, std::min(n, std::min(kMaxShards, R->B)));
+  R->shards.assign(n, Shard());
+  for (int k = 0; k < n; ++k) {
+    Shard& sh = R->shards[k];
+    if (k == 0) {
+      sh.stream = R->ctx->stream;
+    } else {
+      if (R->extra_streams[k] == nullptr)
+        CCO_HIP(hipStreamCreateWithFlags(&R->extra_streams[k], hipStreamNonBlocking));
+      sh.stream = R->extra_streams[k];
+    }
+    sh.lo = (int)((int64_t)R->B * k / n);
+    sh.count = (int)((int64_t)R->B * (k + 1) / n) - sh.lo;
+    sh.dev = R->dev;
+    sh.dev.active = R->dev.active + sh.lo;
+    sh.dev.n_active = R->shard_counts + k;
+    sh.dev.progress = R->progress_dev + k;
+    sh.progress_host = R->progress_host + k;
+    sh.slots_ub = sh.count;
+    sh.steps = 0;
+    *sh.progress_host = ~0ull;  // no step of this run has finished
+    launch_compact(sh.dev, R->ctx->stream, sh.lo, sh.count, 0u);  // tag 0 is never read
+  }
+  if (R->fork_ev == nullptr) CCO_HIP(hipEventCreateWithFlags(&R->fork_ev, hipEventDisableTiming));
+  CCO_HIP(hipGetLastError());
+  R->steps = 0;
+  R->launches = 0;
+  return CCO_OK;
+}
+
+// Steps of a shard still queued after the last one its progress word reports
+// finished.  The word (completed steps, running lanes) also lowers the
+// shard's launch bound as lanes stop (the count only falls during a run),
+// without draining the stream.  `first` is the shard's step count when the
+// current call began: only words tagged after it are this call's.
+int64_t shard_in_flight(Shard& sh, int64_t first) {
+  const unsigned long long w = __atomic_load_n(sh.progress_host, __ATOMIC_ACQUIRE);
+  const int64_t tag = (int64_t)(w >> 32);
+  int64_t done = first;
+  if (w != ~0ull && tag > first && tag <= sh.steps) {
+    done = tag;
+    sh.slots_ub = std::min(sh.slots_ub, (int)(w & 0xffffffffu));
+  }
+  return sh.steps - done;
+}
+
+// Steps are queued at most this many ahead of the device, per shard.
+constexpr int64_t kRunAhead = 16;
 
 int relax_sync_count(cco_relax* R, int* n_active) {
   CCO_HIP(hipMemcpyAsync(R->host_count, R->dev.n_active, sizeof(int), hipMemcpyDeviceToHost,
@@ -962,8 +1082,7 @@ static int relax_create_impl(cco_ctx* ctx, cco_energy* E, int B, int N, int d, i
     v.lhoff = d_lhoff;
     v.lnd = d_lnd;
   }
-  if (rc == 0 && hipHostMalloc((void**)&R->host_count, sizeof(int)) != hipSuccess)
-    rc = fail(CCO_ERR_DEVICE, "hipHostMalloc failed");
+  if (rc == 0) rc = alloc_host_words(R);
   if (rc) {
     cco_relax_destroy(R);
     return rc;
@@ -981,6 +1100,10 @@ int cco_relax_destroy(cco_relax* R) {
                     &R->pack_idx, &R->un_lay, &R->un_geo})
     dev_release(*b);
   if (R->host_count) (void)hipHostFree(R->host_count);
+  if (R->progress_host) (void)hipHostFree(R->progress_host);
+  for (hipStream_t s : R->extra_streams)
+    if (s != nullptr) (void)hipStreamDestroy(s);
+  if (R->fork_ev) (void)hipEventDestroy(R->fork_ev);
   for (hipEvent_t e : R->evs) (void)hipEventDestroy(e);
   delete R;
   return CCO_OK;
@@ -1000,7 +1123,7 @@ int cco_relax_init_dev(cco_relax* R, const double* s0_dev) {
                      R->method);
   CCO_HIP(hipGetLastError());
   R->slots_ub = R->B;
-  R->steps = 0;
+  if (int rc_sh = setup_shards(R)) return rc_sh;
   R->lane_evals = 0;
   R->ready = true;
   return CCO_OK;
@@ -1021,7 +1144,7 @@ int cco_relax_init(cco_relax* R, const double* s0_host) {
   CCO_HIP(hipGetLastError());
   CCO_HIP(hipStreamSynchronize(st));
   R->slots_ub = R->B;
-  R->steps = 0;
+  if (int rc_sh = setup_shards(R)) return rc_sh;
   R->lane_evals = 0;
   R->ready = true;
   return CCO_OK;
@@ -1032,46 +1155,103 @@ int cco_relax_step(cco_relax* R, int64_t n_evals, int* n_active) {
   if (!R->ready) return fail(CCO_ERR_STATE, "session not initialised");
   if (!R->E) return fail(CCO_ERR_STATE, "external-energy session: use pending/submit");
   DevScope ds(R->ctx->device);
-  hipStream_t st = R->ctx->stream;
-  int64_t done = 0;
+  const int nsh = (int)R->shards.size();
+  if (nsh > 1) {  // the other shards' streams start after the work queued on the context stream
+    CCO_HIP(hipEventRecord(R->fork_ev, R->ctx->stream));
+    for (int k = 1; k < nsh; ++k) CCO_HIP(hipStreamWaitEvent(R->shards[k].stream, R->fork_ev, 0));
+  }
   constexpr int kEv = kStages + 1;
-  const size_t need = R->profile ? (size_t)kEv * (size_t)std::max<int64_t>(n_evals, 0) : 0;
-  while (R->evs.size() < need) {
-    hipEvent_t ev;
-    CCO_HIP(hipEventCreate(&ev));
-    R->evs.push_back(ev);
+  int64_t first[kMaxShards];
+  bool live[kMaxShards], force[kMaxShards];
+  for (int k = 0; k < nsh; ++k) {
+    first[k] = R->shards[k].steps;
+    live[k] = R->shards[k].slots_ub > 0;
+    force[k] = false;
   }
-  std::vector<int> seq_ids;
-  for (int64_t i = 0; i < n_evals && R->slots_ub > 0; ++i) {
-    int seq[kStages];
-    const int ns = step_stages(R, seq);
-    for (int q = 0; q < ns; ++q) {
-      if (R->profile && q == 0) CCO_HIP(hipEventRecord(R->evs[kEv * i], st));
-      relax_stage(R, seq[q]);
-      if (R->profile) {
-        CCO_HIP(hipEventRecord(R->evs[kEv * i + q + 1], st));
-        seq_ids.push_back(seq[q]);
+  std::vector<int> seq_ids;  // per profiled step: its stage ids, then -1
+  size_t n_prof = 0;
+  // Round robin over the shards: queue a step on every shard that has fewer
+  // than kRunAhead steps in flight; stop a shard when its lanes have all
+  // stopped or it has queued n_evals steps.  When no shard can queue, wait;
+  // a stream that went idle without its word catching up may queue one step.
+  for (int64_t idle = 0;;) {
+    bool pending = false, queued = false;
+    for (int k = 0; k < nsh; ++k) {
+      if (!live[k]) continue;
+      Shard& sh = R->shards[k];
+      if (sh.steps - first[k] >= n_evals) {
+        live[k] = false;
+        continue;
+      }
+      const int64_t in_flight = shard_in_flight(sh, first[k]);
+      if (sh.slots_ub == 0) {
+        live[k] = false;
+        continue;
+      }
+      pending = true;
+      if (in_flight >= kRunAhead && !force[k]) continue;
+      force[k] = false;
+      const bool prof = R->profile_every > 0 && sh.steps % R->profile_every == 0;
+      while (prof && R->evs.size() < kEv * (n_prof + 1)) {
+        hipEvent_t ev;
+        CCO_HIP(hipEventCreate(&ev));
+        R->evs.push_back(ev);
+      }
+      int seq[kStages];
+      const int ns = step_stages(R, seq);
+      for (int q = 0; q < ns; ++q) {
+        if (prof && q == 0) CCO_HIP(hipEventRecord(R->evs[kEv * n_prof], sh.stream));
+        relax_stage(R, sh, seq[q]);
+        if (prof) {
+          CCO_HIP(hipEventRecord(R->evs[kEv * n_prof + q + 1], sh.stream));
+          seq_ids.push_back(seq[q]);
+        }
+      }
+      if (prof) {
+        seq_ids.push_back(-1);
+        ++n_prof;
+      }
+      CCO_HIP(hipGetLastError());
+      R->md_need_init = false;
+      sh.steps += 1;
+      R->launches += 1;
+      queued = true;
+    }
+    if (!pending) break;
+    if (queued) {
+      idle = 0;
+      continue;
+    }
+    if ((++idle & 1023) == 0) {
+      for (int k = 0; k < nsh; ++k) {
+        if (!live[k]) continue;
+        const hipError_t q = hipStreamQuery(R->shards[k].stream);
+        if (q == hipSuccess) force[k] = true;
+        else if (q != hipErrorNotReady) return fail(CCO_ERR_DEVICE, hipGetErrorString(q));
       }
     }
-    if (R->profile) seq_ids.push_back(-1);
-    CCO_HIP(hipGetLastError());
-    R->md_need_init = false;
-    R->steps += 1;
-    ++done;
+    std::this_thread::yield();
   }
-  int rc = relax_sync_count(R, n_active);
-  if (rc) return rc;
-  if (R->profile) {
-    size_t p = 0;
-    for (int64_t i = 0; i < done; ++i) {
-      for (int q = 0; p < seq_ids.size() && seq_ids[p] >= 0; ++q, ++p) {
-        float ms = 0.0f;
-        CCO_HIP(hipEventElapsedTime(&ms, R->evs[kEv * i + q], R->evs[kEv * i + q + 1]));
-        R->stage_ms[seq_ids[p]] += ms;
-        R->stage_n[seq_ids[p]] += 1;
-      }
-      ++p;  // step separator
+  for (int k = 0; k < nsh; ++k) CCO_HIP(hipStreamSynchronize(R->shards[k].stream));
+  CCO_HIP(hipMemcpy(R->host_count, R->shard_counts, sizeof(int) * nsh, hipMemcpyDeviceToHost));
+  int total = 0;
+  for (int k = 0; k < nsh; ++k) {
+    Shard& sh = R->shards[k];
+    sh.slots_ub = R->host_count[k];
+    total += sh.slots_ub;
+    R->steps = std::max(R->steps, sh.steps);
+  }
+  R->slots_ub = total;
+  if (n_active) *n_active = total;
+  size_t p = 0;
+  for (size_t i = 0; i < n_prof; ++i) {
+    for (int q = 0; p < seq_ids.size() && seq_ids[p] >= 0; ++q, ++p) {
+      float ms = 0.0f;
+      CCO_HIP(hipEventElapsedTime(&ms, R->evs[kEv * i + q], R->evs[kEv * i + q + 1]));
+      R->stage_ms[seq_ids[p]] += ms;
+      R->stage_n[seq_ids[p]] += 1;
     }
+    ++p;  // step separator
   }
   return CCO_OK;
 }
@@ -1115,6 +1295,10 @@ int cco_relax_get_option(cco_relax* R, int key, int64_t* value) {
     *value = R->variant_run;
     return CCO_OK;
   }
+  if (key == CCO_OPT_SHARDS) {
+    *value = (int64_t)R->shards.size();
+    return CCO_OK;
+  }
   return fail(CCO_ERR_ARG, "unknown option key");
 }
 
@@ -1128,6 +1312,11 @@ int cco_relax_set_option(cco_relax* R, int key, int64_t value) {
       return fail(CCO_ERR_ARG,
                   "Gram-form advance needs a uniform L-BFGS session with memory <= 128");
     R->advance_variant = (int)value;
+    return CCO_OK;
+  }
+  if (key == CCO_OPT_SHARDS) {
+    if (value < 0 || value > kMaxShards) return fail(CCO_ERR_ARG, "shards must be 0 (auto) to 4");
+    R->shard_req = (int)value;
     return CCO_OK;
   }
   return fail(CCO_ERR_ARG, "unknown option key");
@@ -1171,13 +1360,14 @@ int cco_relax_counters(cco_relax* R, int64_t* out4) {
   out4[0] = ev;
   out4[1] = it;
   out4[2] = pairs;
-  out4[3] = R->steps;
+  out4[3] = R->launches;
   return CCO_OK;
 }
 
-int cco_relax_set_profiling(cco_relax* R, int on) {
+int cco_relax_set_profiling(cco_relax* R, int every) {
   if (!R) return fail(CCO_ERR_ARG, "relax is NULL");
-  R->profile = on != 0;
+  if (every < 0) return fail(CCO_ERR_ARG, "profiling period must be >= 0");
+  R->profile_every = every;
   for (int s = 0; s < kStages; ++s) {
     R->stage_ms[s] = 0.0;
     R->stage_n[s] = 0;
@@ -1279,12 +1469,9 @@ int cco_relax_run(cco_relax* R, int64_t max_evals, int sync_every) {
   while (n_active > 0) {
     if (R->steps >= max_evals)
       return fail(CCO_ERR_BOUND, "relax did not finish within the round bound");
-    // shorter rounds once most lanes have stopped: steps launched after the
-    // last lane stops are wasted, while a host sync costs a few steps' time
-    int64_t every = sync_every;
-    if (n_active * 4 <= R->B) every = std::min<int64_t>(every, 64);
-    if (n_active * 32 <= R->B) every = std::min<int64_t>(every, 16);
-    const int64_t chunk = std::min<int64_t>(every, max_evals - R->steps);
+    // each call stops queueing steps as soon as the device reports that no
+    // lane runs, so the chunk length only bounds the time between syncs
+    const int64_t chunk = std::min<int64_t>(sync_every, max_evals - R->steps);
     int rc = cco_relax_step(R, chunk, &n_active);
     if (rc) return rc;
   }
@@ -1337,6 +1524,7 @@ int cco_relax_submit(cco_relax* R, const double* phi_host, const double* grad_ho
   rc = relax_advance(R);
   if (rc) return rc;
   R->steps += 1;
+  R->launches += 1;
   CCO_HIP(hipStreamSynchronize(st));
   return CCO_OK;
 }
@@ -1490,8 +1678,7 @@ static int finish_session(cco_relax* R) {
     if (rc == 0) rc = dev_reserve(R->cbox, sizeof(double2) * (size_t)R->B * R->E->geo.Mbox);
     if (rc == 0) rc = reserve_reduce(R->rphi, R->B, R->E->geo);
   }
-  if (rc == 0 && hipHostMalloc((void**)&R->host_count, sizeof(int)) != hipSuccess)
-    rc = fail(CCO_ERR_DEVICE, "hipHostMalloc failed");
+  if (rc == 0) rc = alloc_host_words(R);
   return rc;
 }
 
@@ -1596,7 +1783,7 @@ int cco_flow_init(cco_relax* R, const double* s0_host, const double* dt0_host) {
   CCO_HIP(hipGetLastError());
   CCO_HIP(hipStreamSynchronize(st));
   R->slots_ub = R->B;
-  R->steps = 0;
+  if (int rc_sh = setup_shards(R)) return rc_sh;
   R->ready = true;
   return CCO_OK;
 }
@@ -1724,7 +1911,7 @@ int cco_md_init(cco_relax* R, const double* r0, const double* v0, const double* 
   CCO_HIP(hipGetLastError());
   CCO_HIP(hipStreamSynchronize(st));
   R->slots_ub = R->B;
-  R->steps = 0;
+  if (int rc_sh = setup_shards(R)) return rc_sh;
   R->ready = true;
   R->md_need_init = true;
   // the initial force evaluation (no drift): e0 = Phi + KE(v0)

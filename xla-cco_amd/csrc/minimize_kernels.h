@@ -115,6 +115,9 @@ struct RelaxDev {
   int* prec;      // [B] 1 = split-precision evaluations (null: all FP64)
   int* active;    // [B]
   int* n_active;  // [1]
+  // mapped host word (null: none) that compact_kernel sets to
+  // (completed steps << 32) | running lanes after every step
+  unsigned long long* progress;
   double* s_eval; // [B][stride] trial point (energy input, s coordinates)
   double* g_eval; // [B][stride] energy gradient at the trial point
   double* phi_eval;  // [B]
@@ -185,7 +188,7 @@ inline int gram_stride_for(int m) { return 2 * m * m + 4 * m + 8; }
 void launch_lbfgs_gram(const RelaxDev& r, const LbfgsCfg& c, int slots_ub, int part,
                        hipStream_t st);
 int lbfgs_register_ept(int nd);  // elements per thread of the register variant, 0 if too large
-void launch_compact(const RelaxDev& r, hipStream_t st);
+void launch_compact(const RelaxDev& r, hipStream_t st, int lo, int count, unsigned step_tag);
 void launch_md_prepare(const RelaxDev& r, const MdCfg& c, int slots_ub, hipStream_t st);
 void launch_md_finish(const RelaxDev& r, const MdCfg& c, int slots_ub, hipStream_t st);
 void launch_flow_advance(const RelaxDev& r, const FlowCfg& c, int slots_ub, hipStream_t st);

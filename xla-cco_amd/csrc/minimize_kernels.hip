@@ -36,18 +36,21 @@ __device__ __forceinline__ double wrap01(double v) {
 }
 
 // ---------------------------------------------------------------------------
-// compaction: active list = lanes with status RUNNING, in lane order
+// compaction: active list = lanes lo .. lo + count - 1 with status RUNNING,
+// in lane order (one list per shard of a session)
 // ---------------------------------------------------------------------------
 __global__ __launch_bounds__(1024) void compact_kernel(const int* __restrict__ status,
                                                        int* __restrict__ active,
-                                                       int* __restrict__ n_active, int B) {
+                                                       int* __restrict__ n_active, int lo,
+                                                       int count, unsigned long long* progress,
+                                                       unsigned step_tag) {
   __shared__ int cnt[1024];
   const int t = threadIdx.x;
-  const int per = (B + 1023) / 1024;
-  const int lo = min(B, t * per);
-  const int hi = min(B, lo + per);
+  const int per = (count + 1023) / 1024;
+  const int a = lo + min(count, t * per);
+  const int e = min(lo + count, a + per);
   int c = 0;
-  for (int i = lo; i < hi; ++i) c += (status[i] == ST_RUNNING);
+  for (int i = a; i < e; ++i) c += (status[i] == ST_RUNNING);
   cnt[t] = c;
   __syncthreads();
   for (int off = 1; off < 1024; off <<= 1) {
@@ -57,14 +60,20 @@ __global__ __launch_bounds__(1024) void compact_kernel(const int* __restrict__ s
     __syncthreads();
   }
   int pos = cnt[t] - c;
-  for (int i = lo; i < hi; ++i)
+  for (int i = a; i < e; ++i)
     if (status[i] == ST_RUNNING) active[pos++] = i;
-  if (t == 1023) *n_active = cnt[1023];
+  if (t == 1023) {
+    *n_active = cnt[1023];
+    // the host reads this word while later steps are still queued
+    if (progress != nullptr)
+      __hip_atomic_store(progress, ((unsigned long long)step_tag << 32) | (unsigned)cnt[1023],
+                         __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
+  }
 }
 
-void launch_compact(const RelaxDev& r, hipStream_t st) {
+void launch_compact(const RelaxDev& r, hipStream_t st, int lo, int count, unsigned step_tag) {
   hipLaunchKernelGGL(compact_kernel, dim3(1), dim3(1024), 0, st, r.status, r.active, r.n_active,
-                     r.B);
+                     lo, count, r.progress, step_tag);
 }
 
 // ---------------------------------------------------------------------------
@@ -595,51 +604,62 @@ constexpr int kRegBlock = 1024;
 constexpr int kRegWaves = kRegBlock / kWave;
 
 struct Red2 {
-  double* buf;  // [2][2 * kRegWaves]
+  double* buf;  // [2][3 * kRegWaves]
   int parity;
 };
 
+// Sum of the kRegWaves wave partials p[stride * w] (w = lane % 16 of every
+// row) with row shifts: lane 15 ends with the pairwise tree
+// ((p0 + p1) + (p2 + p3)) + ... up to the order of each addition's operands,
+// which does not change a floating-point sum.  Uniform result.
+static_assert(kRegWaves == 16, "the partial sums fill one 16-lane DPP row");
+__device__ __forceinline__ double partials_sum(const double* p, int stride) {
+  double x = p[stride * (threadIdx.x % 16)];
+  x += dpp_d<0x111, 0xf, true>(x);  // row_shr:1
+  x += dpp_d<0x112, 0xf, true>(x);  // row_shr:2
+  x += dpp_d<0x114, 0xf, true>(x);  // row_shr:4
+  x += dpp_d<0x118, 0xf, true>(x);  // row_shr:8
+  return readlane_d(x, 15);
+}
+
 __device__ __forceinline__ double reg_sum(double v, Red2& R) {
   v = wave_sum(v);
-  double* b = R.buf + R.parity * 2 * kRegWaves;
+  double* b = R.buf + R.parity * 3 * kRegWaves;
   R.parity ^= 1;
   if ((threadIdx.x % kWave) == 0) b[threadIdx.x / kWave] = v;
   __syncthreads();
-  double t[kRegWaves];
-#pragma unroll
-  for (int i = 0; i < kRegWaves; ++i) t[i] = b[i];
-#pragma unroll
-  for (int w = 1; w < kRegWaves; w *= 2)
-#pragma unroll
-    for (int i = 0; i + w < kRegWaves; i += 2 * w) t[i] += t[i + w];
-  return t[0];
+  return partials_sum(b, 1);
 }
 
 __device__ __forceinline__ void reg_sum2(double& v0, double& v1, Red2& R) {
   v0 = wave_sum(v0);
   v1 = wave_sum(v1);
-  double* b = R.buf + R.parity * 2 * kRegWaves;
+  double* b = R.buf + R.parity * 3 * kRegWaves;
   R.parity ^= 1;
   if ((threadIdx.x % kWave) == 0) {
     b[2 * (threadIdx.x / kWave)] = v0;
     b[2 * (threadIdx.x / kWave) + 1] = v1;
   }
   __syncthreads();
-  double t0[kRegWaves], t1[kRegWaves];
-#pragma unroll
-  for (int i = 0; i < kRegWaves; ++i) {
-    t0[i] = b[2 * i];
-    t1[i] = b[2 * i + 1];
+  v0 = partials_sum(b, 2);
+  v1 = partials_sum(b + 1, 2);
+}
+
+__device__ __forceinline__ void reg_sum3(double& v0, double& v1, double& v2, Red2& R) {
+  v0 = wave_sum(v0);
+  v1 = wave_sum(v1);
+  v2 = wave_sum(v2);
+  double* b = R.buf + R.parity * 3 * kRegWaves;
+  R.parity ^= 1;
+  if ((threadIdx.x % kWave) == 0) {
+    b[3 * (threadIdx.x / kWave)] = v0;
+    b[3 * (threadIdx.x / kWave) + 1] = v1;
+    b[3 * (threadIdx.x / kWave) + 2] = v2;
   }
-#pragma unroll
-  for (int w = 1; w < kRegWaves; w *= 2)
-#pragma unroll
-    for (int i = 0; i + w < kRegWaves; i += 2 * w) {
-      t0[i] += t0[i + w];
-      t1[i] += t1[i + w];
-    }
-  v0 = t0[0];
-  v1 = t1[0];
+  __syncthreads();
+  v0 = partials_sum(b, 3);
+  v1 = partials_sum(b + 1, 3);
+  v2 = partials_sum(b + 2, 3);
 }
 
 template <int EPT>
@@ -710,76 +730,146 @@ __device__ void lbfgs_start_reg(const RelaxDev& r, const LbfgsCfg& c, int lane, 
     rhos[t] = (sl == slot_new) ? rho_new : rho[sl];
   }
   __syncthreads();
-  // the next PF pairs are in flight while one is reduced
-  constexpr int PF = EPT <= 2 ? 3 : (EPT == 4 ? 2 : 1);
-  double bs[PF][EPT], by[PF][EPT];
-  auto load_into = [&](double (&ds)[EPT], double (&dy)[EPT], int t) {
-    const int sl = (midx + t) % m;
-    const double* Si = S + (size_t)sl * nd;
-    const double* Yi = Y + (size_t)sl * nd;
+  if constexpr (EPT <= 4) {
+    // Pairs stream D ahead of the one being reduced.  Every load is
+    // unconditional (element indices clamped into the lane's vector, values
+    // past its end masked at use; pair indices clamped into the valid range)
+    // and both loops run to a multiple of D, the extra pairs with zero
+    // weight (which leaves q unchanged), so the loop body is straight-line
+    // and each use waits for its own pair only, not for the pairs in flight.
+    constexpr int D = EPT == 1 ? 6 : (EPT == 2 ? 4 : 2);
+    int ic[EPT];
+    bool mk[EPT];
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
       const int i = tid + e * kRegBlock;
-      ds[e] = i < nd ? Si[i] : 0.0;
-      dy[e] = i < nd ? Yi[i] : 0.0;
+      mk[e] = i < nd;
+      ic[e] = mk[e] ? i : nd - 1;
     }
-  };
-  // first loop: newest -> oldest (k = 0 is age m - 1)
+    double bs[D][EPT], by[D][EPT];
+    auto load_pair = [&](double (&ds)[EPT], double (&dy)[EPT], int t) {
+      const int sl = (midx + t) % m;
+      const double* Si = S + (size_t)sl * nd;
+      const double* Yi = Y + (size_t)sl * nd;
 #pragma unroll
-  for (int p = 0; p < PF; ++p)
-    if (p < nvalid) load_into(bs[p], by[p], m - 1 - p);
-  for (int k0 = 0; k0 < nvalid; k0 += PF) {
+      for (int e = 0; e < EPT; ++e) {
+        ds[e] = Si[ic[e]];
+        dy[e] = Yi[ic[e]];
+      }
+    };
+    const int npad = ((nvalid + D - 1) / D) * D;
+    const int last = nvalid - 1;
+    // first loop: newest -> oldest (pair k has age m - 1 - k)
+    if (nvalid > 0) {
 #pragma unroll
-    for (int p = 0; p < PF; ++p) {
-      const int k = k0 + p;
-      if (k < nvalid) {
-        const int t = m - 1 - k;
+      for (int p = 0; p < D; ++p) load_pair(bs[p], by[p], m - 1 - min(p, last));
+    }
+    for (int k0 = 0; k0 < npad; k0 += D) {
+#pragma unroll
+      for (int p = 0; p < D; ++p) {
+        const int k = k0 + p;
+        const int t = m - 1 - min(k, last);
+        const double w = k < nvalid ? rhos[t] : 0.0;
         double sc[EPT], yc[EPT];
 #pragma unroll
         for (int e = 0; e < EPT; ++e) {
           sc[e] = bs[p][e];
           yc[e] = by[p][e];
         }
-        if (k + PF < nvalid) load_into(bs[p], by[p], t - PF);
+        load_pair(bs[p], by[p], m - 1 - min(k + D, last));
         double dot = 0.0;
 #pragma unroll
-        for (int e = 0; e < EPT; ++e) dot += sc[e] * q[e];
+        for (int e = 0; e < EPT; ++e) dot += sc[e] * q[e];  // q is zero past the vector's end
         dot = reg_sum(dot, R);
-        const double a = rhos[t] * dot;
-        if (tid == 0) alphas[t] = a;
+        const double a = w * dot;
+        if (tid == 0 && k < nvalid) alphas[t] = a;
 #pragma unroll
-        for (int e = 0; e < EPT; ++e) q[e] = q[e] + (-a) * yc[e];
+        for (int e = 0; e < EPT; ++e) q[e] = mk[e] ? q[e] + (-a) * yc[e] : 0.0;
       }
     }
-  }
 #pragma unroll
-  for (int e = 0; e < EPT; ++e) q[e] = gamma * q[e];
-  // second loop: oldest -> newest (k = 0 is age m - nvalid)
+    for (int e = 0; e < EPT; ++e) q[e] = gamma * q[e];
+    // second loop: oldest -> newest (pair k has age m - nvalid + k)
+    if (nvalid > 0) {
 #pragma unroll
-  for (int p = 0; p < PF; ++p)
-    if (p < nvalid) load_into(bs[p], by[p], m - nvalid + p);
-  for (int k0 = 0; k0 < nvalid; k0 += PF) {
+      for (int p = 0; p < D; ++p) load_pair(bs[p], by[p], m - nvalid + min(p, last));
+    }
+    for (int k0 = 0; k0 < npad; k0 += D) {
 #pragma unroll
-    for (int p = 0; p < PF; ++p) {
-      const int k = k0 + p;
-      if (k < nvalid) {
-        const int t = m - nvalid + k;
+      for (int p = 0; p < D; ++p) {
+        const int k = k0 + p;
+        const int t = m - nvalid + min(k, last);
         double sc[EPT], yc[EPT];
 #pragma unroll
         for (int e = 0; e < EPT; ++e) {
           sc[e] = bs[p][e];
           yc[e] = by[p][e];
         }
-        if (k + PF < nvalid) load_into(bs[p], by[p], t + PF);
+        load_pair(bs[p], by[p], m - nvalid + min(k + D, last));
         double dot = 0.0;
 #pragma unroll
         for (int e = 0; e < EPT; ++e) dot += yc[e] * q[e];
         dot = reg_sum(dot, R);
-        const double bcoef = rhos[t] * dot;
-        const double coef = alphas[t] - bcoef;
+        const double coef = k < nvalid ? alphas[t] - rhos[t] * dot : 0.0;
 #pragma unroll
-        for (int e = 0; e < EPT; ++e) q[e] = q[e] + coef * sc[e];
+        for (int e = 0; e < EPT; ++e) q[e] = mk[e] ? q[e] + coef * sc[e] : 0.0;
       }
+    }
+  } else {
+    // One pair per reduction, one pair in flight (vectors of 8 elements per
+    // thread leave no registers for more).
+    double bs[EPT], by[EPT];
+    auto load_into = [&](double (&ds)[EPT], double (&dy)[EPT], int t) {
+      const int sl = (midx + t) % m;
+      const double* Si = S + (size_t)sl * nd;
+      const double* Yi = Y + (size_t)sl * nd;
+  #pragma unroll
+      for (int e = 0; e < EPT; ++e) {
+        const int i = tid + e * kRegBlock;
+        ds[e] = i < nd ? Si[i] : 0.0;
+        dy[e] = i < nd ? Yi[i] : 0.0;
+      }
+    };
+    // first loop: newest -> oldest (k = 0 is age m - 1)
+    if (nvalid > 0) load_into(bs, by, m - 1);
+    for (int k = 0; k < nvalid; ++k) {
+      const int t = m - 1 - k;
+      double sc[EPT], yc[EPT];
+  #pragma unroll
+      for (int e = 0; e < EPT; ++e) {
+        sc[e] = bs[e];
+        yc[e] = by[e];
+      }
+      if (k + 1 < nvalid) load_into(bs, by, t - 1);
+      double dot = 0.0;
+  #pragma unroll
+      for (int e = 0; e < EPT; ++e) dot += sc[e] * q[e];
+      dot = reg_sum(dot, R);
+      const double a = rhos[t] * dot;
+      if (tid == 0) alphas[t] = a;
+  #pragma unroll
+      for (int e = 0; e < EPT; ++e) q[e] = q[e] + (-a) * yc[e];
+    }
+  #pragma unroll
+    for (int e = 0; e < EPT; ++e) q[e] = gamma * q[e];
+    // second loop: oldest -> newest (k = 0 is age m - nvalid)
+    if (nvalid > 0) load_into(bs, by, m - nvalid);
+    for (int k = 0; k < nvalid; ++k) {
+      const int t = m - nvalid + k;
+      double sc[EPT], yc[EPT];
+  #pragma unroll
+      for (int e = 0; e < EPT; ++e) {
+        sc[e] = bs[e];
+        yc[e] = by[e];
+      }
+      if (k + 1 < nvalid) load_into(bs, by, t + 1);
+      double dot = 0.0;
+  #pragma unroll
+      for (int e = 0; e < EPT; ++e) dot += yc[e] * q[e];
+      dot = reg_sum(dot, R);
+      const double coef = alphas[t] - rhos[t] * dot;
+  #pragma unroll
+      for (int e = 0; e < EPT; ++e) q[e] = q[e] + coef * sc[e];
     }
   }
   double slope = 0.0;
@@ -822,7 +912,7 @@ __device__ void lbfgs_start_reg(const RelaxDev& r, const LbfgsCfg& c, int lane, 
 
 template <int EPT>
 __global__ __launch_bounds__(kRegBlock) void lbfgs_advance_reg_kernel(RelaxDev r, LbfgsCfg c) {
-  __shared__ double redbuf[4 * kRegWaves];
+  __shared__ double redbuf[6 * kRegWaves];
   __shared__ double alphas[1024];
   __shared__ double rhos[1024];
   Red2 R{redbuf, 0};
