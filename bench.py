@@ -272,58 +272,60 @@ def main() -> int:
         one_step(inputs[w])
     ctx.synchronize()
 
-    _native.check(L.cco_relax_set_profiling(sess.handle, 0 if args.no_profile else args.profile_every))
-    converged = 0
-    lane_evals = 0
-    iters = 0
-    pairs = 0
-    steps_total = 0
-    elapsed = 0.0
-    for k in range(args.steps):
-        ctx.synchronize()
-        t0 = time.perf_counter()
-        one_step(inputs[args.warmup + k])
-        ctx.synchronize()
-        elapsed += time.perf_counter() - t0
-        e = out_e.to_host()
-        converged += int(np.sum(e < tol))
-        cnt = (C.c_int64 * 4)()
-        _native.check(L.cco_relax_counters(sess.handle, cnt))
-        lane_evals += cnt[0]
-        iters += cnt[1]
-        pairs += cnt[2]
-        steps_total += cnt[3]
-    opt = C.c_int64(0)
-    _native.check(L.cco_relax_get_option(sess.handle, 1, C.byref(opt)))
-    variant_run = int(opt.value)
-    _native.check(L.cco_relax_get_option(sess.handle, 2, C.byref(opt)))
-    shards_run = int(opt.value)
-    ms = (C.c_double * len(STAGES))()
-    launches = (C.c_int64 * len(STAGES))()
-    _native.check(L.cco_relax_profile(sess.handle, ms, launches))
+    def timed(shards: int, first_input: int, nsteps: int) -> dict:
+        """nsteps timed relaxes with `shards` lane shards; totals, counters and stage times."""
+        _native.check(L.cco_relax_set_option(sess.handle, 2, shards))
+        _native.check(L.cco_relax_set_profiling(sess.handle, 0 if args.no_profile else args.profile_every))
+        r = {"converged": 0, "lane_evals": 0, "iters": 0, "pairs": 0, "launches": 0, "elapsed": 0.0}
+        for k in range(nsteps):
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            one_step(inputs[first_input + k])
+            ctx.synchronize()
+            r["elapsed"] += time.perf_counter() - t0
+            e = out_e.to_host()
+            r["converged"] += int(np.sum(e < tol))
+            cnt = (C.c_int64 * 4)()
+            _native.check(L.cco_relax_counters(sess.handle, cnt))
+            r["lane_evals"] += cnt[0]
+            r["iters"] += cnt[1]
+            r["pairs"] += cnt[2]
+            r["launches"] += cnt[3]
+        opt = C.c_int64(0)
+        _native.check(L.cco_relax_get_option(sess.handle, 1, C.byref(opt)))
+        r["variant"] = int(opt.value)
+        _native.check(L.cco_relax_get_option(sess.handle, 2, C.byref(opt)))
+        r["shards"] = int(opt.value)
+        ms = (C.c_double * len(STAGES))()
+        launches = (C.c_int64 * len(STAGES))()
+        _native.check(L.cco_relax_profile(sess.handle, ms, launches))
+        r["ms"], r["timed_launches"] = list(ms), list(launches)
+        return r
 
-    value = converged / elapsed
-    evals_per_conv = lane_evals / max(converged, 1)
     nd = n * d
-    # algorithmic work per launch of each kernel
-    lane_evals_per_launch = lane_evals / max(steps_total, 1)
-    gram = variant_run == 3
-    per_step = 1.0 / max(steps_total, 1)
-    work = {
-        "rho_kernel": ("mfma", 8.0 * M * n * lane_evals_per_launch),
-        "force_kernel": ("mfma", 8.0 * M * n * d * lane_evals_per_launch),
-        # fused two-loop: s_i, y_i read twice per visited pair; Gram form: line search only
-        "lbfgs_advance_kernel": (
-            "hbm",
-            ((0.0 if gram else 32.0 * nd * pairs) + 8.0 * nd * (3 * lane_evals + 12 * iters)) * per_step,
-        ),
-        # Gram form: one read of s_i, y_i per pair in each of the dot and direction passes
-        "lbfgs_gram_dot_kernel": ("hbm", (16.0 * nd * pairs + 24.0 * nd * iters) * per_step),
-        "lbfgs_gram_comb_kernel": ("hbm", (16.0 * nd * pairs + 48.0 * nd * iters) * per_step),
-    }
-    roofline = None
-    stage_avg = {}
-    if not args.no_profile and launches[0] > 0:
+
+    def roofline_of(r: dict):
+        """Per-kernel averages and the roofline of the kernel with the largest total time."""
+        if args.no_profile or r["timed_launches"][0] == 0:
+            return {}, None
+        ms, launches = r["ms"], r["timed_launches"]
+        # algorithmic work per launch of each kernel
+        per_step = 1.0 / max(r["launches"], 1)
+        lane_evals_per_launch = r["lane_evals"] * per_step
+        gram = r["variant"] == 3
+        work = {
+            "rho_kernel": ("mfma", 8.0 * M * n * lane_evals_per_launch),
+            "force_kernel": ("mfma", 8.0 * M * n * d * lane_evals_per_launch),
+            # fused two-loop: s_i, y_i read twice per visited pair; Gram form: line search only
+            "lbfgs_advance_kernel": (
+                "hbm",
+                ((0.0 if gram else 32.0 * nd * r["pairs"]) + 8.0 * nd * (3 * r["lane_evals"] + 12 * r["iters"]))
+                * per_step,
+            ),
+            # Gram form: one read of s_i, y_i per pair in each of the dot and direction passes
+            "lbfgs_gram_dot_kernel": ("hbm", (16.0 * nd * r["pairs"] + 24.0 * nd * r["iters"]) * per_step),
+            "lbfgs_gram_comb_kernel": ("hbm", (16.0 * nd * r["pairs"] + 48.0 * nd * r["iters"]) * per_step),
+        }
         stage_avg = {STAGES[i]: ms[i] / launches[i] for i in range(len(STAGES)) if launches[i] > 0}
         dominant = max((i for i in range(len(STAGES)) if launches[i] > 0), key=lambda i: ms[i])
         name = STAGES[dominant]
@@ -339,12 +341,11 @@ def main() -> int:
         tfile = ROOT / "profiles" / "traffic.json"
         if tfile.exists():
             try:
-                tdat = json.loads(tfile.read_text()).get(name)
-                if tdat and tdat.get("workload") == f"d{d}_n{n}_chi{chi}_b{B}_s{shards_run}":
-                    traffic = tdat.get("bytes_per_launch")
+                tdat = json.loads(tfile.read_text()).get(f"d{d}_n{n}_chi{chi}_b{B}_s{r['shards']}", {})
+                traffic = tdat.get(name, {}).get("bytes_per_launch")
             except (ValueError, AttributeError):
                 traffic = None
-        roofline = {
+        return stage_avg, {
             "kernel": name,
             "bound": bound_kind,
             "achieved": achieved,
@@ -354,10 +355,22 @@ def main() -> int:
             "traffic": traffic,
             "avg_launch_ms": avg_ms,
             "work_per_launch": per_launch,
-            # lane shards run on separate streams: a launch covers 1/streams of the
-            # lanes and its duration includes co-running with the other shards' kernels
-            "streams": shards_run,
+            # lane shards run on separate streams: a launch covers 1/streams of the lanes
+            # and its event-timed duration includes waiting for and sharing the CUs with
+            # the other shards' kernels
+            "streams": r["shards"],
         }
+
+    main_run = timed(args.shards, args.warmup, args.steps)
+    converged, lane_evals, elapsed = main_run["converged"], main_run["lane_evals"], main_run["elapsed"]
+    variant_run, shards_run = main_run["variant"], main_run["shards"]
+    value = converged / elapsed
+    evals_per_conv = lane_evals / max(converged, 1)
+    stage_avg, roofline = roofline_of(main_run)
+    # the same kernels alone on one stream (one more timed relax of the first input)
+    roofline_one = None
+    if roofline is not None and shards_run > 1:
+        _, roofline_one = roofline_of(timed(1, args.warmup, 1))
 
     cpu = None
     if not args.no_cpu:
@@ -395,6 +408,7 @@ def main() -> int:
         "evals_per_converged": evals_per_conv,
         "kernel_avg_ms": stage_avg,
         "roofline": roofline,
+        "roofline_one_stream": roofline_one,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))
