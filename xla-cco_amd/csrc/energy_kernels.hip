@@ -346,8 +346,8 @@ __device__ __forceinline__ void force_member(const int4 m, const double2 B, cons
 #pragma unroll 4
   for (int r = 0; r < m.y; ++r) {
     const double2 c = Cm[r];
-    const double uu = c.y * e.x - c.x * e.y;
-    P += uu;
+    P = fma(c.y, e.x, P);  // P += Im(C conj e), folded into two FMAs
+    P = fma(-c.x, e.y, P);
     Q += P;
     e = cmul(e, z1);
   }
