@@ -187,6 +187,12 @@ int cco_relax_precision_state(cco_relax* r, int* active_mode, int* lanes_split);
  * FP64 (0, default); *applied reports whether the k-set allows it. */
 int cco_energy_set_precision(cco_energy* e, int mode, int* applied);
 
+/* Closed-form Hessians d^2 Phi / ds_ja ds_lb of B configurations (s:
+ * B x N x d), each flattened row-major to (N d) x (N d) in h.  Replaces
+ * StealthyEnergy.hessian (src/xla_cco/energy.py:272-299). */
+int cco_hessian(cco_energy* e, const double* s_host, int B, int N, double* h_host);
+int cco_hessian_dev(cco_energy* e, const double* s_dev, int B, int N, double* h_dev);
+
 /* Gradient flow dr/dt = -grad_r Phi in the Cartesian metric of basis
  * (d x d host, rows = lattice vectors), Dormand-Prince 5(4) with error
  * control on the RMS Cartesian step and on the clocks (t, arclength).

@@ -18,6 +18,10 @@ Parity pins (what checks each function; tests/test_oracle_*.py):
   single-particle closed form Phi = sum(w)/V, a lattice below its first shell
   (Phi ~ 0), a central finite difference of the energy, translation
   invariance (sum of gradients ~ 0) and the C++ Cartesian force relation.
+* ``hessian``: central finite differences of ``energy_grad`` (the FD pin of
+  the gradient carries over), symmetry, and the translation null space (rows
+  sum to zero over the particles of each axis, as Phi is translation
+  invariant), which a dropped diagonal term or a transposed block fails.
 * ``descent``: an independent transcription of the C++ loop (the one that
   recomputes f1 and g every iteration, test_descent.py:52-89), the 40-step
   floor (1e-3 * 0.5^40 < 1e-15) and the budget rules (max_evals // 2).
@@ -178,6 +182,31 @@ def energy_grad(
         coef = w[lo : lo + chunk] * (im[lo : lo + chunk] * np.cos(th) - re[lo : lo + chunk] * np.sin(th))
         acc = acc + coef @ nc
     return value, (2.0 * TWO_PI / volume) * acc, re, im
+
+
+def hessian(s: np.ndarray, n: np.ndarray, w: np.ndarray, volume: float) -> np.ndarray:
+    """Closed-form Hessian d^2 Phi / ds_ja ds_lb, flattened row-major to (N d, N d).
+
+    energy.py:272-299: with theta_kj = 2 pi n_k.s_j,
+    H[(j,a),(l,b)] = (2 (2 pi)^2 / V) sum_k w_k n_ka n_kb
+                     [cos th_kj cos th_kl + sin th_kj sin th_kl
+                      - delta_jl (Re rho_k cos th_kj + Im rho_k sin th_kj)].
+    """
+    s = np.asarray(s, dtype=np.float64)
+    n = np.asarray(n, dtype=np.float64)
+    N, d = s.shape
+    th = TWO_PI * (s @ n.T)  # (N, M)
+    c, sn = np.cos(th), np.sin(th)
+    re, im = c.sum(axis=0), sn.sum(axis=0)
+    pref = 2.0 * TWO_PI**2 / volume
+    core = re * c + im * sn  # (N, M)
+    h = np.zeros((N, d, N, d))
+    for a in range(d):
+        for b in range(d):
+            wab = w * n[:, a] * n[:, b]
+            pair = (c * wab) @ c.T + (sn * wab) @ sn.T
+            h[:, a, :, b] = pref * (pair - np.diag(core @ wab))
+    return h.reshape(N * d, N * d)
 
 
 def cartesian_forces(

@@ -199,3 +199,30 @@ def test_verlet_step_with_gpu_force_fn(gpu_ctx) -> None:
     r1, v1, out = verlet_step(r, v, f, fn, 0.01)
     ref = O.verlet_nve(vg, basis, r, v, 0.01, 1)
     assert np.max(np.abs(r1 - ref["r"])) < 1e-13 and np.max(np.abs(v1 - ref["v"])) < 1e-12
+
+
+def test_harmonic_sample_matches_oracle_hessian(gpu_ctx) -> None:
+    """thermal.py:118-153 with the GPU Hessian equals the same draw from the oracle's Hessian."""
+    from oracle import oracle as O
+    from xla_cco import kvectors as K
+    from xla_cco.box import cubic_box
+    from xla_cco.dynamics.thermal import harmonic_sample
+    from xla_cco.energy import AbstractEnergy, make_stealthy_energy
+
+    n, d = 24, 2
+    box = cubic_box(n, d)
+    ks = K.enumerate_kvectors(box.basis, 0.0, 4.2)
+    e = make_stealthy_energy(box, ks)
+
+    class OracleHessian(AbstractEnergy):
+        def __call__(self, s):  # type: ignore[no-untyped-def]
+            return O.energy_grad(s, ks.n, ks.weights, box.volume)[0]
+
+        def hessian(self, s):  # type: ignore[no-untyped-def]
+            return O.hessian(s, ks.n, ks.weights, box.volume)
+
+    s = np.random.default_rng(50).uniform(size=(n, d))
+    a = harmonic_sample(np.random.default_rng(7), e, s, box.basis, 1e-3)
+    b = harmonic_sample(np.random.default_rng(7), OracleHessian(), s, box.basis, 1e-3)
+    assert np.allclose(a, b, rtol=0, atol=1e-9)
+    assert not np.allclose(a, s @ box.basis, atol=1e-6)

@@ -148,3 +148,46 @@ def test_rejects_float32_and_bad_shapes(gpu_ctx) -> None:
         e.value_and_grad(np.zeros((8, 3)))
     with pytest.raises(ValueError):
         make_stealthy_energy(box, K.enumerate_kvectors(box.basis, 0.0, 0.1))
+
+
+# ---------------- Hessian (energy.py:272-299) ----------------
+
+RTOL_H = 1e-11  # of max |H|; the pair term is a K = 2M FP64 product
+
+
+@pytest.mark.parametrize("n,d,k_outer", [(1, 2, 9.0), (12, 1, 9.0), (16, 2, 4.0), (30, 3, 3.4), (150, 2, 5.5)])
+def test_hessian_matches_oracle(gpu_ctx, n: int, d: int, k_outer: float) -> None:
+    e, ks, box = _energy(n, d, k_outer)
+    s = np.random.default_rng(40 + n).uniform(size=(n, d))
+    H = e.hessian(s)
+    Ho = O.hessian(s, ks.n, ks.weights, box.volume)
+    assert H.shape == (n * d, n * d)
+    # the size of the summed terms: a vanishing Hessian (one particle) is judged against it
+    terms = 2.0 * (2 * np.pi) ** 2 / box.volume * float(np.sum(ks.weights * np.sum(ks.n**2, axis=1))) * n
+    scale = max(np.abs(Ho).max(), terms)
+    assert np.abs(H - Ho).max() <= RTOL_H * scale, np.abs(H - Ho).max() / scale
+    assert np.array_equal(H, H.T) or np.abs(H - H.T).max() <= 1e-13 * scale
+
+
+def test_hessian_batch_equals_single_and_null_space(gpu_ctx) -> None:
+    e, ks, box = _energy(40, 2, 4.6)
+    s = np.random.default_rng(41).uniform(size=(3, 40, 2))
+    Hb = e.hessian_batch(s)
+    for i in range(3):
+        assert np.array_equal(Hb[i], e.hessian(s[i]))
+    scale = np.abs(Hb[0]).max()
+    for b in range(2):
+        t = np.zeros((40, 2))
+        t[:, b] = 1.0
+        assert np.abs(Hb[0] @ t.reshape(-1)).max() < 1e-10 * scale
+    # the gradient's finite difference along one coordinate is the matching column
+    h = 1e-6
+    sp, sm = s[0].copy(), s[0].copy()
+    sp[5, 1] += h
+    sm[5, 1] -= h
+    col = (e.value_and_grad(sp)[1] - e.value_and_grad(sm)[1]).reshape(-1) / (2 * h)
+    assert np.allclose(Hb[0][:, 5 * 2 + 1], col, atol=1e-5 * scale)
+    with pytest.raises(TypeError):
+        e.hessian(s[0].astype(np.float32))
+    with pytest.raises(ValueError):
+        e.hessian(s[0][:, :1].copy())

@@ -189,3 +189,63 @@ def test_a_planted_sign_error_is_caught() -> None:
     fd = (O.energy_grad(sp, n, w, vol)[0] - O.energy_grad(sm, n, w, vol)[0]) / (2 * h)
     assert abs(good[0, 0] - fd) < 1e-6
     assert abs(bad[0, 0] - fd) > 1e-3
+
+
+# ---------------- Hessian (energy.py:272-299) ----------------
+
+
+def _fd_hessian(s, n, w, vol, h=1e-6):
+    N, d = s.shape
+    out = np.zeros((N * d, N * d))
+    for j in range(N):
+        for a in range(d):
+            sp, sm = s.copy(), s.copy()
+            sp[j, a] += h
+            sm[j, a] -= h
+            gp = O.energy_grad(sp, n, w, vol)[1].reshape(-1)
+            gm = O.energy_grad(sm, n, w, vol)[1].reshape(-1)
+            out[:, j * d + a] = (gp - gm) / (2 * h)
+    return out
+
+
+@pytest.mark.parametrize("N,d,k_outer", [(8, 2, 4.0), (6, 3, 3.2), (12, 1, 9.0)])
+def test_hessian_matches_finite_difference_of_gradient(N: int, d: int, k_outer: float) -> None:
+    n, w, vol, _ = small_kset(N, d, k_outer)
+    s = np.random.default_rng(30 + N).uniform(size=(N, d))
+    H = O.hessian(s, n, w, vol)
+    fd = _fd_hessian(s, n, w, vol)
+    assert np.allclose(H, fd, atol=2e-6 * max(1.0, np.abs(H).max()))
+
+
+def test_hessian_symmetric_with_translation_null_space() -> None:
+    n, w, vol, _ = small_kset(10, 3, 3.5)
+    s = np.random.default_rng(31).uniform(size=(10, 3))
+    H = O.hessian(s, n, w, vol)
+    scale = np.abs(H).max()
+    assert np.allclose(H, H.T, atol=1e-13 * scale)
+    # Phi(s + c) = Phi(s): the Hessian maps a uniform translation along any axis to zero
+    for b in range(3):
+        t = np.zeros((10, 3))
+        t[:, b] = 1.0
+        assert np.abs(H @ t.reshape(-1)).max() < 1e-11 * scale
+
+
+def test_hessian_planted_missing_diagonal_is_caught() -> None:
+    """Dropping the delta_jl term (planted in a copy) fails the translation pin and the FD pin."""
+    N, d = 7, 2
+    n, w, vol, _ = small_kset(N, d, 4.0)
+    s = np.random.default_rng(32).uniform(size=(N, d))
+    nf = n.astype(np.float64)
+    th = 2 * math.pi * (s @ nf.T)
+    c, sn = np.cos(th), np.sin(th)
+    pref = 2.0 * (2 * math.pi) ** 2 / vol
+    bad = np.zeros((N, d, N, d))
+    for a in range(d):
+        for b in range(d):
+            wab = w * nf[:, a] * nf[:, b]
+            bad[:, a, :, b] = pref * ((c * wab) @ c.T + (sn * wab) @ sn.T)
+    bad = bad.reshape(N * d, N * d)
+    t = np.zeros((N, d))
+    t[:, 0] = 1.0
+    assert np.abs(bad @ t.reshape(-1)).max() > 1e-3 * np.abs(bad).max()
+    assert not np.allclose(bad, _fd_hessian(s, n, w, vol), atol=1e-4)

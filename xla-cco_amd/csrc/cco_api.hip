@@ -30,6 +30,7 @@ using namespace cco;
 struct cco_ctx {
   int device;
   hipStream_t stream;
+  void* blas = nullptr;  // rocBLAS handle, created on first use (Hessians)
 };
 
 // RAII device scope: make the context's device current for the call.
@@ -71,6 +72,10 @@ struct cco_energy {
   int ident_B = 0;
   int precision = 0;  // 1: split-precision MFMA evaluations (when the k-set allows)
   int prec_B = 0;     // lanes of prec filled with 1
+  // the k-set as given (n_k1, n_k2, n_k3, w_k per row) and the Hessian workspaces
+  std::vector<double> kn_host;
+  DevBuf kn, hs, hX, hXw, hG, hrho, hD, hH;
+  bool kn_ready = false;
 };
 
 namespace {
@@ -717,6 +722,7 @@ int cco_ctx_destroy(cco_ctx* ctx) {
   if (!ctx) return CCO_OK;
   DevScope ds(ctx->device);
   (void)hipStreamSynchronize(ctx->stream);
+  blas_destroy(ctx->blas);
   (void)hipStreamDestroy(ctx->stream);
   delete ctx;
   return CCO_OK;
@@ -771,6 +777,11 @@ int cco_energy_create(cco_ctx* ctx, const int64_t* n, const double* w, int M, in
   DevScope ds(ctx->device);
   cco_energy* E = new cco_energy();
   E->ctx = ctx;
+  E->kn_host.assign((size_t)4 * M, 0.0);
+  for (int k = 0; k < M; ++k) {
+    for (int a = 0; a < d; ++a) E->kn_host[4 * (size_t)k + a] = (double)n[(size_t)k * d + a];
+    E->kn_host[4 * (size_t)k + 3] = w[k];
+  }
   int rc = build_geometry(E, n, w, M, d, volume, pad_to);
   if (rc) {
     dev_release(E->tables);
@@ -786,9 +797,55 @@ int cco_energy_destroy(cco_energy* E) {
   DevScope ds(E->ctx->device);
   (void)hipStreamSynchronize(E->ctx->stream);
   for (DevBuf* b : {&E->tables, &E->s, &E->phi, &E->grad, &E->re, &E->im, &E->part, &E->cbox,
-                    &E->active, &E->n_active, &E->rphi, &E->prec})
+                    &E->active, &E->n_active, &E->rphi, &E->prec, &E->kn, &E->hs, &E->hX, &E->hXw,
+                    &E->hG, &E->hrho, &E->hD, &E->hH})
     dev_release(*b);
   delete E;
+  return CCO_OK;
+}
+
+int cco_hessian_dev(cco_energy* E, const double* s_dev, int B, int N, double* h_dev) {
+  if (!E || !s_dev || !h_dev) return fail(CCO_ERR_ARG, "NULL argument");
+  if (B < 1 || N < 1) return fail(CCO_ERR_ARG, "need B >= 1 and N >= 1");
+  DevScope ds(E->ctx->device);
+  cco_ctx* ctx = E->ctx;
+  if (ctx->blas == nullptr && (ctx->blas = blas_create()) == nullptr)
+    return fail(CCO_ERR_DEVICE, "rocblas_create_handle failed");
+  const int M = E->geo.M, d = E->geo.d;
+  hipStream_t st = ctx->stream;
+  int rc;
+  if (!E->kn_ready) {
+    if ((rc = dev_reserve(E->kn, sizeof(double) * E->kn_host.size()))) return rc;
+    CCO_HIP(hipMemcpyAsync(E->kn.p, E->kn_host.data(), sizeof(double) * E->kn_host.size(),
+                           hipMemcpyHostToDevice, st));
+    E->kn_ready = true;
+  }
+  const size_t xs = (size_t)B * 2 * M * N;
+  if ((rc = dev_reserve(E->hX, sizeof(double) * xs))) return rc;
+  if ((rc = dev_reserve(E->hXw, sizeof(double) * xs))) return rc;
+  if ((rc = dev_reserve(E->hG, sizeof(double) * B * (size_t)N * N))) return rc;
+  if ((rc = dev_reserve(E->hrho, sizeof(double) * B * (size_t)2 * M))) return rc;
+  if ((rc = dev_reserve(E->hD, sizeof(double) * B * (size_t)N))) return rc;
+  if (hessian_batch(st, ctx->blas, s_dev, B, N, d, M, (const double*)E->kn.p, E->geo.volume,
+                    (double*)E->hX.p, (double*)E->hXw.p, (double*)E->hG.p, (double*)E->hrho.p,
+                    (double*)E->hD.p, h_dev) != 0)
+    return fail(CCO_ERR_DEVICE, "Hessian launch failed (rocBLAS or kernel)");
+  return CCO_OK;
+}
+
+int cco_hessian(cco_energy* E, const double* s_host, int B, int N, double* h_host) {
+  if (!E || !s_host || !h_host) return fail(CCO_ERR_ARG, "NULL argument");
+  if (B < 1 || N < 1) return fail(CCO_ERR_ARG, "need B >= 1 and N >= 1");
+  DevScope ds(E->ctx->device);
+  hipStream_t st = E->ctx->stream;
+  const size_t nd = (size_t)N * E->geo.d;
+  int rc;
+  if ((rc = dev_reserve(E->hs, sizeof(double) * B * nd))) return rc;
+  if ((rc = dev_reserve(E->hH, sizeof(double) * B * nd * nd))) return rc;
+  CCO_HIP(hipMemcpyAsync(E->hs.p, s_host, sizeof(double) * B * nd, hipMemcpyHostToDevice, st));
+  if ((rc = cco_hessian_dev(E, (const double*)E->hs.p, B, N, (double*)E->hH.p))) return rc;
+  CCO_HIP(hipMemcpyAsync(h_host, E->hH.p, sizeof(double) * B * nd * nd, hipMemcpyDeviceToHost, st));
+  CCO_HIP(hipStreamSynchronize(st));
   return CCO_OK;
 }
 

@@ -158,6 +158,16 @@ size_t mp_lds_bytes(const Geo& g);
 int mp_force_batch_for(const Geo& g);  // 16-column tiles per force batch (0: does not fit)
 void launch_energy_mixed(int stage, const Geo& g, const Batch& b, int slots_ub, hipStream_t st);
 
+// Closed-form Hessians of B configurations (hessian.hip).  kn: n_k1, n_k2,
+// n_k3, w_k per k (M rows); workspaces X, Xw: B x 2M x N, G: B x N x N,
+// rho: B x 2M, D: B x N; H: B x (N d)^2 row-major.  blas is a rocBLAS handle
+// (blas_create).  Returns 0 on success.
+int hessian_batch(hipStream_t st, void* blas, const double* s, int B, int N, int d, int M,
+                  const double* kn, double volume, double* X, double* Xw, double* G,
+                  double* rho, double* D, double* H);
+void* blas_create();
+void blas_destroy(void* blas);
+
 // ---------------- small helpers ----------------
 
 struct Error {

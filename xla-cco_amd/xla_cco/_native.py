@@ -140,6 +140,8 @@ SIGNATURES = {
     "cco_relax_set_precision": (C.c_int, [_P, C.c_int, C.c_double]),
     "cco_relax_precision_state": (C.c_int, [_P, C.POINTER(C.c_int), C.POINTER(C.c_int)]),
     "cco_energy_set_precision": (C.c_int, [_P, C.c_int, C.POINTER(C.c_int)]),
+    "cco_hessian": (C.c_int, [_P, _PD, C.c_int, C.c_int, _PD]),
+    "cco_hessian_dev": (C.c_int, [_P, _P, C.c_int, C.c_int, _P]),
     "cco_flow_create": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int, _P, _PD, C.POINTER(_P)]),
     "cco_flow_init": (C.c_int, [_P, _PD, _PD]),
     "cco_flow_result": (C.c_int, [_P, _PD, _PD, _PD, _PD, _PD, _PI64, _PI64, _PI64]),

@@ -43,6 +43,12 @@ class AbstractEnergy(abc.ABC):
             f"{type(self).__name__} must implement value_and_grad (no autodiff backend)"
         )
 
+    def hessian(self, s: np.ndarray) -> np.ndarray:
+        """d^2 Phi / ds ds flattened row-major to (n d, n d) (energy.py:134-137)."""
+        raise NotImplementedError(
+            f"{type(self).__name__} must implement hessian (no autodiff backend)"
+        )
+
 
 class StealthyEnergy(AbstractEnergy):
     """Stealthy energy over a padded k-set (energy.py:140-184).
@@ -199,6 +205,26 @@ class StealthyEnergy(AbstractEnergy):
         phi, g, re, im = self._eval(s[None], grad=True, density=True)
         assert g is not None and re is not None and im is not None
         return float(phi[0]), g[0], re[0], im[0]
+
+    def hessian(self, s: np.ndarray) -> np.ndarray:
+        """Closed-form Hessian flattened row-major to (n d, n d) (energy.py:272-299).
+
+        H[(j,a),(l,b)] = (2 (2 pi)^2 / V) sum_k w_k n_ka n_kb [cos th_kj cos th_kl
+        + sin th_kj sin th_kl - delta_jl (Re rho_k cos th_kj + Im rho_k sin th_kj)],
+        computed on the GPU (one FP64 GEMM per block pair).
+        """
+        s = self._check(s, batched=False)
+        return self.hessian_batch(s[None])[0]
+
+    def hessian_batch(self, s: np.ndarray) -> np.ndarray:
+        """Hessians of a batch (b, n, d) -> (b, n d, n d)."""
+        s = self._check(s, batched=True)
+        b, n, d = s.shape
+        out = np.empty((b, n * d, n * d))
+        _native.check(
+            _native.lib().cco_hessian(self.handle(), _native.dptr(s), b, n, _native.dptr(out))
+        )
+        return out
 
     # ----- batched forms (one launch for all configurations) -----
     def energy_batch(self, s: np.ndarray) -> np.ndarray:
