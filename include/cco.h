@@ -209,6 +209,11 @@ typedef struct {
 
 int cco_flow_create(cco_ctx* ctx, cco_energy* e, int B, int N, int d,
                     const cco_flow_settings* fs, const double* basis_host, cco_relax** out);
+/* Integration method before init: 0 Dormand-Prince 5(4) (default), 1 ROS2
+ * (flow.py:367-398: L-stable Rosenbrock with the exact Hessian, time
+ * parametrization only, built-in energy only: each step factors
+ * I - gamma h J per lane on the GPU). */
+int cco_flow_set_method(cco_relax* r, int method);
 /* s0: B x N x d; dt0: B first steps (NULL: dt_init), as gradient_flow's dt0. */
 int cco_flow_init(cco_relax* r, const double* s0_host, const double* dt0_host);
 /* FlowResult fields (flow.py:138-154): unwrapped s, energy, t, length, next

@@ -704,6 +704,7 @@ class FlowSettings:
     rtol: float = 1e-6
     atol: float = 1e-14
     parametrization: str = "time"
+    method: str = "dopri5"
     dt_init: float = 1e-3
     dt_min: float = 1e-14
     dt_max: float = math.inf
@@ -715,12 +716,30 @@ class FlowSettings:
     max_records: int = 0
 
 
-def gradient_flow(vg: ValueGrad, s0: np.ndarray, basis: np.ndarray, st: FlowSettings) -> Dict[str, object]:
-    """Dormand-Prince 5(4) on dr/dt = -grad_r Phi (flow.py:212-481), one configuration.
+_ROS_GAMMA = 1.0 + 1.0 / math.sqrt(2.0)
 
-    Returns a dict with s, energy, t, length, dt, n_steps, n_evals, status and,
-    when recording, the trajectory rows (s, t, length, energy).
+
+def gradient_flow(
+    vg: ValueGrad,
+    s0: np.ndarray,
+    basis: np.ndarray,
+    st: FlowSettings,
+    hess: Optional[Callable[[np.ndarray], np.ndarray]] = None,
+) -> Dict[str, object]:
+    """Gradient flow dr/dt = -grad_r Phi (flow.py:212-481), one configuration.
+
+    ``st.method`` "dopri5": Dormand-Prince 5(4) (flow.py:346-364).
+    "rosenbrock": ROS2 (flow.py:367-398) with the exact Hessian ``hess(s)``
+    (row-major (n d, n d)): W = I - gamma h J, J = dv/ds = -H (I (x) inv(B B^T)),
+    (W) k1 = v(s), (W) k2 = v(s + h k1) - 2 k1, s+ = s + h (3/2 k1 + 1/2 k2),
+    error W^-1 (h (k1 + k2) / 2), order 2, clocks (h, |chord|).  Step control
+    (flow.py:401-451) is shared.  Returns a dict with s, energy, t, length,
+    dt, n_steps, n_evals, status and, when recording, the trajectory rows
+    (s, t, length, energy).
     """
+    rosenbrock = st.method == "rosenbrock"
+    if rosenbrock and hess is None:
+        raise ValueError("method='rosenbrock' needs the Hessian")
     arclength = st.parametrization == "arclength"
     metric_inv = np.linalg.inv(basis @ basis.T)
     inv_t = np.linalg.inv(basis).T
@@ -759,6 +778,36 @@ def gradient_flow(vg: ValueGrad, s0: np.ndarray, basis: np.ndarray, st: FlowSett
     rows = [(s.copy(), 0.0, 0.0, ev[0])] if st.record_every > 0 else []
     while code == STATUS_RUNNING:
         h = dt if arclength else min(dt, st.t_max - t)
+        if rosenbrock:
+            n, d = s.shape
+            H = hess(s).reshape(n, d, n, d)
+            jac = -np.einsum("ibjc,ba->iajc", H, metric_inv).reshape(n * d, n * d)
+            W = np.eye(n * d) - _ROS_GAMMA * h * jac
+            k1 = np.linalg.solve(W, ev[2].reshape(-1)).reshape(n, d)
+            ev1 = velocity(s + h * k1)
+            k2 = np.linalg.solve(W, (ev1[2] - 2.0 * k1).reshape(-1)).reshape(n, d)
+            s_new = s + h * (1.5 * k1 + 0.5 * k2)
+            ev_new = velocity(s_new)
+            chord = math.sqrt(float(np.sum(((s_new - s) @ basis) ** 2)))
+            err_f = np.linalg.solve(W, (0.5 * h * (k1 + k2)).reshape(-1)).reshape(n, d)
+            err = scaled_error(err_f, s_new - s)
+            with np.errstate(all="ignore"):
+                factor = float(np.clip(0.9 * err ** (-1.0 / 2), 0.2, 5.0))
+            if not math.isfinite(factor):
+                factor = 0.2
+            accept = err <= 1.0
+            dt = min(h * factor, st.dt_max)
+            if accept:
+                s, ev = s_new, ev_new
+                t += h
+                length += chord
+                steps += 1
+                if st.record_every > 0 and steps % st.record_every == 0 and len(rows) < st.max_records:
+                    rows.append((s.copy(), t, length, ev[0]))
+            tries += 1
+            n_evals += 2
+            code = status(ev[0], ev[1], ev[4], t, dt, steps, tries)
+            continue
         evs = [ev]
         for i in range(1, 6):
             incr = sum(a * e[2] for a, e in zip(_DP_A[i], evs))

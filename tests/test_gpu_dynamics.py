@@ -118,8 +118,50 @@ def test_flow_rejects_bad_input(gpu_ctx) -> None:
         gradient_flow(e, np.zeros((4, 1), np.float32), basis, FlowSettings())
     with pytest.raises(ValueError):
         gradient_flow(e, np.zeros((4, 1)), basis, FlowSettings(rtol=0.0))
-    with pytest.raises(NotImplementedError):
-        gradient_flow(e, np.zeros((4, 1)), basis, FlowSettings(method="rosenbrock"))
+    with pytest.raises(NotImplementedError):  # ROS2 needs the GPU Hessian of a StealthyEnergy
+        gradient_flow(_Quadratic(np.zeros((1, 2)), 1.0), np.ones((1, 2)), np.eye(2),
+                      FlowSettings(method="rosenbrock"))
+    with pytest.raises(ValueError, match="rosenbrock"):
+        gradient_flow(e, np.zeros((4, 1)), basis, FlowSettings(method="rosenbrock", parametrization="arclength"))
+
+
+# ---------------- ROS2 (flow.py:367-398) ----------------
+
+
+@pytest.mark.parametrize("n,d,shells,t_frac", [(16, 1, 4, 0.5), (24, 2, 2, 1.0)])
+def test_rosenbrock_matches_oracle(gpu_ctx, n: int, d: int, shells: int, t_frac: float) -> None:
+    e, basis, vg = _stealthy(n, d, shells)
+    ks_n, ks_w = e.n_vectors, e.weights
+    vol = float(e.volume)
+    s0 = np.random.default_rng(60 + n).uniform(size=(n, d))
+    full = O.gradient_flow(vg, s0, basis, O.FlowSettings(energy_tol=1e-12))
+    st = dict(method="rosenbrock", rtol=1e-3, atol=1e-10, t_max=t_frac * float(full["t"]), max_steps=5000)
+    ref = O.gradient_flow(vg, s0, basis, O.FlowSettings(**st), hess=lambda s: O.hessian(s, ks_n, ks_w, vol))
+    res = gradient_flow(e, s0, basis, FlowSettings(**st))
+    assert int(res.status) == ref["status"]
+    assert int(res.n_steps) == ref["n_steps"] and int(res.n_evals) == ref["n_evals"]
+    assert np.max(np.abs(res.s - ref["s"])) < 1e-9
+    assert float(res.t) == pytest.approx(ref["t"], rel=1e-12)
+    assert float(res.length) == pytest.approx(ref["length"], rel=1e-9)
+    # the same flow as a tight DOPRI5 at the same time
+    dop = gradient_flow(e, s0, basis, FlowSettings(rtol=1e-10, t_max=float(res.t)))
+    assert np.max(np.abs(res.s - dop.s)) < 1e-5
+
+
+def test_rosenbrock_batch_equals_single_and_records(gpu_ctx) -> None:
+    e, basis, _ = _stealthy(20, 2, 2)
+    s0 = np.random.default_rng(61).uniform(size=(3, 20, 2))
+    fs = FlowSettings(method="rosenbrock", rtol=0.1, atol=1e-4, energy_tol=1e-6, max_steps=3000,
+                      record_every=5, max_records=40)
+    batched = gradient_flow_batch(e, s0, basis, fs)
+    assert np.all(batched.status == STATUS_ENERGY_TOL) and np.all(batched.energy < 1e-6)
+    for i in range(3):
+        one = gradient_flow(e, s0[i], basis, fs)
+        assert np.array_equal(batched.s[i], one.s)
+        assert int(batched.n_steps[i]) == int(one.n_steps)
+        k = int(one.trajectory.count)
+        assert k >= 2 and np.all(np.diff(one.trajectory.t[:k]) > 0)
+        assert np.all(np.diff(one.trajectory.energy[:k]) <= 1e-12)
 
 
 # ---------------- velocity Verlet ----------------

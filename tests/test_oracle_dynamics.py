@@ -2,7 +2,10 @@
 
 Flow: the closed-form linear flow of a quadratic in a cubic cell
 (s(t) = a + (s0 - a) exp(-c t / L^2), test_flow.py:70-93), monotone energy,
-a stealthy ground state.  Verlet: the exact discrete solution of the
+a stealthy ground state; ROS2 (test_flow.py:243-295): the same closed form, agreement
+with DOPRI5 at equal times on a stealthy energy, far fewer steps than DOPRI5 on a stiff
+quadratic, where a planted sign error in its Jacobian never converges.
+Verlet: the exact discrete solution of the
 harmonic oscillator (cos theta = 1 - (w h)^2 / 2), second-order global error,
 time reversal.
 """
@@ -97,3 +100,63 @@ def test_verlet_second_order_and_time_reversal() -> None:
     fwd = O.verlet_nve(vg, basis, r0, v0, 0.01, 100)
     back = O.verlet_nve(vg, basis, fwd["r"], -fwd["v"], 0.01, 100)
     assert np.max(np.abs(back["r"] - r0)) < 1e-11
+
+
+
+# ---------------- ROS2 (flow.py:367-398) ----------------
+
+
+def _quadratic_hess(c: np.ndarray, n: int):
+    """Hessian of 1/2 sum_ij c_j (s_ij - a_ij)^2: diag(c) per particle."""
+    return lambda s: np.diag(np.tile(np.asarray(c, dtype=np.float64), n))
+
+
+def _quadratic_vg_axes(a: np.ndarray, c: np.ndarray):
+    def vg(s):
+        d = s - a
+        return float(0.5 * np.sum(c * d * d)), c * d
+
+    return vg
+
+
+def test_rosenbrock_linear_flow_closed_form() -> None:
+    a = np.array([[0.3, 0.6]])
+    s0 = np.array([[0.9, 0.1]])
+    c = np.array([1.5, 1.5])
+    st = O.FlowSettings(method="rosenbrock", rtol=1e-3, atol=1e-10, t_max=1.3)
+    res = O.gradient_flow(_quadratic_vg_axes(a, c), s0, np.eye(2) * 1.2, st, hess=_quadratic_hess(c, 1))
+    expected = a + (s0 - a) * np.exp(-1.5 * 1.3 / 1.2**2)
+    assert res["t"] == pytest.approx(1.3, rel=1e-12)
+    assert np.max(np.abs(res["s"] - expected)) < 1e-5
+
+
+@pytest.mark.parametrize("frac", [0.1, 0.5, 0.9])
+def test_rosenbrock_agrees_with_dopri(frac: float) -> None:
+    side = 16.0
+    nv, w, vol, basis = small_kset(16, 1, 2.0 * np.pi / side * 4.5)
+    vg = O.make_value_grad(nv, w, vol)
+    s0 = np.random.default_rng(6).uniform(size=(16, 1))
+    full = O.gradient_flow(vg, s0, basis, O.FlowSettings(energy_tol=1e-12))
+    t_stop = frac * float(full["t"])
+    ref = O.gradient_flow(vg, s0, basis, O.FlowSettings(rtol=1e-10, t_max=t_stop))
+    ros = O.gradient_flow(vg, s0, basis, O.FlowSettings(method="rosenbrock", rtol=1e-3, atol=1e-10, t_max=t_stop),
+                          hess=lambda s: O.hessian(s, nv, w, vol))
+    assert np.max(np.abs(ros["s"] - ref["s"])) < 1e-5
+
+
+def test_rosenbrock_needs_fewer_steps_when_stiff() -> None:
+    a = np.array([[0.3, 0.6]])
+    s0 = np.array([[0.4, 0.9]])
+    c = np.array([1e4, 1.0])
+    vg = _quadratic_vg_axes(a, c)
+    dop = O.gradient_flow(vg, s0, np.eye(2), O.FlowSettings(energy_tol=1e-20, max_steps=400_000))
+    ros = O.gradient_flow(vg, s0, np.eye(2), O.FlowSettings(method="rosenbrock", rtol=1e-3, atol=1e-8,
+                                                            energy_tol=1e-20, max_steps=400_000),
+                          hess=_quadratic_hess(c, 1))
+    assert dop["status"] == ros["status"] == O.STATUS_ENERGY_TOL
+    assert ros["n_steps"] < dop["n_steps"] / 5
+    # a Jacobian of the wrong sign (planted in the Hessian callable) loses the stiff mode
+    bad = O.gradient_flow(vg, s0, np.eye(2), O.FlowSettings(method="rosenbrock", rtol=1e-3, atol=1e-8,
+                                                            energy_tol=1e-20, max_steps=20_000),
+                          hess=lambda s: -_quadratic_hess(c, 1)(s))
+    assert bad["status"] != O.STATUS_ENERGY_TOL and bad["energy"] > 1e-6

@@ -410,6 +410,10 @@ struct cco_relax {
   size_t vec_total = 0;     // doubles of one position array (B N d when uniform)
   DevBuf un_lay, un_geo;
   int tiles_rho = 0, tiles_reduce = 0, tiles_force = 0, rho_lds = 0, dmask = 0;
+  // flow by ROS2: positions, Hessians / LU factors, right-hand sides, attempted
+  // steps, pivots, factorisation info and the chunk's lane count
+  bool ros2 = false;
+  DevBuf ros_S, ros_H, ros_rhs, ros_dts, ros_ipiv, ros_info, ros_cnt;
   std::vector<hipEvent_t> evs;
   double stage_ms[kStages] = {};
   int64_t stage_n[kStages] = {};
@@ -804,15 +808,15 @@ int cco_energy_destroy(cco_energy* E) {
   return CCO_OK;
 }
 
-int cco_hessian_dev(cco_energy* E, const double* s_dev, int B, int N, double* h_dev) {
-  if (!E || !s_dev || !h_dev) return fail(CCO_ERR_ARG, "NULL argument");
-  if (B < 1 || N < 1) return fail(CCO_ERR_ARG, "need B >= 1 and N >= 1");
-  DevScope ds(E->ctx->device);
+namespace {
+
+// Hessians of B configurations (device s, device h) on stream st with the
+// energy's workspaces and its context's rocBLAS handle.
+int hessian_on(cco_energy* E, hipStream_t st, const double* s_dev, int B, int N, double* h_dev) {
   cco_ctx* ctx = E->ctx;
   if (ctx->blas == nullptr && (ctx->blas = blas_create()) == nullptr)
     return fail(CCO_ERR_DEVICE, "rocblas_create_handle failed");
   const int M = E->geo.M, d = E->geo.d;
-  hipStream_t st = ctx->stream;
   int rc;
   if (!E->kn_ready) {
     if ((rc = dev_reserve(E->kn, sizeof(double) * E->kn_host.size()))) return rc;
@@ -831,6 +835,15 @@ int cco_hessian_dev(cco_energy* E, const double* s_dev, int B, int N, double* h_
                     (double*)E->hD.p, h_dev) != 0)
     return fail(CCO_ERR_DEVICE, "Hessian launch failed (rocBLAS or kernel)");
   return CCO_OK;
+}
+
+}  // namespace
+
+int cco_hessian_dev(cco_energy* E, const double* s_dev, int B, int N, double* h_dev) {
+  if (!E || !s_dev || !h_dev) return fail(CCO_ERR_ARG, "NULL argument");
+  if (B < 1 || N < 1) return fail(CCO_ERR_ARG, "need B >= 1 and N >= 1");
+  DevScope ds(E->ctx->device);
+  return hessian_on(E, E->ctx->stream, s_dev, B, N, h_dev);
 }
 
 int cco_hessian(cco_energy* E, const double* s_host, int B, int N, double* h_host) {
@@ -1154,7 +1167,8 @@ int cco_relax_destroy(cco_relax* R) {
   (void)hipStreamSynchronize(R->ctx->stream);
   for (DevBuf& b : R->bufs) dev_release(b);
   for (DevBuf* b : {&R->part, &R->cbox, &R->rphi, &R->out_s, &R->out_scalars, &R->pack,
-                    &R->pack_idx, &R->un_lay, &R->un_geo})
+                    &R->pack_idx, &R->un_lay, &R->un_geo, &R->ros_S, &R->ros_H, &R->ros_rhs,
+                    &R->ros_dts, &R->ros_ipiv, &R->ros_info, &R->ros_cnt})
     dev_release(*b);
   if (R->host_count) (void)hipHostFree(R->host_count);
   if (R->progress_host) (void)hipHostFree(R->progress_host);
@@ -1519,9 +1533,86 @@ int cco_copy_dev(cco_ctx* ctx, void* dst_dev, const void* src_dev, size_t bytes,
   return CCO_OK;
 }
 
+namespace {
+
+__global__ void set_int_kernel(int* p, int v) { *p = v; }
+
+// Gradient flow by ROS2 (flow.py:367-398): after the first evaluation (the
+// flow's init step), each round attempts one step for every running lane,
+// in chunks of lanes whose (N d)^2 factors fit kRosBytes: Hessian, W in
+// place, LU, k1, v(s + h k1), k2, v(candidate), the error solve and the step
+// control, then one compaction and one sync.  Two evaluations per round.
+constexpr size_t kRosBytes = (size_t)2 << 30;
+
+int flow_run_ros2(cco_relax* R, int64_t max_evals) {
+  if (!R->ready) return fail(CCO_ERR_STATE, "session not initialised");
+  DevScope ds(R->ctx->device);
+  int na = R->slots_ub;
+  int rc;
+  if (R->steps == 0 && (rc = cco_relax_step(R, 1, &na))) return rc;
+  cco_ctx* ctx = R->ctx;
+  hipStream_t st = ctx->stream;
+  if (ctx->blas == nullptr && (ctx->blas = blas_create()) == nullptr)
+    return fail(CCO_ERR_DEVICE, "rocblas_create_handle failed");
+  const int nd = R->dev.nd;
+  const size_t mat = (size_t)nd * nd;
+  const int chunk = (int)std::max<size_t>(1, std::min<size_t>(R->B, kRosBytes / (mat * sizeof(double))));
+  if ((rc = dev_reserve(R->ros_S, sizeof(double) * chunk * (size_t)nd))) return rc;
+  if ((rc = dev_reserve(R->ros_H, sizeof(double) * chunk * mat))) return rc;
+  if ((rc = dev_reserve(R->ros_rhs, sizeof(double) * chunk * (size_t)nd))) return rc;
+  if ((rc = dev_reserve(R->ros_dts, sizeof(double) * chunk))) return rc;
+  if ((rc = dev_reserve(R->ros_ipiv, sizeof(int) * chunk * (size_t)nd))) return rc;
+  if ((rc = dev_reserve(R->ros_info, sizeof(int) * chunk))) return rc;
+  if ((rc = dev_reserve(R->ros_cnt, sizeof(int)))) return rc;
+  double* S = (double*)R->ros_S.p;
+  double* H = (double*)R->ros_H.p;
+  double* rhs = (double*)R->ros_rhs.p;
+  double* dts = (double*)R->ros_dts.p;
+  int* ipiv = (int*)R->ros_ipiv.p;
+  int* info = (int*)R->ros_info.p;
+  int* cnt = (int*)R->ros_cnt.p;
+  while (na > 0) {
+    if (R->steps >= max_evals)
+      return fail(CCO_ERR_BOUND, "flow did not finish within the round bound");
+    for (int c0 = 0; c0 < na; c0 += chunk) {
+      const int cn = std::min(chunk, na - c0);
+      hipLaunchKernelGGL(set_int_kernel, dim3(1), dim3(1), 0, st, cnt, cn);
+      RelaxDev dv = R->dev;
+      dv.active = R->dev.active + c0;
+      dv.n_active = cnt;
+      dv.progress = nullptr;
+      const Batch eb = relax_batch(R, dv);
+      launch_ros_gather(dv, R->fc, cn, S, dts, st);
+      if ((rc = hessian_on(R->E, st, S, cn, R->N, H))) return rc;
+      launch_ros_build(H, dts, cnt, R->fc, nd, cn, st);
+      if (lu_factor_batched(ctx->blas, st, H, nd, ipiv, info, cn))
+        return fail(CCO_ERR_DEVICE, "rocsolver getrf failed");
+      for (int q = 0; q < 3; ++q) {
+        launch_ros_rhs(dv, R->fc, rhs, dts, q, cn, st);
+        if (lu_solve_batched(ctx->blas, st, H, nd, ipiv, rhs, cn))
+          return fail(CCO_ERR_DEVICE, "rocsolver getrs failed");
+        if (q == 2) break;
+        launch_ros_point(dv, rhs, dts, q, cn, st);
+        for (int stage = STG_RHO; stage <= STG_FORCE; ++stage)
+          launch_energy_stage(stage, R->E->geo, eb, cn, st);
+      }
+      launch_ros_finish(dv, R->fc, rhs, dts, cn, st);
+      CCO_HIP(hipGetLastError());
+    }
+    R->steps += 2;
+    R->launches += 2;
+    launch_compact(whole_view(R).dev, st, 0, R->B, 0u);
+    if ((rc = relax_sync_count(R, &na))) return rc;
+  }
+  return CCO_OK;
+}
+
+}  // namespace
+
 int cco_relax_run(cco_relax* R, int64_t max_evals, int sync_every) {
   if (!R) return fail(CCO_ERR_ARG, "relax is NULL");
   if (sync_every < 1) return fail(CCO_ERR_ARG, "sync_every must be >= 1");
+  if (R->method == METHOD_FLOW && R->ros2) return flow_run_ros2(R, max_evals);
   int n_active = R->slots_ub;
   while (n_active > 0) {
     if (R->steps >= max_evals)
@@ -1816,6 +1907,18 @@ int cco_flow_create(cco_ctx* ctx, cco_energy* E, int B, int N, int d,
     return rc;
   }
   *out = R;
+  return CCO_OK;
+}
+
+int cco_flow_set_method(cco_relax* R, int method) {
+  if (!R) return fail(CCO_ERR_ARG, "NULL argument");
+  if (R->method != METHOD_FLOW) return fail(CCO_ERR_STATE, "not a flow session");
+  if (method != 0 && method != 1) return fail(CCO_ERR_ARG, "method must be 0 (dopri5) or 1 (rosenbrock)");
+  if (method == 1 && R->fc.arclength)
+    return fail(CCO_ERR_ARG, "method='rosenbrock' integrates the time parametrization");
+  if (method == 1 && R->E == nullptr)
+    return fail(CCO_ERR_ARG, "method='rosenbrock' needs the built-in stealthy energy (GPU Hessian)");
+  R->ros2 = method == 1;
   return CCO_OK;
 }
 

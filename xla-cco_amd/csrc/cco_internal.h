@@ -167,6 +167,11 @@ int hessian_batch(hipStream_t st, void* blas, const double* s, int B, int N, int
                   double* rho, double* D, double* H);
 void* blas_create();
 void blas_destroy(void* blas);
+// Batched LU (hessian.hip): column-major n x n matrices n^2 apart.
+int lu_factor_batched(void* blas, hipStream_t st, double* A, int n, int* ipiv, int* info,
+                      int batch);
+int lu_solve_batched(void* blas, hipStream_t st, double* A, int n, const int* ipiv, double* B,
+                     int batch);
 
 // ---------------- small helpers ----------------
 

@@ -380,4 +380,204 @@ void launch_flow_advance(const RelaxDev& r, const FlowCfg& c, int slots_ub, hipS
   hipLaunchKernelGGL(flow_advance_kernel, dim3(slots_ub), dim3(kDynBlock), 0, st, r, c);
 }
 
+// ---------------------------------------------------------------------------
+// ROS2 gradient flow (flow.py:367-398, step control :401-451).  A round
+// attempts one step for every lane of a chunk of the active list; the host
+// runs the Hessian, the LU factorisation and the solves (rocSOLVER) between
+// these kernels.  Slot-indexed buffers: S / rhs [slot][nd], dts [slot].
+// Per lane, k holds v(s) (row 0), k1 (1), v at s + h k1 (2), k2 (3) and
+// v at the candidate (5).
+// ---------------------------------------------------------------------------
+constexpr double kRosGamma = 1.0 + 0.70710678118654752440;  // 1 + 1 / sqrt(2)
+
+// S[slot] = s[lane]; dts[slot] = the attempted step min(dt, t_max - t).
+__global__ __launch_bounds__(kDynBlock) void ros_gather_kernel(RelaxDev R, FlowCfg c, double* S,
+                                                               double* dts) {
+  const int slot = blockIdx.x;
+  if (slot >= *R.n_active) return;
+  const int lane = R.active[slot];
+  const int nd = R.nd;
+  const double* s = R.s + (size_t)lane * R.stride;
+  for (int e = threadIdx.x; e < nd; e += blockDim.x) S[(size_t)slot * nd + e] = s[e];
+  if (threadIdx.x == 0) {
+    const FlowLane L = R.fl[lane];
+    dts[slot] = fmin(L.dt, c.t_max - L.t);
+  }
+}
+
+// In place on the symmetric Hessian H (row-major, (n d)^2 per slot): the
+// column-major W = I - gamma h J with J = -H (I (x) inv(B B^T)), i.e.
+// W[(i,a),(j,c)] = delta + gamma h sum_b H[(i,b),(j,c)] Minv[b][a]; entry
+// ((i,a), col) of the column-major W sits where H[col][(i,a)] = H[(i,a)][col]
+// sits, so each thread rewrites the d entries it read.
+__global__ void ros_build_kernel(double* H, const double* dts, const int* n_active, FlowCfg c,
+                                 int nd) {
+  const int slot = blockIdx.z;
+  if (slot >= *n_active) return;
+  const int col = blockIdx.y;
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int d = c.d;
+  if (i * d >= nd) return;
+  double* colp = H + (size_t)slot * nd * nd + (size_t)col * nd + (size_t)i * d;
+  const double gh = kRosGamma * dts[slot];
+  double h[3];
+  for (int b = 0; b < d; ++b) h[b] = colp[b];
+  for (int a = 0; a < d; ++a) {
+    double acc = 0.0;
+    for (int b = 0; b < d; ++b) acc = acc + h[b] * c.metric_inv[b * d + a];
+    colp[a] = (col == i * d + a ? 1.0 : 0.0) + gh * acc;
+  }
+}
+
+// mode 0: rhs = v(s) (k row 0).
+// mode 1: k2's right-hand side: k row 2 = v(s + h k1) from g_eval, rhs = that - 2 k1.
+// mode 2: the error's right-hand side, rhs = h (k1 + k2) / 2.
+__global__ __launch_bounds__(kDynBlock) void ros_rhs_kernel(RelaxDev R, FlowCfg c, double* rhs,
+                                                            const double* dts, int mode) {
+  __shared__ double red[16];
+  const int slot = blockIdx.x;
+  if (slot >= *R.n_active) return;
+  const int lane = R.active[slot];
+  const int nd = R.nd;
+  const double* k = R.k + (size_t)lane * 7 * R.stride;
+  double* r = rhs + (size_t)slot * nd;
+  if (mode == 0) {
+    for (int e = threadIdx.x; e < nd; e += blockDim.x) r[e] = k[e];
+  } else if (mode == 1) {
+    double* k2row = R.k + ((size_t)lane * 7 + 2) * R.stride;
+    double gn;
+    bool gz;
+    flow_velocity(R.g_eval + (size_t)lane * R.stride, k2row, c, nd / c.d, red, gn, gz);
+    __syncthreads();
+    for (int e = threadIdx.x; e < nd; e += blockDim.x)
+      r[e] = k2row[e] - 2.0 * k[(size_t)R.stride + e];
+  } else {
+    const double h = dts[slot];
+    for (int e = threadIdx.x; e < nd; e += blockDim.x)
+      r[e] = 0.5 * h * (k[(size_t)R.stride + e] + k[(size_t)3 * R.stride + e]);
+  }
+}
+
+// mode 0: k1 = rhs, s_eval = s + h k1.  mode 1: k2 = rhs, s_eval = s + h (3/2 k1 + 1/2 k2).
+__global__ __launch_bounds__(kDynBlock) void ros_point_kernel(RelaxDev R, const double* rhs,
+                                                              const double* dts, int mode) {
+  const int slot = blockIdx.x;
+  if (slot >= *R.n_active) return;
+  const int lane = R.active[slot];
+  const int nd = R.nd;
+  const size_t off = (size_t)lane * R.stride;
+  double* k = R.k + (size_t)lane * 7 * R.stride;
+  const double* r = rhs + (size_t)slot * nd;
+  const double h = dts[slot];
+  for (int e = threadIdx.x; e < nd; e += blockDim.x) {
+    if (mode == 0) {
+      k[(size_t)R.stride + e] = r[e];
+      R.s_eval[off + e] = R.s[off + e] + h * r[e];
+    } else {
+      k[(size_t)3 * R.stride + e] = r[e];
+      R.s_eval[off + e] = R.s[off + e] + h * (1.5 * k[(size_t)R.stride + e] + 0.5 * r[e]);
+    }
+  }
+}
+
+// The candidate s_eval was evaluated; rhs holds W^-1 (h (k1 + k2) / 2).
+__global__ __launch_bounds__(kDynBlock) void ros_finish_kernel(RelaxDev R, FlowCfg c,
+                                                               const double* rhs,
+                                                               const double* dts) {
+  __shared__ double red[16];
+  const int slot = blockIdx.x;
+  if (slot >= *R.n_active) return;
+  const int lane = R.active[slot];
+  const int nd = R.nd;
+  const int d = c.d;
+  const int n = nd / d;
+  const size_t off = (size_t)lane * R.stride;
+  double* s = R.s + off;
+  const double* se = R.s_eval + off;
+  double* k = R.k + (size_t)lane * 7 * R.stride;
+  double* k5 = k + (size_t)5 * R.stride;
+  const double* errf = rhs + (size_t)slot * nd;
+  FlowLane L = R.fl[lane];
+  const double dt = dts[slot];
+  double gn;
+  bool gz;
+  flow_velocity(R.g_eval + off, k5, c, n, red, gn, gz);
+  double mv = 0.0, me = 0.0;
+  for (int j = threadIdx.x; j < n; j += blockDim.x) {
+    for (int a = 0; a < d; ++a) {
+      double cs = 0.0, ce = 0.0;
+      for (int b = 0; b < d; ++b) {
+        cs = cs + (se[j * d + b] - s[j * d + b]) * c.basis[b * d + a];
+        ce = ce + errf[j * d + b] * c.basis[b * d + a];
+      }
+      mv += cs * cs;
+      me += ce * ce;
+    }
+  }
+  mv = block_sum<kDynBlock>(mv, red);
+  me = block_sum<kDynBlock>(me, red);
+  const double move = sqrt(mv / (double)nd);
+  const double err = sqrt(me / (double)nd) / (c.atol + c.rtol * move);
+  const double chord = sqrt(mv);
+  const bool accept = err <= 1.0;
+  double factor = 0.9 * pow(err, -0.5);
+  factor = fmin(fmax(factor, 0.2), 5.0);
+  if (!isfinite(factor) || factor != factor) factor = 0.2;
+  const double dt_next = fmin(dt * factor, c.dt_max);
+  __syncthreads();
+  if (accept) {
+    for (int e = threadIdx.x; e < nd; e += blockDim.x) {
+      s[e] = se[e];
+      k[e] = k5[e];
+    }
+    L.f = R.phi_eval[lane];
+    L.gnorm = gn;
+    L.gzero = gz;
+    L.t = L.t + dt;
+    L.length = L.length + chord;
+    L.var = L.var + dt;
+    L.n_steps += 1;
+  }
+  L.n_tries += 1;
+  L.n_evals += 2;
+  L.dt = dt_next;
+  const int code = flow_status(L.f, L.gzero, L.gnorm, L.t, dt_next, L.n_steps, L.n_tries, c);
+  __syncthreads();
+  if (accept && c.record_every > 0 && (L.n_steps % c.record_every) == 0 &&
+      L.rec_count < c.record_cap)
+    flow_record(R, c, lane, L, s, nd);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    R.fl[lane] = L;
+    R.status[lane] = code;
+  }
+}
+
+void launch_ros_gather(const RelaxDev& r, const FlowCfg& c, int slots, double* S, double* dts,
+                       hipStream_t st) {
+  hipLaunchKernelGGL(ros_gather_kernel, dim3(slots), dim3(kDynBlock), 0, st, r, c, S, dts);
+}
+
+void launch_ros_build(double* H, const double* dts, const int* n_active, const FlowCfg& c,
+                      int nd, int slots, hipStream_t st) {
+  const int n = nd / c.d;
+  hipLaunchKernelGGL(ros_build_kernel, dim3((n + 255) / 256, nd, slots), dim3(256), 0, st, H,
+                     dts, n_active, c, nd);
+}
+
+void launch_ros_rhs(const RelaxDev& r, const FlowCfg& c, double* rhs, const double* dts, int mode,
+                    int slots, hipStream_t st) {
+  hipLaunchKernelGGL(ros_rhs_kernel, dim3(slots), dim3(kDynBlock), 0, st, r, c, rhs, dts, mode);
+}
+
+void launch_ros_point(const RelaxDev& r, const double* rhs, const double* dts, int mode, int slots,
+                      hipStream_t st) {
+  hipLaunchKernelGGL(ros_point_kernel, dim3(slots), dim3(kDynBlock), 0, st, r, rhs, dts, mode);
+}
+
+void launch_ros_finish(const RelaxDev& r, const FlowCfg& c, const double* rhs, const double* dts,
+                       int slots, hipStream_t st) {
+  hipLaunchKernelGGL(ros_finish_kernel, dim3(slots), dim3(kDynBlock), 0, st, r, c, rhs, dts);
+}
+
 }  // namespace cco

@@ -14,6 +14,7 @@
 // layout are the kernels below.  Blocks (a, b) and (b, a) share G (it is
 // symmetric and w_ab = w_ba), so d (d + 1) / 2 products cover a Hessian.
 #include <rocblas/rocblas.h>
+#include <rocsolver/rocsolver.h>
 
 #include "cco_internal.h"
 
@@ -107,6 +108,29 @@ void* blas_create() {
 
 void blas_destroy(void* blas) {
   if (blas != nullptr) (void)rocblas_destroy_handle((rocblas_handle)blas);
+}
+
+// LU factorisation of `batch` column-major n x n matrices stored n^2 apart
+// (partial pivoting, rocSOLVER), and solves with one right-hand side each.
+int lu_factor_batched(void* blas, hipStream_t st, double* A, int n, int* ipiv, int* info,
+                      int batch) {
+  rocblas_handle h = (rocblas_handle)blas;
+  if (rocblas_set_stream(h, st) != rocblas_status_success) return 1;
+  return rocsolver_dgetrf_strided_batched(h, n, n, A, n, (rocblas_stride)n * n, ipiv,
+                                          (rocblas_stride)n, info, batch) == rocblas_status_success
+             ? 0
+             : 2;
+}
+
+int lu_solve_batched(void* blas, hipStream_t st, double* A, int n, const int* ipiv, double* B,
+                     int batch) {
+  rocblas_handle h = (rocblas_handle)blas;
+  if (rocblas_set_stream(h, st) != rocblas_status_success) return 1;
+  return rocsolver_dgetrs_strided_batched(h, rocblas_operation_none, n, 1, A, n,
+                                          (rocblas_stride)n * n, ipiv, (rocblas_stride)n, B, n,
+                                          (rocblas_stride)n, batch) == rocblas_status_success
+             ? 0
+             : 2;
 }
 
 int hessian_batch(hipStream_t st, void* blas, const double* s, int B, int N, int d, int M,

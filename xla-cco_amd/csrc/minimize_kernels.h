@@ -192,6 +192,17 @@ void launch_compact(const RelaxDev& r, hipStream_t st, int lo, int count, unsign
 void launch_md_prepare(const RelaxDev& r, const MdCfg& c, int slots_ub, hipStream_t st);
 void launch_md_finish(const RelaxDev& r, const MdCfg& c, int slots_ub, hipStream_t st);
 void launch_flow_advance(const RelaxDev& r, const FlowCfg& c, int slots_ub, hipStream_t st);
+// ROS2 flow round kernels (dynamics_kernels.hip)
+void launch_ros_gather(const RelaxDev& r, const FlowCfg& c, int slots, double* S, double* dts,
+                       hipStream_t st);
+void launch_ros_build(double* H, const double* dts, const int* n_active, const FlowCfg& c,
+                      int nd, int slots, hipStream_t st);
+void launch_ros_rhs(const RelaxDev& r, const FlowCfg& c, double* rhs, const double* dts, int mode,
+                    int slots, hipStream_t st);
+void launch_ros_point(const RelaxDev& r, const double* rhs, const double* dts, int mode, int slots,
+                      hipStream_t st);
+void launch_ros_finish(const RelaxDev& r, const FlowCfg& c, const double* rhs, const double* dts,
+                       int slots, hipStream_t st);
 void launch_finalize(const RelaxDev& r, int method, const LbfgsCfg& lc, const RelaxOutDev& o,
                      hipStream_t st);
 

@@ -143,6 +143,7 @@ SIGNATURES = {
     "cco_hessian": (C.c_int, [_P, _PD, C.c_int, C.c_int, _PD]),
     "cco_hessian_dev": (C.c_int, [_P, _P, C.c_int, C.c_int, _P]),
     "cco_flow_create": (C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int, _P, _PD, C.POINTER(_P)]),
+    "cco_flow_set_method": (C.c_int, [_P, C.c_int]),
     "cco_flow_init": (C.c_int, [_P, _PD, _PD]),
     "cco_flow_result": (C.c_int, [_P, _PD, _PD, _PD, _PD, _PD, _PI64, _PI64, _PI64]),
     "cco_flow_trajectory": (C.c_int, [_P, _PI64, _PD, _PD, _PD, _PD]),

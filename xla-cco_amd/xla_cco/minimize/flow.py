@@ -1,4 +1,4 @@
-"""Gradient flow dr/dt = -grad_r Phi, Dormand-Prince 5(4) on the GPU.
+"""Gradient flow dr/dt = -grad_r Phi on the GPU: Dormand-Prince 5(4) or ROS2.
 
 Restates src/xla_cco/minimize/flow.py:93-481 (``method="dopri5"``): error
 control on the RMS Cartesian displacement (atol + rtol |step|) and on the
@@ -8,8 +8,13 @@ energy / gradient / time / budget / step floor.  Every stage of every lane
 runs in ``flow_advance_kernel``; energies are HIP kernels for a
 ``StealthyEnergy`` or host callbacks for another ``AbstractEnergy``.
 
-``method="rosenbrock"`` (ROS2 with the exact Hessian) needs the closed-form
-Hessian, which this build does not provide; it raises NotImplementedError.
+``method="rosenbrock"`` (flow.py:367-398): L-stable ROS2 with the exact
+Hessian, time parametrization only.  Each round factors W = I - gamma h J
+(J = -H (I (x) inv(B B^T))) for every running lane on the GPU (closed-form
+Hessian, rocSOLVER LU, three solves, two energy evaluations) and applies the
+same step control with the order-2 exponent.  It needs a ``StealthyEnergy``
+(another ``AbstractEnergy`` raises NotImplementedError: its energy and
+Hessian live on the host).
 """
 
 from __future__ import annotations
@@ -99,10 +104,7 @@ def _validate(s0: np.ndarray, basis: np.ndarray, settings: FlowSettings) -> None
         raise ValueError("record_every and max_records must be >= 0")
     if (settings.record_every > 0) != (settings.max_records > 1):
         raise ValueError("recording needs record_every > 0 and max_records >= 2")
-    if settings.method == "rosenbrock":
-        raise NotImplementedError(
-            "method='rosenbrock' needs the closed-form Hessian, which this build does not provide"
-        )
+
 
 
 def gradient_flow_batch(
@@ -137,11 +139,18 @@ def gradient_flow_batch(
                           C.byref(fs), _native.dptr(bas), C.byref(h))
     )
     try:
+        if settings.method == "rosenbrock":
+            if not builtin:
+                raise NotImplementedError(
+                    "method='rosenbrock' runs on the GPU Hessian of a StealthyEnergy; "
+                    f"{type(energy).__name__} is evaluated on the host"
+                )
+            _native.check(L.cco_flow_set_method(h, 1))
         dt0a = None
         if dt0 is not None:
             dt0a = np.ascontiguousarray(np.broadcast_to(np.asarray(dt0, dtype=np.float64), (b,)))
         _native.check(L.cco_flow_init(h, _native.dptr(s0), _native.dptr(dt0a) if dt0a is not None else None))
-        bound = 1 + 6 * 4 * int(settings.max_steps) + 6
+        bound = 1 + 6 * 4 * int(settings.max_steps) + 6  # evaluations (ROS2 uses 2 per attempt)
         if builtin:
             _native.check(L.cco_relax_run(h, bound, 256))
         else:
