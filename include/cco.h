@@ -245,6 +245,49 @@ int cco_md_run(cco_relax* r, int64_t n_steps, int sync_every, int* n_active);
 int cco_md_state(cco_relax* r, double* r_host, double* v_host, double* f_host,
                  double* scalars_host, int64_t* ints_host);
 
+/* Schemes, thermostats and tracked density waves of an MD session
+ * (src/xla_cco/dynamics/state.py:57-88, run.py:157-537, integrators.py:
+ * 99-176, observables.py).  scheme 0 verlet, 1 andersen (a velocity reset
+ * after each Verlet step: andersen_mode 0 count, 1 bernoulli), 2 baoab
+ * (Langevin, friction gamma); the drift guard applies to verlet only.  The
+ * n_waves tracked wavevectors (kvec, n_waves x d, Cartesian) start with the
+ * n_constrained rows that are the energy's first k rows (their rho comes from
+ * the force pass); the rest are summed every wind_every steps.  Each step
+ * updates rho_prev (COM-corrected unless verlet), the winding sums (only
+ * where S = |rho|^2 / N exceeds phase_floor at both ends), the S sums, a
+ * snapshot at every step in record_steps (strictly increasing, at most 256)
+ * and the rho trace every record_rho_every steps (max_rho_records rows).
+ * Random draws: Philox-4x32-10 with keys split per step (the host module
+ * xla_cco/dynamics/rng.py defines the same draws). */
+typedef struct {
+  double andersen_p, gamma, phase_floor;
+  int32_t scheme, andersen_mode, wind_every, record_rho_every, max_rho_records, n_record_steps;
+} cco_md_settings;
+int cco_md_configure(cco_relax* r, const cco_md_settings* ms, int n_waves, int n_constrained,
+                     const double* kvec_host, const int64_t* record_steps_host);
+/* The complete lane state (MDState, state.py:129-165) in host arrays with a
+ * leading lane axis.  lane_f64 B x 10: phi, temperature, time, reset_acc, e0,
+ * max_drift, sum_kinetic, sum_phi, sum_phi2, kinetic; lane_i64 B x 5: step,
+ * n_sum, status, snapshot count, trace count; key B x 2.  Per wave (w):
+ * rho_prev B x w x 2, wind_abs, wind_net, n_valid, sum_s B x w; snapshots
+ * (nr rows): step, time, phi, kinetic B x nr, wind_abs, wind_net B x nr x w,
+ * rho B x nr x w x 2, r B x nr x N x d; trace (q rows): rho B x q x w x 2,
+ * time B x q.  cco_md_put replaces the device state (lanes whose status is
+ * not running stay frozen); cco_md_get reads it. */
+typedef struct {
+  double *r, *v, *f, *lane_f64;
+  int64_t* lane_i64;
+  uint32_t* key;
+  double *rho_prev, *wind_abs, *wind_net;
+  int64_t* n_valid;
+  double* sum_s;
+  int64_t* snap_step;
+  double *snap_time, *snap_phi, *snap_kinetic, *snap_wind_abs, *snap_wind_net, *snap_rho, *snap_r;
+  double *trace_rho, *trace_time;
+} cco_md_state_io;
+int cco_md_put(cco_relax* r, const cco_md_state_io* s);
+int cco_md_get(cco_relax* r, cco_md_state_io* s);
+
 /* Isotropic structure-factor moments: for the D directions n_c (D x d
  * int64) with L_c multiples each, k = m n_c (m = 1..L_c, directions in
  * order, multiples contiguous), tot[k] = sum over the B configurations of

@@ -33,6 +33,14 @@ Parity pins (what checks each function; tests/test_oracle_*.py):
 * ``zoom_linesearch``: the accepted step satisfies the strong-Wolfe or the
   approximate-decrease condition it claims; a convex quadratic's exact
   minimiser along the line is bracketed.
+* ``md_lane`` / ``md_run`` (schemes, tracked waves, winding, records):
+  Verlet equals ``verlet_nve``; BAOAB at gamma = 0 and Andersen at p = 0 equal
+  Verlet; the count mode fires exactly every 1/p steps; ballistic motion
+  (no forces, one common velocity u) winds every wave by wrap(k . u dt) per
+  step (wind_every steps at once for the untracked-every-step rows) and by
+  zero once COM-corrected, with S constant; BAOAB samples <x^2> = T / k of a
+  harmonic well at a large step (exact for BAOAB); snapshots and trace rows
+  equal the states of shorter runs; a drift stop freezes the lane.
 * ``lbfgs``: the reference's outcome-level tests (SPD quadratic to 1e-18 and
   within 1e-8 of the minimum, d=2 N=64 chi=0.25 below 1e-20, three passes equal
   three chained single passes, exact evaluation count, stagnation and
@@ -911,6 +919,174 @@ def verlet_nve(
 
 
 # ============================================================
+# MD WITH THERMOSTATS AND TRACKED WAVES (dynamics/run.py:259-537,
+# integrators.py:99-176, observables.py:56-125)
+# ============================================================
+
+
+@dataclasses.dataclass
+class MDConfig:
+    """The MDSettings fields the step reads (dynamics/state.py:57-88)."""
+
+    dt: float
+    scheme: str = "verlet"
+    andersen_p: float = 0.01
+    andersen_mode: str = "count"
+    gamma: float = 0.0
+    wind_every: int = 1
+    record_steps: Tuple[int, ...] = ()
+    record_rho_every: int = 0
+    max_rho_records: int = 0
+    phase_floor: float = 1e-16
+    drift_tol: float = math.inf
+
+
+def tracked_rho(r: np.ndarray, kvec: np.ndarray) -> np.ndarray:
+    """rho_k = sum_j exp(i k . r_j) as (w, 2) by the trig sum over particles (observables.py:56-78)."""
+    out = np.zeros((kvec.shape[0], 2))
+    for i in range(kvec.shape[0]):
+        th = r @ kvec[i]
+        out[i, 0] = np.cos(th).sum()
+        out[i, 1] = np.sin(th).sum()
+    return out
+
+
+def com_rotate(rho: np.ndarray, kvec: np.ndarray, r_cm: np.ndarray) -> np.ndarray:
+    """rho_k exp(-i k . R_cm) (observables.py:89-98)."""
+    out = np.empty_like(rho)
+    for i in range(rho.shape[0]):
+        ph = float(kvec[i] @ r_cm)
+        c, s_ = math.cos(ph), math.sin(ph)
+        out[i, 0] = rho[i, 0] * c + rho[i, 1] * s_
+        out[i, 1] = rho[i, 1] * c - rho[i, 0] * s_
+    return out
+
+
+def md_lane(vg: ValueGrad, basis: np.ndarray, r0: np.ndarray, v0: np.ndarray, temperature: float,
+            key, kvec: np.ndarray, cfg: MDConfig) -> Dict[str, object]:
+    """One lane at step 0 (init_state, run.py:300-352): forces, rho_prev, empty buffers."""
+    inv = np.linalg.inv(basis)
+    phi, g = vg(r0 @ inv)
+    n, d = r0.shape
+    w = kvec.shape[0]
+    rho = tracked_rho(r0, kvec)
+    if cfg.scheme != "verlet":
+        rho = com_rotate(rho, kvec, r0.mean(axis=0))
+    nr, q = len(cfg.record_steps), cfg.max_rho_records
+    return {
+        "r": r0.copy(), "v": v0.copy(), "f": -g @ inv.T, "phi": float(phi), "temperature": float(temperature),
+        "key": (int(key[0]), int(key[1])), "step": 0, "time": 0.0, "reset_acc": 0.0, "rho_prev": rho,
+        "wind_abs": np.zeros(w), "wind_net": np.zeros(w), "n_valid": np.zeros(w, np.int64),
+        "e0": float(phi) + 0.5 * float(np.sum(v0 * v0)), "max_drift": 0.0, "sum_kinetic": 0.0,
+        "sum_phi": 0.0, "sum_phi2": 0.0, "sum_s": np.zeros(w), "n_sum": 0, "status": STATUS_RUNNING,
+        "snap_step": np.full(nr, -1, np.int64), "snap_time": np.zeros(nr), "snap_wind_abs": np.zeros((nr, w)),
+        "snap_wind_net": np.zeros((nr, w)), "snap_rho": np.zeros((nr, w, 2)), "snap_r": np.zeros((nr, n, d)),
+        "snap_phi": np.zeros(nr), "snap_kinetic": np.zeros(nr), "snap_count": 0,
+        "trace_rho": np.zeros((q, w, 2)), "trace_time": np.zeros(q), "trace_count": 0,
+    }
+
+
+def md_run(vg: ValueGrad, basis: np.ndarray, lane: Dict[str, object], kvec: np.ndarray, nc: int,
+           cfg: MDConfig, n_steps: int) -> Dict[str, object]:
+    """n_steps steps of one lane (run.py:415-537); a lane that leaves STATUS_RUNNING stops.
+
+    Per step: the scheme (verlet; andersen = verlet then a reset from
+    (key', k_pick, k_draw, k_coin) = split(key, 4); baoab = B A O A B with
+    (key', k_xi) = split(key, 2)), rho of the tracked waves (the first nc
+    every step, the rest on steps with step % wind_every == 0, else their
+    previous value), COM correction of the fresh rows unless verlet, winding
+    where S = |rho|^2 / N exceeds phase_floor at both ends, the sums (sum_s
+    over every row), the drift guard (verlet), then the snapshot (step in
+    record_steps) and the trace row (step % record_rho_every == 0).
+    """
+    st = {k: (v.copy() if isinstance(v, np.ndarray) else v) for k, v in lane.items()}
+    inv = np.linalg.inv(basis)
+    n, d = st["r"].shape
+    w = kvec.shape[0]
+    dt, half = cfg.dt, 0.5 * cfg.dt
+    for _ in range(n_steps):
+        if st["status"] != STATUS_RUNNING:
+            break
+        key, T = st["key"], st["temperature"]
+        r, v, f = st["r"], st["v"], st["f"]
+        if cfg.scheme == "baoab":
+            key, k_xi = rng_split(key, 2)
+            c = math.exp(-cfg.gamma * dt)
+            var = -math.expm1(-2.0 * cfg.gamma * dt)
+            v = v + half * f
+            r = r + half * v
+            v = c * v + math.sqrt(var * T) * rng_normals(k_xi, n * d).reshape(n, d)
+            r = r + half * v
+            phi, g = vg(r @ inv)
+            f = -g @ inv.T
+            v = v + half * f
+        else:
+            vh = v + half * f
+            r = r + dt * vh
+            phi, g = vg(r @ inv)
+            f = -g @ inv.T
+            v = vh + half * f
+            if cfg.scheme == "andersen":
+                key, k_pick, k_draw, k_coin = rng_split(key, 4)
+                if cfg.andersen_mode == "count":
+                    st["reset_acc"] += cfg.andersen_p
+                    fire = st["reset_acc"] >= 1.0
+                    if fire:
+                        st["reset_acc"] -= 1.0
+                else:
+                    fire = rng_uniform(k_coin) < cfg.andersen_p
+                if fire:
+                    v = v.copy()
+                    v[rng_randint(k_pick, n)] = math.sqrt(T) * rng_normals(k_draw, d)
+        step = st["step"] + 1
+        wind = step % cfg.wind_every == 0
+        fresh = tracked_rho(r, kvec)
+        if cfg.scheme != "verlet":
+            fresh = com_rotate(fresh, kvec, r.mean(axis=0))
+        rho_prev = st["rho_prev"]
+        for i in range(w):
+            if not (i < nc or wind):
+                continue
+            o_re, o_im = rho_prev[i]
+            n_re, n_im = fresh[i]
+            delta = math.atan2(n_im * o_re - n_re * o_im, n_re * o_re + n_im * o_im)
+            if (o_re * o_re + o_im * o_im) / n > cfg.phase_floor and (n_re * n_re + n_im * n_im) / n > cfg.phase_floor:
+                st["wind_abs"][i] += abs(delta)
+                st["wind_net"][i] += delta
+                st["n_valid"][i] += 1
+            rho_prev[i] = fresh[i]
+        ke = 0.5 * float(np.sum(v * v))
+        drift = abs(float(phi) + ke - st["e0"])
+        if cfg.scheme == "verlet" and math.isfinite(cfg.drift_tol):
+            if not drift <= cfg.drift_tol * 0.5 * d * (n - 1) * T:
+                st["status"] = STATUS_DRIFT
+        st.update(r=r, v=v, f=f, phi=float(phi), key=key, step=step, time=st["time"] + dt)
+        st["max_drift"] = _nanmax(st["max_drift"], drift)
+        st["sum_kinetic"] += ke
+        st["sum_phi"] += float(phi)
+        st["sum_phi2"] += float(phi) * float(phi)
+        st["sum_s"] = st["sum_s"] + (rho_prev[:, 0] ** 2 + rho_prev[:, 1] ** 2) / n
+        st["n_sum"] += 1
+        if step in cfg.record_steps:
+            j = cfg.record_steps.index(step)
+            st["snap_step"][j] = step
+            st["snap_time"][j] = st["time"]
+            st["snap_wind_abs"][j] = st["wind_abs"]
+            st["snap_wind_net"][j] = st["wind_net"]
+            st["snap_rho"][j] = rho_prev
+            st["snap_r"][j] = r
+            st["snap_phi"][j] = float(phi)
+            st["snap_kinetic"][j] = ke
+            st["snap_count"] += 1
+        every, cap = cfg.record_rho_every, cfg.max_rho_records
+        if every > 0 and cap > 0 and step % every == 0 and st["trace_count"] < cap:
+            st["trace_rho"][st["trace_count"]] = rho_prev
+            st["trace_time"][st["trace_count"]] = st["time"]
+            st["trace_count"] += 1
+    return st
+
+
+# ============================================================
 # ANALYSIS (tests/oracle.py:158-209, analysis/structure.py:111-201)
 # ============================================================
 
@@ -966,3 +1142,61 @@ def isotropic_kset(basis: np.ndarray, k_dir_max: float, k_max: float) -> Tuple[n
             ksq.append(q)
             m += 1
     return np.array(rows, dtype=np.int64), np.array(ksq)
+
+
+# ============================================================
+# COUNTER-BASED DRAWS (the thermostats' random numbers)
+# ============================================================
+#
+# Philox-4x32-10 (Salmon, Moraes, Dror and Shaw, SC 2011) in plain Python
+# integers, and the split / normal / uniform / integer draws built on it.
+# Pinned by the published known-answer vectors of Philox-4x32-10 and by the
+# moments and Kolmogorov-Smirnov statistics of the draws.
+
+_PH_M = (0xD2511F53, 0xCD9E8D57)
+_PH_W = (0x9E3779B9, 0xBB67AE85)
+TAG_SPLIT, TAG_NORMAL, TAG_UNIF = 0x53504C54, 0x4E524D4C, 0x554E4946
+
+
+def philox4x32_10(ctr: Tuple[int, int, int, int], key: Tuple[int, int]) -> Tuple[int, int, int, int]:
+    c0, c1, c2, c3 = (int(x) & 0xFFFFFFFF for x in ctr)
+    k0, k1 = (int(x) & 0xFFFFFFFF for x in key)
+    for _ in range(10):
+        p0 = _PH_M[0] * c0
+        p1 = _PH_M[1] * c2
+        c0, c1, c2, c3 = (p1 >> 32) ^ c1 ^ k0, p1 & 0xFFFFFFFF, (p0 >> 32) ^ c3 ^ k1, p0 & 0xFFFFFFFF
+        k0, k1 = (k0 + _PH_W[0]) & 0xFFFFFFFF, (k1 + _PH_W[1]) & 0xFFFFFFFF
+    return c0, c1, c2, c3
+
+
+def rng_split(key, num: int):
+    words: List[int] = []
+    i = 0
+    while len(words) < 2 * num:
+        words += list(philox4x32_10((i, 0, 0, TAG_SPLIT), key))
+        i += 1
+    return [(words[2 * j], words[2 * j + 1]) for j in range(num)]
+
+
+def _u53(hi: int, lo: int) -> float:
+    return ((hi >> 5) * 67108864 + (lo >> 6)) / 9007199254740992.0
+
+
+def rng_normals(key, count: int) -> np.ndarray:
+    out = []
+    for i in range((count + 1) // 2):
+        w = philox4x32_10((i, 0, 0, TAG_NORMAL), key)
+        u1, u2 = _u53(w[0], w[1]), _u53(w[2], w[3])
+        rho = math.sqrt(-2.0 * math.log1p(-u1))
+        out += [rho * math.cos(2.0 * math.pi * u2), rho * math.sin(2.0 * math.pi * u2)]
+    return np.array(out[:count])
+
+
+def rng_uniform(key) -> float:
+    w = philox4x32_10((0, 0, 0, TAG_UNIF), key)
+    return _u53(w[0], w[1])
+
+
+def rng_randint(key, n: int) -> int:
+    w = philox4x32_10((0, 0, 0, TAG_UNIF), key)
+    return min(int(math.floor(_u53(w[2], w[3]) * n)), n - 1)

@@ -16,7 +16,7 @@ import pytest
 from oracle import oracle as O
 from xla_cco import kvectors as K
 from xla_cco.box import cubic_box
-from xla_cco.dynamics import MDSettings, init_state, make_force_fn, simulate, verlet_step
+from xla_cco.dynamics import MDSettings, Waves, init_state, make_force_fn, prng_key, simulate, verlet_step
 from xla_cco.energy import AbstractEnergy, make_stealthy_energy
 from xla_cco.minimize.flow import FlowSettings, gradient_flow, gradient_flow_batch
 from xla_cco.minimize.result import STATUS_BUDGET, STATUS_DRIFT, STATUS_ENERGY_TOL, STATUS_GRAD_TOL
@@ -174,14 +174,31 @@ def _md_system(n: int = 64, d: int = 2, chi: float = 0.2):
     return e, box.basis, O.make_value_grad(ks.n, ks.weights, box.volume)
 
 
+def _energy_waves(e, basis) -> Waves:
+    """The energy's own k-set (its padding rows dropped) as the tracked waves, all constrained."""
+    nv = np.asarray(e.n_vectors, dtype=np.float64)
+    nv = nv[: int(np.count_nonzero(np.any(nv != 0.0, axis=1)))]
+    return Waves(n_vectors=nv, kvec=nv @ (2.0 * np.pi * np.linalg.inv(basis).T),
+                 constrained=np.ones(nv.shape[0], bool), n_constrained=nv.shape[0])
+
+
+def _start(e, basis, r0, v0, temperature, st):
+    keys = np.stack([prng_key(i) for i in range(r0.shape[0])])
+    return init_state(e, _energy_waves(e, basis), basis, r0, v0, temperature, keys, st)
+
+
+def _run(e, basis, st, state, n):
+    return simulate(e, _energy_waves(e, basis), basis, st, state, n).state
+
+
 def test_md_matches_oracle(gpu_ctx) -> None:
     e, basis, vg = _md_system()
     rng = np.random.default_rng(0)
     r0 = rng.uniform(size=(3, 64, 2)) @ basis
     v0 = 0.01 * rng.normal(size=(3, 64, 2))
     st = MDSettings(dt=0.01, remove_com=False, drift_tol=math.inf)
-    state = init_state(e, basis, r0, v0, 1e-4, st)
-    out = simulate(state, 60, st).state
+    state = _start(e, basis, r0, v0, 1e-4, st)
+    out = _run(e, basis, st, state, 60)
     for i in range(3):
         ref = O.verlet_nve(vg, basis, r0[i], v0[i], 0.01, 60)
         assert int(out.step[i]) == 60 and int(out.status[i]) == 0
@@ -199,8 +216,8 @@ def test_md_time_reversal_on_device(gpu_ctx) -> None:
     r0 = rng.uniform(size=(2, 40, 3)) @ basis
     v0 = 0.02 * rng.normal(size=(2, 40, 3))
     st = MDSettings(dt=0.01, remove_com=False, drift_tol=math.inf)
-    fwd = simulate(init_state(e, basis, r0, v0, 1.0, st), 150, st).state
-    back = simulate(init_state(e, basis, fwd.r, -fwd.v, 1.0, st), 150, st).state
+    fwd = _run(e, basis, st, _start(e, basis, r0, v0, 1.0, st), 150)
+    back = _run(e, basis, st, _start(e, basis, fwd.r, -fwd.v, 1.0, st), 150)
     assert np.max(np.abs(back.r - r0)) < 1e-10
 
 
@@ -211,11 +228,11 @@ def test_md_drift_guard_freezes_lane(gpu_ctx) -> None:
     v0 = 0.01 * rng.normal(size=(2, 64, 2))
     # huge dt on random (high-energy) states: |E - e0| explodes at once
     st = MDSettings(dt=0.5, remove_com=False, drift_tol=1e-3)
-    out = simulate(init_state(e, basis, r0, v0, 1e-6, st), 40, st).state
+    out = _run(e, basis, st, _start(e, basis, r0, v0, 1e-6, st), 40)
     assert np.all(out.status == STATUS_DRIFT)
     assert np.all(out.step < 40)
     frozen = out.r.copy()
-    again = simulate(out, 5, st).state
+    again = _run(e, basis, st, out, 5)
     assert np.array_equal(again.r, frozen)
 
 
@@ -225,8 +242,10 @@ def test_md_com_removal_and_energy_conservation(gpu_ctx) -> None:
     r0 = rng.uniform(size=(1, 64, 2)) @ basis
     v0 = 1e-3 * rng.normal(size=(1, 64, 2)) + 0.5
     st = MDSettings(dt=0.005, remove_com=True, drift_tol=math.inf)
-    state = init_state(e, basis, r0, v0, 1e-6, st)
-    out = simulate(state, 200, st).state
+    from xla_cco.dynamics import begin_segment
+
+    state = begin_segment(_start(e, basis, r0, v0, 1e-6, st), st, reset_sums=True, reset_winding=False)
+    out = _run(e, basis, st, state, 200)
     assert np.abs(out.v[0].sum(axis=0)).max() < 1e-12
     assert out.max_drift[0] < 5e-3 * abs(out.e0[0])
 

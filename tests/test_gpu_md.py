@@ -1,0 +1,217 @@
+"""Device MD (schemes, thermostats, tracked waves, records) against the oracle.
+
+Mirrors tests/unit/test_md_run.py and test_md_thermostats.py of the
+reference: every scheme over several chunks with tracked waves (constrained
+and every-other-step ones), snapshots and the rho trace; lanes at different
+temperatures, one stopped by the drift guard; begin_segment with a pending
+rho_prev; checkpoints; md_chunk; select_dt.  The device and the oracle draw
+the same Philox numbers, so trajectories, keys and counters agree lane by lane.
+"""
+
+from __future__ import annotations
+
+import dataclasses
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from xla_cco import kvectors as K
+from xla_cco.box import cubic_box
+from xla_cco.dynamics import (
+    MDSettings,
+    begin_segment,
+    init_state,
+    lane,
+    load_state,
+    make_waves,
+    md_chunk,
+    prng_key,
+    read_meta,
+    select_dt,
+    simulate,
+)
+from xla_cco.dynamics.state import leaves
+from xla_cco.energy import make_stealthy_energy
+from xla_cco.minimize.result import STATUS_DRIFT, STATUS_RUNNING
+
+pytestmark = pytest.mark.gpu
+
+
+def _system(n: int = 32, d: int = 2, chi: float = 0.2, track: float = 1.4):
+    box = cubic_box(n, d)
+    k_c = K.k_outer_from_ka(K.ka_from_chi(chi, 0.05, d), 0.05, d)
+    ks = K.enumerate_kvectors(box.basis, 0.0, k_c)
+    e = make_stealthy_energy(box, ks)
+    waves = make_waves(box.basis, k_c, track * k_c)
+    assert waves.kvec.shape[0] > waves.n_constrained
+    return e, box.basis, waves, O.make_value_grad(ks.n, ks.weights, box.volume)
+
+
+def _cfg(st: MDSettings) -> O.MDConfig:
+    return O.MDConfig(dt=st.dt, scheme=st.scheme, andersen_p=st.andersen_p, andersen_mode=st.andersen_mode,
+                      gamma=st.gamma, wind_every=st.wind_every, record_steps=tuple(st.record_steps),
+                      record_rho_every=st.record_rho_every, max_rho_records=st.max_rho_records,
+                      phase_floor=st.phase_floor, drift_tol=st.drift_tol)
+
+
+def _compare(dev, ref, i: int, tol: float = 1e-9) -> None:
+    """Lane i of a device state against an oracle lane dict."""
+    assert tuple(int(x) for x in dev.key[i]) == ref["key"]
+    for name in ("step", "n_sum", "status"):
+        assert int(getattr(dev, name)[i]) == ref[name], name
+    assert np.array_equal(dev.n_valid[i], ref["n_valid"])
+    assert int(dev.snap.count[i]) == ref["snap_count"] and int(dev.trace.count[i]) == ref["trace_count"]
+    assert np.array_equal(dev.snap.step[i], ref["snap_step"])
+
+    def close(a, b, what, scale=1.0):
+        err = np.max(np.abs(np.asarray(a) - np.asarray(b))) if np.size(a) else 0.0
+        assert err <= tol * scale, f"{what}: {err}"
+
+    close(dev.r[i], ref["r"], "r")
+    close(dev.v[i], ref["v"], "v")
+    close(dev.f[i], ref["f"], "f", 10.0)
+    n = dev.r.shape[1]
+    close(dev.phi[i], ref["phi"], "phi", max(1.0, abs(ref["phi"])))
+    close(dev.time[i], ref["time"], "time")
+    close(dev.reset_acc[i], ref["reset_acc"], "reset_acc")
+    close(dev.e0[i], ref["e0"], "e0", max(1.0, abs(ref["e0"])))
+    close(dev.rho_prev[i], ref["rho_prev"], "rho_prev", n)
+    close(dev.wind_abs[i], ref["wind_abs"], "wind_abs", 100.0)
+    close(dev.wind_net[i], ref["wind_net"], "wind_net", 100.0)
+    close(dev.sum_s[i], ref["sum_s"], "sum_s", n * max(1, ref["n_sum"]))
+    for name in ("sum_kinetic", "sum_phi", "sum_phi2", "max_drift"):
+        close(getattr(dev, name)[i], ref[name], name, max(1.0, abs(ref[name])))
+    close(dev.snap.r[i], ref["snap_r"], "snap.r")
+    close(dev.snap.rho[i], ref["snap_rho"], "snap.rho", n)
+    close(dev.snap.wind_net[i], ref["snap_wind_net"], "snap.wind_net", 100.0)
+    close(dev.snap.phi[i], ref["snap_phi"], "snap.phi", 10.0)
+    close(dev.snap.kinetic[i], ref["snap_kinetic"], "snap.kinetic", 10.0)
+    close(dev.snap.time[i], ref["snap_time"], "snap.time")
+    close(dev.trace.rho[i], ref["trace_rho"], "trace.rho", n)
+    close(dev.trace.time[i], ref["trace_time"], "trace.time")
+
+
+SCHEMES = [
+    dict(scheme="verlet"),
+    dict(scheme="andersen", andersen_p=0.3, andersen_mode="count"),
+    dict(scheme="andersen", andersen_p=0.4, andersen_mode="bernoulli"),
+    dict(scheme="baoab", gamma=0.8),
+]
+
+
+@pytest.mark.parametrize("kw", SCHEMES, ids=lambda kw: f"{kw['scheme']}-{kw.get('andersen_mode', '')}")
+def test_schemes_match_oracle(gpu_ctx, kw) -> None:
+    e, basis, waves, vg = _system()
+    st = MDSettings(dt=0.02, chunk_steps=7, wind_every=2, record_steps=(3, 10, 25), record_rho_every=4,
+                    max_rho_records=5, drift_tol=0.5, **kw)
+    rng = np.random.default_rng(0)
+    B, n, d = 3, 32, 2
+    r0 = rng.uniform(size=(B, n, d)) @ basis
+    temps = np.array([0.02, 0.05, 1e-14])  # the last lane trips the Verlet drift guard
+    v0 = rng.normal(size=(B, n, d)) * np.sqrt(np.array([0.02, 0.05, 0.05]))[:, None, None]
+    keys = np.stack([prng_key(100 + i) for i in range(B)])
+    state = init_state(e, waves, basis, r0, v0, temps, keys, st)
+    out = simulate(e, waves, basis, st, state, 30).state
+    cfg = _cfg(st)
+    nc = waves.n_constrained
+    for i in range(B):
+        ref = O.md_run(vg, basis, O.md_lane(vg, basis, r0[i], v0[i], temps[i], tuple(int(x) for x in keys[i]),
+                                            waves.kvec, cfg), waves.kvec, nc, cfg, 30)
+        _compare(out, ref, i)
+    if st.scheme == "verlet":
+        assert out.status[2] == STATUS_DRIFT and out.step[2] < 30
+    assert np.all(out.status[:2] == STATUS_RUNNING) and np.all(out.step[:2] == 30)
+
+
+def test_segment_with_pending_rho(gpu_ctx) -> None:
+    """begin_segment(reset_winding) marks rho_prev pending; the next run recomputes it first."""
+    e, basis, waves, vg = _system(n=24)
+    st = MDSettings(dt=0.02, scheme="baoab", gamma=0.5, chunk_steps=5, record_rho_every=3, max_rho_records=8)
+    rng = np.random.default_rng(1)
+    r0 = rng.uniform(size=(2, 24, 2)) @ basis
+    v0 = 0.2 * rng.normal(size=(2, 24, 2))
+    keys = np.stack([prng_key(7), prng_key(8)])
+    first = simulate(e, waves, basis, st, init_state(e, waves, basis, r0, v0, 0.04, keys, st), 9).state
+    seg = begin_segment(first, st, reset_sums=True, reset_winding=True)
+    assert np.all(np.isnan(seg.rho_prev))
+    assert np.array_equal(simulate(e, waves, basis, st, seg, 0).state.rho_prev, seg.rho_prev, equal_nan=True)
+    out = simulate(e, waves, basis, st, seg, 8).state
+    cfg = _cfg(st)
+    for i in range(2):
+        ref = O.md_run(vg, basis, O.md_lane(vg, basis, r0[i], v0[i], 0.04, tuple(int(x) for x in keys[i]),
+                                            waves.kvec, cfg), waves.kvec, waves.n_constrained, cfg, 9)
+        # begin_segment on the oracle lane: COM momentum out, e0 anew, sums and winding zeroed,
+        # rho_prev recomputed from the current positions (COM-corrected: baoab)
+        v = ref["v"] - ref["v"].mean(axis=0)
+        ref.update(v=v, e0=ref["phi"] + 0.5 * float(np.sum(v * v)), max_drift=0.0, sum_kinetic=0.0, sum_phi=0.0,
+                   sum_phi2=0.0, sum_s=np.zeros_like(ref["sum_s"]), n_sum=0, wind_abs=np.zeros_like(ref["wind_abs"]),
+                   wind_net=np.zeros_like(ref["wind_net"]), n_valid=np.zeros_like(ref["n_valid"]),
+                   rho_prev=O.com_rotate(O.tracked_rho(ref["r"], waves.kvec), waves.kvec, ref["r"].mean(axis=0)))
+        ref = O.md_run(vg, basis, ref, waves.kvec, waves.n_constrained, cfg, 8)
+        _compare(out, ref, i)
+
+
+def test_checkpoint_and_md_chunk(gpu_ctx, tmp_path) -> None:
+    e, basis, waves, _ = _system(n=20)
+    st = MDSettings(dt=0.02, scheme="andersen", andersen_p=0.5, chunk_steps=4, record_steps=(2, 9))
+    rng = np.random.default_rng(2)
+    r0 = rng.uniform(size=(2, 20, 2)) @ basis
+    v0 = 0.1 * rng.normal(size=(2, 20, 2))
+    keys = np.stack([prng_key(1), prng_key(2)])
+    state = init_state(e, waves, basis, r0, v0, 0.01, keys, st)
+    path = tmp_path / "md.npz"
+    res = simulate(e, waves, basis, st, state, 10, checkpoint=path, checkpoint_every_s=1e-9)
+    assert read_meta(path) == {"source": "xla_cco.dynamics.run.simulate", "steps_done": 10, "n_steps": 10}
+    back, _ = load_state(path, res.state)
+    for a, b in zip(leaves(back), leaves(res.state)):
+        assert np.array_equal(a, b)
+    assert np.array_equal(back.snap.r, res.state.snap.r) and res.trace is None
+    # resuming from the checkpoint equals running on
+    more = simulate(e, waves, basis, st, back, 5).state
+    cont = simulate(e, waves, basis, st, res.state, 5).state
+    assert np.array_equal(more.r, cont.r) and np.array_equal(more.key, cont.key)
+    # md_chunk: one lane, at most chunk_steps steps, equal to its lane of the batch
+    one = md_chunk(e, waves, basis, st, lane(state, 1), 100)
+    four = simulate(e, waves, basis, st, state, 4).state
+    assert int(one.step) == 4
+    assert np.array_equal(one.r, four.r[1]) and np.array_equal(one.key, four.key[1])
+    init_one = init_state(e, waves, basis, r0[1], v0[1], 0.01, keys[1], st)
+    assert np.allclose(init_one.rho_prev, state.rho_prev[1], rtol=0, atol=1e-10)
+    with pytest.raises(ValueError):
+        simulate(e, waves, basis, st, init_one, 3)  # unbatched
+
+
+def test_waves_must_match_the_energy(gpu_ctx) -> None:
+    e, basis, waves, _ = _system(n=20)
+    nc = waves.n_constrained
+    order = np.r_[np.arange(nc)[::-1], np.arange(nc, waves.kvec.shape[0])]  # constrained rows reversed
+    bad = dataclasses.replace(waves, n_vectors=waves.n_vectors[order], kvec=waves.kvec[order])
+    st = MDSettings(dt=0.02)
+    r0 = np.random.default_rng(3).uniform(size=(1, 20, 2)) @ basis
+    with pytest.raises(ValueError):
+        init_state(e, bad, basis, r0, np.zeros_like(r0), 0.01, np.stack([prng_key(0)]), st)
+
+
+def test_select_dt_matches_oracle_drift(gpu_ctx) -> None:
+    e, basis, waves, vg = _system(n=24, chi=0.1)
+    rng = np.random.default_rng(4)
+    r0 = rng.uniform(size=(2, 24, 2)) @ basis
+    v0 = 0.05 * rng.normal(size=(2, 24, 2))
+    keys = np.stack([prng_key(0), prng_key(1)])
+    state = init_state(e, waves, basis, r0, v0, 0.0025, keys, MDSettings())
+    ladder, n_steps = (0.4, 0.1, 0.02), 50
+
+    def rel_drift(dt: float) -> float:
+        worst = 0.0
+        for i in range(2):
+            ref = O.verlet_nve(vg, basis, r0[i], v0[i], dt, n_steps)
+            e_last = ref["phi"] + 0.5 * float(np.sum(ref["v"] ** 2))
+            worst = max(worst, abs(e_last - ref["e0"]) / (ref["sum_kinetic"] / n_steps))
+        return worst
+
+    drifts = {dt: rel_drift(dt) for dt in ladder}
+    limit = math.sqrt(drifts[0.1] * drifts[0.02]) if drifts[0.1] > drifts[0.02] else 2.0 * drifts[0.02]
+    want = max(dt for dt in ladder if drifts[dt] < limit)
+    assert select_dt(e, waves, basis, state, ladder, n_steps, limit) == want

@@ -137,7 +137,7 @@ def test_mixed_md_energy_drift_vs_fp64(gpu_ctx) -> None:
     stays stable and its energy drift is reported against the FP64 run."""
     import math
 
-    from xla_cco.dynamics import MDSettings, init_state, simulate
+    from xla_cco.dynamics import MDSettings, Waves, init_state, prng_key, simulate
 
     box, ks = _system(400, 3, 0.3)
     e = make_stealthy_energy(box, ks)
@@ -145,10 +145,15 @@ def test_mixed_md_energy_drift_vs_fp64(gpu_ctx) -> None:
     res = relax(e, s0, LBFGSSettings(energy_tol=1e-20))
     r0 = res.s @ box.basis
     v0 = np.random.default_rng(22).normal(scale=math.sqrt(1e-4), size=r0.shape)
+    nv = np.asarray(ks.n, dtype=np.float64)
+    waves = Waves(n_vectors=nv, kvec=nv @ (2.0 * np.pi * np.linalg.inv(box.basis).T),
+                  constrained=np.ones(nv.shape[0], bool), n_constrained=nv.shape[0])
+    keys = np.stack([prng_key(1), prng_key(2)])
     out = {}
     for prec in ("fp64", "mixed"):
         st = MDSettings(dt=0.01, remove_com=False, drift_tol=math.inf, precision=prec)
-        out[prec] = simulate(init_state(e, box.basis, r0, v0, 1e-4, st), 400, st).state
+        lanes = init_state(e, waves, box.basis, r0, v0, 1e-4, keys, st)
+        out[prec] = simulate(e, waves, box.basis, st, lanes, 400).state
     a, m = out["fp64"], out["mixed"]
     assert np.all(m.step == 400) and np.all(m.status == 0)
     kin = 0.5 * np.sum(v0**2, axis=(1, 2))

@@ -414,6 +414,10 @@ struct cco_relax {
   // steps, pivots, factorisation info and the chunk's lane count
   bool ros2 = false;
   DevBuf ros_S, ros_H, ros_rhs, ros_dts, ros_ipiv, ros_info, ros_cnt;
+  // MD observables (cco_md_configure): device buffers behind md_obs and the
+  // waves / recorded steps behind mc.kvec, mc.rec_steps
+  MdObs md_obs{};
+  std::vector<DevBuf> md_bufs;
   std::vector<hipEvent_t> evs;
   double stage_ms[kStages] = {};
   int64_t stage_n[kStages] = {};
@@ -502,6 +506,10 @@ Batch relax_batch(cco_relax* R, const RelaxDev& dv) {
   b.cbox = (double2*)R->cbox.p;
   b.phi_part = (double*)R->rphi.p;
   b.prec = R->dev.prec;
+  if (R->method == METHOD_MD && R->mc.w > 0) {  // the tracked constrained waves read re / im
+    b.re = R->md_obs.re;
+    b.im = R->md_obs.im;
+  }
   if (R->uneven) {
     b.lay = (const LaneLayout*)R->un_lay.p;
     b.geos = (const Geo*)R->un_geo.p;
@@ -528,7 +536,7 @@ void relax_stage(cco_relax* R, const Shard& sh, int stage) {
     else if (R->method == METHOD_FLOW)
       launch_flow_advance(dv, R->fc, ub, st);
     else
-      launch_md_finish(dv, R->mc, ub, st);
+      launch_md_finish(dv, R->mc, R->md_obs, ub, st);
   } else if (stage == STG_COMPACT) {
     launch_compact(dv, st, sh.lo, sh.count, (unsigned)(sh.steps + 1));
   } else if (stage >= STG_GRAM_DOT) {
@@ -1166,6 +1174,7 @@ int cco_relax_destroy(cco_relax* R) {
   DevScope ds(R->ctx->device);
   (void)hipStreamSynchronize(R->ctx->stream);
   for (DevBuf& b : R->bufs) dev_release(b);
+  for (DevBuf& b : R->md_bufs) dev_release(b);
   for (DevBuf* b : {&R->part, &R->cbox, &R->rphi, &R->out_s, &R->out_scalars, &R->pack,
                     &R->pack_idx, &R->un_lay, &R->un_geo, &R->ros_S, &R->ros_H, &R->ros_rhs,
                     &R->ros_dts, &R->ros_ipiv, &R->ros_info, &R->ros_cnt})
@@ -2130,6 +2139,245 @@ int cco_md_state(cco_relax* R, double* r, double* v, double* f, double* scalars,
   return CCO_OK;
 }
 
+
+extern "C++" {
+namespace {
+
+template <typename T>
+int md_alloc(cco_relax* R, T*& out, size_t count) {
+  DevBuf b;
+  int rc = dev_reserve(b, std::max<size_t>(count, 1) * sizeof(T));
+  if (rc) return rc;
+  CCO_HIP(hipMemsetAsync(b.p, 0, b.bytes, R->ctx->stream));
+  R->md_bufs.push_back(b);
+  out = (T*)b.p;
+  return CCO_OK;
+}
+
+template <typename T>
+int md_copy(T* dst, const T* src, size_t count, hipMemcpyKind kind) {
+  if (count == 0 || dst == nullptr || src == nullptr) return CCO_OK;
+  CCO_HIP(hipMemcpy(dst, src, sizeof(T) * count, kind));
+  return CCO_OK;
+}
+
+}  // namespace
+}  // extern "C++"
+
+int cco_md_configure(cco_relax* R, const cco_md_settings* ms, int n_waves, int n_constrained,
+                     const double* kvec_host, const int64_t* record_steps_host) {
+  if (!R || !ms) return fail(CCO_ERR_ARG, "NULL argument");
+  if (R->method != METHOD_MD) return fail(CCO_ERR_STATE, "not an MD session");
+  if (ms->scheme < 0 || ms->scheme > 2) return fail(CCO_ERR_ARG, "scheme must be 0, 1 or 2");
+  if (ms->andersen_mode < 0 || ms->andersen_mode > 1)
+    return fail(CCO_ERR_ARG, "andersen_mode must be 0 (count) or 1 (bernoulli)");
+  if (!(ms->andersen_p >= 0.0 && ms->andersen_p <= 1.0) || !(ms->gamma >= 0.0) ||
+      !std::isfinite(ms->gamma))
+    return fail(CCO_ERR_ARG, "need 0 <= andersen_p <= 1 and a finite gamma >= 0");
+  if (ms->wind_every < 1 || ms->record_rho_every < 0 || ms->max_rho_records < 0 ||
+      ms->n_record_steps < 0 || ms->n_record_steps > 256 || !(ms->phase_floor >= 0.0))
+    return fail(CCO_ERR_ARG, "bad wind_every, trace, record or phase_floor setting");
+  if (n_waves < 0 || n_constrained < 0 || n_constrained > n_waves ||
+      n_constrained > R->E->geo.M || (n_waves > 0 && !kvec_host) ||
+      (ms->n_record_steps > 0 && !record_steps_host))
+    return fail(CCO_ERR_ARG, "bad waves (need n_constrained <= n_waves, <= M) or NULL arrays");
+  for (int i = 1; i < ms->n_record_steps; ++i)
+    if (record_steps_host[i] <= record_steps_host[i - 1])
+      return fail(CCO_ERR_ARG, "record_steps must be strictly increasing");
+  DevScope ds(R->ctx->device);
+  for (DevBuf& b : R->md_bufs) dev_release(b);
+  R->md_bufs.clear();
+  MdCfg& c = R->mc;
+  c.scheme = ms->scheme;
+  c.andersen_mode = ms->andersen_mode;
+  c.andersen_p = ms->andersen_p;
+  c.ou_c = std::exp(-ms->gamma * c.dt);
+  c.ou_var = -std::expm1(-2.0 * ms->gamma * c.dt);  // 1 - c^2 without cancellation
+  c.phase_floor = ms->phase_floor;
+  c.w = n_waves;
+  c.nc = n_constrained;
+  c.wind_every = ms->wind_every;
+  c.m_pad = R->E->geo.M_pad;
+  c.n_rec = ms->n_record_steps;
+  c.rho_every = ms->record_rho_every;
+  c.rho_cap = ms->record_rho_every > 0 ? ms->max_rho_records : 0;
+  const size_t B = R->B, w = n_waves, nr = c.n_rec, q = c.rho_cap;
+  const size_t nd = (size_t)R->N * R->d;
+  MdObs& o = R->md_obs;
+  o = MdObs{};
+  int rc = CCO_OK;
+  double* kvec = nullptr;
+  long long* rec = nullptr;
+  double *re = nullptr, *im = nullptr;
+  auto go = [&](auto*& p, size_t cnt) {
+    if (rc == 0) rc = md_alloc(R, p, cnt);
+  };
+  go(kvec, w * R->d);
+  go(rec, nr);
+  go(o.rho_prev, B * w * 2);
+  go(o.wind_abs, B * w);
+  go(o.wind_net, B * w);
+  go(o.n_valid, B * w);
+  go(o.sum_s, B * w);
+  go(re, B * c.m_pad);
+  go(im, B * c.m_pad);
+  go(o.snap_step, B * nr);
+  go(o.snap_time, B * nr);
+  go(o.snap_phi, B * nr);
+  go(o.snap_kin, B * nr);
+  go(o.snap_wabs, B * nr * w);
+  go(o.snap_wnet, B * nr * w);
+  go(o.snap_rho, B * nr * w * 2);
+  go(o.snap_r, B * nr * nd);
+  go(o.tr_rho, B * q * w * 2);
+  go(o.tr_time, B * q);
+  if (rc) return rc;
+  o.re = re;
+  o.im = im;
+  if (w > 0) CCO_HIP(hipMemcpy(kvec, kvec_host, sizeof(double) * w * R->d, hipMemcpyHostToDevice));
+  if (nr > 0)
+    CCO_HIP(hipMemcpy(rec, record_steps_host, sizeof(long long) * nr, hipMemcpyHostToDevice));
+  c.kvec = kvec;
+  c.rec_steps = rec;
+  CCO_HIP(hipStreamSynchronize(R->ctx->stream));
+  return CCO_OK;
+}
+
+int cco_md_put(cco_relax* R, const cco_md_state_io* s) {
+  if (!R || !s || !s->r || !s->v || !s->f || !s->lane_f64 || !s->lane_i64 || !s->key)
+    return fail(CCO_ERR_ARG, "NULL argument");
+  if (R->method != METHOD_MD) return fail(CCO_ERR_STATE, "not an MD session");
+  DevScope ds(R->ctx->device);
+  hipStream_t st = R->ctx->stream;
+  CCO_HIP(hipStreamSynchronize(st));
+  const size_t B = R->B, nd = (size_t)R->dev.stride;
+  const MdCfg& c = R->mc;
+  const size_t w = c.w, nr = c.n_rec, q = c.rho_cap;
+  int rc;
+  if ((rc = md_copy(R->dev.r, s->r, B * nd, hipMemcpyHostToDevice))) return rc;
+  if ((rc = md_copy(R->dev.v, s->v, B * nd, hipMemcpyHostToDevice))) return rc;
+  if ((rc = md_copy(R->dev.f, s->f, B * nd, hipMemcpyHostToDevice))) return rc;
+  std::vector<MdLane> L(B);
+  std::vector<int> status(B);
+  for (size_t b = 0; b < B; ++b) {
+    const double* x = s->lane_f64 + b * 10;
+    const int64_t* y = s->lane_i64 + b * 5;
+    MdLane& l = L[b];
+    l = MdLane{};
+    l.phi = x[0];
+    l.temperature = x[1];
+    l.time = x[2];
+    l.reset_acc = x[3];
+    l.e0 = x[4];
+    l.max_drift = x[5];
+    l.sum_kinetic = x[6];
+    l.sum_phi = x[7];
+    l.sum_phi2 = x[8];
+    l.kinetic = x[9];
+    l.step = y[0];
+    l.n_sum = y[1];
+    status[b] = (int)y[2];
+    l.snap_count = y[3];
+    l.trace_count = y[4];
+    l.key[0] = s->key[2 * b];
+    l.key[1] = s->key[2 * b + 1];
+    l.mode = MODE_EVAL;
+  }
+  if ((rc = md_copy(R->dev.ml, L.data(), B, hipMemcpyHostToDevice))) return rc;
+  if ((rc = md_copy(R->dev.status, status.data(), B, hipMemcpyHostToDevice))) return rc;
+  const MdObs& o = R->md_obs;
+  if (w > 0) {
+    if ((rc = md_copy(o.rho_prev, s->rho_prev, B * w * 2, hipMemcpyHostToDevice))) return rc;
+    if ((rc = md_copy(o.wind_abs, s->wind_abs, B * w, hipMemcpyHostToDevice))) return rc;
+    if ((rc = md_copy(o.wind_net, s->wind_net, B * w, hipMemcpyHostToDevice))) return rc;
+    if ((rc = md_copy(o.n_valid, (const long long*)s->n_valid, B * w, hipMemcpyHostToDevice)))
+      return rc;
+    if ((rc = md_copy(o.sum_s, s->sum_s, B * w, hipMemcpyHostToDevice))) return rc;
+    if ((rc = md_copy(o.snap_step, (const long long*)s->snap_step, B * nr, hipMemcpyHostToDevice)))
+      return rc;
+    if ((rc = md_copy(o.snap_time, s->snap_time, B * nr, hipMemcpyHostToDevice))) return rc;
+    if ((rc = md_copy(o.snap_phi, s->snap_phi, B * nr, hipMemcpyHostToDevice))) return rc;
+    if ((rc = md_copy(o.snap_kin, s->snap_kinetic, B * nr, hipMemcpyHostToDevice))) return rc;
+    if ((rc = md_copy(o.snap_wabs, s->snap_wind_abs, B * nr * w, hipMemcpyHostToDevice))) return rc;
+    if ((rc = md_copy(o.snap_wnet, s->snap_wind_net, B * nr * w, hipMemcpyHostToDevice))) return rc;
+    if ((rc = md_copy(o.snap_rho, s->snap_rho, B * nr * w * 2, hipMemcpyHostToDevice))) return rc;
+    if ((rc = md_copy(o.snap_r, s->snap_r, B * nr * nd, hipMemcpyHostToDevice))) return rc;
+    if ((rc = md_copy(o.tr_rho, s->trace_rho, B * q * w * 2, hipMemcpyHostToDevice))) return rc;
+    if ((rc = md_copy(o.tr_time, s->trace_time, B * q, hipMemcpyHostToDevice))) return rc;
+  }
+  R->variant_run = resolve_variant(R);
+  R->dev.prec = resolve_prec(R);
+  if (R->dev.prec != nullptr)
+    hipLaunchKernelGGL(fill_int_kernel, dim3((R->B + 255) / 256), dim3(256), 0, st, R->dev.prec,
+                       R->B, 1);
+  R->slots_ub = R->B;
+  if ((rc = setup_shards(R))) return rc;  // active lists from the uploaded status
+  CCO_HIP(hipStreamSynchronize(st));
+  R->ready = true;
+  R->md_need_init = false;
+  return CCO_OK;
+}
+
+int cco_md_get(cco_relax* R, cco_md_state_io* s) {
+  if (!R || !s || !s->r || !s->v || !s->f || !s->lane_f64 || !s->lane_i64 || !s->key)
+    return fail(CCO_ERR_ARG, "NULL argument");
+  if (R->method != METHOD_MD) return fail(CCO_ERR_STATE, "not an MD session");
+  DevScope ds(R->ctx->device);
+  CCO_HIP(hipStreamSynchronize(R->ctx->stream));
+  const size_t B = R->B, nd = (size_t)R->dev.stride;
+  const MdCfg& c = R->mc;
+  const size_t w = c.w, nr = c.n_rec, q = c.rho_cap;
+  int rc;
+  if ((rc = md_copy(s->r, R->dev.r, B * nd, hipMemcpyDeviceToHost))) return rc;
+  if ((rc = md_copy(s->v, R->dev.v, B * nd, hipMemcpyDeviceToHost))) return rc;
+  if ((rc = md_copy(s->f, R->dev.f, B * nd, hipMemcpyDeviceToHost))) return rc;
+  std::vector<MdLane> L(B);
+  std::vector<int> status(B);
+  if ((rc = md_copy(L.data(), R->dev.ml, B, hipMemcpyDeviceToHost))) return rc;
+  if ((rc = md_copy(status.data(), R->dev.status, B, hipMemcpyDeviceToHost))) return rc;
+  for (size_t b = 0; b < B; ++b) {
+    double* x = s->lane_f64 + b * 10;
+    int64_t* y = s->lane_i64 + b * 5;
+    const MdLane& l = L[b];
+    x[0] = l.phi;
+    x[1] = l.temperature;
+    x[2] = l.time;
+    x[3] = l.reset_acc;
+    x[4] = l.e0;
+    x[5] = l.max_drift;
+    x[6] = l.sum_kinetic;
+    x[7] = l.sum_phi;
+    x[8] = l.sum_phi2;
+    x[9] = l.kinetic;
+    y[0] = l.step;
+    y[1] = l.n_sum;
+    y[2] = status[b];
+    y[3] = l.snap_count;
+    y[4] = l.trace_count;
+    s->key[2 * b] = l.key[0];
+    s->key[2 * b + 1] = l.key[1];
+  }
+  const MdObs& o = R->md_obs;
+  if (w > 0) {
+    if ((rc = md_copy(s->rho_prev, o.rho_prev, B * w * 2, hipMemcpyDeviceToHost))) return rc;
+    if ((rc = md_copy(s->wind_abs, o.wind_abs, B * w, hipMemcpyDeviceToHost))) return rc;
+    if ((rc = md_copy(s->wind_net, o.wind_net, B * w, hipMemcpyDeviceToHost))) return rc;
+    if ((rc = md_copy((long long*)s->n_valid, o.n_valid, B * w, hipMemcpyDeviceToHost))) return rc;
+    if ((rc = md_copy(s->sum_s, o.sum_s, B * w, hipMemcpyDeviceToHost))) return rc;
+    if ((rc = md_copy((long long*)s->snap_step, o.snap_step, B * nr, hipMemcpyDeviceToHost)))
+      return rc;
+    if ((rc = md_copy(s->snap_time, o.snap_time, B * nr, hipMemcpyDeviceToHost))) return rc;
+    if ((rc = md_copy(s->snap_phi, o.snap_phi, B * nr, hipMemcpyDeviceToHost))) return rc;
+    if ((rc = md_copy(s->snap_kinetic, o.snap_kin, B * nr, hipMemcpyDeviceToHost))) return rc;
+    if ((rc = md_copy(s->snap_wind_abs, o.snap_wabs, B * nr * w, hipMemcpyDeviceToHost))) return rc;
+    if ((rc = md_copy(s->snap_wind_net, o.snap_wnet, B * nr * w, hipMemcpyDeviceToHost))) return rc;
+    if ((rc = md_copy(s->snap_rho, o.snap_rho, B * nr * w * 2, hipMemcpyDeviceToHost))) return rc;
+    if ((rc = md_copy(s->snap_r, o.snap_r, B * nr * nd, hipMemcpyDeviceToHost))) return rc;
+    if ((rc = md_copy(s->trace_rho, o.tr_rho, B * q * w * 2, hipMemcpyDeviceToHost))) return rc;
+    if ((rc = md_copy(s->trace_time, o.tr_time, B * q, hipMemcpyDeviceToHost))) return rc;
+  }
+  return CCO_OK;
+}
 
 // ---------------------------------------------------------------------------
 // ensemble analysis

@@ -26,9 +26,72 @@ __device__ __forceinline__ double nanmax_d(double a, double b) {
 }
 
 // ---------------------------------------------------------------------------
-// velocity Verlet
+// counter-based draws (Philox-4x32-10, Salmon et al. SC 2011): the same
+// split / normal / uniform / integer definitions as the host module
+// xla_cco/dynamics/rng.py, so host and device thermostat steps draw alike.
 // ---------------------------------------------------------------------------
-// Half kick and drift: v <- v + dt/2 f, r <- r + dt v, s_eval = r inv(B).
+constexpr unsigned kTagSplit = 0x53504C54u, kTagNormal = 0x4E524D4Cu, kTagUnif = 0x554E4946u;
+
+__device__ inline void philox(unsigned c[4], unsigned k0, unsigned k1) {
+  for (int r = 0; r < 10; ++r) {
+    const unsigned long long p0 = 0xD2511F53ull * c[0];
+    const unsigned long long p1 = 0xCD9E8D57ull * c[2];
+    const unsigned n0 = (unsigned)(p1 >> 32) ^ c[1] ^ k0;
+    const unsigned n1 = (unsigned)p1;
+    const unsigned n2 = (unsigned)(p0 >> 32) ^ c[3] ^ k1;
+    const unsigned n3 = (unsigned)p0;
+    c[0] = n0;
+    c[1] = n1;
+    c[2] = n2;
+    c[3] = n3;
+    k0 += 0x9E3779B9u;
+    k1 += 0xBB67AE85u;
+  }
+}
+
+// keys[j] = words 2j, 2j + 1 of philox((i, 0, 0, TAG_SPLIT), key), num <= 4
+__device__ inline void rng_split(const unsigned key[2], int num, unsigned out[][2]) {
+  unsigned w[8];
+  for (int i = 0; i < (2 * num + 3) / 4; ++i) {
+    unsigned c[4] = {(unsigned)i, 0u, 0u, kTagSplit};
+    philox(c, key[0], key[1]);
+    for (int q = 0; q < 4; ++q) w[4 * i + q] = c[q];
+  }
+  for (int j = 0; j < num; ++j) {
+    out[j][0] = w[2 * j];
+    out[j][1] = w[2 * j + 1];
+  }
+}
+
+__device__ inline double u53(unsigned hi, unsigned lo) {
+  return ((double)(hi >> 5) * 67108864.0 + (double)(lo >> 6)) * (1.0 / 9007199254740992.0);
+}
+
+// Normal draw number e of the stream of `key` (Box-Muller pair e / 2).
+__device__ inline double rng_normal(const unsigned key[2], int e) {
+  unsigned c[4] = {(unsigned)(e >> 1), 0u, 0u, kTagNormal};
+  philox(c, key[0], key[1]);
+  const double u1 = u53(c[0], c[1]);
+  const double u2 = u53(c[2], c[3]);
+  const double rho = sqrt(-2.0 * log1p(-u1));
+  return (e & 1) ? rho * sin(2.0 * M_PI * u2) : rho * cos(2.0 * M_PI * u2);
+}
+
+__device__ inline void rng_unif_words(const unsigned key[2], unsigned c[4]) {
+  c[0] = 0u;
+  c[1] = 0u;
+  c[2] = 0u;
+  c[3] = kTagUnif;
+  philox(c, key[0], key[1]);
+}
+
+// ---------------------------------------------------------------------------
+// molecular dynamics (dynamics/integrators.py:83-176, run.py:383-537)
+// ---------------------------------------------------------------------------
+// verlet / andersen: v <- v + dt/2 f, r <- r + dt v.  baoab: v <- v + dt/2 f,
+// r <- r + dt/2 v, v <- c v + sqrt((1 - c^2) T) xi, r <- r + dt/2 v, with
+// (key', k_xi) = split(key) and xi the normal stream of k_xi in (j, a)
+// order.  Then s_eval = r inv(B).
 __global__ __launch_bounds__(kDynBlock) void md_prepare_kernel(RelaxDev R, MdCfg c) {
   const int slot = blockIdx.x;
   if (slot >= *R.n_active) return;
@@ -39,13 +102,28 @@ __global__ __launch_bounds__(kDynBlock) void md_prepare_kernel(RelaxDev R, MdCfg
   const double* f = R.f + off;
   double* se = R.s_eval + off;
   const int d = c.d;
+  MdLane L = R.ml[lane];
+  unsigned ks[2][2];
+  double amp = 0.0;
+  if (c.scheme == 2) {
+    rng_split(L.key, 2, ks);
+    amp = sqrt(c.ou_var * L.temperature);
+  }
   for (int j = threadIdx.x; j < c.n; j += blockDim.x) {
     double rn[3];
     for (int a = 0; a < d; ++a) {
-      const double vh = v[j * d + a] + c.half * f[j * d + a];
-      v[j * d + a] = vh;
-      rn[a] = r[j * d + a] + c.dt * vh;
-      r[j * d + a] = rn[a];
+      const int e = j * d + a;
+      double vh = v[e] + c.half * f[e];
+      if (c.scheme == 2) {
+        double x = r[e] + c.half * vh;
+        vh = c.ou_c * vh + amp * rng_normal(ks[1], e);
+        x = x + c.half * vh;
+        rn[a] = x;
+      } else {
+        rn[a] = r[e] + c.dt * vh;
+      }
+      v[e] = vh;
+      r[e] = rn[a];
     }
     for (int a = 0; a < d; ++a) {
       double sa = 0.0;
@@ -53,36 +131,72 @@ __global__ __launch_bounds__(kDynBlock) void md_prepare_kernel(RelaxDev R, MdCfg
       se[j * d + a] = sa;
     }
   }
+  if (c.scheme == 2) {
+    __syncthreads();  // every wave has read the old key
+    if (threadIdx.x == 0) {
+      R.ml[lane].key[0] = ks[0][0];
+      R.ml[lane].key[1] = ks[0][1];
+    }
+  }
 }
 
-// Second half kick with the new forces, kinetic energy, drift guard and sums.
-__global__ __launch_bounds__(kDynBlock) void md_finish_kernel(RelaxDev R, MdCfg c) {
+// Second half kick with the new forces; andersen: reset one particle's
+// velocity (count: when the accumulator reaches 1; bernoulli: with
+// probability p) from (key', k_pick, k_draw, k_coin) = split(key, 4); the
+// kinetic energy, the drift guard (verlet), the running sums; then the
+// tracked waves: constrained rho from the force pass, the others summed on
+// cadence steps, COM-corrected unless verlet, winding, S sums, snapshots
+// at the recorded steps and the rho trace.
+__global__ __launch_bounds__(kDynBlock) void md_finish_kernel(RelaxDev R, MdCfg c, MdObs o) {
   __shared__ double red[16];
+  __shared__ double cm[3];
   const int slot = blockIdx.x;
   if (slot >= *R.n_active) return;
   const int lane = R.active[slot];
   const size_t off = (size_t)lane * R.stride;
+  double* r = R.r + off;
   double* v = R.v + off;
   double* f = R.f + off;
   const double* g = R.g_eval + off;
   MdLane L = R.ml[lane];
   const int d = c.d;
   const bool init = L.mode == MODE_INIT;
-  double ke = 0.0;
   for (int j = threadIdx.x; j < c.n; j += blockDim.x) {
     for (int a = 0; a < d; ++a) {
       double acc = 0.0;
       for (int b = 0; b < d; ++b) acc = acc + g[j * d + b] * c.inv_t[b * d + a];
       const double fa = -acc;
       f[j * d + a] = fa;
-      double va = v[j * d + a];
-      if (!init) {
-        va = va + c.half * fa;
-        v[j * d + a] = va;
-      }
-      ke += va * va;
+      if (!init) v[j * d + a] = v[j * d + a] + c.half * fa;
     }
   }
+  if (!init && c.scheme == 1) {
+    unsigned ks[4][2];
+    rng_split(L.key, 4, ks);
+    bool fire;
+    if (c.andersen_mode == 0) {
+      L.reset_acc = L.reset_acc + c.andersen_p;
+      fire = L.reset_acc >= 1.0;
+      if (fire) L.reset_acc = L.reset_acc - 1.0;
+    } else {
+      unsigned w[4];
+      rng_unif_words(ks[3], w);
+      fire = u53(w[0], w[1]) < c.andersen_p;
+    }
+    if (fire) {
+      unsigned w[4];
+      rng_unif_words(ks[1], w);
+      int pick = (int)floor(u53(w[2], w[3]) * (double)c.n);
+      if (pick > c.n - 1) pick = c.n - 1;
+      __syncthreads();  // every half kick is written before the row is replaced
+      if ((int)threadIdx.x < d) v[(size_t)pick * d + threadIdx.x] = sqrt(L.temperature) * rng_normal(ks[2], threadIdx.x);
+    }
+    L.key[0] = ks[0][0];
+    L.key[1] = ks[0][1];
+  }
+  __syncthreads();
+  double ke = 0.0;
+  for (int e = threadIdx.x; e < c.n * d; e += blockDim.x) ke += v[e] * v[e];
   ke = 0.5 * block_sum<kDynBlock>(ke, red);
   const double phi = R.phi_eval[lane];
   int code = ST_RUNNING;
@@ -93,7 +207,7 @@ __global__ __launch_bounds__(kDynBlock) void md_finish_kernel(RelaxDev R, MdCfg 
     L.mode = MODE_EVAL;
   } else {
     const double drift = fabs(phi + ke - L.e0);
-    if (isfinite(c.drift_tol)) {
+    if (c.scheme == 0 && isfinite(c.drift_tol)) {
       const double limit = c.drift_tol * 0.5 * (double)d * (double)(c.n - 1) * L.temperature;
       if (!(drift <= limit)) code = ST_DRIFT;
     }
@@ -107,6 +221,99 @@ __global__ __launch_bounds__(kDynBlock) void md_finish_kernel(RelaxDev R, MdCfg 
     L.sum_phi2 = L.sum_phi2 + phi * phi;
     L.n_sum += 1;
   }
+  if (!init && c.w > 0) {
+    const bool corrected = c.scheme != 0;
+    const bool wind = c.w > c.nc && (c.wind_every == 1 || L.step % c.wind_every == 0);
+    if (corrected) {
+      for (int a = 0; a < d; ++a) {
+        double s = 0.0;
+        for (int j = threadIdx.x; j < c.n; j += blockDim.x) s += r[(size_t)j * d + a];
+        s = block_sum<kDynBlock>(s, red);
+        if (threadIdx.x == 0) cm[a] = s / (double)c.n;
+      }
+      __syncthreads();
+    }
+    double* rp = o.rho_prev + (size_t)lane * c.w * 2;
+    double* wa = o.wind_abs + (size_t)lane * c.w;
+    double* wn = o.wind_net + (size_t)lane * c.w;
+    long long* nv = o.n_valid + (size_t)lane * c.w;
+    double* ss = o.sum_s + (size_t)lane * c.w;
+    for (int i = threadIdx.x; i < c.w; i += blockDim.x) {
+      const double* k = c.kvec + (size_t)i * d;
+      const bool active = i < c.nc || wind;
+      double nre = rp[2 * i], nim = rp[2 * i + 1];
+      if (i < c.nc) {
+        nre = o.re[(size_t)lane * c.m_pad + i];
+        nim = o.im[(size_t)lane * c.m_pad + i];
+      } else if (wind) {
+        nre = 0.0;
+        nim = 0.0;
+        for (int j = 0; j < c.n; ++j) {
+          double th = 0.0;
+          for (int a = 0; a < d; ++a) th = th + k[a] * r[(size_t)j * d + a];
+          double sn, cs;
+          sincos(th, &sn, &cs);
+          nre += cs;
+          nim += sn;
+        }
+      }
+      if (active && corrected) {
+        double ph = 0.0;
+        for (int a = 0; a < d; ++a) ph = ph + k[a] * cm[a];
+        double sn, cs;
+        sincos(ph, &sn, &cs);
+        const double cre = nre * cs + nim * sn;
+        const double cim = nim * cs - nre * sn;
+        nre = cre;
+        nim = cim;
+      }
+      if (active) {
+        const double ore = rp[2 * i], oim = rp[2 * i + 1];
+        const double delta = atan2(nim * ore - nre * oim, nre * ore + nim * oim);
+        const double s_old = (ore * ore + oim * oim) / (double)c.n;
+        const double s_new = (nre * nre + nim * nim) / (double)c.n;
+        if (s_old > c.phase_floor && s_new > c.phase_floor) {
+          wa[i] = wa[i] + fabs(delta);
+          wn[i] = wn[i] + delta;
+          nv[i] += 1;
+        }
+        rp[2 * i] = nre;
+        rp[2 * i + 1] = nim;
+      }
+      ss[i] = ss[i] + (rp[2 * i] * rp[2 * i] + rp[2 * i + 1] * rp[2 * i + 1]) / (double)c.n;
+    }
+    __syncthreads();
+    // snapshot at a recorded step (run.py:477-503): the state after this step
+    int jrec = -1;
+    for (int q = 0; q < c.n_rec; ++q)
+      if (c.rec_steps[q] == L.step) jrec = q;
+    if (jrec >= 0) {
+      const size_t row = (size_t)lane * c.n_rec + jrec;
+      for (int i = threadIdx.x; i < c.w; i += blockDim.x) {
+        o.snap_wabs[row * c.w + i] = wa[i];
+        o.snap_wnet[row * c.w + i] = wn[i];
+        o.snap_rho[(row * c.w + i) * 2] = rp[2 * i];
+        o.snap_rho[(row * c.w + i) * 2 + 1] = rp[2 * i + 1];
+      }
+      for (int e = threadIdx.x; e < c.n * d; e += blockDim.x) o.snap_r[row * (size_t)c.n * d + e] = r[e];
+      if (threadIdx.x == 0) {
+        o.snap_step[row] = L.step;
+        o.snap_time[row] = L.time;
+        o.snap_phi[row] = L.phi;
+        o.snap_kin[row] = ke;
+      }
+      L.snap_count += 1;
+    }
+    if (c.rho_every > 0 && L.step % c.rho_every == 0 && L.trace_count < c.rho_cap) {
+      const size_t row = (size_t)lane * c.rho_cap + L.trace_count;
+      for (int i = threadIdx.x; i < c.w; i += blockDim.x) {
+        o.tr_rho[(row * c.w + i) * 2] = rp[2 * i];
+        o.tr_rho[(row * c.w + i) * 2 + 1] = rp[2 * i + 1];
+      }
+      if (threadIdx.x == 0) o.tr_time[row] = L.time;
+      L.trace_count += 1;
+    }
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
     R.ml[lane] = L;
@@ -119,9 +326,10 @@ void launch_md_prepare(const RelaxDev& r, const MdCfg& c, int slots_ub, hipStrea
   hipLaunchKernelGGL(md_prepare_kernel, dim3(slots_ub), dim3(kDynBlock), 0, st, r, c);
 }
 
-void launch_md_finish(const RelaxDev& r, const MdCfg& c, int slots_ub, hipStream_t st) {
+void launch_md_finish(const RelaxDev& r, const MdCfg& c, const MdObs& o, int slots_ub,
+                      hipStream_t st) {
   if (slots_ub <= 0) return;
-  hipLaunchKernelGGL(md_finish_kernel, dim3(slots_ub), dim3(kDynBlock), 0, st, r, c);
+  hipLaunchKernelGGL(md_finish_kernel, dim3(slots_ub), dim3(kDynBlock), 0, st, r, c, o);
 }
 
 // ---------------------------------------------------------------------------

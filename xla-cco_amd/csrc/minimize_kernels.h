@@ -64,19 +64,53 @@ struct LbfgsLane {
   LsState ls;
 };
 
-// Velocity-Verlet NVE settings (src/xla_cco/dynamics/state.py:57-88, verlet scheme).
+// MD settings (src/xla_cco/dynamics/state.py:57-88).  scheme 0 verlet, 1
+// andersen (verlet then a velocity reset), 2 baoab (Langevin); andersen_mode
+// 0 count, 1 bernoulli.  Waves: w tracked wavevectors kvec [w][d]
+// (Cartesian), the first nc of which are the energy's first k rows (their
+// rho comes from the force pass); the rest are summed every wind_every steps.
 struct MdCfg {
   double dt, half;
   double drift_tol;   // +inf disables the drift guard
   double inv[9];      // inv(B), row-major d x d (s = r inv(B))
   double inv_t[9];    // inv(B)^T (F = -g inv(B)^T)
   int d, n;
+  int scheme, andersen_mode;
+  double andersen_p, ou_c, ou_var;  // BAOAB: c = exp(-gamma dt), 1 - c^2
+  double phase_floor;
+  int w, nc, wind_every, m_pad;     // m_pad: row stride of the force pass's re/im
+  int n_rec, rho_every, rho_cap, pad2;
+  const double* kvec;               // [w][d]
+  const long long* rec_steps;       // [n_rec], strictly increasing
 };
 
 struct MdLane {
   double phi, e0, max_drift, sum_kinetic, sum_phi, sum_phi2, time, temperature, kinetic;
-  long long step, n_sum;
+  double reset_acc;
+  long long step, n_sum, snap_count, trace_count;
+  unsigned int key[2];
   int mode, pad;
+};
+
+// Per-lane observables buffers of an MD session (null w: none).
+struct MdObs {
+  double* rho_prev;   // [B][w][2]
+  double* wind_abs;   // [B][w]
+  double* wind_net;   // [B][w]
+  long long* n_valid; // [B][w]
+  double* sum_s;      // [B][w]
+  double* re;         // [B][m_pad] force-pass densities
+  double* im;
+  long long* snap_step;  // [B][n_rec]
+  double* snap_time;     // [B][n_rec]
+  double* snap_phi;      // [B][n_rec]
+  double* snap_kin;      // [B][n_rec]
+  double* snap_wabs;     // [B][n_rec][w]
+  double* snap_wnet;     // [B][n_rec][w]
+  double* snap_rho;      // [B][n_rec][w][2]
+  double* snap_r;        // [B][n_rec][N d]
+  double* tr_rho;        // [B][rho_cap][w][2]
+  double* tr_time;       // [B][rho_cap]
 };
 
 // Dormand-Prince 5(4) gradient flow settings (src/xla_cco/minimize/flow.py:93-125).
@@ -190,7 +224,8 @@ void launch_lbfgs_gram(const RelaxDev& r, const LbfgsCfg& c, int slots_ub, int p
 int lbfgs_register_ept(int nd);  // elements per thread of the register variant, 0 if too large
 void launch_compact(const RelaxDev& r, hipStream_t st, int lo, int count, unsigned step_tag);
 void launch_md_prepare(const RelaxDev& r, const MdCfg& c, int slots_ub, hipStream_t st);
-void launch_md_finish(const RelaxDev& r, const MdCfg& c, int slots_ub, hipStream_t st);
+void launch_md_finish(const RelaxDev& r, const MdCfg& c, const MdObs& o, int slots_ub,
+                      hipStream_t st);
 void launch_flow_advance(const RelaxDev& r, const FlowCfg& c, int slots_ub, hipStream_t st);
 // ROS2 flow round kernels (dynamics_kernels.hip)
 void launch_ros_gather(const RelaxDev& r, const FlowCfg& c, int slots, double* S, double* dts,

@@ -78,6 +78,50 @@ class LBFGSSettingsC(C.Structure):
     ]
 
 
+class MDSettingsC(C.Structure):
+    _fields_ = [
+        ("andersen_p", C.c_double),
+        ("gamma", C.c_double),
+        ("phase_floor", C.c_double),
+        ("scheme", C.c_int32),
+        ("andersen_mode", C.c_int32),
+        ("wind_every", C.c_int32),
+        ("record_rho_every", C.c_int32),
+        ("max_rho_records", C.c_int32),
+        ("n_record_steps", C.c_int32),
+    ]
+
+
+_PDn = C.POINTER(C.c_double)
+_PI64n = C.POINTER(C.c_int64)
+
+
+class MDStateIOC(C.Structure):
+    _fields_ = [
+        ("r", _PDn),
+        ("v", _PDn),
+        ("f", _PDn),
+        ("lane_f64", _PDn),
+        ("lane_i64", _PI64n),
+        ("key", C.POINTER(C.c_uint32)),
+        ("rho_prev", _PDn),
+        ("wind_abs", _PDn),
+        ("wind_net", _PDn),
+        ("n_valid", _PI64n),
+        ("sum_s", _PDn),
+        ("snap_step", _PI64n),
+        ("snap_time", _PDn),
+        ("snap_phi", _PDn),
+        ("snap_kinetic", _PDn),
+        ("snap_wind_abs", _PDn),
+        ("snap_wind_net", _PDn),
+        ("snap_rho", _PDn),
+        ("snap_r", _PDn),
+        ("trace_rho", _PDn),
+        ("trace_time", _PDn),
+    ]
+
+
 _P = C.c_void_p
 _PD = C.POINTER(C.c_double)
 _PI64 = C.POINTER(C.c_int64)
@@ -153,6 +197,11 @@ SIGNATURES = {
     "cco_md_init": (C.c_int, [_P, _PD, _PD, _PD]),
     "cco_md_run": (C.c_int, [_P, C.c_int64, C.c_int, _PI]),
     "cco_md_state": (C.c_int, [_P, _PD, _PD, _PD, _PD, _PI64]),
+    "cco_md_configure": (
+        C.c_int, [_P, C.POINTER(MDSettingsC), C.c_int, C.c_int, _PD, _PI64]
+    ),
+    "cco_md_put": (C.c_int, [_P, C.POINTER(MDStateIOC)]),
+    "cco_md_get": (C.c_int, [_P, C.POINTER(MDStateIOC)]),
     "cco_sk_moments": (
         C.c_int, [_P, _PD, C.c_int, C.c_int, C.c_int, _PI64, _PI32, C.c_int, _PD, _PD]
     ),
@@ -211,6 +260,11 @@ def dptr(a: np.ndarray) -> "C._Pointer[C.c_double]":
 def i64ptr(a: np.ndarray) -> "C._Pointer[C.c_int64]":
     assert a.dtype == np.int64 and a.flags["C_CONTIGUOUS"]
     return a.ctypes.data_as(_PI64)
+
+
+def u32ptr(a: np.ndarray) -> "C._Pointer[C.c_uint32]":
+    assert a.dtype == np.uint32 and a.flags["C_CONTIGUOUS"]
+    return a.ctypes.data_as(C.POINTER(C.c_uint32))
 
 
 class Context:

@@ -1,29 +1,36 @@
 """Kinetic energy, Maxwell-Boltzmann velocities and harmonic draws (mass 1, k_B T = T).
 
-src/xla_cco/dynamics/thermal.py:47-153.  The reference draws velocities with
-JAX's threefry generator; here they come from a NumPy ``Generator`` (the
-distribution is the same, the streams are not).  ``harmonic_sample`` draws
-with the caller's NumPy generator, as the reference does.
+src/xla_cco/dynamics/thermal.py:47-153.  Velocities come from a Philox key
+(xla_cco.dynamics.rng, the functional key threading of the reference's JAX
+keys; the distribution is the reference's, the stream is not).
+``harmonic_sample`` draws with the caller's NumPy generator, as the
+reference does.
 """
 
 from __future__ import annotations
 
 import math
-from typing import Union
 
 import numpy as np
 
 from xla_cco._types import require_float64
+from xla_cco.dynamics import rng as _rng
+
+require_key = _rng.require_key
 
 
-def maxwell_boltzmann(
-    rng: Union[np.random.Generator, int], n: int, d: int, temperature: float, remove_com: bool
-) -> np.ndarray:
-    """Velocities (n, d) ~ N(0, T) per component; zero total momentum if asked."""
+def maxwell_boltzmann(key: np.ndarray, n: int, d: int, temperature: float, remove_com: bool) -> np.ndarray:
+    """Velocities (n, d) ~ N(0, T) per component; zero total momentum if asked (thermal.py:73-97).
+
+    The components are sqrt(T) normals(key, n d) in row-major order; with
+    ``remove_com`` the mean velocity is subtracted (d (n - 1) degrees of
+    freedom remain).  Raises ValueError unless n, d >= 1 and TypeError on a
+    key that is not uint32 of shape (2,).
+    """
     if n < 1 or d < 1:
         raise ValueError(f"n and d must be positive, got n={n}, d={d}")
-    gen = rng if isinstance(rng, np.random.Generator) else np.random.default_rng(rng)
-    v = np.sqrt(float(temperature)) * gen.standard_normal((n, d))
+    require_key(key)
+    v = math.sqrt(float(temperature)) * _rng.normals(np.asarray(key), n * d).reshape(n, d)
     if remove_com:
         v = v - np.mean(v, axis=0, keepdims=True)
     return v
@@ -36,7 +43,9 @@ def kinetic_energy(v: np.ndarray) -> float:
 
 
 def kinetic_temperature(v: np.ndarray, dof: int) -> float:
-    """2 KE / dof."""
+    """Instantaneous temperature 2 KE / dof (dof = d (n - 1) without COM motion)."""
+    if dof < 1:
+        raise ValueError(f"dof must be positive, got {dof}")
     return 2.0 * kinetic_energy(v) / dof
 
 
