@@ -226,24 +226,20 @@ int cco_flow_result(cco_relax* r, double* s_host, double* energy_host, double* t
 int cco_flow_trajectory(cco_relax* r, int64_t* count_host, double* s_host, double* t_host,
                         double* length_host, double* energy_host);
 
-/* NVE velocity Verlet, mass 1, Cartesian positions r (B x N x d), forces at
- * s = r inv(B): replaces verlet_step (src/xla_cco/dynamics/integrators.py:83-96)
- * with make_force_fn (dynamics/run.py:219-244) and the verlet branch of
- * _step (run.py:415-442): kinetic energy, drift guard |Phi + KE - e0| <=
- * drift_tol * d (N - 1) T / 2 (else status 7, lane frozen; drift_tol = inf
- * disables it), running sums.  cco_md_init evaluates the initial forces and
- * e0 = Phi + KE(v0) (init_state, run.py:300-381). */
+/* A molecular-dynamics session of B lanes of N particles in d dimensions
+ * (mass 1, Cartesian positions r, forces at s = r inv(B), F = -g inv(B)^T:
+ * make_force_fn, src/xla_cco/dynamics/run.py:229-256).  dt is the step,
+ * drift_tol bounds |Phi + KE - e0| by drift_tol d (N - 1) T / 2 for the
+ * verlet scheme (status 7 and the lane stops; inf disables the guard).
+ * cco_md_configure sets the scheme and the tracked waves, cco_md_put
+ * uploads the lanes, cco_md_run advances them, cco_md_get reads them back;
+ * release with cco_relax_destroy. */
 int cco_md_create(cco_ctx* ctx, cco_energy* e, int B, int N, int d, const double* basis_host,
                   double dt, double drift_tol, cco_relax** out);
-int cco_md_init(cco_relax* r, const double* r0_host, const double* v0_host,
-                const double* temperature_host);
-/* n_steps Verlet steps for every running lane (host sync every sync_every). */
+/* n_steps steps of every running lane (replaces the chunk loop of
+ * simulate, run.py:599-654); the host waits for the device every
+ * sync_every steps; n_active = lanes still running. */
 int cco_md_run(cco_relax* r, int64_t n_steps, int sync_every, int* n_active);
-/* r, v, f (B x N x d, NULL to skip); scalars B x 9: phi, kinetic, e0,
- * max_drift, sum_kinetic, sum_phi, sum_phi2, time, temperature; ints B x 3:
- * step, n_sum, status. */
-int cco_md_state(cco_relax* r, double* r_host, double* v_host, double* f_host,
-                 double* scalars_host, int64_t* ints_host);
 
 /* Schemes, thermostats and tracked density waves of an MD session
  * (src/xla_cco/dynamics/state.py:57-88, run.py:157-537, integrators.py:

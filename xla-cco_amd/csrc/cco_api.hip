@@ -380,7 +380,6 @@ struct cco_relax {
   LbfgsCfg lc;
   FlowCfg fc;
   MdCfg mc;
-  bool md_need_init = false;  // the first MD step evaluates r0 without a drift
   double* flow_dt0 = nullptr;  // device copy of per-lane first steps
   RelaxDev dev;
   std::vector<DevBuf> bufs;
@@ -474,17 +473,10 @@ __global__ void relax_init_kernel(RelaxDev r, int method) {
     LbfgsLane L = {};
     L.mode = MODE_INIT;
     r.ll[lane] = L;
-  } else if (method == CCO_METHOD_FLOW) {
+  } else {
     FlowLane L = {};
     L.mode = MODE_INIT;
     r.fl[lane] = L;
-  } else {
-    MdLane L = r.ml[lane];  // temperature set by the host before init
-    const double temp = L.temperature;
-    L = MdLane{};
-    L.temperature = temp;
-    L.mode = MODE_INIT;
-    r.ml[lane] = L;
   }
 }
 
@@ -563,7 +555,7 @@ Shard whole_view(const cco_relax* R) {
 // Stage sequence of one evaluation step of the session's method.
 int step_stages(const cco_relax* R, int* out) {
   int n = 0;
-  if (R->method == METHOD_MD && !R->md_need_init) out[n++] = STG_PREPARE;
+  if (R->method == METHOD_MD) out[n++] = STG_PREPARE;
   out[n++] = STG_RHO;
   out[n++] = STG_REDUCE;
   out[n++] = STG_FORCE;
@@ -1292,7 +1284,6 @@ int cco_relax_step(cco_relax* R, int64_t n_evals, int* n_active) {
         ++n_prof;
       }
       CCO_HIP(hipGetLastError());
-      R->md_need_init = false;
       sh.steps += 1;
       R->launches += 1;
       queued = true;
@@ -2047,47 +2038,6 @@ int cco_md_create(cco_ctx* ctx, cco_energy* E, int B, int N, int d, const double
   return CCO_OK;
 }
 
-int cco_md_init(cco_relax* R, const double* r0, const double* v0, const double* temperature) {
-  if (!R || !r0 || !v0 || !temperature) return fail(CCO_ERR_ARG, "NULL argument");
-  if (R->method != METHOD_MD) return fail(CCO_ERR_STATE, "not an MD session");
-  DevScope ds(R->ctx->device);
-  hipStream_t st = R->ctx->stream;
-  const int d = R->d;
-  const size_t nd = (size_t)R->dev.stride;
-  const size_t B = R->B;
-  // s = r0 inv(B) for the first force evaluation (same sum order as md_prepare_kernel)
-  std::vector<double> s(B * nd);
-  for (size_t b = 0; b < B; ++b)
-    for (int j = 0; j < R->N; ++j)
-      for (int a = 0; a < d; ++a) {
-        double acc = 0.0;
-        for (int q = 0; q < d; ++q) acc = acc + r0[b * nd + (size_t)j * d + q] * R->mc.inv[q * d + a];
-        s[b * nd + (size_t)j * d + a] = acc;
-      }
-  std::vector<MdLane> L(B);
-  for (size_t b = 0; b < B; ++b) {
-    L[b] = MdLane{};
-    L[b].temperature = temperature[b];
-  }
-  CCO_HIP(hipMemcpyAsync(R->dev.r, r0, sizeof(double) * B * nd, hipMemcpyHostToDevice, st));
-  CCO_HIP(hipMemcpyAsync(R->dev.v, v0, sizeof(double) * B * nd, hipMemcpyHostToDevice, st));
-  CCO_HIP(hipMemcpyAsync(R->dev.s_eval, s.data(), sizeof(double) * B * nd, hipMemcpyHostToDevice, st));
-  CCO_HIP(hipMemcpyAsync(R->dev.ml, L.data(), sizeof(MdLane) * B, hipMemcpyHostToDevice, st));
-  R->variant_run = resolve_variant(R);
-  R->dev.prec = resolve_prec(R);
-  hipLaunchKernelGGL(relax_init_kernel, dim3((R->B + 255) / 256), dim3(256), 0, st, R->dev,
-                     R->method);
-  CCO_HIP(hipGetLastError());
-  CCO_HIP(hipStreamSynchronize(st));
-  R->slots_ub = R->B;
-  if (int rc_sh = setup_shards(R)) return rc_sh;
-  R->ready = true;
-  R->md_need_init = true;
-  // the initial force evaluation (no drift): e0 = Phi + KE(v0)
-  int na = 0;
-  return cco_relax_step(R, 1, &na);
-}
-
 int cco_md_run(cco_relax* R, int64_t n_steps, int sync_every, int* n_active) {
   if (!R) return fail(CCO_ERR_ARG, "NULL argument");
   if (R->method != METHOD_MD) return fail(CCO_ERR_STATE, "not an MD session");
@@ -2103,42 +2053,6 @@ int cco_md_run(cco_relax* R, int64_t n_steps, int sync_every, int* n_active) {
   if (n_active) *n_active = na;
   return CCO_OK;
 }
-
-int cco_md_state(cco_relax* R, double* r, double* v, double* f, double* scalars, int64_t* ints) {
-  if (!R) return fail(CCO_ERR_ARG, "NULL argument");
-  if (R->method != METHOD_MD) return fail(CCO_ERR_STATE, "not an MD session");
-  DevScope ds(R->ctx->device);
-  const size_t B = R->B;
-  const size_t bytes = sizeof(double) * B * R->dev.stride;
-  if (r) CCO_HIP(hipMemcpy(r, R->dev.r, bytes, hipMemcpyDeviceToHost));
-  if (v) CCO_HIP(hipMemcpy(v, R->dev.v, bytes, hipMemcpyDeviceToHost));
-  if (f) CCO_HIP(hipMemcpy(f, R->dev.f, bytes, hipMemcpyDeviceToHost));
-  std::vector<MdLane> L(B);
-  std::vector<int> stv(B);
-  CCO_HIP(hipMemcpy(L.data(), R->dev.ml, sizeof(MdLane) * B, hipMemcpyDeviceToHost));
-  CCO_HIP(hipMemcpy(stv.data(), R->dev.status, sizeof(int) * B, hipMemcpyDeviceToHost));
-  for (size_t b = 0; b < B; ++b) {
-    if (scalars) {
-      double* o = scalars + b * 9;
-      o[0] = L[b].phi;
-      o[1] = L[b].kinetic;
-      o[2] = L[b].e0;
-      o[3] = L[b].max_drift;
-      o[4] = L[b].sum_kinetic;
-      o[5] = L[b].sum_phi;
-      o[6] = L[b].sum_phi2;
-      o[7] = L[b].time;
-      o[8] = L[b].temperature;
-    }
-    if (ints) {
-      ints[b * 3 + 0] = L[b].step;
-      ints[b * 3 + 1] = L[b].n_sum;
-      ints[b * 3 + 2] = stv[b];
-    }
-  }
-  return CCO_OK;
-}
-
 
 extern "C++" {
 namespace {
@@ -2314,7 +2228,6 @@ int cco_md_put(cco_relax* R, const cco_md_state_io* s) {
   if ((rc = setup_shards(R))) return rc;  // active lists from the uploaded status
   CCO_HIP(hipStreamSynchronize(st));
   R->ready = true;
-  R->md_need_init = false;
   return CCO_OK;
 }
 

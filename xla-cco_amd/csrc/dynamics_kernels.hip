@@ -1,11 +1,13 @@
-// Device-side batched dynamics: velocity-Verlet NVE steps and the
-// Dormand-Prince 5(4) gradient flow, sharing the energy kernels.
+// Device-side batched dynamics: MD steps (velocity Verlet, Andersen resets,
+// BAOAB Langevin) with the tracked density waves, the Dormand-Prince 5(4)
+// and ROS2 gradient flows, sharing the energy kernels.
 //
 // Restated algorithms:
-//   Verlet  : src/xla_cco/dynamics/integrators.py:83-96 (verlet_step),
-//             src/xla_cco/dynamics/run.py:219-244 (force at s = r inv(B),
-//             F = -g inv(B)^T), :415-442 (_step: KE, drift guard, sums),
-//             src/xla_cco/dynamics/thermal.py:100-106 (KE = sum v^2 / 2)
+//   MD      : src/xla_cco/dynamics/integrators.py:83-176 (verlet_step,
+//             andersen_reset, baoab_step), run.py:229-256 (force at
+//             s = r inv(B), F = -g inv(B)^T), :259-291 (tracked rho),
+//             :415-503 (_step, _record), observables.py:56-125 (densities,
+//             COM correction, winding), thermal.py:100-106 (KE)
 //   flow    : src/xla_cco/minimize/flow.py:212-251 (status, velocity),
 //             :337-364 (scaled error, DOPRI5 proposal), :401-451 (step)
 // One workgroup per active lane; threads own whole particles (d components)
@@ -160,17 +162,16 @@ __global__ __launch_bounds__(kDynBlock) void md_finish_kernel(RelaxDev R, MdCfg 
   const double* g = R.g_eval + off;
   MdLane L = R.ml[lane];
   const int d = c.d;
-  const bool init = L.mode == MODE_INIT;
   for (int j = threadIdx.x; j < c.n; j += blockDim.x) {
     for (int a = 0; a < d; ++a) {
       double acc = 0.0;
       for (int b = 0; b < d; ++b) acc = acc + g[j * d + b] * c.inv_t[b * d + a];
       const double fa = -acc;
       f[j * d + a] = fa;
-      if (!init) v[j * d + a] = v[j * d + a] + c.half * fa;
+      v[j * d + a] = v[j * d + a] + c.half * fa;
     }
   }
-  if (!init && c.scheme == 1) {
+  if (c.scheme == 1) {
     unsigned ks[4][2];
     rng_split(L.key, 4, ks);
     bool fire;
@@ -200,28 +201,21 @@ __global__ __launch_bounds__(kDynBlock) void md_finish_kernel(RelaxDev R, MdCfg 
   ke = 0.5 * block_sum<kDynBlock>(ke, red);
   const double phi = R.phi_eval[lane];
   int code = ST_RUNNING;
-  if (init) {
-    L.phi = phi;
-    L.kinetic = ke;
-    L.e0 = phi + ke;
-    L.mode = MODE_EVAL;
-  } else {
-    const double drift = fabs(phi + ke - L.e0);
-    if (c.scheme == 0 && isfinite(c.drift_tol)) {
-      const double limit = c.drift_tol * 0.5 * (double)d * (double)(c.n - 1) * L.temperature;
-      if (!(drift <= limit)) code = ST_DRIFT;
-    }
-    L.phi = phi;
-    L.kinetic = ke;
-    L.step += 1;
-    L.time = L.time + c.dt;
-    L.max_drift = nanmax_d(L.max_drift, drift);
-    L.sum_kinetic = L.sum_kinetic + ke;
-    L.sum_phi = L.sum_phi + phi;
-    L.sum_phi2 = L.sum_phi2 + phi * phi;
-    L.n_sum += 1;
+  const double drift = fabs(phi + ke - L.e0);
+  if (c.scheme == 0 && isfinite(c.drift_tol)) {
+    const double limit = c.drift_tol * 0.5 * (double)d * (double)(c.n - 1) * L.temperature;
+    if (!(drift <= limit)) code = ST_DRIFT;
   }
-  if (!init && c.w > 0) {
+  L.phi = phi;
+  L.kinetic = ke;
+  L.step += 1;
+  L.time = L.time + c.dt;
+  L.max_drift = nanmax_d(L.max_drift, drift);
+  L.sum_kinetic = L.sum_kinetic + ke;
+  L.sum_phi = L.sum_phi + phi;
+  L.sum_phi2 = L.sum_phi2 + phi * phi;
+  L.n_sum += 1;
+  if (c.w > 0) {
     const bool corrected = c.scheme != 0;
     const bool wind = c.w > c.nc && (c.wind_every == 1 || L.step % c.wind_every == 0);
     if (corrected) {

@@ -194,9 +194,7 @@ SIGNATURES = {
     "cco_md_create": (
         C.c_int, [_P, _P, C.c_int, C.c_int, C.c_int, _PD, C.c_double, C.c_double, C.POINTER(_P)]
     ),
-    "cco_md_init": (C.c_int, [_P, _PD, _PD, _PD]),
     "cco_md_run": (C.c_int, [_P, C.c_int64, C.c_int, _PI]),
-    "cco_md_state": (C.c_int, [_P, _PD, _PD, _PD, _PD, _PI64]),
     "cco_md_configure": (
         C.c_int, [_P, C.POINTER(MDSettingsC), C.c_int, C.c_int, _PD, _PI64]
     ),
