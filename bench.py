@@ -197,7 +197,7 @@ def main() -> int:
         "--shards",
         type=int,
         default=0,
-        help="lane shards, one stream each (0 auto: 2 at this batch size; 1 = a single stream)",
+        help="lane shards, one stream each (0 auto: 3 at this batch size; 1 = a single stream)",
     )
     ap.add_argument(
         "--uneven",

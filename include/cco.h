@@ -159,11 +159,12 @@ int cco_relax_counters(cco_relax* r, int64_t* out4);
  * 1 L-BFGS vectors in global memory, 2 register-resident (N*d <= 8192),
  * 3 Gram form (memory <= 128).  Takes effect at the next cco_relax_init. */
 #define CCO_OPT_ADVANCE_VARIANT 1
-/* CCO_OPT_SHARDS: lanes split into this many shards (0 auto: 2 for
- * built-in-energy descent and L-BFGS sessions of 64 or more lanes, else 1;
- * at most 4), each run on its own stream so that one shard's HBM-bound
- * L-BFGS update overlaps another's energy evaluation.  A lane's results do
- * not depend on the shard count.  Takes effect at the next init. */
+/* CCO_OPT_SHARDS: lanes split into this many shards (0 auto, for
+ * built-in-energy descent and L-BFGS sessions: one shard per 32 lanes, at
+ * most 3; at most 4 when set), each run on its own stream so that one
+ * shard's HBM-bound L-BFGS update overlaps another's energy evaluation.  A
+ * lane's results do not depend on the shard count.  Takes effect at the next
+ * init. */
 #define CCO_OPT_SHARDS 2
 int cco_relax_set_option(cco_relax* r, int key, int64_t value);
 /* The value in force: for CCO_OPT_ADVANCE_VARIANT the variant latched by the

@@ -361,7 +361,7 @@ constexpr int METHOD_MD = CCO_METHOD_VERLET;
 // the vector units.  Every kernel indexes lanes, so a lane's arithmetic does
 // not depend on its shard.
 constexpr int kMaxShards = 4;
-constexpr int kShardMinLanes = 32;  // two shards by default from 2 x this many lanes
+constexpr int kShardMinLanes = 32;  // k shards by default from k x this many lanes (k <= 3)
 
 struct Shard {
   hipStream_t stream = nullptr;
@@ -625,7 +625,7 @@ int alloc_host_words(cco_relax* R) {
 int setup_shards(cco_relax* R) {
   int n = 1;
   if (R->E != nullptr && (R->method == CCO_METHOD_DESCENT || R->method == CCO_METHOD_LBFGS))
-    n = R->shard_req > 0 ? R->shard_req : (R->B >= 2 * kShardMinLanes ? 2 : 1);
+    n = R->shard_req > 0 ? R->shard_req : std::min(3, R->B / kShardMinLanes);
   n = std::max(1, std::min(n, std::min(kMaxShards, R->B)));
   R->shards.assign(n, Shard());
   for (int k = 0; k < n; ++k) {
