@@ -592,77 +592,74 @@ __global__ __launch_bounds__(BLOCK) void lbfgs_advance_kernel(RelaxDev r, LbfgsC
 }
 
 // ---------------------------------------------------------------------------
-// L-BFGS advance with register-resident vectors: one 1024-thread workgroup
-// per lane, element i of every vector owned by thread i % 1024.  The two-loop
-// recursion keeps q in registers, keeps the next 1-3 (s, y) pairs in flight
-// while one is reduced (enough bytes in flight per CU to stream at HBM rate
-// with one lane per CU), and pays one barrier per pair (double-buffered
-// reduction slots).
-// Same arithmetic, in the same order per element, as lbfgs_advance_kernel.
+// L-BFGS advance with register-resident vectors: one workgroup of NT threads
+// per lane, element reg_elem(tid, e) of every vector owned by thread tid.
+// The two-loop recursion keeps q in registers, keeps the next pairs (s, y) in
+// flight while one is reduced (enough bytes in flight per CU to stream at HBM
+// rate with one lane per CU), and pays one barrier per pair (double-buffered
+// reduction slots).  The per-pair chain (dot, wave sum, LDS, barrier, sum of
+// the wave partials) sets a lane's latency: 512 threads with 16-byte history
+// loads (V = 2, even N d up to 2048 in a uniform batch) run it in about 60% of
+// the time of 1024 threads, which serve larger vectors and uneven batches.
+// Same arithmetic per element as lbfgs_advance_kernel; the dot products are
+// summed in another order.
 // ---------------------------------------------------------------------------
-constexpr int kRegBlock = 1024;
-constexpr int kRegWaves = kRegBlock / kWave;
+constexpr int kRegBlock = 1024;  // the largest register-resident workgroup
+constexpr size_t kLdsPerCu = 160 * 1024;  // gfx950
+
+// Element e of thread tid in a register-resident vector of NT threads.  V = 2:
+// each thread owns adjacent element pairs (one 16-byte load per pair).
+template <int NT, int V>
+__device__ __forceinline__ int reg_elem(int tid, int e) {
+  return V == 2 ? (e >> 1) * (2 * NT) + 2 * tid + (e & 1) : e * NT + tid;
+}
 
 struct Red2 {
-  double* buf;  // [2][3 * kRegWaves]
+  double* buf;  // [2][3 * W]
   int parity;
 };
 
-// Sum of the kRegWaves wave partials p[stride * w] (w = lane % 16 of every
-// row) with row shifts: lane 15 ends with the pairwise tree
-// ((p0 + p1) + (p2 + p3)) + ... up to the order of each addition's operands,
-// which does not change a floating-point sum.  Uniform result.
-static_assert(kRegWaves == 16, "the partial sums fill one 16-lane DPP row");
+// Sum of the W wave partials p[stride * w] (w = lane % W of every row) with
+// row shifts: lane W - 1 ends with the pairwise tree ((p0 + p1) + (p2 + p3)) + ...
+// up to the order of each addition's operands, which does not change a
+// floating-point sum.  Uniform result.
+template <int W>
 __device__ __forceinline__ double partials_sum(const double* p, int stride) {
-  double x = p[stride * (threadIdx.x % 16)];
+  static_assert(W == 8 || W == 16, "the wave partials fill one 16-lane DPP row");
+  double x = p[stride * (threadIdx.x % W)];
   x += dpp_d<0x111, 0xf, true>(x);  // row_shr:1
   x += dpp_d<0x112, 0xf, true>(x);  // row_shr:2
   x += dpp_d<0x114, 0xf, true>(x);  // row_shr:4
-  x += dpp_d<0x118, 0xf, true>(x);  // row_shr:8
-  return readlane_d(x, 15);
+  if constexpr (W == 16) x += dpp_d<0x118, 0xf, true>(x);  // row_shr:8
+  return readlane_d(x, W - 1);
 }
 
+template <int W>
 __device__ __forceinline__ double reg_sum(double v, Red2& R) {
   v = wave_sum(v);
-  double* b = R.buf + R.parity * 3 * kRegWaves;
+  double* b = R.buf + R.parity * 3 * W;
   R.parity ^= 1;
   if ((threadIdx.x % kWave) == 0) b[threadIdx.x / kWave] = v;
   __syncthreads();
-  return partials_sum(b, 1);
+  return partials_sum<W>(b, 1);
 }
 
+template <int W>
 __device__ __forceinline__ void reg_sum2(double& v0, double& v1, Red2& R) {
   v0 = wave_sum(v0);
   v1 = wave_sum(v1);
-  double* b = R.buf + R.parity * 3 * kRegWaves;
+  double* b = R.buf + R.parity * 3 * W;
   R.parity ^= 1;
   if ((threadIdx.x % kWave) == 0) {
     b[2 * (threadIdx.x / kWave)] = v0;
     b[2 * (threadIdx.x / kWave) + 1] = v1;
   }
   __syncthreads();
-  v0 = partials_sum(b, 2);
-  v1 = partials_sum(b + 1, 2);
+  v0 = partials_sum<W>(b, 2);
+  v1 = partials_sum<W>(b + 1, 2);
 }
 
-__device__ __forceinline__ void reg_sum3(double& v0, double& v1, double& v2, Red2& R) {
-  v0 = wave_sum(v0);
-  v1 = wave_sum(v1);
-  v2 = wave_sum(v2);
-  double* b = R.buf + R.parity * 3 * kRegWaves;
-  R.parity ^= 1;
-  if ((threadIdx.x % kWave) == 0) {
-    b[3 * (threadIdx.x / kWave)] = v0;
-    b[3 * (threadIdx.x / kWave) + 1] = v1;
-    b[3 * (threadIdx.x / kWave) + 2] = v2;
-  }
-  __syncthreads();
-  v0 = partials_sum(b, 3);
-  v1 = partials_sum(b + 1, 3);
-  v2 = partials_sum(b + 2, 3);
-}
-
-template <int EPT>
+template <int NT, int EPT, int V>
 __device__ void lbfgs_start_reg(const RelaxDev& r, const LbfgsCfg& c, int lane, LbfgsLane& L,
                                 Red2& R, double* alphas, double* rhos) {
   const int nd = rv_nd(r, lane);
@@ -691,7 +688,7 @@ __device__ void lbfgs_start_reg(const RelaxDev& r, const LbfgsCfg& c, int lane, 
     double sy = 0.0, yy = 0.0;
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
-      const int i = tid + e * kRegBlock;
+      const int i = reg_elem<NT, V>(tid, e);
       q[e] = 0.0;
       if (i < nd) {
         const double gi = gx[i];
@@ -704,7 +701,7 @@ __device__ void lbfgs_start_reg(const RelaxDev& r, const LbfgsCfg& c, int lane, 
         q[e] = gi;
       }
     }
-    reg_sum2(sy, yy, R);
+    reg_sum2<NT / kWave>(sy, yy, R);
     rho_new = (sy == 0.0) ? 0.0 : 1.0 / sy;
     if (tid == 0) rho[slot] = rho_new;
     gamma = (yy > 0.0) ? sy / yy : 1.0;
@@ -712,20 +709,20 @@ __device__ void lbfgs_start_reg(const RelaxDev& r, const LbfgsCfg& c, int lane, 
     double gg = 0.0;
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
-      const int i = tid + e * kRegBlock;
+      const int i = reg_elem<NT, V>(tid, e);
       q[e] = 0.0;
       if (i < nd) {
         q[e] = gx[i];
         gg += q[e] * q[e];
       }
     }
-    gg = reg_sum(gg, R);
+    gg = reg_sum<NT / kWave>(gg, R);
     gamma = nanmin(1.0, 1.0 / sqrt(gg));
   }
   const int nvalid = count < m ? count : m;
   const int midx = count % m;
   // rho of every valid pair by age, staged once in LDS (off the critical path)
-  for (int t = m - nvalid + tid; t < m; t += kRegBlock) {
+  for (int t = m - nvalid + tid; t < m; t += NT) {
     const int sl = (midx + t) % m;
     rhos[t] = (sl == slot_new) ? rho_new : rho[sl];
   }
@@ -737,24 +734,37 @@ __device__ void lbfgs_start_reg(const RelaxDev& r, const LbfgsCfg& c, int lane, 
     // and both loops run to a multiple of D, the extra pairs with zero
     // weight (which leaves q unchanged), so the loop body is straight-line
     // and each use waits for its own pair only, not for the pairs in flight.
-    constexpr int D = EPT == 1 ? 6 : (EPT == 2 ? 4 : 2);
+    constexpr int D = NT == 512 ? 4 : (EPT == 1 ? 6 : (EPT == 2 ? 4 : 2));
     int ic[EPT];
     bool mk[EPT];
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
-      const int i = tid + e * kRegBlock;
+      const int i = reg_elem<NT, V>(tid, e);
       mk[e] = i < nd;
-      ic[e] = mk[e] ? i : nd - 1;
+      // V = 2 keeps pairs whole: an even element and its successor (nd is even)
+      ic[e] = mk[e] ? i : (V == 2 ? nd - 2 + (e & 1) : nd - 1);
     }
     double bs[D][EPT], by[D][EPT];
     auto load_pair = [&](double (&ds)[EPT], double (&dy)[EPT], int t) {
       const int sl = (midx + t) % m;
       const double* Si = S + (size_t)sl * nd;
       const double* Yi = Y + (size_t)sl * nd;
+      if constexpr (V == 2) {
 #pragma unroll
-      for (int e = 0; e < EPT; ++e) {
-        ds[e] = Si[ic[e]];
-        dy[e] = Yi[ic[e]];
+        for (int e = 0; e < EPT; e += 2) {
+          const double2 a = *reinterpret_cast<const double2*>(Si + ic[e]);
+          const double2 b = *reinterpret_cast<const double2*>(Yi + ic[e]);
+          ds[e] = a.x;
+          ds[e + 1] = a.y;
+          dy[e] = b.x;
+          dy[e + 1] = b.y;
+        }
+      } else {
+#pragma unroll
+        for (int e = 0; e < EPT; ++e) {
+          ds[e] = Si[ic[e]];
+          dy[e] = Yi[ic[e]];
+        }
       }
     };
     const int npad = ((nvalid + D - 1) / D) * D;
@@ -780,7 +790,7 @@ __device__ void lbfgs_start_reg(const RelaxDev& r, const LbfgsCfg& c, int lane, 
         double dot = 0.0;
 #pragma unroll
         for (int e = 0; e < EPT; ++e) dot += sc[e] * q[e];  // q is zero past the vector's end
-        dot = reg_sum(dot, R);
+        dot = reg_sum<NT / kWave>(dot, R);
         const double a = w * dot;
         if (tid == 0 && k < nvalid) alphas[t] = a;
 #pragma unroll
@@ -809,7 +819,7 @@ __device__ void lbfgs_start_reg(const RelaxDev& r, const LbfgsCfg& c, int lane, 
         double dot = 0.0;
 #pragma unroll
         for (int e = 0; e < EPT; ++e) dot += yc[e] * q[e];
-        dot = reg_sum(dot, R);
+        dot = reg_sum<NT / kWave>(dot, R);
         const double coef = k < nvalid ? alphas[t] - rhos[t] * dot : 0.0;
 #pragma unroll
         for (int e = 0; e < EPT; ++e) q[e] = mk[e] ? q[e] + coef * sc[e] : 0.0;
@@ -825,7 +835,7 @@ __device__ void lbfgs_start_reg(const RelaxDev& r, const LbfgsCfg& c, int lane, 
       const double* Yi = Y + (size_t)sl * nd;
   #pragma unroll
       for (int e = 0; e < EPT; ++e) {
-        const int i = tid + e * kRegBlock;
+        const int i = reg_elem<NT, V>(tid, e);
         ds[e] = i < nd ? Si[i] : 0.0;
         dy[e] = i < nd ? Yi[i] : 0.0;
       }
@@ -844,7 +854,7 @@ __device__ void lbfgs_start_reg(const RelaxDev& r, const LbfgsCfg& c, int lane, 
       double dot = 0.0;
   #pragma unroll
       for (int e = 0; e < EPT; ++e) dot += sc[e] * q[e];
-      dot = reg_sum(dot, R);
+      dot = reg_sum<NT / kWave>(dot, R);
       const double a = rhos[t] * dot;
       if (tid == 0) alphas[t] = a;
   #pragma unroll
@@ -866,7 +876,7 @@ __device__ void lbfgs_start_reg(const RelaxDev& r, const LbfgsCfg& c, int lane, 
       double dot = 0.0;
   #pragma unroll
       for (int e = 0; e < EPT; ++e) dot += yc[e] * q[e];
-      dot = reg_sum(dot, R);
+      dot = reg_sum<NT / kWave>(dot, R);
       const double coef = alphas[t] - rhos[t] * dot;
   #pragma unroll
       for (int e = 0; e < EPT; ++e) q[e] = q[e] + coef * sc[e];
@@ -875,7 +885,7 @@ __device__ void lbfgs_start_reg(const RelaxDev& r, const LbfgsCfg& c, int lane, 
   double slope = 0.0;
 #pragma unroll
   for (int e = 0; e < EPT; ++e) {
-    const int i = tid + e * kRegBlock;
+    const int i = reg_elem<NT, V>(tid, e);
     if (i < nd) {
       const double ui = -q[e];
       q[e] = ui;
@@ -886,7 +896,7 @@ __device__ void lbfgs_start_reg(const RelaxDev& r, const LbfgsCfg& c, int lane, 
       slope += ui * gi;
     }
   }
-  slope = reg_sum(slope, R);
+  slope = reg_sum<NT / kWave>(slope, R);
   L.opt_count = count + 1;
   L.pairs += nvalid;
   LsState& Q = L.ls;
@@ -905,14 +915,15 @@ __device__ void lbfgs_start_reg(const RelaxDev& r, const LbfgsCfg& c, int lane, 
   double* se = r.s_eval + off;
 #pragma unroll
   for (int e = 0; e < EPT; ++e) {
-    const int i = tid + e * kRegBlock;
+    const int i = reg_elem<NT, V>(tid, e);
     if (i < nd) se[i] = (x[i] + a * q[e]) / rs;
   }
 }
 
-template <int EPT>
-__global__ __launch_bounds__(kRegBlock) void lbfgs_advance_reg_kernel(RelaxDev r, LbfgsCfg c) {
-  __shared__ double redbuf[6 * kRegWaves];
+template <int NT, int EPT, int V>
+__global__ __launch_bounds__(NT) void lbfgs_advance_reg_kernel(RelaxDev r, LbfgsCfg c) {
+  constexpr int W = NT / kWave;
+  __shared__ double redbuf[6 * W];
   __shared__ double alphas[1024];
   __shared__ double rhos[1024];
   Red2 R{redbuf, 0};
@@ -937,7 +948,7 @@ __global__ __launch_bounds__(kRegBlock) void lbfgs_advance_reg_kernel(RelaxDev r
     const double r0 = c.rescales[0];
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
-      const int i = tid + e * kRegBlock;
+      const int i = reg_elem<NT, V>(tid, e);
       if (i < nd) {
         x[i] = s[i] * r0;
         gx[i] = ge[i] / r0;
@@ -958,7 +969,7 @@ __global__ __launch_bounds__(kRegBlock) void lbfgs_advance_reg_kernel(RelaxDev r
       const size_t trow = rv_trow(r, lane, c.record_cap, 0);
 #pragma unroll
       for (int e = 0; e < EPT; ++e) {
-        const int i = tid + e * kRegBlock;
+        const int i = reg_elem<NT, V>(tid, e);
         if (i < nd) r.tr_s[trow + i] = wrap01(s[i]);
       }
       if (tid == 0) {
@@ -975,7 +986,7 @@ __global__ __launch_bounds__(kRegBlock) void lbfgs_advance_reg_kernel(RelaxDev r
     double sa = 0.0;
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
-      const int i = tid + e * kRegBlock;
+      const int i = reg_elem<NT, V>(tid, e);
       gt[e] = 0.0;
       uu[e] = 0.0;
       if (i < nd) {
@@ -984,12 +995,12 @@ __global__ __launch_bounds__(kRegBlock) void lbfgs_advance_reg_kernel(RelaxDev r
         sa += gt[e] * uu[e];
       }
     }
-    sa = reg_sum(sa, R);
+    sa = reg_sum<NT / kWave>(sa, R);
     const int flags = ls_consume(L.ls, fe, sa, c.max_linesearch_steps);
     if (flags & 1) {
 #pragma unroll
       for (int e = 0; e < EPT; ++e) {
-        const int i = tid + e * kRegBlock;
+        const int i = reg_elem<NT, V>(tid, e);
         if (i < nd) sg[i] = gt[e];
       }
     }
@@ -1005,7 +1016,7 @@ __global__ __launch_bounds__(kRegBlock) void lbfgs_advance_reg_kernel(RelaxDev r
       double gv[EPT];
 #pragma unroll
       for (int e = 0; e < EPT; ++e) {
-        const int i = tid + e * kRegBlock;
+        const int i = reg_elem<NT, V>(tid, e);
         gv[e] = 0.0;
         if (i < nd) {
           if (!fail) {
@@ -1023,7 +1034,7 @@ __global__ __launch_bounds__(kRegBlock) void lbfgs_advance_reg_kernel(RelaxDev r
       if (!fail) L.f = value;
       L.stag = (L.f < f_old) ? 0 : L.stag + 1;
       L.pass_evals += evals;
-      gg = reg_sum(gg, R);
+      gg = reg_sum<NT / kWave>(gg, R);
       const double gnorm = sqrt(gg) * rs;
       int glob = (gnorm < c.grad_tol) ? ST_GRAD_TOL : ST_RUNNING;
       if (L.f < c.energy_tol) glob = ST_ENERGY_TOL;
@@ -1039,7 +1050,7 @@ __global__ __launch_bounds__(kRegBlock) void lbfgs_advance_reg_kernel(RelaxDev r
         const double rn = c.rescales[L.pass_idx + 1];
 #pragma unroll
         for (int e = 0; e < EPT; ++e) {
-          const int i = tid + e * kRegBlock;
+          const int i = reg_elem<NT, V>(tid, e);
           if (i < nd) {
             x[i] = s[i] * rn;
             gx[i] = gv[e] * rs / rn;
@@ -1059,7 +1070,7 @@ __global__ __launch_bounds__(kRegBlock) void lbfgs_advance_reg_kernel(RelaxDev r
         const size_t trow = rv_trow(r, lane, c.record_cap, L.rec_count);
 #pragma unroll
         for (int e = 0; e < EPT; ++e) {
-          const int i = tid + e * kRegBlock;
+          const int i = reg_elem<NT, V>(tid, e);
           if (i < nd) r.tr_s[trow + i] = wrap01(s[i]);
         }
         if (tid == 0) {
@@ -1075,12 +1086,12 @@ __global__ __launch_bounds__(kRegBlock) void lbfgs_advance_reg_kernel(RelaxDev r
       double* se = r.s_eval + off;
 #pragma unroll
       for (int e = 0; e < EPT; ++e) {
-        const int i = tid + e * kRegBlock;
+        const int i = reg_elem<NT, V>(tid, e);
         if (i < nd) se[i] = (x[i] + a * uu[e]) / rs;
       }
     }
   }
-  if (start) lbfgs_start_reg<EPT>(r, c, lane, L, R, alphas, rhos);
+  if (start) lbfgs_start_reg<NT, EPT, V>(r, c, lane, L, R, alphas, rhos);
   if (tid == 0) {
     r.ll[lane] = L;
     r.status[lane] = code;
@@ -1109,18 +1120,37 @@ void launch_lbfgs_advance(const RelaxDev& r, const LbfgsCfg& c, int slots_ub, hi
                        st, r, c);
     return;
   }
+  // 512 threads and 16-byte history loads: uniform batches of even N d <= 2048
+  // (every lane's vectors and history rows then start 16-byte aligned).  One
+  // such workgroup per CU: without the LDS reservation the dispatcher packs
+  // two or three lanes onto a CU while the other shards' kernels hold the
+  // rest, and lanes that share a CU share its memory pipeline (measured 25%
+  // slower per launch); more than half of the CU's LDS keeps a second one out.
+  if (r.loff == nullptr && r.nd % 2 == 0 && r.nd <= 2048) {
+    constexpr size_t kStatic = sizeof(double) * (6 * (512 / kWave) + 2 * 1024);
+    const size_t pad = kLdsPerCu / 2 + 1024 - kStatic;
+    if (r.nd <= 1024)
+      hipLaunchKernelGGL((lbfgs_advance_reg_kernel<512, 2, 2>), dim3(slots_ub), dim3(512), pad, st, r, c);
+    else
+      hipLaunchKernelGGL((lbfgs_advance_reg_kernel<512, 4, 2>), dim3(slots_ub), dim3(512), pad, st, r, c);
+    return;
+  }
   switch (ept) {
     case 1:
-      hipLaunchKernelGGL(lbfgs_advance_reg_kernel<1>, dim3(slots_ub), dim3(kRegBlock), 0, st, r, c);
+      hipLaunchKernelGGL((lbfgs_advance_reg_kernel<kRegBlock, 1, 1>), dim3(slots_ub), dim3(kRegBlock),
+                         0, st, r, c);
       break;
     case 2:
-      hipLaunchKernelGGL(lbfgs_advance_reg_kernel<2>, dim3(slots_ub), dim3(kRegBlock), 0, st, r, c);
+      hipLaunchKernelGGL((lbfgs_advance_reg_kernel<kRegBlock, 2, 1>), dim3(slots_ub), dim3(kRegBlock),
+                         0, st, r, c);
       break;
     case 4:
-      hipLaunchKernelGGL(lbfgs_advance_reg_kernel<4>, dim3(slots_ub), dim3(kRegBlock), 0, st, r, c);
+      hipLaunchKernelGGL((lbfgs_advance_reg_kernel<kRegBlock, 4, 1>), dim3(slots_ub), dim3(kRegBlock),
+                         0, st, r, c);
       break;
     default:
-      hipLaunchKernelGGL(lbfgs_advance_reg_kernel<8>, dim3(slots_ub), dim3(kRegBlock), 0, st, r, c);
+      hipLaunchKernelGGL((lbfgs_advance_reg_kernel<kRegBlock, 8, 1>), dim3(slots_ub), dim3(kRegBlock),
+                         0, st, r, c);
       break;
   }
 }
