@@ -69,12 +69,19 @@ def cpu_baseline(ks, volume: float, d: int, n: int, evals_per_converged: float, 
     wall = time.perf_counter() - t0
     evals = float(ne.sum() + 0 * threads)  # n_evals counts the initial evaluation as well
     eps = evals / wall
+    model = "unknown"
+    try:
+        with open("/proc/cpuinfo") as fh:
+            model = next((ln.split(":", 1)[1].strip() for ln in fh if ln.startswith("model name")), model)
+    except OSError:  # pragma: no cover
+        pass
     return {
         "value": eps / evals_per_converged,
         "unit": "configs/s",
         "cores": threads,
         "kind": "port",
         "evals_per_s": eps,
+        "host": {"cpu_model": model, "nproc": os.cpu_count(), "affinity_cpus": len(os.sched_getaffinity(0))},
         "sample": (
             f"C oracle (oracle/cco_oracle.c, trig per (k,j), same L-BFGS) on {threads} configurations "
             f"of the workload, one per OpenMP thread, capped at {cap} energy+force evaluations each "
@@ -205,6 +212,8 @@ def main() -> int:
         help="time the uneven batch instead: 512 configurations, N ~ U{100..2000}, "
         "d ~ U{2,3}, chi ~ U{0.1,0.2,0.3,0.45}, one session",
     )
+    ap.add_argument("--no-mixed", action="store_true",
+                    help="skip the split-precision relax reported beside the FP64 value")
     ap.add_argument(
         "--calibrate-copy",
         type=int,
@@ -372,6 +381,29 @@ def main() -> int:
     if roofline is not None and shards_run > 1:
         _, roofline_one = roofline_of(timed(1, args.warmup, 1))
 
+    # the metric's comparison: the same relax with split-precision evaluations
+    # (lanes turn FP64 at E <= switch), one warm and one timed relax of the first
+    # timed input, in a session of its own
+    mixed = None
+    if args.precision == "fp64" and not args.no_mixed:
+        ms_ = RelaxSession(energy, B, n, d, LBFGSSettings(energy_tol=tol, precision="mixed",
+                                                           precision_switch=args.switch))
+        _native.check(L.cco_relax_set_option(ms_.handle, 2, args.shards))
+        for _ in range(2):
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            _native.check(L.cco_relax_init_dev(ms_.handle, inputs[args.warmup].ptr))
+            _native.check(L.cco_relax_run(ms_.handle, bound, 256))
+            _native.check(L.cco_relax_result_dev(ms_.handle, out_s.ptr, out_e.ptr, out_g.ptr, out_i[0].ptr,
+                                                 out_i[1].ptr, out_i[2].ptr))
+            ctx.synchronize()
+            t_mixed = time.perf_counter() - t0
+        conv_m = int(np.sum(out_e.to_host() < tol))
+        mixed = {"value": conv_m / t_mixed, "unit": "configs/s", "converged": f"{conv_m}/{B}",
+                 "vs_fp64": (conv_m / t_mixed) / value, "switch": args.switch,
+                 "note": "split-precision rho and forces (fp16 hi+lo MFMA) until E <= switch, then FP64"}
+        ms_.close()
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(ks, box.volume, d, n, evals_per_conv, args.cpu_seconds)
@@ -409,6 +441,7 @@ def main() -> int:
         "kernel_avg_ms": stage_avg,
         "roofline": roofline,
         "roofline_one_stream": roofline_one,
+        "mixed_vs_fp64": mixed,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))
