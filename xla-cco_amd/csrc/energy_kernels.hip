@@ -94,12 +94,27 @@ __device__ __forceinline__ void rho_unit_rows(double2 (&acc)[kRowSeg], const dou
     }
     acc[0].x += e0.x + e1.x;
     acc[0].y += e0.y + e1.y;
-#pragma unroll
-    for (int r = 1; r < NR; ++r) {
+    if constexpr (NR >= 2) {
+      // row 1 by a complex multiply, rows 2.. by the three-term recurrence
+      // e_{r+1} = 2 Re(z) e_r - e_{r-1} (z^{r+1} + z^{r-1} = 2 Re(z) z^r for
+      // |z| = 1): two FMAs per particle and row instead of four operations
+      double2 p0 = e0, p1 = e1;
       e0 = cmul(e0, y0);
       e1 = cmul(e1, y1);
-      acc[r].x += e0.x + e1.x;
-      acc[r].y += e0.y + e1.y;
+      acc[1].x += e0.x + e1.x;
+      acc[1].y += e0.y + e1.y;
+      const double w0 = 2.0 * y0.x, w1 = 2.0 * y1.x;
+#pragma unroll
+      for (int r = 2; r < NR; ++r) {
+        const double2 q0 = make_double2(fma(w0, e0.x, -p0.x), fma(w0, e0.y, -p0.y));
+        const double2 q1 = make_double2(fma(w1, e1.x, -p1.x), fma(w1, e1.y, -p1.y));
+        p0 = e0;
+        p1 = e1;
+        e0 = q0;
+        e1 = q1;
+        acc[r].x += e0.x + e1.x;
+        acc[r].y += e0.y + e1.y;
+      }
     }
     a0 = na0;
     a1 = na1;
@@ -339,17 +354,29 @@ __device__ __forceinline__ void force_member(const int4 m, const double2 B, cons
                                              const double2* __restrict__ C, double& g1,
                                              double& g2, double& g3) {
   if (m.y == 0) return;
+  // rows advance by the three-term recurrence e_{r+1} = 2 Re(z) e_r - e_{r-1}
+  // (|z| = 1): two FMAs per row instead of a complex multiply.  The pair
+  // (e_{r-1}, e_r) restarts from e_r conj(z) every 8 rows, so rounding errors
+  // grow linearly along a long column (as for a chain of complex multiplies),
+  // not quadratically as an unbroken recurrence near z = +-1 would let them.
   double2 e = B;
+  const double w = 2.0 * z1.x;
   double P = 0.0;
   double Q = 0.0;
   const double2* Cm = C + m.x;
+  for (int r0 = 0; r0 < m.y; r0 += 8) {
+    double2 ep = cmul(e, cconj(z1));
+    const int nr = min(8, m.y - r0);
 #pragma unroll 4
-  for (int r = 0; r < m.y; ++r) {
-    const double2 c = Cm[r];
-    P = fma(c.y, e.x, P);  // P += Im(C conj e), folded into two FMAs
-    P = fma(-c.x, e.y, P);
-    Q += P;
-    e = cmul(e, z1);
+    for (int r = 0; r < nr; ++r) {
+      const double2 c = Cm[r0 + r];
+      P = fma(c.y, e.x, P);  // P += Im(C conj e), folded into two FMAs
+      P = fma(-c.x, e.y, P);
+      Q += P;
+      const double2 en = make_double2(fma(w, e.x, -ep.x), fma(w, e.y, -ep.y));
+      ep = e;
+      e = en;
+    }
   }
   g1 += (double)m.y * P - Q;
   if (D >= 2) g2 += (double)m.z * P;
