@@ -172,6 +172,71 @@ def bench_uneven(args) -> int:
     return 0
 
 
+def bench_md(args) -> int:
+    """NVE MD steps of 32 lanes at d=3, N=10000, chi=0.45 (the large MD shape): lane-steps/s."""
+    from xla_cco import _native, kvectors as K
+    from xla_cco.box import cubic_box
+    from xla_cco.dynamics import MDSettings, init_state, make_waves, prng_key
+    from xla_cco.dynamics.run import MDSession
+    from xla_cco.energy import make_stealthy_energy
+
+    ctx = _native.context(0)
+    d, n, chi, B = 3, 10000, 0.45, 32
+    box = cubic_box(n, d)
+    k_c = K.k_outer_from_ka(K.ka_from_chi(chi, 0.05, d), 0.05, d)
+    ks = K.enumerate_kvectors(box.basis, 0.0, k_c)
+    M = int(ks.n.shape[0])
+    energy = make_stealthy_energy(box, ks)
+    waves = make_waves(box.basis, k_c, k_c)  # the constrained waves: rho from the force pass
+    rng = np.random.default_rng(31)
+    r0 = rng.uniform(size=(B, n, d)) @ box.basis
+    v0 = rng.normal(scale=1e-2, size=(B, n, d))  # T = 1e-4 per component
+    keys = np.stack([prng_key(i) for i in range(B)])
+    out = {}
+    for prec in ("fp64", "mixed"):
+        st = MDSettings(dt=0.01, remove_com=False, chunk_steps=10_000, drift_tol=float("inf"), precision=prec)
+        lanes = init_state(energy, waves, box.basis, r0, v0, 1e-4, keys, st)
+        # one device session holds the lanes; the timed region is the device steps only
+        dev = MDSession(energy, waves, box.basis, st, B, n, d)
+        dev.put(lanes)
+        dev.run(args.warmup)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        dev.run(args.steps)
+        ctx.synchronize()
+        out[prec] = (time.perf_counter() - t0, dev.get(lanes))
+        dev.close()
+    wall, res = out["fp64"]
+    wall_m, res_m = out["mixed"]
+    lane_steps = B * args.steps
+    flop = 8.0 * M * n * (1 + d) * lane_steps
+    e0 = res.phi + 0.5 * np.sum(res.v * res.v, axis=(1, 2))
+    e0m = res_m.phi + 0.5 * np.sum(res_m.v * res_m.v, axis=(1, 2))
+    line = {
+        "metric": "MD lane-steps/s (one energy+force evaluation per lane-step)",
+        "value": lane_steps / wall,
+        "unit": "lane-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"NVE velocity Verlet, {B} lanes, d={d} N={n} chi={chi} (M={M}), dt=0.01, "
+                               "uniform random positions, v ~ N(0, 1e-4)", "lanes": B, "M": M},
+        "flop_rate_tflops": flop / wall / 1e12,
+        "frac_of_fp64_peak": flop / wall / 1e12 / PEAK_FP64_TFLOPS,
+        "mixed": {"value": lane_steps / wall_m, "vs_fp64": wall / wall_m,
+                  "energy_diff_rel": float(np.max(np.abs(e0m - e0) / np.abs(e0)))},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line))
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -212,6 +277,8 @@ def main() -> int:
         help="time the uneven batch instead: 512 configurations, N ~ U{100..2000}, "
         "d ~ U{2,3}, chi ~ U{0.1,0.2,0.3,0.45}, one session",
     )
+    ap.add_argument("--md", action="store_true",
+                    help="time NVE MD at d=3 N=10000 chi=0.45 with 32 lanes instead (steps = MD steps)")
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the split-precision relax reported beside the FP64 value")
     ap.add_argument(
@@ -235,6 +302,8 @@ def main() -> int:
 
     if args.uneven:
         return bench_uneven(args)
+    if args.md:
+        return bench_md(args)
 
     L = _native.lib()
     ctx = _native.context(0)

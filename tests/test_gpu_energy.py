@@ -60,10 +60,11 @@ def test_value_grad_density_matches_oracle(gpu_ctx, n: int, d: int, k_outer: flo
         _check(e, ks, box.volume, s)
 
 
-@pytest.mark.parametrize("d,n,chi", [(2, 1000, 0.45), (3, 400, 0.3), (2, 1500, 0.2), (1, 800, 0.45)])
+@pytest.mark.parametrize("d,n,chi", [(2, 1000, 0.45), (3, 400, 0.3), (2, 1500, 0.2), (1, 800, 0.45), (2, 3000, 0.1)])
 def test_bench_shapes_match_oracle(gpu_ctx, d: int, n: int, chi: float) -> None:
     """Shapes with several rho splits, column tiles and row segments; d = 1 puts all
-    M = 359 rows in one column (the longest phase recurrences)."""
+    M = 359 rows in one column (the longest phase recurrences); N = 3000 takes
+    256-particle splits."""
     nv, w, vol, basis = chi_kset(n, d, chi)
     box = cubic_box(n, d)
     ks = K.enumerate_kvectors(box.basis, 0.0, K.k_outer_from_ka(K.ka_from_chi(chi, 0.05, d), 0.05, d))

@@ -387,3 +387,19 @@ def test_relax_shards_bitwise(gpu_ctx, method: str) -> None:
     _native.check(L.cco_relax_get_option(sess.handle, 2, C.byref(v)))
     assert int(v.value) == 2
     sess.close()
+
+
+def test_relax_long_splits_initial_energy(gpu_ctx) -> None:
+    """Sessions above N = 2048 use longer rho splits: the recorded energy at s0 matches the oracle."""
+    from xla_cco.minimize import RelaxSession
+
+    e, vg, _ = _poisson(2600, 2, 0.05)
+    s0 = np.random.default_rng(21).uniform(size=(2, 2600, 2))
+    sess = RelaxSession(e, 2, 2600, 2, LBFGSSettings(max_evals_per_pass=3, record_every=1, max_records=3))
+    sess.init(s0)
+    sess.run(100, 64)
+    out = sess.result()
+    sess.close()
+    for i in range(2):
+        want, _ = vg(s0[i])
+        assert out.trajectory.energy[i, 0] == pytest.approx(want, rel=1e-12)

@@ -80,9 +80,20 @@ struct cco_energy {
 
 namespace {
 
-constexpr int kSplitLen = 128;  // particles per rho split (fixed per lane shape)
+constexpr int kSplitLen = 128;  // particles per rho split up to N = 2048 and in uneven batches
 
-int n_splits_for(int N) { return (N + kSplitLen - 1) / kSplitLen; }
+// Particles per rho split of a uniform batch: 128 up to N = 2048, then longer
+// (a multiple of 128) so that a lane has at most 16 splits.  The partial sums
+// (splits x box entries per lane) are written by the rho kernel and read back
+// by the reduce kernel: 128-particle splits made that 0.8 GB per launch at 32
+// lanes of N = 10000 (d = 3, M = 13540).
+int split_len_for(int N) {
+  constexpr int kMaxSplits = 16;
+  if (N <= kMaxSplits * kSplitLen) return kSplitLen;
+  return ((N + kMaxSplits * kSplitLen - 1) / (kMaxSplits * kSplitLen)) * kSplitLen;
+}
+int n_splits_for(int N) { return (N + split_len_for(N) - 1) / split_len_for(N); }
+int n_splits_uneven(int N) { return (N + kSplitLen - 1) / kSplitLen; }
 int n_ftiles_for(int N) { return (N + kForceTile - 1) / kForceTile; }
 
 // Energy partials of the tiled reduce.
@@ -318,7 +329,7 @@ int energy_batch(cco_energy* E, const double* s_dev, int B, int N, double* phi_d
   b.N = N;
   b.lane_stride = N * g.d;
   b.n_splits = nsp;
-  b.split_len = kSplitLen;
+  b.split_len = split_len_for(N);
   b.n_ftiles = n_ftiles_for(N);
   b.s = s_dev;
   b.grad = grad_dev;
@@ -487,7 +498,7 @@ Batch relax_batch(cco_relax* R, const RelaxDev& dv) {
   b.N = R->N;
   b.lane_stride = R->dev.stride;
   b.n_splits = n_splits_for(R->N);
-  b.split_len = kSplitLen;
+  b.split_len = R->uneven ? kSplitLen : split_len_for(R->N);
   b.n_ftiles = n_ftiles_for(R->N);
   b.s = R->dev.s_eval;
   b.grad = R->dev.g_eval;
@@ -1050,7 +1061,7 @@ static int relax_create_impl(cco_ctx* ctx, cco_energy* E, int B, int N, int d, i
       g.n_utiles = (g.n_units + kWave - 1) / kWave;
       g.rho_halves = 2;
       const int nd = un->N[i] * g.d;
-      const int ns = n_splits_for(un->N[i]);
+      const int ns = n_splits_uneven(un->N[i]);
       h_loff[i] = vt;
       h_lhoff[i] = ht;
       h_lnd[i] = nd;
@@ -1474,7 +1485,7 @@ int cco_energy_profile(cco_energy* E, const double* s_dev, int B, int N, int rep
   b.N = N;
   b.lane_stride = N * g.d;
   b.n_splits = n_splits_for(N);
-  b.split_len = kSplitLen;
+  b.split_len = split_len_for(N);
   b.n_ftiles = n_ftiles_for(N);
   b.s = s_dev;
   b.grad = (double*)E->grad.p;

@@ -286,8 +286,13 @@ def begin_segment(state: MDState, settings: MDSettings, reset_sums: bool, reset_
 
 
 # ----- the device session -----
-class _MDDevice:
-    """One cco_md session holding a batch of lanes, its waves and its settings."""
+class MDSession:
+    """One cco_md device session: a batch of lanes with its waves and settings.
+
+    ``put(state)`` uploads a batched MDState, ``run(n)`` advances every running
+    lane by n steps on the device, ``get(like)`` reads the state back (``like``
+    gives the host arrays' shapes); ``simulate`` drives one per call.
+    """
 
     def __init__(self, energy: StealthyEnergy, waves: Waves, basis: np.ndarray, settings: MDSettings,
                  b: int, n: int, d: int) -> None:
@@ -441,14 +446,14 @@ def simulate(
         raise ValueError(f"need n_steps >= 0 and checkpoint_every_s > 0, got {n_steps}, {checkpoint_every_s}")
     n_steps = int(n_steps)
     done, last = 0, time.monotonic()
-    dev: Optional[_MDDevice] = None
+    dev: Optional[MDSession] = None
     try:
         while done < n_steps and np.any(state.status == STATUS_RUNNING):
             n_this = min(settings.chunk_steps, n_steps - done)
             if dev is None:
                 state = _repair_pending(energy, waves, basis, settings, state)
                 b, n, d = state.r.shape
-                dev = _MDDevice(energy, waves, basis, settings, b, n, d)
+                dev = MDSession(energy, waves, basis, settings, b, n, d)
                 dev.put(state)
             n_active = dev.run(n_this)
             done += n_this
