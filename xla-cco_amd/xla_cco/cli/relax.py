@@ -110,7 +110,7 @@ def validate_args(args: argparse.Namespace) -> None:
     if (args.record_every > 0) != (args.max_records > 0):
         raise ValueError("--record-every and --max-records must both be positive")
     if args.device != "gpu":
-        raise ValueError("--device cpu: this build computes on the GPU only")
+        raise ValueError("--device cpu: the relax runs on the GPU only")
 
 
 def load_input(prefix: Path, dim: int, input_basis: str = "auto") -> Tuple[List[PackedConfig], np.ndarray, np.ndarray]:

@@ -2,7 +2,7 @@
 
 The reference threads JAX threefry keys through its thermostats and
 Maxwell-Boltzmann draws (dynamics/thermal.py:73-97, integrators.py:99-176);
-those streams cannot be reproduced without JAX.  This build uses
+those streams cannot be reproduced without JAX.  This module uses
 Philox-4x32-10 (Salmon, Moraes, Dror and Shaw, "Parallel random numbers: as
 easy as 1, 2, 3", SC 2011) with the same functional key threading: a key is
 two uint32 words, ``split`` derives new keys from it, and draws read a key
