@@ -55,12 +55,18 @@ struct MpTables {
   _Float16* z1p;
   float2* t2;
   float2* t3;
+  double2* zu;  // [3][kMpChunk] the chunk's unit phases per axis (FP64)
   int w2p, w3p;
 };
 
+__host__ __device__ inline size_t mp_zu_offset(int RP, int P2, int P3) {
+  const size_t end = (size_t)4 * RP * kZ1Pitch * sizeof(_Float16) +
+                     (size_t)kMpChunk * (mp_pad(P2 + 1) + mp_pad(P3 + 1)) * sizeof(float2);
+  return (end + 15) & ~(size_t)15;
+}
+
 __host__ __device__ inline size_t mp_table_bytes(int RP, int P2, int P3) {
-  return (size_t)4 * RP * kZ1Pitch * sizeof(_Float16) +
-         (size_t)kMpChunk * (mp_pad(P2 + 1) + mp_pad(P3 + 1)) * sizeof(float2);
+  return mp_zu_offset(RP, P2, P3) + (size_t)3 * kMpChunk * sizeof(double2);
 }
 
 __device__ __forceinline__ MpTables mp_tables(char* smem, const Geo& g) {
@@ -70,6 +76,7 @@ __device__ __forceinline__ MpTables mp_tables(char* smem, const Geo& g) {
   T.w2p = mp_pad(g.P2 + 1);
   T.w3p = mp_pad(g.P3 + 1);
   T.t3 = T.t2 + (size_t)kMpChunk * T.w2p;
+  T.zu = (double2*)(smem + mp_zu_offset(g.RP, g.P2, g.P3));
   return T;
 }
 
@@ -82,24 +89,54 @@ __device__ __forceinline__ float2 cmulf(float2 a, float2 b) {
   return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
-// Tables of particles [jc, jc + nj) (called by every thread of the block).
+// Tables of particles [jc, jc + nj), called by every thread of the block:
+// the unit phases z_a = exp(2 pi i s_a) once per particle and axis, then the
+// power tables in blocks of 8 powers spread over all threads, each block
+// seeded by z^{8q} (repeated squaring) and continued by recurrence, so no
+// thread runs a serial chain longer than 8 + q steps.
 template <int D>
 __device__ void mp_fill(const MpTables& T, const double* s, int jc, int nj, int RP, int P2,
                         int P3) {
-  const double tscale = D == 2 ? 4096.0 : 64.0;
   for (int idx = threadIdx.x; idx < kMpChunk * D; idx += blockDim.x) {
     const int jj = idx % kMpChunk;
     const int ax = idx / kMpChunk;
-    const bool ok = jj < nj;
     double2 z = make_double2(0.0, 0.0);
-    if (ok) {
+    if (jj < nj) {
       double sn, cs;
       sincospi(2.0 * s[(size_t)(jc + jj) * D + ax], &sn, &cs);
       z = make_double2(cs, sn);
     }
-    double2 p = make_double2(ok ? 1.0 : 0.0, 0.0);
+    T.zu[ax * kMpChunk + jj] = z;
+  }
+  __syncthreads();
+  const double tscale = D == 2 ? 4096.0 : 64.0;
+  const int b1 = (RP + 7) / 8;
+  const int b2 = D >= 2 ? (P2 + 8) / 8 : 0;  // powers 0 .. P2
+  const int b3 = D >= 3 ? (P3 + 8) / 8 : 0;
+  for (int idx = threadIdx.x; idx < kMpChunk * (b1 + b2 + b3); idx += blockDim.x) {
+    const int jj = idx % kMpChunk;
+    int q = idx / kMpChunk;
+    int ax = 0;
+    if (q >= b1) {
+      q -= b1;
+      ax = 1;
+      if (q >= b2) {
+        q -= b2;
+        ax = 2;
+      }
+    }
+    const double2 z = T.zu[ax * kMpChunk + jj];
+    double2 p = make_double2(jj < nj ? 1.0 : 0.0, 0.0);  // particles past the chunk: zeros
+    if (q > 0) {
+      double2 z8 = cmul(z, z);
+      z8 = cmul(z8, z8);
+      z8 = cmul(z8, z8);
+      for (int i = 0; i < q; ++i) p = cmul(p, z8);
+    }
+    const int r0 = 8 * q;
     if (ax == 0) {
-      for (int r = 0; r < RP; ++r) {
+      const int r1 = min(r0 + 8, RP);
+      for (int r = r0; r < r1; ++r) {
         const double re = kPhaseScale * p.x, im = kPhaseScale * p.y;
         const _Float16 rh = (_Float16)re, ih = (_Float16)im;
         T.z1p[(0 * RP + r) * kZ1Pitch + jj] = rh;
@@ -110,9 +147,9 @@ __device__ void mp_fill(const MpTables& T, const double* s, int jc, int nj, int 
       }
     } else {
       float2* t = (ax == 1) ? (T.t2 + jj * T.w2p) : (T.t3 + jj * T.w3p);
-      const int P = (ax == 1) ? P2 : P3;
-      for (int q = 0; q <= P; ++q) {
-        t[q] = make_float2((float)(tscale * p.x), (float)(tscale * p.y));
+      const int r1 = min(r0 + 8, (ax == 1 ? P2 : P3) + 1);
+      for (int r = r0; r < r1; ++r) {
+        t[r] = make_float2((float)(tscale * p.x), (float)(tscale * p.y));
         p = cmul(p, z);
       }
     }
