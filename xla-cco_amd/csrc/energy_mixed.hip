@@ -89,13 +89,36 @@ __device__ __forceinline__ float2 cmulf(float2 a, float2 b) {
   return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 
+// Rows [RL, RP) of the z1 planes (past the longest column) enter the GEMMs
+// only against zero C entries or as discarded rho rows: zero them once per
+// kernel so that they are finite, and never compute them.
+__device__ void mp_zero_pad_rows(const MpTables& T, int RL, int RP) {
+  const int n = (RP - RL) * (kZ1Pitch / 2);
+  for (int idx = threadIdx.x; idx < 4 * n; idx += blockDim.x) {
+    const int plane = idx / n;
+    const int e = idx - plane * n;
+    const int r = RL + e / (kZ1Pitch / 2);
+    const int c = e % (kZ1Pitch / 2);
+    reinterpret_cast<unsigned*>(T.z1p + (size_t)(plane * RP + r) * kZ1Pitch)[c] = 0u;
+  }
+}
+
+// hi + lo fp16 split of a phase component scaled by 2^12: one rounding to
+// FP32 (24 bits), then the split (about 22 bits kept)
+__device__ __forceinline__ void split_phase(double x, _Float16& hi, _Float16& lo) {
+  const float f = (float)(kPhaseScale * x);
+  hi = (_Float16)f;
+  lo = (_Float16)(f - (float)hi);
+}
+
 // Tables of particles [jc, jc + nj), called by every thread of the block:
 // the unit phases z_a = exp(2 pi i s_a) once per particle and axis, then the
 // power tables in blocks of 8 powers spread over all threads, each block
 // seeded by z^{8q} (repeated squaring) and continued by recurrence, so no
-// thread runs a serial chain longer than 8 + q steps.
+// thread runs a serial chain longer than 8 + q steps.  z1 powers are built for
+// particle pairs (packed 32-bit LDS stores) and rows < RL only.
 template <int D>
-__device__ void mp_fill(const MpTables& T, const double* s, int jc, int nj, int RP, int P2,
+__device__ void mp_fill(const MpTables& T, const double* s, int jc, int nj, int RL, int RP, int P2,
                         int P3) {
   for (int idx = threadIdx.x; idx < kMpChunk * D; idx += blockDim.x) {
     const int jj = idx % kMpChunk;
@@ -110,45 +133,63 @@ __device__ void mp_fill(const MpTables& T, const double* s, int jc, int nj, int 
   }
   __syncthreads();
   const double tscale = D == 2 ? 4096.0 : 64.0;
-  const int b1 = (RP + 7) / 8;
+  constexpr int kPairs = kMpChunk / 2;
+  const int b1 = (RL + 7) / 8;
   const int b2 = D >= 2 ? (P2 + 8) / 8 : 0;  // powers 0 .. P2
   const int b3 = D >= 3 ? (P3 + 8) / 8 : 0;
-  for (int idx = threadIdx.x; idx < kMpChunk * (b1 + b2 + b3); idx += blockDim.x) {
-    const int jj = idx % kMpChunk;
-    int q = idx / kMpChunk;
-    int ax = 0;
-    if (q >= b1) {
-      q -= b1;
-      ax = 1;
-      if (q >= b2) {
-        q -= b2;
-        ax = 2;
+  const int n1 = kPairs * b1;
+  for (int idx = threadIdx.x; idx < n1 + kMpChunk * (b2 + b3); idx += blockDim.x) {
+    if (idx < n1) {
+      const int jp = idx % kPairs;
+      const int q = idx / kPairs;
+      const double2 za = T.zu[2 * jp], zb = T.zu[2 * jp + 1];
+      double2 pa = make_double2(2 * jp < nj ? 1.0 : 0.0, 0.0);  // particles past the chunk: zeros
+      double2 pb = make_double2(2 * jp + 1 < nj ? 1.0 : 0.0, 0.0);
+      if (q > 0) {
+        double2 a8 = cmul(za, za), b8 = cmul(zb, zb);
+        a8 = cmul(a8, a8);
+        b8 = cmul(b8, b8);
+        a8 = cmul(a8, a8);
+        b8 = cmul(b8, b8);
+        for (int i = 0; i < q; ++i) {
+          pa = cmul(pa, a8);
+          pb = cmul(pb, b8);
+        }
       }
-    }
-    const double2 z = T.zu[ax * kMpChunk + jj];
-    double2 p = make_double2(jj < nj ? 1.0 : 0.0, 0.0);  // particles past the chunk: zeros
-    if (q > 0) {
-      double2 z8 = cmul(z, z);
-      z8 = cmul(z8, z8);
-      z8 = cmul(z8, z8);
-      for (int i = 0; i < q; ++i) p = cmul(p, z8);
-    }
-    const int r0 = 8 * q;
-    if (ax == 0) {
-      const int r1 = min(r0 + 8, RP);
-      for (int r = r0; r < r1; ++r) {
-        const double re = kPhaseScale * p.x, im = kPhaseScale * p.y;
-        const _Float16 rh = (_Float16)re, ih = (_Float16)im;
-        T.z1p[(0 * RP + r) * kZ1Pitch + jj] = rh;
-        T.z1p[(1 * RP + r) * kZ1Pitch + jj] = (_Float16)(re - (double)rh);
-        T.z1p[(2 * RP + r) * kZ1Pitch + jj] = ih;
-        T.z1p[(3 * RP + r) * kZ1Pitch + jj] = (_Float16)(im - (double)ih);
-        p = cmul(p, z);
+      const int r1 = min(8 * q + 8, RL);
+      for (int r = 8 * q; r < r1; ++r) {
+        _Float16 ah, al, bh, bl, ch, cl, dh, dl;
+        split_phase(pa.x, ah, al);
+        split_phase(pb.x, bh, bl);
+        split_phase(pa.y, ch, cl);
+        split_phase(pb.y, dh, dl);
+        typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
+        half2_t* row = reinterpret_cast<half2_t*>(T.z1p + (size_t)r * kZ1Pitch) + jp;
+        const size_t pl = (size_t)RP * kZ1Pitch / 2;  // one plane in half2 units
+        row[0] = half2_t{ah, bh};
+        row[pl] = half2_t{al, bl};
+        row[2 * pl] = half2_t{ch, dh};
+        row[3 * pl] = half2_t{cl, dl};
+        pa = cmul(pa, za);
+        pb = cmul(pb, zb);
       }
     } else {
+      const int e = idx - n1;
+      const int jj = e % kMpChunk;
+      int q = e / kMpChunk;
+      const int ax = q < b2 ? 1 : 2;
+      if (ax == 2) q -= b2;
+      const double2 z = T.zu[ax * kMpChunk + jj];
+      double2 p = make_double2(jj < nj ? 1.0 : 0.0, 0.0);
+      if (q > 0) {
+        double2 z8 = cmul(z, z);
+        z8 = cmul(z8, z8);
+        z8 = cmul(z8, z8);
+        for (int i = 0; i < q; ++i) p = cmul(p, z8);
+      }
       float2* t = (ax == 1) ? (T.t2 + jj * T.w2p) : (T.t3 + jj * T.w3p);
-      const int r1 = min(r0 + 8, (ax == 1 ? P2 : P3) + 1);
-      for (int r = r0; r < r1; ++r) {
+      const int r1 = min(8 * q + 8, (ax == 1 ? P2 : P3) + 1);
+      for (int r = 8 * q; r < r1; ++r) {
         t[r] = make_float2((float)(tscale * p.x), (float)(tscale * p.y));
         p = cmul(p, z);
       }
@@ -225,9 +266,10 @@ __global__ __launch_bounds__(256) void rho_mp_kernel(Geo g, Batch b) {
       sum_im[rt][i] = 0.0;
     }
   const float4_t z4 = {0.f, 0.f, 0.f, 0.f};
+  mp_zero_pad_rows(T, g.P1 + 1, g.RP);  // ordered before use by mp_fill's barrier
   for (int jc = j0; jc < j1; jc += kMpChunk) {
     const int nj = min(kMpChunk, j1 - jc);
-    mp_fill<D>(T, s, jc, nj, g.RP, g.P2, g.P3);
+    mp_fill<D>(T, s, jc, nj, g.P1 + 1, g.RP, g.P2, g.P3);
     __syncthreads();
     // FP32 accumulators over the chunk: hi*hi and cross terms apart
     float4_t rb[NRT], rs[NRT], ib[NRT], is[NRT];
@@ -329,7 +371,8 @@ __global__ __launch_bounds__(256) void force_mp_kernel(Geo g, Batch b) {
   const size_t plane = (size_t)16 * CB * CP;
   const int jc = pt * 64;
   const int nj = min(64, b.N - jc);
-  mp_fill<D>(T, s, jc, nj, RP, g.P2, g.P3);
+  mp_zero_pad_rows(T, g.P1 + 1, RP);  // ordered before use by mp_fill's barrier
+  mp_fill<D>(T, s, jc, nj, g.P1 + 1, RP, g.P2, g.P3);
   // Phi V from the reduce partials; C scale 2^-e bounds max |C| by 1
   const double* pp = b.phi_part + (size_t)lane * g.n_rtiles;
   double phiV = 0.0;
