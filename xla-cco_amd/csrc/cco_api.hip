@@ -68,7 +68,7 @@ struct cco_energy {
   int nmax[3];
   DevBuf tables;
   // batch workspace for cco_value_grad_density*
-  DevBuf s, phi, grad, re, im, part, cbox, active, n_active, rphi, prec;
+  DevBuf s, phi, grad, re, im, part, cbox, active, n_active, rphi, prec, cpl, csig;
   int ident_B = 0;
   int precision = 0;  // 1: split-precision MFMA evaluations (when the k-set allows)
   int prec_B = 0;     // lanes of prec filled with 1
@@ -342,6 +342,10 @@ int energy_batch(cco_energy* E, const double* s_dev, int B, int N, double* phi_d
   b.prec = nullptr;
   if (E->precision == 1 && g.mp_ok) {
     if ((rc = dev_reserve(E->prec, sizeof(int) * (size_t)B))) return rc;
+    if ((rc = dev_reserve(E->cpl, mp_cplane_bytes(g) * (size_t)B))) return rc;
+    if ((rc = dev_reserve(E->csig, sizeof(double) * (size_t)B))) return rc;
+    b.cpl = E->cpl.p;
+    b.csig = (double*)E->csig.p;
     if (E->prec_B < B) {
       hipLaunchKernelGGL(fill_int_kernel, dim3((B + 255) / 256), dim3(256), 0, st, (int*)E->prec.p,
                          B, 1);
@@ -394,7 +398,7 @@ struct cco_relax {
   double* flow_dt0 = nullptr;  // device copy of per-lane first steps
   RelaxDev dev;
   std::vector<DevBuf> bufs;
-  DevBuf part, cbox, rphi, out_s, out_scalars, pack, pack_idx;
+  DevBuf part, cbox, rphi, out_s, out_scalars, pack, pack_idx, cpl, csig;
   int slots_ub = 0;
   int64_t steps = 0;
   int64_t lane_evals = 0;
@@ -509,6 +513,8 @@ Batch relax_batch(cco_relax* R, const RelaxDev& dv) {
   b.cbox = (double2*)R->cbox.p;
   b.phi_part = (double*)R->rphi.p;
   b.prec = R->dev.prec;
+  b.cpl = R->cpl.p;
+  b.csig = (double*)R->csig.p;
   if (R->method == METHOD_MD && R->mc.w > 0) {  // the tracked constrained waves read re / im
     b.re = R->md_obs.re;
     b.im = R->md_obs.im;
@@ -591,6 +597,14 @@ int resolve_variant(const cco_relax* R) {
 
 // Per-lane precision flags for the run being initialised: split precision
 // when asked for, the k-set fits the MFMA kernels and the method uses it.
+// Device buffers of the split-precision C planes, when the run uses them.
+int reserve_mp_planes(cco_relax* R) {
+  if (R->dev.prec == nullptr) return CCO_OK;
+  int rc = dev_reserve(R->cpl, mp_cplane_bytes(R->E->geo) * (size_t)R->B);
+  if (rc == CCO_OK) rc = dev_reserve(R->csig, sizeof(double) * (size_t)R->B);
+  return rc;
+}
+
 int* resolve_prec(const cco_relax* R) {
   if (R->precision != 1 || R->E == nullptr || !R->E->geo.mp_ok || R->uneven) return nullptr;
   if (R->method != CCO_METHOD_LBFGS && R->method != METHOD_MD) return nullptr;
@@ -812,7 +826,7 @@ int cco_energy_destroy(cco_energy* E) {
   DevScope ds(E->ctx->device);
   (void)hipStreamSynchronize(E->ctx->stream);
   for (DevBuf* b : {&E->tables, &E->s, &E->phi, &E->grad, &E->re, &E->im, &E->part, &E->cbox,
-                    &E->active, &E->n_active, &E->rphi, &E->prec, &E->kn, &E->hs, &E->hX, &E->hXw,
+                    &E->active, &E->n_active, &E->rphi, &E->prec, &E->cpl, &E->csig, &E->kn, &E->hs, &E->hX, &E->hXw,
                     &E->hG, &E->hrho, &E->hD, &E->hH})
     dev_release(*b);
   delete E;
@@ -1178,7 +1192,7 @@ int cco_relax_destroy(cco_relax* R) {
   (void)hipStreamSynchronize(R->ctx->stream);
   for (DevBuf& b : R->bufs) dev_release(b);
   for (DevBuf& b : R->md_bufs) dev_release(b);
-  for (DevBuf* b : {&R->part, &R->cbox, &R->rphi, &R->out_s, &R->out_scalars, &R->pack,
+  for (DevBuf* b : {&R->cpl, &R->csig, &R->part, &R->cbox, &R->rphi, &R->out_s, &R->out_scalars, &R->pack,
                     &R->pack_idx, &R->un_lay, &R->un_geo, &R->ros_S, &R->ros_H, &R->ros_rhs,
                     &R->ros_dts, &R->ros_ipiv, &R->ros_info, &R->ros_cnt})
     dev_release(*b);
@@ -1202,6 +1216,7 @@ int cco_relax_init_dev(cco_relax* R, const double* s0_dev) {
   CCO_HIP(hipMemcpyAsync(R->dev.s_eval, s0_dev, bytes, hipMemcpyDeviceToDevice, st));
   R->variant_run = resolve_variant(R);
   R->dev.prec = resolve_prec(R);
+  if (int rc_mp = reserve_mp_planes(R)) return rc_mp;
   hipLaunchKernelGGL(relax_init_kernel, dim3((R->B + 255) / 256), dim3(256), 0, st, R->dev,
                      R->method);
   CCO_HIP(hipGetLastError());
@@ -1222,6 +1237,7 @@ int cco_relax_init(cco_relax* R, const double* s0_host) {
   CCO_HIP(hipMemcpyAsync(R->dev.s_eval, R->dev.s, bytes, hipMemcpyDeviceToDevice, st));
   R->variant_run = resolve_variant(R);
   R->dev.prec = resolve_prec(R);
+  if (int rc_mp = reserve_mp_planes(R)) return rc_mp;
   hipLaunchKernelGGL(relax_init_kernel, dim3((R->B + 255) / 256), dim3(256), 0, st, R->dev,
                      R->method);
   CCO_HIP(hipGetLastError());
@@ -1496,6 +1512,8 @@ int cco_energy_profile(cco_energy* E, const double* s_dev, int B, int N, int rep
   b.cbox = (double2*)E->cbox.p;
   b.phi_part = (double*)E->rphi.p;
   b.prec = (E->precision == 1 && g.mp_ok) ? (const int*)E->prec.p : nullptr;
+  b.cpl = E->cpl.p;
+  b.csig = (double*)E->csig.p;
   hipStream_t st = E->ctx->stream;
   std::vector<hipEvent_t> ev(4 * (size_t)reps);
   for (auto& e : ev) CCO_HIP(hipEventCreate(&e));
@@ -1949,6 +1967,7 @@ int cco_flow_init(cco_relax* R, const double* s0_host, const double* dt0_host) {
   }
   R->variant_run = resolve_variant(R);
   R->dev.prec = resolve_prec(R);
+  if (int rc_mp = reserve_mp_planes(R)) return rc_mp;
   hipLaunchKernelGGL(relax_init_kernel, dim3((R->B + 255) / 256), dim3(256), 0, st, R->dev,
                      R->method);
   CCO_HIP(hipGetLastError());
@@ -2232,6 +2251,7 @@ int cco_md_put(cco_relax* R, const cco_md_state_io* s) {
   }
   R->variant_run = resolve_variant(R);
   R->dev.prec = resolve_prec(R);
+  if (int rc_mp = reserve_mp_planes(R)) return rc_mp;
   if (R->dev.prec != nullptr)
     hipLaunchKernelGGL(fill_int_kernel, dim3((R->B + 255) / 256), dim3(256), 0, st, R->dev.prec,
                        R->B, 1);

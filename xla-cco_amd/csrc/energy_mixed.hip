@@ -228,6 +228,12 @@ size_t mp_lds_bytes(const Geo& g) {
          (size_t)8 * 16 * cb * mp_cpitch(g.RP) * sizeof(_Float16);
 }
 
+// The C planes of a lane, converted once per evaluation by mp_cplanes_kernel:
+// [8][Cn padded to 16][CP] halves (C re_hi, re_lo, im_hi, im_lo, then r C).
+size_t mp_cplane_bytes(const Geo& g) {
+  return (size_t)8 * ((g.Cn + 15) / 16) * 16 * mp_cpitch(g.RP) * sizeof(_Float16);
+}
+
 #define MFMA32(a, b, c) __builtin_amdgcn_mfma_f32_16x16x32_f16((a), (b), (c), 0, 0, 0)
 #define MFMA16(a, b, c) __builtin_amdgcn_mfma_f32_16x16x16f16((a), (b), (c), 0, 0, 0)
 
@@ -346,6 +352,58 @@ __device__ __forceinline__ double row16_sum(double v) {
   return v;
 }
 
+// grid: slots_ub x ceil(Cn / 16); block 256: Phi of the lane, its C scale
+// sigma = 2^-e kCScale with 2^e above the bound w_max sqrt(Phi V / w_min) of
+// max |C|, and the fp16 hi / lo planes of 16 columns of sigma C and sigma r C
+// (rows past a column's length and the pitch padding are zeros).  Every force
+// workgroup of the lane then copies planes instead of converting C itself.
+__global__ __launch_bounds__(256) void mp_cplanes_kernel(Geo g, Batch b) {
+  const int n_ct = (g.Cn + 15) / 16;
+  const int slot = blockIdx.x / n_ct;
+  if (slot >= *b.n_active) return;
+  const int ct = blockIdx.x % n_ct;
+  const int lane = b.active[slot];
+  if (b.prec[lane] == 0) return;
+  const double* pp = b.phi_part + (size_t)lane * g.n_rtiles;
+  double phiV = 0.0;
+  for (int q = 0; q < g.n_rtiles; ++q) phiV += pp[q];
+  const double bound = g.wmax * sqrt(fmax(phiV, 0.0) / g.wmin);
+  const int ex = bound > 0.0 ? ilogb(bound) + 1 : 0;
+  const double sigma = ldexp(1.0, -ex) * kCScale;
+  if (ct == 0 && threadIdx.x == 0) {
+    b.phi[lane] = phiV / g.volume;
+    b.csig[lane] = sigma;
+  }
+  const int RP = g.RP;
+  const int CP = mp_cpitch(RP);
+  const size_t plane = (size_t)n_ct * 16 * CP;
+  _Float16* base = (_Float16*)b.cpl + (size_t)lane * 8 * plane + (size_t)ct * 16 * CP;
+  const double2* C = b.cbox + (size_t)lane * g.Mbox;
+  for (int e = threadIdx.x; e < 16 * CP; e += blockDim.x) {
+    const int cl = e / CP;
+    const int r = e - cl * CP;
+    const int4 oc = g.gcol[ct * 16 + cl];
+    double2 v = make_double2(0.0, 0.0);
+    if (r < oc.y) v = C[oc.x + r];
+    const float xr = (float)(v.x * sigma);
+    const float xi = (float)(v.y * sigma);
+    _Float16 h, lo;
+    _Float16* q = base + e;
+    splitf(xr, h, lo);
+    q[0 * plane] = h;
+    q[1 * plane] = lo;
+    splitf(xi, h, lo);
+    q[2 * plane] = h;
+    q[3 * plane] = lo;
+    splitf(xr * (float)r, h, lo);
+    q[4 * plane] = h;
+    q[5 * plane] = lo;
+    splitf(xi * (float)r, h, lo);
+    q[6 * plane] = h;
+    q[7 * plane] = lo;
+  }
+}
+
 // grid: slots_ub x ceil(N / 64); block: 4 waves, wave w owns particles
 // [64 pt + 16 w, +16).  C is converted once per workgroup into LDS operand
 // planes, a batch of 16-column tiles at a time.
@@ -373,14 +431,7 @@ __global__ __launch_bounds__(256) void force_mp_kernel(Geo g, Batch b) {
   const int nj = min(64, b.N - jc);
   mp_zero_pad_rows(T, g.P1 + 1, RP);  // ordered before use by mp_fill's barrier
   mp_fill<D>(T, s, jc, nj, g.P1 + 1, RP, g.P2, g.P3);
-  // Phi V from the reduce partials; C scale 2^-e bounds max |C| by 1
-  const double* pp = b.phi_part + (size_t)lane * g.n_rtiles;
-  double phiV = 0.0;
-  for (int q = 0; q < g.n_rtiles; ++q) phiV += pp[q];
-  if (pt == 0 && threadIdx.x == 0) b.phi[lane] = phiV / g.volume;
-  const double bound = g.wmax * sqrt(fmax(phiV, 0.0) / g.wmin);
-  const int ex = bound > 0.0 ? ilogb(bound) + 1 : 0;
-  const double sigma = ldexp(1.0, -ex) * kCScale;
+  const double sigma = b.csig[lane];  // written with the C planes (mp_cplanes_kernel)
   __syncthreads();
   // A operand: particle 16 wave + m16 (m), rows 16 rk + 4 g16 + i (k)
   half4_t are_h[NRT], are_l[NRT], aim_h[NRT], aim_l[NRT];
@@ -396,34 +447,22 @@ __global__ __launch_bounds__(256) void force_mp_kernel(Geo g, Batch b) {
       aim_l[rk][i] = T.z1p[(3 * RP + r) * kZ1Pitch + jA];
     }
   }
-  const double2* C = b.cbox + (size_t)lane * g.Mbox;
   double g1[4] = {0.0, 0.0, 0.0, 0.0}, g2[4] = {0.0, 0.0, 0.0, 0.0}, g3[4] = {0.0, 0.0, 0.0, 0.0};
   const int n_ct = (g.Cn + 15) / 16;
   for (int cb0 = 0; cb0 < n_ct; cb0 += CB) {
     const int nb = min(CB, n_ct - cb0);
     __syncthreads();  // the previous batch's planes are no longer read
-    for (int e = threadIdx.x; e < nb * 16 * RP; e += blockDim.x) {
-      const int cl = e / RP;
-      const int r = e - cl * RP;
-      const int4 oc = g.gcol[cb0 * 16 + cl];
-      double2 v = make_double2(0.0, 0.0);
-      if (r < oc.y) v = C[oc.x + r];
-      const float xr = (float)(v.x * sigma);
-      const float xi = (float)(v.y * sigma);
-      _Float16 h, lo;
-      _Float16* q = cpl + (size_t)cl * CP + r;
-      splitf(xr, h, lo);
-      q[0 * plane] = h;
-      q[1 * plane] = lo;
-      splitf(xi, h, lo);
-      q[2 * plane] = h;
-      q[3 * plane] = lo;
-      splitf(xr * (float)r, h, lo);
-      q[4 * plane] = h;
-      q[5 * plane] = lo;
-      splitf(xi * (float)r, h, lo);
-      q[6 * plane] = h;
-      q[7 * plane] = lo;
+    {
+      // the batch's planes: per plane, nb x 16 contiguous columns of CP halves
+      const int per = nb * 16 * CP / 4;  // 8-byte units (CP = RP + 4 is even)
+      const size_t gplane = (size_t)n_ct * 16 * CP;
+      const _Float16* src = (const _Float16*)b.cpl + (size_t)lane * 8 * gplane + (size_t)cb0 * 16 * CP;
+      for (int e = threadIdx.x; e < 8 * per; e += blockDim.x) {
+        const int p = e / per;
+        const int o = e - p * per;
+        reinterpret_cast<uint2*>(cpl + (size_t)p * plane)[o] =
+            reinterpret_cast<const uint2*>(src + (size_t)p * gplane)[o];
+      }
     }
     __syncthreads();
     for (int tb = 0; tb < nb; ++tb) {
@@ -513,6 +552,8 @@ static void launch_mp(int stage, const Geo& g, const Batch& b, int slots_ub, hip
     hipLaunchKernelGGL((rho_mp_kernel<D, NRT>), dim3(slots_ub * g.n_mp_ctiles * b.n_splits),
                        dim3(256), mp_table_bytes(g.RP, g.P2, g.P3), st, g, b);
   } else {
+    hipLaunchKernelGGL(mp_cplanes_kernel, dim3(slots_ub * ((g.Cn + 15) / 16)), dim3(256), 0, st, g,
+                       b);
     hipLaunchKernelGGL((force_mp_kernel<D, NRT>), dim3(slots_ub * ((b.N + 63) / 64)), dim3(256),
                        mp_lds_bytes(g), st, g, b);
   }
