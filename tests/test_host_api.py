@@ -1,0 +1,40 @@
+"""The package's top-level names and the MD API are the reference's (CPU only).
+
+src/xla_cco/__init__.py and src/xla_cco/dynamics/__init__.py list the public
+names a user of the reference imports; each must resolve here (extras allowed).
+"""
+
+from __future__ import annotations
+
+import xla_cco
+import xla_cco.dynamics as dyn
+
+TOP = [
+    "AbstractEnergy", "Box", "DescentSettings", "KSet", "LBFGSSettings", "PackedConfig",
+    "RelaxResult", "StealthyEnergy", "Trajectory", "adaptive_descent", "chi_from_count",
+    "collective_density", "cubic_box", "enumerate_kvectors", "g2_from_histogram",
+    "isotropic_structure_factor", "k_outer_from_ka", "ka_from_chi", "lbfgs",
+    "make_stealthy_energy", "nearest_neighbor_distances", "pair_distance_histogram",
+    "read_configpack", "relax", "require_float64", "stack_fractional", "structure_factor",
+    "unit_ball_volume", "wrap_fractional", "write_configpack", "write_function_table",
+]
+DYNAMICS = [
+    "MDResult", "MDSettings", "MDState", "RhoTrace", "Snapshots", "Waves",
+    "andersen_reset", "baoab_step", "begin_segment", "center_of_mass",
+    "com_correct", "harmonic_sample", "init_state", "kinetic_energy",
+    "kinetic_temperature", "load_state", "make_force_fn", "make_waves",
+    "maxwell_boltzmann", "md_chunk", "save_state", "select_dt", "simulate",
+    "verlet_step", "wave_densities", "winding_update",
+]
+
+
+def test_top_level_names() -> None:
+    missing = [n for n in TOP if not hasattr(xla_cco, n)]
+    assert not missing, missing
+    assert set(TOP) <= set(xla_cco.__all__)
+
+
+def test_dynamics_names() -> None:
+    missing = [n for n in DYNAMICS if not hasattr(dyn, n)]
+    assert not missing, missing
+    assert set(DYNAMICS) <= set(dyn.__all__)

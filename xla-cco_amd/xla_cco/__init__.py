@@ -6,6 +6,8 @@ All arrays are float64.  The compute path is the HIP library
 """
 
 from xla_cco._types import require_float64
+from xla_cco.analysis.pairs import g2_from_histogram, nearest_neighbor_distances, pair_distance_histogram
+from xla_cco.analysis.structure import isotropic_structure_factor, structure_factor
 from xla_cco.box import Box, cubic_box, wrap_fractional
 from xla_cco.energy import (
     AbstractEnergy,
@@ -13,6 +15,8 @@ from xla_cco.energy import (
     collective_density,
     make_stealthy_energy,
 )
+from xla_cco.io.configpack import PackedConfig, read_configpack, stack_fractional, write_configpack
+from xla_cco.io.tables import write_function_table
 from xla_cco.kvectors import (
     KSet,
     chi_from_count,
@@ -49,4 +53,14 @@ __all__ = [
     "lbfgs",
     "RelaxResult",
     "Trajectory",
+    "PackedConfig",
+    "read_configpack",
+    "stack_fractional",
+    "write_configpack",
+    "write_function_table",
+    "g2_from_histogram",
+    "nearest_neighbor_distances",
+    "pair_distance_histogram",
+    "isotropic_structure_factor",
+    "structure_factor",
 ]
