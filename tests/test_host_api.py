@@ -38,3 +38,19 @@ def test_dynamics_names() -> None:
     missing = [n for n in DYNAMICS if not hasattr(dyn, n)]
     assert not missing, missing
     assert set(DYNAMICS) <= set(dyn.__all__)
+
+
+def test_auto_chunk_size_and_cli_helpers() -> None:
+    import numpy as np
+    import pytest
+
+    from xla_cco.cli.relax import select_device, wrap_fractional_batch
+    from xla_cco.energy import auto_chunk_size
+
+    assert auto_chunk_size(1000, 894) is None  # the (M, N) matrix fits 1 GiB
+    assert auto_chunk_size(100_000, 20_000) == (1 << 30) // (100_000 * 96)
+    assert auto_chunk_size(10, 10, budget_bytes=96 * 10 * 3) == 3
+    w = wrap_fractional_batch(np.array([[[1.5, -0.25], [2.0, 0.999]]]))
+    assert np.array_equal(w, [[[0.5, 0.75], [0.0, 0.999]]])
+    with pytest.raises(ValueError):
+        select_device("cpu")

@@ -175,6 +175,26 @@ def make_settings(args: argparse.Namespace) -> Settings:
     return DescentSettings(grow=args.grow, **common)
 
 
+def select_device(device: str) -> str:
+    """Check the requested device and return its platform name (cli/relax.py:203-230).
+
+    ``gpu`` opens the HIP context of device 0 (RuntimeError naming the cause
+    when no GPU is visible); ``cpu`` raises ValueError: the relax runs on the
+    GPU only, so the selection never falls back to a CPU path.
+    """
+    assert device in ("gpu", "cpu"), "device must be gpu or cpu"
+    if device == "cpu":
+        raise ValueError("--device cpu: the relax runs on the GPU only")
+    _native.context(0)
+    return "gpu"
+
+
+def wrap_fractional_batch(s: np.ndarray) -> np.ndarray:
+    """Wrap a (b, n, d) batch of fractional positions into [0, 1) (cli/relax.py:361-364)."""
+    assert np.ndim(s) == 3, "s must have shape (b, n, d)"
+    return wrap_fractional(np.asarray(s))
+
+
 def relax_batches(energy: Any, s0: np.ndarray, settings: Settings, batch_size: int) -> Dict[str, np.ndarray]:
     parts: List[Dict[str, np.ndarray]] = []
     for lo in range(0, s0.shape[0], batch_size):

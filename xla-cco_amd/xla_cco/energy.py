@@ -265,6 +265,21 @@ def collective_density(
     return re[0], im[0]
 
 
+def auto_chunk_size(
+    n_particles: int, n_vectors: int, budget_bytes: int = 1 << 30, bytes_per_pair: int = 96
+) -> Optional[int]:
+    """A ``chunk_size`` whose (c, N) work arrays fit ``budget_bytes`` (energy.py:86-101).
+
+    None when the whole (M, N) matrix fits.  Kept for interface parity: the
+    GPU kernels tile (k, j) themselves, so ``chunk_size`` never changes the
+    arithmetic here.
+    """
+    assert n_particles >= 1 and n_vectors >= 1, "sizes must be positive"
+    assert budget_bytes >= 1 and bytes_per_pair >= 1, "budget must be positive"
+    size = max(1, budget_bytes // (n_particles * bytes_per_pair))
+    return None if size >= n_vectors else size
+
+
 def make_stealthy_energy(
     box: Box, kset: KSet, chunk_size: Optional[int] = None, pad_to: int = 64
 ) -> StealthyEnergy:
