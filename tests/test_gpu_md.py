@@ -20,6 +20,7 @@ from oracle import oracle as O
 from xla_cco import kvectors as K
 from xla_cco.box import cubic_box
 from xla_cco.dynamics import (
+    MDSession,
     MDSettings,
     begin_segment,
     init_state,
@@ -192,6 +193,39 @@ def test_waves_must_match_the_energy(gpu_ctx) -> None:
     r0 = np.random.default_rng(3).uniform(size=(1, 20, 2)) @ basis
     with pytest.raises(ValueError):
         init_state(e, bad, basis, r0, np.zeros_like(r0), 0.01, np.stack([prng_key(0)]), st)
+
+
+def test_session_call_order_and_counters(gpu_ctx) -> None:
+    """cco_md_run / cco_md_get before a state is put are STATE errors; counters that
+    would index past the snapshot or trace rows are rejected before any copy."""
+    e, basis, waves, _ = _system(n=20)
+    st = MDSettings(dt=0.02, record_steps=(2,), record_rho_every=2, max_rho_records=3, drift_tol=math.inf)
+    r0 = np.random.default_rng(5).uniform(size=(2, 20, 2)) @ basis
+    state = init_state(e, waves, basis, r0, np.zeros_like(r0), 0.01, np.stack([prng_key(0), prng_key(1)]), st)
+    ses = MDSession(e, waves, basis, st, 2, 20, 2)
+    try:
+        with pytest.raises(RuntimeError, match="cco_md_put must come before"):
+            ses.run(3)
+        with pytest.raises(RuntimeError, match="cco_md_put must come before"):
+            ses.get(state)
+        bad = [dataclasses.replace(state, trace=dataclasses.replace(state.trace, count=np.array([0, -1]))),
+               dataclasses.replace(state, trace=dataclasses.replace(state.trace, count=np.array([4, 0]))),
+               dataclasses.replace(state, snap=dataclasses.replace(state.snap, count=np.array([0, 2]))),
+               dataclasses.replace(state, step=np.array([-1, 0]))]
+        for s in bad:
+            with pytest.raises(ValueError, match="lane"):
+                ses.put(s)
+        with pytest.raises(RuntimeError):
+            ses.run(1)  # a rejected put leaves no state to run
+        ses.put(state)
+        n_active = ses.run(4)
+        back = ses.get(state)
+        ref = simulate(e, waves, basis, st, state, 4).state
+        assert np.array_equal(back.r, ref.r) and np.array_equal(back.trace.rho, ref.trace.rho)
+        assert np.array_equal(back.status, ref.status) and n_active == 2
+        assert np.array_equal(back.step, [4, 4]) and np.array_equal(back.trace.count, [2, 2])
+    finally:
+        ses.close()
 
 
 def test_select_dt_matches_oracle_drift(gpu_ctx) -> None:

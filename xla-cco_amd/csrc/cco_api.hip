@@ -395,6 +395,7 @@ struct cco_relax {
   LbfgsCfg lc;
   FlowCfg fc;
   MdCfg mc;
+  int md_stage = 0;  // MD sessions: 0 created, 1 configured, 2 state uploaded
   double* flow_dt0 = nullptr;  // device copy of per-lane first steps
   RelaxDev dev;
   std::vector<DevBuf> bufs;
@@ -2072,6 +2073,7 @@ int cco_md_run(cco_relax* R, int64_t n_steps, int sync_every, int* n_active) {
   if (!R) return fail(CCO_ERR_ARG, "NULL argument");
   if (R->method != METHOD_MD) return fail(CCO_ERR_STATE, "not an MD session");
   if (n_steps < 0 || sync_every < 1) return fail(CCO_ERR_ARG, "need n_steps >= 0, sync_every >= 1");
+  if (R->md_stage < 2) return fail(CCO_ERR_STATE, "cco_md_put must come before cco_md_run");
   int na = R->slots_ub;
   int64_t done = 0;
   while (done < n_steps && na > 0) {
@@ -2129,6 +2131,8 @@ int cco_md_configure(cco_relax* R, const cco_md_settings* ms, int n_waves, int n
     if (record_steps_host[i] <= record_steps_host[i - 1])
       return fail(CCO_ERR_ARG, "record_steps must be strictly increasing");
   DevScope ds(R->ctx->device);
+  R->md_stage = 0;  // the observable buffers are replaced: the state must be put again
+  R->ready = false;
   for (DevBuf& b : R->md_bufs) dev_release(b);
   R->md_bufs.clear();
   MdCfg& c = R->mc;
@@ -2184,6 +2188,7 @@ int cco_md_configure(cco_relax* R, const cco_md_settings* ms, int n_waves, int n
   c.kvec = kvec;
   c.rec_steps = rec;
   CCO_HIP(hipStreamSynchronize(R->ctx->stream));
+  R->md_stage = 1;
   return CCO_OK;
 }
 
@@ -2191,12 +2196,25 @@ int cco_md_put(cco_relax* R, const cco_md_state_io* s) {
   if (!R || !s || !s->r || !s->v || !s->f || !s->lane_f64 || !s->lane_i64 || !s->key)
     return fail(CCO_ERR_ARG, "NULL argument");
   if (R->method != METHOD_MD) return fail(CCO_ERR_STATE, "not an MD session");
+  if (R->md_stage < 1) return fail(CCO_ERR_STATE, "cco_md_configure must come before cco_md_put");
   DevScope ds(R->ctx->device);
   hipStream_t st = R->ctx->stream;
   CCO_HIP(hipStreamSynchronize(st));
   const size_t B = R->B, nd = (size_t)R->dev.stride;
   const MdCfg& c = R->mc;
   const size_t w = c.w, nr = c.n_rec, q = c.rho_cap;
+  // the counters index device rows: reject what would address outside them
+  for (size_t b = 0; b < B; ++b) {
+    const int64_t* y = s->lane_i64 + b * 5;
+    if (y[0] < 0 || y[1] < 0 || y[3] < 0 || y[3] > (int64_t)nr || y[4] < 0 || y[4] > (int64_t)q)
+      return fail(CCO_ERR_ARG, "lane " + std::to_string(b) +
+                                   ": need step, n_sum >= 0, 0 <= snapshot count <= " +
+                                   std::to_string(nr) + ", 0 <= trace count <= " + std::to_string(q));
+    if (y[2] != (int64_t)(int32_t)y[2])
+      return fail(CCO_ERR_ARG, "lane " + std::to_string(b) + ": status " + std::to_string(y[2]) +
+                                   " does not fit 32 bits");
+  }
+  R->md_stage = 1;  // until every copy below has landed
   int rc;
   if ((rc = md_copy(R->dev.r, s->r, B * nd, hipMemcpyHostToDevice))) return rc;
   if ((rc = md_copy(R->dev.v, s->v, B * nd, hipMemcpyHostToDevice))) return rc;
@@ -2259,6 +2277,7 @@ int cco_md_put(cco_relax* R, const cco_md_state_io* s) {
   if ((rc = setup_shards(R))) return rc;  // active lists from the uploaded status
   CCO_HIP(hipStreamSynchronize(st));
   R->ready = true;
+  R->md_stage = 2;
   return CCO_OK;
 }
 
@@ -2266,6 +2285,7 @@ int cco_md_get(cco_relax* R, cco_md_state_io* s) {
   if (!R || !s || !s->r || !s->v || !s->f || !s->lane_f64 || !s->lane_i64 || !s->key)
     return fail(CCO_ERR_ARG, "NULL argument");
   if (R->method != METHOD_MD) return fail(CCO_ERR_STATE, "not an MD session");
+  if (R->md_stage < 2) return fail(CCO_ERR_STATE, "cco_md_put must come before cco_md_get");
   DevScope ds(R->ctx->device);
   CCO_HIP(hipStreamSynchronize(R->ctx->stream));
   const size_t B = R->B, nd = (size_t)R->dev.stride;

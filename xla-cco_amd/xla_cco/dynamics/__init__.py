@@ -14,6 +14,7 @@ from xla_cco.dynamics.integrators import andersen_reset, baoab_step, verlet_step
 from xla_cco.dynamics.observables import center_of_mass, com_correct, wave_densities, winding_update
 from xla_cco.dynamics.rng import prng_key
 from xla_cco.dynamics.run import (
+    MDSession,
     begin_segment,
     init_state,
     make_force_fn,
@@ -38,5 +39,5 @@ __all__ = [
     "kinetic_temperature", "load_state", "make_force_fn", "make_waves",
     "maxwell_boltzmann", "md_chunk", "save_state", "select_dt", "simulate",
     "verlet_step", "wave_densities", "winding_update", "CheckpointError",
-    "read_meta", "prng_key", "cartesian_hessian", "lane", "stack",
+    "read_meta", "prng_key", "cartesian_hessian", "lane", "stack", "MDSession",
 ]

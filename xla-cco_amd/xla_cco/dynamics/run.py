@@ -351,7 +351,8 @@ class MDSession:
         for k, x in a.items():
             want = np.uint32 if k == "key" else (np.int64 if k in ("lane_i64", "n_valid", "snap_step") else np.float64)
             x = np.ascontiguousarray(np.array(x, dtype=want, copy=True))
-            assert x.shape[0] == b, f"{k} has no lane axis"
+            if x.ndim == 0 or x.shape[0] != b:
+                raise ValueError(f"{k} needs a leading axis of {b} lanes, got shape {x.shape}")
             a[k] = x
         io = _native.MDStateIOC()
         for k, x in a.items():
@@ -530,6 +531,7 @@ def select_dt(
 
 __all__ = [
     "MAX_RECORD_STEPS",
+    "MDSession",
     "begin_segment",
     "init_state",
     "make_force_fn",
