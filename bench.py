@@ -36,7 +36,8 @@ PEAK_FP64_TFLOPS = 78.6  # MI355X dense FP64 (vector and matrix)
 PEAK_HBM_GBS = 8000.0
 METRIC = "converged configs/s (E<1e-20), mixed vs FP64 on 1 GPU; energy+force evals/s"
 STAGES = ("rho_kernel", "reduce_kernel", "force_kernel", "lbfgs_advance_kernel", "compact_kernel",
-          "md_prepare_kernel", "lbfgs_gram_dot_kernel", "lbfgs_gram_solve_kernel", "lbfgs_gram_comb_kernel")
+          "md_prepare_kernel", "lbfgs_gram_dot_kernel", "lbfgs_gram_solve_kernel", "lbfgs_gram_comb_kernel",
+          "md_tracked_rho_reduce")
 
 
 def build_workload(d: int, n: int, chi: float):
@@ -187,14 +188,17 @@ def bench_md(args) -> int:
     ks = K.enumerate_kvectors(box.basis, 0.0, k_c)
     M = int(ks.n.shape[0])
     energy = make_stealthy_energy(box, ks)
-    waves = make_waves(box.basis, k_c, k_c)  # the constrained waves: rho from the force pass
+    # the constrained waves take their rho from the force pass; the others (|k| up to
+    # md_track K, every step) from the tracked-wave rho pass
+    waves = make_waves(box.basis, k_c, args.md_track * k_c)
     rng = np.random.default_rng(31)
     r0 = rng.uniform(size=(B, n, d)) @ box.basis
     v0 = rng.normal(scale=1e-2, size=(B, n, d))  # T = 1e-4 per component
     keys = np.stack([prng_key(i) for i in range(B)])
     out = {}
     for prec in ("fp64", "mixed"):
-        st = MDSettings(dt=0.01, remove_com=False, chunk_steps=10_000, drift_tol=float("inf"), precision=prec)
+        st = MDSettings(dt=0.01, remove_com=False, chunk_steps=10_000, drift_tol=float("inf"), precision=prec,
+                        wind_every=1)
         lanes = init_state(energy, waves, box.basis, r0, v0, 1e-4, keys, st)
         # one device session holds the lanes; the timed region is the device steps only
         dev = MDSession(energy, waves, box.basis, st, B, n, d)
@@ -209,7 +213,8 @@ def bench_md(args) -> int:
     wall, res = out["fp64"]
     wall_m, res_m = out["mixed"]
     lane_steps = B * args.steps
-    flop = 8.0 * M * n * (1 + d) * lane_steps
+    m_trk = int(waves.kvec.shape[0]) - int(waves.n_constrained)  # waves summed by the tracked pass
+    flop = 8.0 * (M * n * (1 + d) + m_trk * n) * lane_steps
     e0 = res.phi + 0.5 * np.sum(res.v * res.v, axis=(1, 2))
     e0m = res_m.phi + 0.5 * np.sum(res_m.v * res_m.v, axis=(1, 2))
     line = {
@@ -226,7 +231,8 @@ def bench_md(args) -> int:
         "dtype": "f64",
         "data": "synthetic",
         "config": {"workload": f"NVE velocity Verlet, {B} lanes, d={d} N={n} chi={chi} (M={M}), dt=0.01, "
-                               "uniform random positions, v ~ N(0, 1e-4)", "lanes": B, "M": M},
+                               "uniform random positions, v ~ N(0, 1e-4)", "lanes": B, "M": M,
+                   "tracked_waves": int(waves.kvec.shape[0]), "k_track_over_K": args.md_track},
         "flop_rate_tflops": flop / wall / 1e12,
         "frac_of_fp64_peak": flop / wall / 1e12 / PEAK_FP64_TFLOPS,
         "mixed": {"value": lane_steps / wall_m, "vs_fp64": wall / wall_m,
@@ -279,6 +285,9 @@ def main() -> int:
     )
     ap.add_argument("--md", action="store_true",
                     help="time NVE MD at d=3 N=10000 chi=0.45 with 32 lanes instead (steps = MD steps)")
+    ap.add_argument("--md-track", type=float, default=1.0,
+                    help="with --md: track every wave with |k| < md_track K, wound every step "
+                         "(1: the constrained waves only; the reference MD command line tracks 2K)")
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the split-precision relax reported beside the FP64 value")
     ap.add_argument(

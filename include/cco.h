@@ -311,10 +311,11 @@ int cco_pair_stats(cco_ctx* ctx, const double* s_host, int B, int N, int d,
  * between launches cost device time, so a period above 1 keeps them from
  * slowing the run they measure): stages rho, reduce, force, advance (or MD
  * finish; the line-search kernel of the Gram form), compact, MD prepare,
- * Gram dot products, Gram recursion, Gram direction (ms summed, timed
- * launches counted; CCO_PROFILE_STAGES entries each). */
+ * Gram dot products, Gram recursion, Gram direction, MD tracked-wave rho +
+ * reduce (ms summed, timed launches counted; CCO_PROFILE_STAGES entries
+ * each). */
 int cco_relax_set_profiling(cco_relax* r, int every);
-#define CCO_PROFILE_STAGES 9
+#define CCO_PROFILE_STAGES 10
 int cco_relax_profile(cco_relax* r, double* ms_sum, int64_t* launches);
 /* reps timed evaluations of B x N on device data: avg ms of the rho, reduce
  * and force kernels and of the whole evaluation (avg_ms[0..3]). */

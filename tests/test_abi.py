@@ -71,3 +71,18 @@ def test_version_and_device_count_callable() -> None:
     c = ctypes.c_int(-1)
     assert L.cco_device_count(ctypes.byref(c)) == 0
     assert c.value >= 0
+
+
+def test_profile_stage_count_agrees() -> None:
+    """cco_relax_profile fills CCO_PROFILE_STAGES entries: the ctypes constant and
+    bench.py's stage names must size their arrays alike."""
+    import sys
+
+    from xla_cco import _native
+
+    m = re.search(r"#define CCO_PROFILE_STAGES (\d+)", HEADER.read_text())
+    assert m is not None
+    sys.path.insert(0, str(ROOT))
+    import bench
+
+    assert int(m.group(1)) == _native.CCO_PROFILE_STAGES == len(bench.STAGES)

@@ -10,6 +10,7 @@ the same Philox numbers, so trajectories, keys and counters agree lane by lane.
 
 from __future__ import annotations
 
+import ctypes as C
 import dataclasses
 import math
 
@@ -17,6 +18,7 @@ import numpy as np
 import pytest
 
 from oracle import oracle as O
+from xla_cco import _native
 from xla_cco import kvectors as K
 from xla_cco.box import cubic_box
 from xla_cco.dynamics import (
@@ -193,6 +195,49 @@ def test_waves_must_match_the_energy(gpu_ctx) -> None:
     r0 = np.random.default_rng(3).uniform(size=(1, 20, 2)) @ basis
     with pytest.raises(ValueError):
         init_state(e, bad, basis, r0, np.zeros_like(r0), 0.01, np.stack([prng_key(0)]), st)
+
+
+@pytest.mark.parametrize("case", ["lattice-d3-2K", "off-lattice"])
+def test_unconstrained_waves_match_oracle(gpu_ctx, case) -> None:
+    """Waves on the reciprocal lattice take the factorised rho pass (here d = 3 with
+    every wave |k| < 2K, the reference CLI's default, over several particle splits);
+    waves off it are summed particle by particle.  Both equal the oracle's direct sums."""
+    if case == "lattice-d3-2K":
+        e, basis, waves, vg = _system(n=300, d=3, chi=0.1, track=2.0)
+        n, d, steps = 300, 3, 6
+        st = MDSettings(dt=0.02, scheme="andersen", andersen_p=0.5, wind_every=1, record_steps=(4,),
+                        record_rho_every=2, max_rho_records=4, drift_tol=math.inf)
+    else:
+        e, basis, waves, vg = _system(n=24)
+        nc = waves.n_constrained
+        kvec = waves.kvec.copy()
+        kvec[nc:] *= 1.01  # off the lattice
+        waves = dataclasses.replace(waves, kvec=kvec)
+        n, d, steps = 24, 2, 9
+        st = MDSettings(dt=0.02, scheme="baoab", gamma=0.5, wind_every=3, record_rho_every=3, max_rho_records=4)
+    rng = np.random.default_rng(6)
+    r0 = rng.uniform(size=(2, n, d)) @ basis
+    v0 = 0.05 * rng.normal(size=(2, n, d))
+    keys = np.stack([prng_key(11), prng_key(12)])
+    out = simulate(e, waves, basis, st, init_state(e, waves, basis, r0, v0, 0.0025, keys, st), steps).state
+    cfg = _cfg(st)
+    for i in range(2):
+        ref = O.md_run(vg, basis, O.md_lane(vg, basis, r0[i], v0[i], 0.0025, tuple(int(x) for x in keys[i]),
+                                            waves.kvec, cfg), waves.kvec, waves.n_constrained, cfg, steps)
+        _compare(out, ref, i)
+    assert np.all(out.n_valid[:, waves.n_constrained:] > 0)
+    # which path ran: the tracked pass is profiling stage 9
+    ses = MDSession(e, waves, basis, st, 2, n, d)
+    try:
+        ses.put(init_state(e, waves, basis, r0, v0, 0.0025, keys, st))
+        _native.check(_native.lib().cco_relax_set_profiling(ses.handle, 1))
+        ses.run(3)
+        ms = (C.c_double * _native.CCO_PROFILE_STAGES)()
+        launches = (C.c_int64 * _native.CCO_PROFILE_STAGES)()
+        _native.check(_native.lib().cco_relax_profile(ses.handle, ms, launches))
+        assert launches[9] == (3 if case == "lattice-d3-2K" else 0)
+    finally:
+        ses.close()
 
 
 def test_session_call_order_and_counters(gpu_ctx) -> None:

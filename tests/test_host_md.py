@@ -237,3 +237,19 @@ def test_checkpoint_round_trip(tmp_path) -> None:
         ck.save_state(p, "state", {})
     back, _ = ck.load_state(p, st)  # a failed save kept the previous file
     assert np.array_equal(back.r, st.r)
+
+
+def test_lattice_rows_recover_n() -> None:
+    """The tracked-wave density pass takes waves whose n = B k / 2 pi is integral."""
+    from xla_cco.dynamics.run import _lattice_rows, make_waves
+
+    for d, basis in ((2, np.array([[3.0, 0.0], [1.2, 2.5]])), (3, 4.0 * np.eye(3))):
+        waves = make_waves(basis, 2.5, 5.0)
+        n = _lattice_rows(basis, waves.kvec)
+        assert n is not None and np.array_equal(n, waves.n_vectors)
+        off = waves.kvec.copy()
+        off[-1] *= 1.0 + 1e-7
+        assert _lattice_rows(basis, off) is None
+        zero = waves.kvec.copy()
+        zero[0] = 0.0
+        assert _lattice_rows(basis, zero) is None

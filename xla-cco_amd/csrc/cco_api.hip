@@ -366,7 +366,8 @@ int energy_batch(cco_energy* E, const double* s_dev, int B, int N, double* phi_d
 // Stages of one evaluation step; profiling accumulates per stage id.
 constexpr int kStages = CCO_PROFILE_STAGES;
 enum StageId { STG_RHO = 0, STG_REDUCE = 1, STG_FORCE = 2, STG_ADVANCE = 3, STG_COMPACT = 4,
-               STG_PREPARE = 5, STG_GRAM_DOT = 6, STG_GRAM_SOLVE = 7, STG_GRAM_COMB = 8 };
+               STG_PREPARE = 5, STG_GRAM_DOT = 6, STG_GRAM_SOLVE = 7, STG_GRAM_COMB = 8,
+               STG_TRACK = 9 };
 constexpr int METHOD_FLOW = CCO_METHOD_FLOW;
 constexpr int METHOD_MD = CCO_METHOD_VERLET;
 
@@ -433,6 +434,13 @@ struct cco_relax {
   // waves / recorded steps behind mc.kvec, mc.rec_steps
   MdObs md_obs{};
   std::vector<DevBuf> md_bufs;
+  double md_basis[9] = {};     // lattice vectors as rows (cco_md_create)
+  // unconstrained tracked waves on the reciprocal lattice: their k-set as an
+  // internal energy (unit weights; only its rho and reduce run) and that
+  // pass's partial sums, box values and energy parts (in md_bufs)
+  cco_energy* trk = nullptr;
+  double2 *trk_part = nullptr, *trk_cbox = nullptr;
+  double* trk_rphi = nullptr;
   std::vector<hipEvent_t> evs;
   double stage_ms[kStages] = {};
   int64_t stage_n[kStages] = {};
@@ -532,6 +540,30 @@ Batch relax_batch(cco_relax* R, const RelaxDev& dv) {
   return b;
 }
 
+// The tracked-wave pass of an MD step: FP64 rho + reduce of the unconstrained
+// waves at the positions md_prepare wrote, for the lanes whose cadence flag
+// md_prepare cleared; Re/Im rho land in md_obs.trk_re / trk_im.
+Batch track_batch(cco_relax* R, const RelaxDev& dv) {
+  Batch b;
+  b.active = dv.active;
+  b.n_active = dv.n_active;
+  b.N = R->N;
+  b.lane_stride = R->dev.stride;
+  b.n_splits = n_splits_for(R->N);
+  b.split_len = split_len_for(R->N);
+  b.n_ftiles = n_ftiles_for(R->N);
+  b.s = R->dev.s_eval;
+  b.grad = nullptr;
+  b.phi = nullptr;
+  b.re = R->md_obs.trk_re;
+  b.im = R->md_obs.trk_im;
+  b.part = R->trk_part;
+  b.cbox = R->trk_cbox;
+  b.phi_part = R->trk_rphi;
+  b.skip = R->mc.trk_skip;
+  return b;
+}
+
 void relax_stage(cco_relax* R, const Shard& sh, int stage) {
   hipStream_t st = sh.stream;
   const RelaxDev& dv = sh.dev;
@@ -549,6 +581,8 @@ void relax_stage(cco_relax* R, const Shard& sh, int stage) {
       launch_md_finish(dv, R->mc, R->md_obs, ub, st);
   } else if (stage == STG_COMPACT) {
     launch_compact(dv, st, sh.lo, sh.count, (unsigned)(sh.steps + 1));
+  } else if (stage == STG_TRACK) {
+    launch_rho_reduce(R->trk->geo, track_batch(R, dv), ub, st);
   } else if (stage >= STG_GRAM_DOT) {
     launch_lbfgs_gram(dv, R->lc, ub, stage - STG_GRAM_DOT + 1, st);
   } else {
@@ -574,6 +608,7 @@ Shard whole_view(const cco_relax* R) {
 int step_stages(const cco_relax* R, int* out) {
   int n = 0;
   if (R->method == METHOD_MD) out[n++] = STG_PREPARE;
+  if (R->method == METHOD_MD && R->trk != nullptr) out[n++] = STG_TRACK;
   out[n++] = STG_RHO;
   out[n++] = STG_REDUCE;
   out[n++] = STG_FORCE;
@@ -1193,6 +1228,7 @@ int cco_relax_destroy(cco_relax* R) {
   (void)hipStreamSynchronize(R->ctx->stream);
   for (DevBuf& b : R->bufs) dev_release(b);
   for (DevBuf& b : R->md_bufs) dev_release(b);
+  if (R->trk) cco_energy_destroy(R->trk);
   for (DevBuf* b : {&R->cpl, &R->csig, &R->part, &R->cbox, &R->rphi, &R->out_s, &R->out_scalars, &R->pack,
                     &R->pack_idx, &R->un_lay, &R->un_geo, &R->ros_S, &R->ros_H, &R->ros_rhs,
                     &R->ros_dts, &R->ros_ipiv, &R->ros_info, &R->ros_cnt})
@@ -2044,6 +2080,7 @@ int cco_md_create(cco_ctx* ctx, cco_energy* E, int B, int N, int d, const double
     }
   c.d = d;
   c.n = N;
+  for (int i = 0; i < d * d; ++i) R->md_basis[i] = basis[i];
   RelaxDev& v = R->dev;
   const size_t vec = (size_t)B * v.stride;
   auto A = [&](auto*& p, size_t cnt) {
@@ -2110,6 +2147,64 @@ int md_copy(T* dst, const T* src, size_t count, hipMemcpyKind kind) {
 }  // namespace
 }  // extern "C++"
 
+extern "C++" {
+namespace {
+
+// The unconstrained waves k_i (i = nc .. w-1) as a k-set of their own when
+// every one lies on the reciprocal lattice: n = B k / (2 pi) integral to
+// 1e-9, so exp(i k.r) = exp(2 pi i n.s) and the factorised rho kernel sums
+// them (O(w N) complex multiplies instead of w N sincos).  Waves off the
+// lattice, or a set too wide for the rho kernel's LDS tables, stay with
+// md_finish's direct sums.
+int md_tracked_pass(cco_relax* R, int n_waves, int n_constrained, const double* kvec_host) {
+  const int d = R->d, mt = n_waves - n_constrained;
+  if (mt <= 0) return CCO_OK;
+  std::vector<int64_t> nt((size_t)mt * d);
+  std::vector<double> ones(mt, 1.0);
+  for (int i = 0; i < mt; ++i) {
+    const double* k = kvec_host + (size_t)(n_constrained + i) * d;
+    bool nonzero = false;
+    for (int a = 0; a < d; ++a) {
+      double x = 0.0;
+      for (int b = 0; b < d; ++b) x += R->md_basis[a * d + b] * k[b];
+      x /= 2.0 * M_PI;
+      const double r = std::nearbyint(x);
+      if (!(std::fabs(x - r) <= 1e-9 * std::max(1.0, std::fabs(r))) || std::fabs(r) > 1e6)
+        return CCO_OK;  // off the lattice: direct sums
+      nt[(size_t)i * d + a] = (int64_t)r;
+      nonzero = nonzero || r != 0.0;
+    }
+    if (!nonzero) return CCO_OK;
+  }
+  cco_energy* T = nullptr;
+  const int rc_t = cco_energy_create(R->ctx, nt.data(), ones.data(), mt, d, 1.0, 16, &T);
+  if (rc_t == CCO_ERR_ARG) {
+    g_last_error.clear();  // too wide for the LDS tables: direct sums
+    return CCO_OK;
+  }
+  if (rc_t != CCO_OK) return rc_t;
+  R->trk = T;
+  const Geo& g = T->geo;
+  const size_t B = R->B;
+  MdObs& o = R->md_obs;
+  int rc = CCO_OK;
+  auto go = [&](auto*& p, size_t cnt) {
+    if (rc == 0) rc = md_alloc(R, p, cnt);
+  };
+  go(R->trk_part, B * n_splits_for(R->N) * g.Mbox);
+  go(R->trk_cbox, B * g.Mbox);
+  go(R->trk_rphi, B * g.n_rtiles);
+  go(o.trk_re, B * g.M_pad);
+  go(o.trk_im, B * g.M_pad);
+  go(R->mc.trk_skip, B);
+  if (rc) return rc;
+  R->mc.trk_m_pad = g.M_pad;
+  return CCO_OK;
+}
+
+}  // namespace
+}  // extern "C++"
+
 int cco_md_configure(cco_relax* R, const cco_md_settings* ms, int n_waves, int n_constrained,
                      const double* kvec_host, const int64_t* record_steps_host) {
   if (!R || !ms) return fail(CCO_ERR_ARG, "NULL argument");
@@ -2135,7 +2230,13 @@ int cco_md_configure(cco_relax* R, const cco_md_settings* ms, int n_waves, int n
   R->ready = false;
   for (DevBuf& b : R->md_bufs) dev_release(b);
   R->md_bufs.clear();
+  if (R->trk) cco_energy_destroy(R->trk);
+  R->trk = nullptr;
+  R->trk_part = R->trk_cbox = nullptr;
+  R->trk_rphi = nullptr;
   MdCfg& c = R->mc;
+  c.trk_skip = nullptr;
+  c.trk_m_pad = 0;
   c.scheme = ms->scheme;
   c.andersen_mode = ms->andersen_mode;
   c.andersen_p = ms->andersen_p;
@@ -2187,6 +2288,7 @@ int cco_md_configure(cco_relax* R, const cco_md_settings* ms, int n_waves, int n
     CCO_HIP(hipMemcpy(rec, record_steps_host, sizeof(long long) * nr, hipMemcpyHostToDevice));
   c.kvec = kvec;
   c.rec_steps = rec;
+  if ((rc = md_tracked_pass(R, n_waves, n_constrained, kvec_host))) return rc;
   CCO_HIP(hipStreamSynchronize(R->ctx->stream));
   R->md_stage = 1;
   return CCO_OK;

@@ -93,6 +93,7 @@ struct Batch {
   double2* cbox;         // w * rho per box entry [lane][Mbox]
   double* phi_part;      // energy partials [lane][n_rtiles]
   const int* prec = nullptr;  // [lane] 1 = split-precision MFMA path (null: every lane FP64)
+  const int* skip = nullptr;  // [lane] nonzero: rho and reduce leave the lane alone (null: none)
   void* cpl = nullptr;        // split-precision C planes [lane][8][Cn padded to 16][RP + 4] fp16
   double* csig = nullptr;     // [lane] the C scale of those planes
   // uneven batches (null: every lane uses the launch's Geo and the strides above)
@@ -148,6 +149,8 @@ __device__ __forceinline__ LaneIO lane_io(const Geo& g, const Batch& b, int lane
 
 void launch_energy(const Geo& g, const Batch& b, int slots_ub, bool with_grad,
                    hipStream_t stream);
+// FP64 rho and reduce only (Re/Im rho in k order into b.re / b.im), uniform batches
+void launch_rho_reduce(const Geo& g, const Batch& b, int slots_ub, hipStream_t stream);
 void launch_energy_stage(int stage, const Geo& g, const Batch& b, int slots_ub, hipStream_t st);
 size_t mp_cplane_bytes(const Geo& g);  // split-precision C planes of one lane
 void launch_copy(const void* src, void* dst, size_t bytes, hipStream_t st);

@@ -140,6 +140,10 @@ __global__ __launch_bounds__(kDynBlock) void md_prepare_kernel(RelaxDev R, MdCfg
       R.ml[lane].key[1] = ks[0][1];
     }
   }
+  // the step md_finish is about to complete is L.step + 1: a cadence step
+  // sums the unconstrained waves (the same test as md_finish's)
+  if (c.trk_skip != nullptr && threadIdx.x == 0)
+    c.trk_skip[lane] = (c.wind_every == 1 || (L.step + 1) % c.wind_every == 0) ? 0 : 1;
 }
 
 // Second half kick with the new forces; andersen: reset one particle's
@@ -239,6 +243,9 @@ __global__ __launch_bounds__(kDynBlock) void md_finish_kernel(RelaxDev R, MdCfg 
       if (i < c.nc) {
         nre = o.re[(size_t)lane * c.m_pad + i];
         nim = o.im[(size_t)lane * c.m_pad + i];
+      } else if (wind && o.trk_re != nullptr) {
+        nre = o.trk_re[(size_t)lane * c.trk_m_pad + (i - c.nc)];
+        nim = o.trk_im[(size_t)lane * c.trk_m_pad + (i - c.nc)];
       } else if (wind) {
         nre = 0.0;
         nim = 0.0;

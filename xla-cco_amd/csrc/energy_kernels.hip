@@ -136,6 +136,7 @@ __global__ __launch_bounds__(H * kRhoUnitsMax) void rho_kernel(Geo g0, Batch b) 
   const int t = blockIdx.x % tiles_per_lane;
   const int lane = b.active[slot];
   if (b.prec != nullptr && b.prec[lane]) return;  // split-precision lane
+  if (b.skip != nullptr && b.skip[lane]) return;
   const Geo g = b.lay != nullptr ? b.geos[lane] : g0;
   if (g.d != D) return;  // uneven batches launch once per dimension
   const LaneIO io = lane_io(g, b, lane);
@@ -290,6 +291,7 @@ __global__ __launch_bounds__(kReduceTile) void reduce_kernel(Geo g0, Batch b) {
   if (slot >= *b.n_active) return;
   const int tile = blockIdx.x % tiles_per_lane;
   const int lane = b.active[slot];
+  if (b.skip != nullptr && b.skip[lane]) return;  // uniform per workgroup, before any barrier
   const Geo g = b.lay != nullptr ? b.geos[lane] : g0;
   if (tile >= g.n_rtiles) return;  // uniform per workgroup, before any barrier
   const LaneIO io = lane_io(g, b, lane);
@@ -533,6 +535,15 @@ void launch_energy_stage(int stage, const Geo& g, const Batch& b, int slots_ub, 
   if (g.d == 2) launch_stage<2>(stage, g, b, slots_ub, st);
   if (g.d == 3) launch_stage<3>(stage, g, b, slots_ub, st);
   if (stage != 1) launch_energy_mixed(stage, g, b, slots_ub, st);
+}
+
+void launch_rho_reduce(const Geo& g, const Batch& b, int slots_ub, hipStream_t st) {
+  if (slots_ub <= 0) return;
+  for (int stage = 0; stage < 2; ++stage) {
+    if (g.d == 1) launch_stage<1>(stage, g, b, slots_ub, st);
+    if (g.d == 2) launch_stage<2>(stage, g, b, slots_ub, st);
+    if (g.d == 3) launch_stage<3>(stage, g, b, slots_ub, st);
+  }
 }
 
 void launch_energy(const Geo& g, const Batch& b, int slots_ub, bool with_grad,

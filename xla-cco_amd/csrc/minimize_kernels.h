@@ -82,6 +82,11 @@ struct MdCfg {
   int n_rec, rho_every, rho_cap, pad2;
   const double* kvec;               // [w][d]
   const long long* rec_steps;       // [n_rec], strictly increasing
+  // waves nc .. w-1 on the reciprocal lattice: their rho comes from a rho +
+  // reduce pass over their own k-set, run for the lanes whose flag md_prepare
+  // clears (cadence steps); null: md_finish sums them particle by particle
+  int* trk_skip;                    // [B]
+  int trk_m_pad, pad3;              // row stride of o.trk_re / o.trk_im
 };
 
 struct MdLane {
@@ -101,6 +106,8 @@ struct MdObs {
   double* sum_s;      // [B][w]
   double* re;         // [B][m_pad] force-pass densities
   double* im;
+  double* trk_re;     // [B][trk_m_pad] densities of waves nc .. w-1 (tracked pass)
+  double* trk_im;
   long long* snap_step;  // [B][n_rec]
   double* snap_time;     // [B][n_rec]
   double* snap_phi;      // [B][n_rec]

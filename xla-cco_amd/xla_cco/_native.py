@@ -23,6 +23,7 @@ CCO_ERR_DTYPE = 2
 CCO_ERR_DEVICE = 3
 CCO_ERR_BOUND = 4
 CCO_ERR_STATE = 5
+CCO_PROFILE_STAGES = 10  # include/cco.h: entries of cco_relax_profile's arrays
 
 METHOD_DESCENT = 0
 METHOD_LBFGS = 1

@@ -158,16 +158,49 @@ def _forces_batch(energy: StealthyEnergy, inv: np.ndarray, r: np.ndarray):  # ty
     return phi, -g @ inv.T, re, im
 
 
+def _lattice_rows(basis: np.ndarray, kvec: np.ndarray) -> Optional[np.ndarray]:
+    """n = B k / (2 pi) per row when every row is integral (to 1e-9) and nonzero, else None."""
+    x = np.asarray(kvec) @ np.asarray(basis, dtype=np.float64).T / (2.0 * math.pi)
+    n = np.rint(x)
+    if not np.all(np.abs(x - n) <= 1e-9 * np.maximum(1.0, np.abs(n))) or np.any(np.all(n == 0.0, axis=1)):
+        return None
+    return n
+
+
+def _unconstrained_rho(energy: StealthyEnergy, waves: Waves, basis: np.ndarray, inv: np.ndarray,
+                       r: np.ndarray) -> np.ndarray:
+    """rho (b, w - nc, 2) of the unconstrained waves for a batch of positions.
+
+    Waves on the reciprocal lattice go through the device density pass over
+    their own k-set (exp(i k.r) = exp(2 pi i n.s)), as the MD step does;
+    others, or a set too wide for that pass, are summed on the host.
+    """
+    nc = waves.n_constrained
+    kv = np.asarray(waves.kvec[nc:])
+    n_t = _lattice_rows(basis, kv)
+    if n_t is not None:
+        try:
+            t = StealthyEnergy(n_t, np.ones(n_t.shape[0]), 1.0, device=energy.device)
+            _, _, re, im = t._eval(np.ascontiguousarray(r @ inv), grad=False, density=True)
+        except ValueError:
+            pass  # too wide for the density pass's tables
+        else:
+            assert re is not None and im is not None
+            return np.stack([re, im], axis=-1)
+    return np.stack([wave_densities(r[i], kv, energy.chunk_size) for i in range(r.shape[0])])
+
+
 def _lane_rho(r: np.ndarray, re: np.ndarray, im: np.ndarray, waves: Waves, scheme: str,
-              chunk: Optional[int]) -> np.ndarray:
-    """rho (w, 2) of one lane with every wave fresh (run.py:259-291, wind = True)."""
+              rho_u: np.ndarray) -> np.ndarray:
+    """rho (w, 2) of one lane with every wave fresh (run.py:259-291, wind = True);
+    rho_u holds the unconstrained waves' densities."""
     nc = waves.n_constrained
     w = int(waves.kvec.shape[0])
     rho = np.empty((w, 2))
     rho[:nc, 0] = re[:nc]
     rho[:nc, 1] = im[:nc]
     if w > nc:
-        rho[nc:] = wave_densities(r, np.asarray(waves.kvec[nc:]), chunk)
+        rho[nc:] = rho_u
     if scheme != "verlet":
         rho = com_correct(rho, np.asarray(waves.kvec), center_of_mass(r))
     return rho
@@ -236,7 +269,8 @@ def init_state(
     _check_waves(energy, waves)
     inv = _basis_inverse(energy, basis)
     phi, f, re, im = _forces_batch(energy, inv, rb)
-    rho = np.stack([_lane_rho(rb[i], re[i], im[i], waves, settings.scheme, energy.chunk_size) for i in range(b)])
+    rho_u = _unconstrained_rho(energy, waves, basis, inv, rb)
+    rho = np.stack([_lane_rho(rb[i], re[i], im[i], waves, settings.scheme, rho_u[i]) for i in range(b)])
     snap, trace = _empty_buffers((b,), n, d, w, settings)
     ke = 0.5 * np.sum(vb * vb, axis=(1, 2))
     zf = np.zeros(b)
@@ -397,9 +431,10 @@ def _repair_pending(energy: StealthyEnergy, waves: Waves, basis: np.ndarray, set
         return st
     inv = _basis_inverse(energy, basis)
     _, _, re, im = _forces_batch(energy, inv, st.r[lanes])
+    rho_u = _unconstrained_rho(energy, waves, basis, inv, st.r[lanes])
     rho_prev = st.rho_prev.copy()
     for q, i in enumerate(lanes):
-        rho = _lane_rho(st.r[i], re[q], im[q], waves, settings.scheme, energy.chunk_size)
+        rho = _lane_rho(st.r[i], re[q], im[q], waves, settings.scheme, rho_u[q])
         rho_prev[i] = np.where(pend[i][:, None], rho, rho_prev[i])
     return dataclasses.replace(st, rho_prev=rho_prev)
 
