@@ -2280,6 +2280,8 @@ int cco_md_configure(cco_relax* R, const cco_md_settings* ms, int n_waves, int n
   go(o.snap_r, B * nr * nd);
   go(o.tr_rho, B * q * w * 2);
   go(o.tr_time, B * q);
+  go(o.cm, B * 3);
+  go(o.rows, B * 2);
   if (rc) return rc;
   o.re = re;
   o.im = im;

@@ -149,13 +149,11 @@ __global__ __launch_bounds__(kDynBlock) void md_prepare_kernel(RelaxDev R, MdCfg
 // Second half kick with the new forces; andersen: reset one particle's
 // velocity (count: when the accumulator reaches 1; bernoulli: with
 // probability p) from (key', k_pick, k_draw, k_coin) = split(key, 4); the
-// kinetic energy, the drift guard (verlet), the running sums; then the
-// tracked waves: constrained rho from the force pass, the others summed on
-// cadence steps, COM-corrected unless verlet, winding, S sums, snapshots
-// at the recorded steps and the rho trace.
+// kinetic energy, the drift guard (verlet), the running sums; then what
+// the tracked waves need per lane (centre of mass, this step's snapshot and
+// trace rows with their scalar fields) for md_waves_kernel.
 __global__ __launch_bounds__(kDynBlock) void md_finish_kernel(RelaxDev R, MdCfg c, MdObs o) {
   __shared__ double red[16];
-  __shared__ double cm[3];
   const int slot = blockIdx.x;
   if (slot >= *R.n_active) return;
   const int lane = R.active[slot];
@@ -220,82 +218,25 @@ __global__ __launch_bounds__(kDynBlock) void md_finish_kernel(RelaxDev R, MdCfg 
   L.sum_phi2 = L.sum_phi2 + phi * phi;
   L.n_sum += 1;
   if (c.w > 0) {
-    const bool corrected = c.scheme != 0;
-    const bool wind = c.w > c.nc && (c.wind_every == 1 || L.step % c.wind_every == 0);
-    if (corrected) {
+    // per lane: the centre of mass (COM-corrected schemes), this step's
+    // snapshot row (run.py:477-503: the state after this step) and trace
+    // row with their scalar fields; md_waves_kernel does the per-wave part
+    if (c.scheme != 0) {
       for (int a = 0; a < d; ++a) {
         double s = 0.0;
         for (int j = threadIdx.x; j < c.n; j += blockDim.x) s += r[(size_t)j * d + a];
         s = block_sum<kDynBlock>(s, red);
-        if (threadIdx.x == 0) cm[a] = s / (double)c.n;
+        if (threadIdx.x == 0) o.cm[(size_t)lane * 3 + a] = s / (double)c.n;
       }
-      __syncthreads();
     }
-    double* rp = o.rho_prev + (size_t)lane * c.w * 2;
-    double* wa = o.wind_abs + (size_t)lane * c.w;
-    double* wn = o.wind_net + (size_t)lane * c.w;
-    long long* nv = o.n_valid + (size_t)lane * c.w;
-    double* ss = o.sum_s + (size_t)lane * c.w;
-    for (int i = threadIdx.x; i < c.w; i += blockDim.x) {
-      const double* k = c.kvec + (size_t)i * d;
-      const bool active = i < c.nc || wind;
-      double nre = rp[2 * i], nim = rp[2 * i + 1];
-      if (i < c.nc) {
-        nre = o.re[(size_t)lane * c.m_pad + i];
-        nim = o.im[(size_t)lane * c.m_pad + i];
-      } else if (wind && o.trk_re != nullptr) {
-        nre = o.trk_re[(size_t)lane * c.trk_m_pad + (i - c.nc)];
-        nim = o.trk_im[(size_t)lane * c.trk_m_pad + (i - c.nc)];
-      } else if (wind) {
-        nre = 0.0;
-        nim = 0.0;
-        for (int j = 0; j < c.n; ++j) {
-          double th = 0.0;
-          for (int a = 0; a < d; ++a) th = th + k[a] * r[(size_t)j * d + a];
-          double sn, cs;
-          sincos(th, &sn, &cs);
-          nre += cs;
-          nim += sn;
-        }
-      }
-      if (active && corrected) {
-        double ph = 0.0;
-        for (int a = 0; a < d; ++a) ph = ph + k[a] * cm[a];
-        double sn, cs;
-        sincos(ph, &sn, &cs);
-        const double cre = nre * cs + nim * sn;
-        const double cim = nim * cs - nre * sn;
-        nre = cre;
-        nim = cim;
-      }
-      if (active) {
-        const double ore = rp[2 * i], oim = rp[2 * i + 1];
-        const double delta = atan2(nim * ore - nre * oim, nre * ore + nim * oim);
-        const double s_old = (ore * ore + oim * oim) / (double)c.n;
-        const double s_new = (nre * nre + nim * nim) / (double)c.n;
-        if (s_old > c.phase_floor && s_new > c.phase_floor) {
-          wa[i] = wa[i] + fabs(delta);
-          wn[i] = wn[i] + delta;
-          nv[i] += 1;
-        }
-        rp[2 * i] = nre;
-        rp[2 * i + 1] = nim;
-      }
-      ss[i] = ss[i] + (rp[2 * i] * rp[2 * i] + rp[2 * i + 1] * rp[2 * i + 1]) / (double)c.n;
-    }
-    __syncthreads();
-    // snapshot at a recorded step (run.py:477-503): the state after this step
     int jrec = -1;
     for (int q = 0; q < c.n_rec; ++q)
       if (c.rec_steps[q] == L.step) jrec = q;
+    int trow = -1;
+    if (c.rho_every > 0 && L.step % c.rho_every == 0 && L.trace_count < c.rho_cap)
+      trow = (int)L.trace_count;
     if (jrec >= 0) {
       const size_t row = (size_t)lane * c.n_rec + jrec;
-      for (int i = threadIdx.x; i < c.w; i += blockDim.x) {
-        o.snap_wabs[row * c.w + i] = wa[i];
-        o.snap_wnet[row * c.w + i] = wn[i];
-        o.snap_rho[(row * c.w + i) * 2] = rp[2 * i];
-        o.snap_rho[(row * c.w + i) * 2 + 1] = rp[2 * i + 1];
-      }
       for (int e = threadIdx.x; e < c.n * d; e += blockDim.x) o.snap_r[row * (size_t)c.n * d + e] = r[e];
       if (threadIdx.x == 0) {
         o.snap_step[row] = L.step;
@@ -305,20 +246,103 @@ __global__ __launch_bounds__(kDynBlock) void md_finish_kernel(RelaxDev R, MdCfg 
       }
       L.snap_count += 1;
     }
-    if (c.rho_every > 0 && L.step % c.rho_every == 0 && L.trace_count < c.rho_cap) {
-      const size_t row = (size_t)lane * c.rho_cap + L.trace_count;
-      for (int i = threadIdx.x; i < c.w; i += blockDim.x) {
-        o.tr_rho[(row * c.w + i) * 2] = rp[2 * i];
-        o.tr_rho[(row * c.w + i) * 2 + 1] = rp[2 * i + 1];
-      }
-      if (threadIdx.x == 0) o.tr_time[row] = L.time;
+    if (trow >= 0) {
+      if (threadIdx.x == 0) o.tr_time[(size_t)lane * c.rho_cap + trow] = L.time;
       L.trace_count += 1;
+    }
+    if (threadIdx.x == 0) {
+      o.rows[2 * lane] = jrec;
+      o.rows[2 * lane + 1] = trow;
     }
   }
   __syncthreads();
   if (threadIdx.x == 0) {
     R.ml[lane] = L;
     R.status[lane] = code;
+  }
+}
+
+
+// The tracked waves of one lane's step, one thread per wave (grid: slots x
+// wave tiles), after md_finish_kernel: the constrained rho from the force
+// pass, the others from the tracked-wave pass (or summed over the particles
+// here when off the lattice) on cadence steps, COM rotation unless verlet,
+// winding where S = |rho|^2 / N passes the phase floor at both ends, the S
+// sums, and the wave columns of this step's snapshot and trace rows.
+__global__ __launch_bounds__(kDynBlock) void md_waves_kernel(RelaxDev R, MdCfg c, MdObs o,
+                                                             int n_wtiles) {
+  const int slot = blockIdx.x / n_wtiles;
+  if (slot >= *R.n_active) return;
+  const int lane = R.active[slot];
+  const int i = (blockIdx.x % n_wtiles) * kDynBlock + (int)threadIdx.x;
+  if (i >= c.w) return;  // no barrier below
+  const int d = c.d;
+  const double* r = R.r + (size_t)lane * R.stride;
+  const long long step = R.ml[lane].step;  // the step md_finish_kernel completed
+  const bool corrected = c.scheme != 0;
+  const bool wind = c.w > c.nc && (c.wind_every == 1 || step % c.wind_every == 0);
+  double* rp = o.rho_prev + (size_t)lane * c.w * 2;
+  double* wa = o.wind_abs + (size_t)lane * c.w;
+  double* wn = o.wind_net + (size_t)lane * c.w;
+  long long* nv = o.n_valid + (size_t)lane * c.w;
+  double* ss = o.sum_s + (size_t)lane * c.w;
+  const double* k = c.kvec + (size_t)i * d;
+  const bool active = i < c.nc || wind;
+  double nre = rp[2 * i], nim = rp[2 * i + 1];
+  if (i < c.nc) {
+    nre = o.re[(size_t)lane * c.m_pad + i];
+    nim = o.im[(size_t)lane * c.m_pad + i];
+  } else if (wind && o.trk_re != nullptr) {
+    nre = o.trk_re[(size_t)lane * c.trk_m_pad + (i - c.nc)];
+    nim = o.trk_im[(size_t)lane * c.trk_m_pad + (i - c.nc)];
+  } else if (wind) {
+    nre = 0.0;
+    nim = 0.0;
+    for (int j = 0; j < c.n; ++j) {
+      double th = 0.0;
+      for (int a = 0; a < d; ++a) th = th + k[a] * r[(size_t)j * d + a];
+      double sn, cs;
+      sincos(th, &sn, &cs);
+      nre += cs;
+      nim += sn;
+    }
+  }
+  if (active && corrected) {
+    double ph = 0.0;
+    for (int a = 0; a < d; ++a) ph = ph + k[a] * o.cm[(size_t)lane * 3 + a];
+    double sn, cs;
+    sincos(ph, &sn, &cs);
+    const double cre = nre * cs + nim * sn;
+    const double cim = nim * cs - nre * sn;
+    nre = cre;
+    nim = cim;
+  }
+  if (active) {
+    const double ore = rp[2 * i], oim = rp[2 * i + 1];
+    const double delta = atan2(nim * ore - nre * oim, nre * ore + nim * oim);
+    const double s_old = (ore * ore + oim * oim) / (double)c.n;
+    const double s_new = (nre * nre + nim * nim) / (double)c.n;
+    if (s_old > c.phase_floor && s_new > c.phase_floor) {
+      wa[i] = wa[i] + fabs(delta);
+      wn[i] = wn[i] + delta;
+      nv[i] += 1;
+    }
+    rp[2 * i] = nre;
+    rp[2 * i + 1] = nim;
+  }
+  ss[i] = ss[i] + (rp[2 * i] * rp[2 * i] + rp[2 * i + 1] * rp[2 * i + 1]) / (double)c.n;
+  const int jrec = o.rows[2 * lane], trow = o.rows[2 * lane + 1];
+  if (jrec >= 0) {
+    const size_t row = (size_t)lane * c.n_rec + jrec;
+    o.snap_wabs[row * c.w + i] = wa[i];
+    o.snap_wnet[row * c.w + i] = wn[i];
+    o.snap_rho[(row * c.w + i) * 2] = rp[2 * i];
+    o.snap_rho[(row * c.w + i) * 2 + 1] = rp[2 * i + 1];
+  }
+  if (trow >= 0) {
+    const size_t row = (size_t)lane * c.rho_cap + trow;
+    o.tr_rho[(row * c.w + i) * 2] = rp[2 * i];
+    o.tr_rho[(row * c.w + i) * 2 + 1] = rp[2 * i + 1];
   }
 }
 
@@ -331,6 +355,11 @@ void launch_md_finish(const RelaxDev& r, const MdCfg& c, const MdObs& o, int slo
                       hipStream_t st) {
   if (slots_ub <= 0) return;
   hipLaunchKernelGGL(md_finish_kernel, dim3(slots_ub), dim3(kDynBlock), 0, st, r, c, o);
+  if (c.w > 0) {
+    const int n_wtiles = (c.w + kDynBlock - 1) / kDynBlock;
+    hipLaunchKernelGGL(md_waves_kernel, dim3(slots_ub * n_wtiles), dim3(kDynBlock), 0, st, r, c, o,
+                       n_wtiles);
+  }
 }
 
 // ---------------------------------------------------------------------------

@@ -108,6 +108,8 @@ struct MdObs {
   double* im;
   double* trk_re;     // [B][trk_m_pad] densities of waves nc .. w-1 (tracked pass)
   double* trk_im;
+  double* cm;         // [B][3] centre of mass at the step (md_finish -> md_waves)
+  int* rows;          // [B][2] the step's snapshot row and trace row, -1: none
   long long* snap_step;  // [B][n_rec]
   double* snap_time;     // [B][n_rec]
   double* snap_phi;      // [B][n_rec]
