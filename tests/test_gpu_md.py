@@ -240,6 +240,31 @@ def test_unconstrained_waves_match_oracle(gpu_ctx, case) -> None:
         ses.close()
 
 
+def test_tracked_pass_at_full_shape(gpu_ctx) -> None:
+    """32 lanes x N = 10000 x d = 3, chi = 0.45, every wave |k| < 2K (108099 waves): the
+    tracked pass runs with fewer particle splits than the energy; its densities after two
+    steps equal direct sums over the returned positions (sampled waves, two lanes)."""
+    d, n, b = 3, 10000, 32
+    box = cubic_box(n, d)
+    k_c = K.k_outer_from_ka(K.ka_from_chi(0.45, 0.05, d), 0.05, d)
+    e = make_stealthy_energy(box, K.enumerate_kvectors(box.basis, 0.0, k_c))
+    waves = make_waves(box.basis, k_c, 2.0 * k_c)
+    rng = np.random.default_rng(9)
+    r0 = rng.uniform(size=(b, n, d)) @ box.basis
+    v0 = rng.normal(scale=1e-2, size=(b, n, d))
+    keys = np.stack([prng_key(i) for i in range(b)])
+    st = MDSettings(dt=0.01, drift_tol=math.inf, wind_every=1)
+    out = simulate(e, waves, box.basis, st, init_state(e, waves, box.basis, r0, v0, 1e-4, keys, st), 2).state
+    nc = waves.n_constrained
+    pick = np.r_[nc, rng.choice(np.arange(nc + 1, waves.kvec.shape[0] - 1), 40, replace=False),
+                 waves.kvec.shape[0] - 1]
+    for i in (0, b - 1):
+        th = out.r[i] @ waves.kvec[pick].T
+        want = np.stack([np.cos(th).sum(axis=0), np.sin(th).sum(axis=0)], axis=-1)
+        assert np.max(np.abs(out.rho_prev[i, pick] - want)) <= 1e-9 * n
+        assert np.all(out.n_valid[i, pick] == 2)
+
+
 def test_session_call_order_and_counters(gpu_ctx) -> None:
     """cco_md_run / cco_md_get before a state is put are STATE errors; counters that
     would index past the snapshot or trace rows are rejected before any copy."""

@@ -439,6 +439,7 @@ struct cco_relax {
   // internal energy (unit weights; only its rho and reduce run) and that
   // pass's partial sums, box values and energy parts (in md_bufs)
   cco_energy* trk = nullptr;
+  int trk_splits = 0, trk_split_len = 0;  // its particle splits (fewer than the energy's)
   double2 *trk_part = nullptr, *trk_cbox = nullptr;
   double* trk_rphi = nullptr;
   std::vector<hipEvent_t> evs;
@@ -549,8 +550,8 @@ Batch track_batch(cco_relax* R, const RelaxDev& dv) {
   b.n_active = dv.n_active;
   b.N = R->N;
   b.lane_stride = R->dev.stride;
-  b.n_splits = n_splits_for(R->N);
-  b.split_len = split_len_for(R->N);
+  b.n_splits = R->trk_splits;
+  b.split_len = R->trk_split_len;
   b.n_ftiles = n_ftiles_for(R->N);
   b.s = R->dev.s_eval;
   b.grad = nullptr;
@@ -2186,12 +2187,19 @@ int md_tracked_pass(cco_relax* R, int n_waves, int n_constrained, const double* 
   R->trk = T;
   const Geo& g = T->geo;
   const size_t B = R->B;
+  // a large wave set has many unit tiles per lane: about 4096 workgroups over
+  // all lanes need fewer particle splits than the energy's, and fewer splits
+  // mean fewer partial sums for the reduce to read
+  const int want = (4096 + R->B * g.n_utiles - 1) / (R->B * g.n_utiles);
+  const int ns = std::max(1, std::min(n_splits_for(R->N), want));
+  R->trk_split_len = ((R->N + ns - 1) / ns + kSplitLen - 1) / kSplitLen * kSplitLen;
+  R->trk_splits = (R->N + R->trk_split_len - 1) / R->trk_split_len;
   MdObs& o = R->md_obs;
   int rc = CCO_OK;
   auto go = [&](auto*& p, size_t cnt) {
     if (rc == 0) rc = md_alloc(R, p, cnt);
   };
-  go(R->trk_part, B * n_splits_for(R->N) * g.Mbox);
+  go(R->trk_part, B * R->trk_splits * g.Mbox);
   go(R->trk_cbox, B * g.Mbox);
   go(R->trk_rphi, B * g.n_rtiles);
   go(o.trk_re, B * g.M_pad);
