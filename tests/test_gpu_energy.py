@@ -60,6 +60,25 @@ def test_value_grad_density_matches_oracle(gpu_ctx, n: int, d: int, k_outer: flo
         _check(e, ks, box.volume, s)
 
 
+@pytest.mark.parametrize("n,d,k_inner,k_outer", [(40, 1, 3.0, 9.0), (200, 2, 2.0, 5.0), (150, 3, 1.5, 3.2),
+                                                  (500, 3, 2.2, 4.4)])
+def test_kset_with_inner_cutoff_matches_oracle(gpu_ctx, n: int, d: int, k_inner: float, k_outer: float) -> None:
+    """A k-set k_inner < |k| < k_outer (enumerate_kvectors' inner cutoff): the rho runs that
+    hold no k-vector (near n1 = 0 inside the inner sphere) write zero sums without work."""
+    box = cubic_box(n, d)
+    ks = K.enumerate_kvectors(box.basis, k_inner, k_outer)
+    assert ks.n.shape[0] > 0 and np.all(np.linalg.norm(ks.n @ (2 * np.pi * np.linalg.inv(box.basis).T), axis=1) > k_inner)
+    e = make_stealthy_energy(box, ks)
+    rng = np.random.default_rng(n + d)
+    _check(e, ks, box.volume, rng.uniform(size=(n, d)))
+    s2 = rng.uniform(size=(3, n, d))
+    phi, g, re, im = e.value_grad_density_batch(s2)
+    for b in range(3):
+        fo, go, reo, imo = O.energy_grad(s2[b], ks.n, ks.weights, box.volume)
+        assert abs(phi[b] - fo) <= RTOL_E * fo + 1e-12
+        assert np.allclose(re[b, : ks.n.shape[0]], reo, rtol=0, atol=1e-11 * np.sqrt(n))
+
+
 @pytest.mark.parametrize("d,n,chi", [(2, 1000, 0.45), (3, 400, 0.3), (2, 1500, 0.2), (1, 800, 0.45), (2, 3000, 0.1)])
 def test_bench_shapes_match_oracle(gpu_ctx, d: int, n: int, chi: float) -> None:
     """Shapes with several rho splits, column tiles and row segments; d = 1 puts all

@@ -169,7 +169,13 @@ int build_geometry(cco_energy* E, const int64_t* n, const double* w, int M, int 
   }
   g.Mbox = off;
   g.Cn = (int)cols.size();
-  // rho units: runs of <= kRowSeg rows of one column, longest first (stable)
+  // box rows holding a k-vector (a k-set with an inner cutoff leaves runs
+  // near n1 = 0 empty)
+  std::vector<char> occ(g.Mbox, 0);
+  for (int k = 0; k < M; ++k) occ[cols[col_index[{n2[k], n3[k]}]].off + n1[k]] = 1;
+  // rho units: runs of <= kRowSeg rows of one column, longest first (stable);
+  // a run without a k-vector is flagged empty and sorted last, so that whole
+  // waves of them write zero sums without the particle loop
   std::vector<int4> unit_tab;
   g.Q = 1;
   for (size_t c = 0; c < cols.size(); ++c) {
@@ -177,11 +183,16 @@ int build_geometry(cco_energy* E, const int64_t* n, const double* w, int M, int 
     g.Q = std::max(g.Q, nq);
     for (int q = 0; q < nq; ++q) {
       const int nr = std::min(kRowSeg, cols[c].L - q * kRowSeg);
-      unit_tab.push_back(make_int4(cols[c].off + q * kRowSeg, nr | (q << 8), cols[c].n2, cols[c].n3));
+      bool any = false;
+      for (int r = 0; r < nr; ++r) any = any || occ[cols[c].off + q * kRowSeg + r];
+      unit_tab.push_back(make_int4(cols[c].off + q * kRowSeg, nr | (q << 8) | (any ? 0 : kUnitEmpty),
+                                   cols[c].n2, cols[c].n3));
     }
   }
+  if (g.Q > 255) return fail(CCO_ERR_ARG, "k-set too long along axis 1 for the rho units");
+  auto eff_rows = [](const int4& a) { return (a.y & kUnitEmpty) ? 0 : (a.y & 0xff); };
   std::stable_sort(unit_tab.begin(), unit_tab.end(),
-                   [](const int4& a, const int4& b) { return (a.y & 0xff) > (b.y & 0xff); });
+                   [&](const int4& a, const int4& b) { return eff_rows(a) > eff_rows(b); });
   g.n_units = (int)unit_tab.size();
   g.n_utiles = (g.n_units + kRhoUnitsMax - 1) / kRhoUnitsMax;
   const int per_tile = (g.n_units + g.n_utiles - 1) / g.n_utiles;

@@ -26,6 +26,7 @@ namespace cco {
 constexpr int kWave = 64;
 constexpr int kRowSeg = 8;       // rows of one rho unit (accumulated in registers)
 constexpr int kRhoChunkMax = 64; // particles staged in LDS per rho chunk (upper bound)
+constexpr int kUnitEmpty = 1 << 30;  // rho unit flag (in .y): the run holds no k-vector
 constexpr int kRhoUnitsMax = 256;  // rho units per workgroup (two threads per unit)
 constexpr int kForceTile = 64;   // particles per force workgroup (one wave per k part)
 constexpr int kForceParts = 4;   // k parts per force workgroup (waves sharing 64 particles)

@@ -149,7 +149,8 @@ __global__ __launch_bounds__(H * kRhoUnitsMax) void rho_kernel(Geo g0, Batch b) 
   const bool uvalid = u < g.n_units;
   const int4 un = uvalid ? g.unit[u] : make_int4(0, 0, 0, 0);
   const int nrows = un.y & 0xff;
-  const int q = un.y >> 8;
+  const int q = (un.y >> 8) & 0xff;
+  const bool work = uvalid && !(un.y & kUnitEmpty);  // empty runs only write zero sums
   const int j0 = split * b.split_len;
   const int j1 = min(io.N, j0 + b.split_len);
   const double* s = io.s;
@@ -157,7 +158,7 @@ __global__ __launch_bounds__(H * kRhoUnitsMax) void rho_kernel(Geo g0, Batch b) 
   // One row count per wave (its longest unit): shorter units also compute the
   // rows past their end, which are never stored, so a wave never loops over
   // the particles once per distinct length.
-  int wave_rows = uvalid ? nrows : 0;
+  int wave_rows = work ? nrows : 0;
 #pragma unroll
   for (int off = 32; off > 0; off >>= 1) wave_rows = max(wave_rows, __shfl_xor(wave_rows, off));
   wave_rows = __builtin_amdgcn_readfirstlane(wave_rows);
@@ -182,7 +183,10 @@ __global__ __launch_bounds__(H * kRhoUnitsMax) void rho_kernel(Geo g0, Batch b) 
   const double sg2 = un.z < 0 ? -1.0 : 1.0;
   const double sg3 = un.w < 0 ? -1.0 : 1.0;
 
-  for (int jc = j0; jc < j1; jc += CH) {
+  // empty runs sort last: a tile whose first run is empty holds no k-vector
+  // and skips the tables (uniform per workgroup), writing zero sums
+  const bool tile_empty = (g.unit[utile * g.UT].y & kUnitEmpty) != 0;
+  for (int jc = j0; jc < j1 && !tile_empty; jc += CH) {
     const int nj = min(CH, j1 - jc);
     const int nje = (nj + 1) & ~1;  // an odd chunk gets a zero particle (CH is even)
     // Work items per particle: the power tables of axes 1 .. D-1 (serial
@@ -232,7 +236,7 @@ __global__ __launch_bounds__(H * kRhoUnitsMax) void rho_kernel(Geo g0, Batch b) 
       }
     }
     __syncthreads();
-    if (uvalid) {
+    if (work) {
       const double2* zr = LZR + q * TL;
       const double2* t2 = LT2 + a2 * TL;
       const double2* t3 = LT3 + a3 * TL;
