@@ -258,9 +258,11 @@ int build_geometry(cco_energy* E, const int64_t* n, const double* w, int M, int 
       g.fpart[p] = make_int2(begin, u);
     }
   }
+  // at most half the CU's 160 KB of LDS, so two workgroups share a CU
+  constexpr size_t kRhoLdsMax = 80 * 1024;
   g.rho_chunk = kRhoChunkMax;
-  while (g.rho_chunk > 16 && rho_lds_bytes(g) > 64 * 1024) g.rho_chunk /= 2;
-  if (rho_lds_bytes(g) > 64 * 1024)
+  while (g.rho_chunk > 16 && rho_lds_bytes(g) > kRhoLdsMax) g.rho_chunk /= 2;
+  if (rho_lds_bytes(g) > kRhoLdsMax)
     return fail(CCO_ERR_ARG, "k-set too wide for the rho kernel's LDS tables (max |n| too large)");
   // split-precision path: per-column table, padded rows, weight range
   // padded to whole 64-column tiles with empty columns (L = 0)
