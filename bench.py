@@ -173,6 +173,75 @@ def bench_uneven(args) -> int:
     return 0
 
 
+def bench_flow(args) -> int:
+    """DOPRI5 gradient flow of 32 lanes at d=3, N=10000, chi=0.45: lane-evaluations/s."""
+    from xla_cco import _native, kvectors as K
+    from xla_cco.box import cubic_box
+    from xla_cco.energy import make_stealthy_energy
+    from xla_cco.minimize.flow import FlowSettings
+
+    ctx = _native.context(0)
+    L = _native.lib()
+    d, n, chi, B = 3, 10000, 0.45, 32
+    box = cubic_box(n, d)
+    k_c = K.k_outer_from_ka(K.ka_from_chi(chi, 0.05, d), 0.05, d)
+    ks = K.enumerate_kvectors(box.basis, 0.0, k_c)
+    M = int(ks.n.shape[0])
+    energy = make_stealthy_energy(box, ks)
+    s0 = np.ascontiguousarray(np.random.default_rng(41).uniform(size=(B, n, d)))
+    fl = FlowSettings()
+    fs = _native.FlowSettingsC(fl.rtol, fl.atol, fl.dt_init, fl.dt_min, fl.dt_max, fl.t_max, fl.energy_tol,
+                               fl.grad_tol, int(fl.max_steps), 0, 0, 0)
+    h = C.c_void_p()
+    bas = np.ascontiguousarray(box.basis)
+    _native.check(L.cco_flow_create(ctx.handle, energy.handle(), B, n, d, C.byref(fs), _native.dptr(bas),
+                                    C.byref(h)))
+    try:
+        _native.check(L.cco_flow_init(h, _native.dptr(s0), None))
+        na = C.c_int(0)
+        _native.check(L.cco_relax_step(h, args.warmup, C.byref(na)))  # evaluations, lanes resident
+        ctx.synchronize()
+        _native.check(L.cco_relax_set_profiling(h, 4))
+        t0 = time.perf_counter()
+        _native.check(L.cco_relax_step(h, args.steps, C.byref(na)))
+        ctx.synchronize()
+        wall = time.perf_counter() - t0
+        ms = (C.c_double * len(STAGES))()
+        launches = (C.c_int64 * len(STAGES))()
+        _native.check(L.cco_relax_profile(h, ms, launches))
+        running = int(na.value)
+    finally:
+        L.cco_relax_destroy(h)
+    stage_avg = {STAGES[i]: ms[i] / launches[i] for i in range(len(STAGES)) if launches[i] > 0}
+    if "lbfgs_advance_kernel" in stage_avg:  # the advance stage of a flow session
+        stage_avg["flow_advance_kernel"] = stage_avg.pop("lbfgs_advance_kernel")
+    lane_evals = B * args.steps
+    flop = 8.0 * M * n * (1 + d) * lane_evals
+    line = {
+        "metric": "gradient-flow lane-evaluations/s (DOPRI5 stages, one energy+force evaluation each)",
+        "value": lane_evals / wall,
+        "unit": "lane-evaluations/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": wall / args.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {"workload": f"DOPRI5 gradient flow (rtol 1e-6, atol 1e-14, time parametrisation), {B} lanes, "
+                               f"d={d} N={n} chi={chi} (M={M}), uniform random positions", "lanes": B, "M": M,
+                   "running_after": running},
+        "flop_rate_tflops": flop / wall / 1e12,
+        "frac_of_fp64_peak": flop / wall / 1e12 / PEAK_FP64_TFLOPS,
+        "kernel_avg_ms": stage_avg,
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line))
+    return 0
+
+
 def bench_md(args) -> int:
     """NVE MD steps of 32 lanes at d=3, N=10000, chi=0.45 (the large MD shape): lane-steps/s."""
     from xla_cco import _native, kvectors as K
@@ -285,6 +354,8 @@ def main() -> int:
     )
     ap.add_argument("--md", action="store_true",
                     help="time NVE MD at d=3 N=10000 chi=0.45 with 32 lanes instead (steps = MD steps)")
+    ap.add_argument("--flow", action="store_true",
+                    help="time the DOPRI5 gradient flow at d=3 N=10000 chi=0.45 with 32 lanes (steps = evaluations)")
     ap.add_argument("--md-track", type=float, default=1.0,
                     help="with --md: track every wave with |k| < md_track K, wound every step "
                          "(1: the constrained waves only; the reference MD command line tracks 2K)")
@@ -311,6 +382,8 @@ def main() -> int:
 
     if args.uneven:
         return bench_uneven(args)
+    if args.flow:
+        return bench_flow(args)
     if args.md:
         return bench_md(args)
 
