@@ -453,7 +453,8 @@ struct cco_relax {
   // pass's partial sums, box values and energy parts (in md_bufs)
   cco_energy* trk = nullptr;
   int trk_splits = 0, trk_split_len = 0;  // its particle splits (fewer than the energy's)
-  double2 *trk_part = nullptr, *trk_cbox = nullptr;
+  double2 *trk_part = nullptr, *trk_cbox = nullptr, *trk_rbox = nullptr;
+  double2* md_rbox = nullptr;  // the force pass's rho per box entry (md_obs.rbox)
   double* trk_rphi = nullptr;
   std::vector<hipEvent_t> evs;
   double stage_ms[kStages] = {};
@@ -539,8 +540,7 @@ Batch relax_batch(cco_relax* R, const RelaxDev& dv) {
   b.cpl = R->cpl.p;
   b.csig = (double*)R->csig.p;
   if (R->method == METHOD_MD && R->mc.w > 0) {  // the tracked constrained waves read re / im
-    b.re = R->md_obs.re;
-    b.im = R->md_obs.im;
+    b.rbox = R->md_rbox;  // md_waves_kernel gathers the constrained waves from it
   }
   if (R->uneven) {
     b.lay = (const LaneLayout*)R->un_lay.p;
@@ -556,7 +556,7 @@ Batch relax_batch(cco_relax* R, const RelaxDev& dv) {
 
 // The tracked-wave pass of an MD step: FP64 rho + reduce of the unconstrained
 // waves at the positions md_prepare wrote, for the lanes whose cadence flag
-// md_prepare cleared; Re/Im rho land in md_obs.trk_re / trk_im.
+// md_prepare cleared; rho per box entry lands in trk_rbox.
 Batch track_batch(cco_relax* R, const RelaxDev& dv) {
   Batch b;
   b.active = dv.active;
@@ -569,8 +569,9 @@ Batch track_batch(cco_relax* R, const RelaxDev& dv) {
   b.s = R->dev.s_eval;
   b.grad = nullptr;
   b.phi = nullptr;
-  b.re = R->md_obs.trk_re;
-  b.im = R->md_obs.trk_im;
+  b.re = nullptr;
+  b.im = nullptr;
+  b.rbox = R->trk_rbox;
   b.part = R->trk_part;
   b.cbox = R->trk_cbox;
   b.phi_part = R->trk_rphi;
@@ -2215,11 +2216,12 @@ int md_tracked_pass(cco_relax* R, int n_waves, int n_constrained, const double* 
   go(R->trk_part, B * R->trk_splits * g.Mbox);
   go(R->trk_cbox, B * g.Mbox);
   go(R->trk_rphi, B * g.n_rtiles);
-  go(o.trk_re, B * g.M_pad);
-  go(o.trk_im, B * g.M_pad);
+  go(R->trk_rbox, B * g.Mbox);
   go(R->mc.trk_skip, B);
   if (rc) return rc;
-  R->mc.trk_m_pad = g.M_pad;
+  R->mc.trk_mbox = g.Mbox;
+  o.trk_rbox = R->trk_rbox;
+  o.trk_kbox = g.kbox;
   return CCO_OK;
 }
 
@@ -2257,7 +2259,9 @@ int cco_md_configure(cco_relax* R, const cco_md_settings* ms, int n_waves, int n
   R->trk_rphi = nullptr;
   MdCfg& c = R->mc;
   c.trk_skip = nullptr;
-  c.trk_m_pad = 0;
+  c.trk_mbox = 0;
+  R->md_rbox = nullptr;
+  R->trk_rbox = nullptr;
   c.scheme = ms->scheme;
   c.andersen_mode = ms->andersen_mode;
   c.andersen_p = ms->andersen_p;
@@ -2267,7 +2271,7 @@ int cco_md_configure(cco_relax* R, const cco_md_settings* ms, int n_waves, int n
   c.w = n_waves;
   c.nc = n_constrained;
   c.wind_every = ms->wind_every;
-  c.m_pad = R->E->geo.M_pad;
+  c.mbox = R->E->geo.Mbox;
   c.n_rec = ms->n_record_steps;
   c.rho_every = ms->record_rho_every;
   c.rho_cap = ms->record_rho_every > 0 ? ms->max_rho_records : 0;
@@ -2278,7 +2282,6 @@ int cco_md_configure(cco_relax* R, const cco_md_settings* ms, int n_waves, int n
   int rc = CCO_OK;
   double* kvec = nullptr;
   long long* rec = nullptr;
-  double *re = nullptr, *im = nullptr;
   auto go = [&](auto*& p, size_t cnt) {
     if (rc == 0) rc = md_alloc(R, p, cnt);
   };
@@ -2289,8 +2292,7 @@ int cco_md_configure(cco_relax* R, const cco_md_settings* ms, int n_waves, int n
   go(o.wind_net, B * w);
   go(o.n_valid, B * w);
   go(o.sum_s, B * w);
-  go(re, B * c.m_pad);
-  go(im, B * c.m_pad);
+  go(R->md_rbox, B * c.mbox);
   go(o.snap_step, B * nr);
   go(o.snap_time, B * nr);
   go(o.snap_phi, B * nr);
@@ -2304,8 +2306,8 @@ int cco_md_configure(cco_relax* R, const cco_md_settings* ms, int n_waves, int n
   go(o.cm, B * 3);
   go(o.rows, B * 2);
   if (rc) return rc;
-  o.re = re;
-  o.im = im;
+  o.rbox = R->md_rbox;
+  o.kbox = R->E->geo.kbox;
   if (w > 0) CCO_HIP(hipMemcpy(kvec, kvec_host, sizeof(double) * w * R->d, hipMemcpyHostToDevice));
   if (nr > 0)
     CCO_HIP(hipMemcpy(rec, record_steps_host, sizeof(long long) * nr, hipMemcpyHostToDevice));

@@ -93,6 +93,7 @@ struct Batch {
   double2* part;         // rho partials [lane][split][Mbox]
   double2* cbox;         // w * rho per box entry [lane][Mbox]
   double* phi_part;      // energy partials [lane][n_rtiles]
+  double2* rbox = nullptr;  // rho per box entry [lane][Mbox] (null: not written; uniform only)
   const int* prec = nullptr;  // [lane] 1 = split-precision MFMA path (null: every lane FP64)
   const int* skip = nullptr;  // [lane] nonzero: rho and reduce leave the lane alone (null: none)
   void* cpl = nullptr;        // split-precision C planes [lane][8][Cn padded to 16][RP + 4] fp16
@@ -114,6 +115,7 @@ struct LaneIO {
   double* phi_part;
   double* re;
   double* im;
+  double2* rbox;
   int N, n_splits, n_ftiles;
 };
 
@@ -132,6 +134,7 @@ __device__ __forceinline__ LaneIO lane_io(const Geo& g, const Batch& b, int lane
     io.phi_part = b.phi_part + (size_t)lane * g.n_rtiles;
     io.re = b.re != nullptr ? b.re + (size_t)lane * g.M_pad : nullptr;
     io.im = b.im != nullptr ? b.im + (size_t)lane * g.M_pad : nullptr;
+    io.rbox = b.rbox != nullptr ? b.rbox + (size_t)lane * g.Mbox : nullptr;
   } else {
     const LaneLayout L = b.lay[lane];
     io.N = L.N;
@@ -144,6 +147,7 @@ __device__ __forceinline__ LaneIO lane_io(const Geo& g, const Batch& b, int lane
     io.phi_part = b.phi_part + L.rt_off;
     io.re = nullptr;
     io.im = nullptr;
+    io.rbox = nullptr;
   }
   return io;
 }

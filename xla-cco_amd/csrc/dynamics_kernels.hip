@@ -290,11 +290,15 @@ __global__ __launch_bounds__(kDynBlock) void md_waves_kernel(RelaxDev R, MdCfg c
   const bool active = i < c.nc || wind;
   double nre = rp[2 * i], nim = rp[2 * i + 1];
   if (i < c.nc) {
-    nre = o.re[(size_t)lane * c.m_pad + i];
-    nim = o.im[(size_t)lane * c.m_pad + i];
-  } else if (wind && o.trk_re != nullptr) {
-    nre = o.trk_re[(size_t)lane * c.trk_m_pad + (i - c.nc)];
-    nim = o.trk_im[(size_t)lane * c.trk_m_pad + (i - c.nc)];
+    const int code = o.kbox[i];
+    const double2 v = o.rbox[(size_t)lane * c.mbox + (code < 0 ? ~code : code)];
+    nre = v.x;
+    nim = code < 0 ? -v.y : v.y;
+  } else if (wind && o.trk_rbox != nullptr) {
+    const int code = o.trk_kbox[i - c.nc];
+    const double2 v = o.trk_rbox[(size_t)lane * c.trk_mbox + (code < 0 ? ~code : code)];
+    nre = v.x;
+    nim = code < 0 ? -v.y : v.y;
   } else if (wind) {
     nre = 0.0;
     nim = 0.0;

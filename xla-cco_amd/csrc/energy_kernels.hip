@@ -312,6 +312,7 @@ __global__ __launch_bounds__(kReduceTile) void reduce_kernel(Geo g0, Batch b) {
     }
     const double w = g.wbox[idx];
     cbox[idx] = make_double2(w * rho.x, w * rho.y);
+    if (io.rbox != nullptr) io.rbox[idx] = rho;  // box order: coalesced, gathered by k later
     acc = w * (rho.x * rho.x + rho.y * rho.y);
   }
   if (io.re != nullptr && idx < g.M_pad) {

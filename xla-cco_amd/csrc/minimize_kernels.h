@@ -78,7 +78,7 @@ struct MdCfg {
   int scheme, andersen_mode;
   double andersen_p, ou_c, ou_var;  // BAOAB: c = exp(-gamma dt), 1 - c^2
   double phase_floor;
-  int w, nc, wind_every, m_pad;     // m_pad: row stride of the force pass's re/im
+  int w, nc, wind_every, mbox;      // mbox: row stride of the force pass's o.rbox
   int n_rec, rho_every, rho_cap, pad2;
   const double* kvec;               // [w][d]
   const long long* rec_steps;       // [n_rec], strictly increasing
@@ -86,7 +86,7 @@ struct MdCfg {
   // reduce pass over their own k-set, run for the lanes whose flag md_prepare
   // clears (cadence steps); null: md_finish sums them particle by particle
   int* trk_skip;                    // [B]
-  int trk_m_pad, pad3;              // row stride of o.trk_re / o.trk_im
+  int trk_mbox, pad3;               // row stride of o.trk_rbox
 };
 
 struct MdLane {
@@ -104,10 +104,12 @@ struct MdObs {
   double* wind_net;   // [B][w]
   long long* n_valid; // [B][w]
   double* sum_s;      // [B][w]
-  double* re;         // [B][m_pad] force-pass densities
-  double* im;
-  double* trk_re;     // [B][trk_m_pad] densities of waves nc .. w-1 (tracked pass)
-  double* trk_im;
+  // densities per box entry (reduce_kernel, coalesced) and each wave's box
+  // entry (kbox: ~j = conjugate of entry j): the force pass's for waves < nc
+  const double2* rbox;  // [B][mbox]
+  const int* kbox;      // [nc]
+  const double2* trk_rbox;  // [B][trk_mbox] tracked pass (waves nc .. w-1; null: none)
+  const int* trk_kbox;      // [w - nc]
   double* cm;         // [B][3] centre of mass at the step (md_finish -> md_waves)
   int* rows;          // [B][2] the step's snapshot row and trace row, -1: none
   long long* snap_step;  // [B][n_rec]
