@@ -20,7 +20,7 @@
 
 namespace cco {
 
-constexpr int kDynBlock = 256;
+constexpr int kDynBlock = 1024;
 
 __device__ __forceinline__ double nanmax_d(double a, double b) {
   if (a != a || b != b) return __longlong_as_double(0x7ff8000000000000LL);
