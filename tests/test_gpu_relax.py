@@ -289,9 +289,10 @@ def test_relax_d3_batch_to_ground_state(gpu_ctx) -> None:
 
 # register forms: N d = 128 and 1800 take 512 threads with 16-byte loads (2 and 4
 # elements per thread), 2100 and the odd 45 take 1024 threads with 8-byte loads
-@pytest.mark.parametrize("n,d", [(64, 2), (600, 3), (700, 3), (45, 1)])
+@pytest.mark.parametrize("n,d", [(64, 2), (600, 3), (700, 3), (45, 1), (1700, 3), (2700, 3)])
 def test_lbfgs_advance_variants_agree(gpu_ctx, n: int, d: int) -> None:
-    """Global-memory, register-resident and Gram-form L-BFGS advance: same path to ~1e-9."""
+    """Global-memory, register-resident and Gram-form L-BFGS advance: same path to ~1e-9.
+    N d = 5100 and 8100 take the register form's 6- and 8-element layouts."""
     from xla_cco import _native
     from xla_cco.minimize import RelaxSession
 

@@ -1101,7 +1101,7 @@ __global__ __launch_bounds__(NT) void lbfgs_advance_reg_kernel(RelaxDev r, Lbfgs
 }
 
 int lbfgs_register_ept(int nd) {
-  for (int e : {1, 2, 4, 8})
+  for (int e : {1, 2, 4, 6, 8})
     if (nd <= e * kRegBlock) return e;
   return 0;
 }
@@ -1146,6 +1146,10 @@ void launch_lbfgs_advance(const RelaxDev& r, const LbfgsCfg& c, int slots_ub, hi
       break;
     case 4:
       hipLaunchKernelGGL((lbfgs_advance_reg_kernel<kRegBlock, 4, 1>), dim3(slots_ub), dim3(kRegBlock),
+                         0, st, r, c);
+      break;
+    case 6:
+      hipLaunchKernelGGL((lbfgs_advance_reg_kernel<kRegBlock, 6, 1>), dim3(slots_ub), dim3(kRegBlock),
                          0, st, r, c);
       break;
     default:
