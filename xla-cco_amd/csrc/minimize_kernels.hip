@@ -297,6 +297,7 @@ __device__ int ls_consume(LsState& L, double va, double sa, int max_steps) {
 
 // Start an L-BFGS iteration: memory update, two-loop recursion, line-search
 // init, first trial point x + 1 * u.  All threads call it.
+template <int BLOCK>
 __device__ void lbfgs_start_iteration(const RelaxDev& r, const LbfgsCfg& c, int lane,
                                       LbfgsLane& L, double* red) {
   const int nd = rv_nd(r, lane);
@@ -329,15 +330,15 @@ __device__ void lbfgs_start_iteration(const RelaxDev& r, const LbfgsCfg& c, int 
       sy += dy * ds;
       yy += dy * dy;
     }
-    sy = block_sum<kAdvBlock>(sy, red);
-    yy = block_sum<kAdvBlock>(yy, red);
+    sy = block_sum<BLOCK>(sy, red);
+    yy = block_sum<BLOCK>(yy, red);
     rho_new = (sy == 0.0) ? 0.0 : 1.0 / sy;
     if (threadIdx.x == 0) rho[slot] = rho_new;
     gamma = (yy > 0.0) ? sy / yy : 1.0;
   } else {
     double gg = 0.0;
     for (int i = threadIdx.x; i < nd; i += blockDim.x) gg += gx[i] * gx[i];
-    gg = block_sum<kAdvBlock>(gg, red);
+    gg = block_sum<BLOCK>(gg, red);
     gamma = nanmin(1.0, 1.0 / sqrt(gg));
   }
   __syncthreads();
@@ -352,7 +353,7 @@ __device__ void lbfgs_start_iteration(const RelaxDev& r, const LbfgsCfg& c, int 
     const double* Yi = Y + (size_t)sl * nd;
     double dot = 0.0;
     for (int i = threadIdx.x; i < nd; i += blockDim.x) dot += Si[i] * u[i];
-    dot = block_sum<kAdvBlock>(dot, red);
+    dot = block_sum<BLOCK>(dot, red);
     const double a = (sl == slot_new ? rho_new : rho[sl]) * dot;
     if (threadIdx.x == 0) alphas[t] = a;
     for (int i = threadIdx.x; i < nd; i += blockDim.x) u[i] = u[i] + (-a) * Yi[i];
@@ -365,7 +366,7 @@ __device__ void lbfgs_start_iteration(const RelaxDev& r, const LbfgsCfg& c, int 
     const double* Yi = Y + (size_t)sl * nd;
     double dot = 0.0;
     for (int i = threadIdx.x; i < nd; i += blockDim.x) dot += Yi[i] * u[i];
-    dot = block_sum<kAdvBlock>(dot, red);
+    dot = block_sum<BLOCK>(dot, red);
     const double bcoef = (sl == slot_new ? rho_new : rho[sl]) * dot;
     const double coef = alphas[t] - bcoef;
     for (int i = threadIdx.x; i < nd; i += blockDim.x) u[i] = u[i] + coef * Si[i];
@@ -379,7 +380,7 @@ __device__ void lbfgs_start_iteration(const RelaxDev& r, const LbfgsCfg& c, int 
     pg[i] = gx[i];
     slope += ui * gx[i];
   }
-  slope = block_sum<kAdvBlock>(slope, red);
+  slope = block_sum<BLOCK>(slope, red);
   L.opt_count = count + 1;
   L.pairs += nvalid;
   LsState& Q = L.ls;
@@ -483,7 +484,7 @@ __global__ __launch_bounds__(BLOCK) void lbfgs_advance_kernel(RelaxDev r, LbfgsC
         lbfgs_gram_begin(r, c, lane, L);
         started = true;
       } else {
-        lbfgs_start_iteration(r, c, lane, L, red);
+        lbfgs_start_iteration<BLOCK>(r, c, lane, L, red);
       }
     }
   } else {
@@ -572,7 +573,7 @@ __global__ __launch_bounds__(BLOCK) void lbfgs_advance_kernel(RelaxDev r, LbfgsC
           lbfgs_gram_begin(r, c, lane, L);
           started = true;
         } else {
-          lbfgs_start_iteration(r, c, lane, L, red);
+          lbfgs_start_iteration<BLOCK>(r, c, lane, L, red);
         }
       }
     } else {
@@ -1116,7 +1117,7 @@ void launch_lbfgs_advance(const RelaxDev& r, const LbfgsCfg& c, int slots_ub, hi
   }
   const int ept = lbfgs_register_ept(r.nd);
   if (variant == 1 || ept == 0) {
-    hipLaunchKernelGGL((lbfgs_advance_kernel<kAdvBlock, false>), dim3(slots_ub), dim3(kAdvBlock), 0,
+    hipLaunchKernelGGL((lbfgs_advance_kernel<kRegBlock, false>), dim3(slots_ub), dim3(kRegBlock), 0,
                        st, r, c);
     return;
   }
