@@ -346,35 +346,61 @@ __device__ void lbfgs_start_iteration(const RelaxDev& r, const LbfgsCfg& c, int 
   const int nvalid = count < m ? count : m;
   const int midx = count % m;
   __shared__ double alphas[1024];
+  // Each pair's update of q is applied in the pass that forms the next pair's
+  // dot product (one pass over q per pair; every element sees the same
+  // operations in the same order as with separate passes).  Elements are
+  // owned by one thread throughout, so q needs no barrier between passes.
   for (int i = threadIdx.x; i < nd; i += blockDim.x) u[i] = gx[i];
+  const double* pend = nullptr;  // the pair vector of the pending update
+  double pc = 0.0;               // its coefficient
   for (int t = m - 1; t >= m - nvalid; --t) {
     const int sl = (midx + t) % m;
     const double* Si = S + (size_t)sl * nd;
-    const double* Yi = Y + (size_t)sl * nd;
     double dot = 0.0;
-    for (int i = threadIdx.x; i < nd; i += blockDim.x) dot += Si[i] * u[i];
+    for (int i = threadIdx.x; i < nd; i += blockDim.x) {
+      double q = u[i];
+      if (pend != nullptr) {
+        q = q + pc * pend[i];
+        u[i] = q;
+      }
+      dot += Si[i] * q;
+    }
     dot = block_sum<BLOCK>(dot, red);
     const double a = (sl == slot_new ? rho_new : rho[sl]) * dot;
     if (threadIdx.x == 0) alphas[t] = a;
-    for (int i = threadIdx.x; i < nd; i += blockDim.x) u[i] = u[i] + (-a) * Yi[i];
+    pend = Y + (size_t)sl * nd;
+    pc = -a;
   }
-  for (int i = threadIdx.x; i < nd; i += blockDim.x) u[i] = gamma * u[i];
+  for (int i = threadIdx.x; i < nd; i += blockDim.x) {
+    double q = u[i];
+    if (pend != nullptr) q = q + pc * pend[i];
+    u[i] = gamma * q;
+  }
   __syncthreads();
+  pend = nullptr;
   for (int t = m - nvalid; t < m; ++t) {
     const int sl = (midx + t) % m;
-    const double* Si = S + (size_t)sl * nd;
     const double* Yi = Y + (size_t)sl * nd;
     double dot = 0.0;
-    for (int i = threadIdx.x; i < nd; i += blockDim.x) dot += Yi[i] * u[i];
+    for (int i = threadIdx.x; i < nd; i += blockDim.x) {
+      double q = u[i];
+      if (pend != nullptr) {
+        q = q + pc * pend[i];
+        u[i] = q;
+      }
+      dot += Yi[i] * q;
+    }
     dot = block_sum<BLOCK>(dot, red);
     const double bcoef = (sl == slot_new ? rho_new : rho[sl]) * dot;
-    const double coef = alphas[t] - bcoef;
-    for (int i = threadIdx.x; i < nd; i += blockDim.x) u[i] = u[i] + coef * Si[i];
+    pend = S + (size_t)sl * nd;
+    pc = alphas[t] - bcoef;
   }
   // direction u = -H g, remember (x, g) for the next memory update
   double slope = 0.0;
   for (int i = threadIdx.x; i < nd; i += blockDim.x) {
-    const double ui = -u[i];
+    double q = u[i];
+    if (pend != nullptr) q = q + pc * pend[i];
+    const double ui = -q;
     u[i] = ui;
     px[i] = x[i];
     pg[i] = gx[i];
