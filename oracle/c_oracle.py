@@ -20,20 +20,25 @@ _PD = C.POINTER(C.c_double)
 _PI64 = C.POINTER(C.c_int64)
 
 
+def load(path: Path) -> C.CDLL:
+    """Open a build of cco_oracle.c and declare its entry points."""
+    h = C.CDLL(str(path))
+    h.oracle_energy_grad_batch.argtypes = [
+        _PD, C.c_int, C.c_int, _PD, _PD, C.c_int, C.c_int, C.c_double, _PD, _PD, C.c_int
+    ]
+    h.oracle_relax.argtypes = [
+        _PD, C.c_int, C.c_int, _PD, _PD, C.c_int, C.c_int, C.c_double, C.c_int,
+        _PD, _PI64, _PD, _PI64, C.c_int64, _PD, _PD, _PI64, _PI64, _PI64, C.c_int,
+    ]
+    return h
+
+
 def lib() -> C.CDLL:
     global _lib
     if _lib is None:
         if not LIB.exists():
             subprocess.run(["make", "-C", str(HERE)], check=True, capture_output=True)
-        h = C.CDLL(str(LIB))
-        h.oracle_energy_grad_batch.argtypes = [
-            _PD, C.c_int, C.c_int, _PD, _PD, C.c_int, C.c_int, C.c_double, _PD, _PD, C.c_int
-        ]
-        h.oracle_relax.argtypes = [
-            _PD, C.c_int, C.c_int, _PD, _PD, C.c_int, C.c_int, C.c_double, C.c_int,
-            _PD, _PI64, _PD, _PI64, C.c_int64, _PD, _PD, _PI64, _PI64, _PI64, C.c_int,
-        ]
-        _lib = h
+        _lib = load(LIB)
     return _lib
 
 
@@ -62,8 +67,12 @@ def relax(
     settings: object,
     eval_cap: int = 2**62,
     threads: int = 1,
+    handle: Optional[C.CDLL] = None,
 ):
-    """Batched relax with an oracle.DescentSettings / oracle.LBFGSSettings."""
+    """Batched relax with an oracle.DescentSettings / oracle.LBFGSSettings.
+
+    ``handle`` selects another build of cco_oracle.c (default: the Makefile's).
+    """
     from oracle import oracle as O
 
     s0 = np.ascontiguousarray(s0, dtype=np.float64)
@@ -92,7 +101,7 @@ def relax(
     ne = np.empty(b, dtype=np.int64)
     ni = np.empty(b, dtype=np.int64)
     st = np.empty(b, dtype=np.int64)
-    lib().oracle_relax(
+    (handle or lib()).oracle_relax(
         _d(s0), b, nn, _d(nf), _d(w), nf.shape[0], d, volume, method,
         _d(dcfg), dint.ctypes.data_as(_PI64), _d(lcfg), lint.ctypes.data_as(_PI64), int(eval_cap),
         _d(s), _d(e), ne.ctypes.data_as(_PI64), ni.ctypes.data_as(_PI64), st.ctypes.data_as(_PI64),

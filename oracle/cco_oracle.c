@@ -24,6 +24,11 @@
 #endif
 
 #define TWO_PI (2.0 * 3.14159265358979323846)
+/* zoom phase ends once |high - low| <= this (optax's float64 threshold); a test
+   rebuilds with another value to show the published outcomes reject it */
+#ifndef LS_INTERVAL_MIN
+#define LS_INTERVAL_MIN 1e-10
+#endif
 
 enum { ST_RUNNING = 0, ST_ENERGY_TOL = 1, ST_STEP_FLOOR = 2, ST_BUDGET = 3, ST_STAGNATED = 4,
        ST_LS_FAIL = 5, ST_GRAD_TOL = 6 };
@@ -297,7 +302,7 @@ static void zoom(phi_x_t* P, const double* x, const double* u, double f0, const 
         ca = ola; cv = olv;
       }
       count++;
-      failed = ((count >= max_steps) || (delta <= 1e-5)) && !done;
+      failed = ((count >= max_steps) || (delta <= LS_INTERVAL_MIN)) && !done;
     }
     (void)hs;
     pa = a; pv = va; ps = sa;

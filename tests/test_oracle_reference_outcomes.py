@@ -1,0 +1,102 @@
+"""The oracle against the outcomes the reference itself published.
+
+CPU only.  benchmarks/results_{cpu,gpu}.json and results_threshold_gpu.json of
+the reference hold, for inputs default_rng(0).random((B, N, d)) that NumPy
+reproduces exactly (benchmarks/bench_energy.py:74-92):
+
+* the mean energy and max |dPhi/ds| of value_and_grad over 4000 lanes at d=3,
+  N=100 and over one lane at d=2, N=10^4, chi=0.45;
+* the mean final energy of relax with DescentSettings(max_evals=400);
+* the total n_evals and mean final energy of relax with
+  LBFGSSettings(max_evals_per_pass=200).
+
+The first two are deterministic to round-off and are checked to 1e-12; the
+L-BFGS totals are checked within the bounds of systems.LBFGS_EVAL_RTOL.  A
+rebuild of the C oracle with the zoom search's old 1e-5 bracket floor is shown
+to fall outside them.
+"""
+
+from __future__ import annotations
+
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+from oracle import c_oracle
+from oracle import oracle as O
+from systems import benchmark_case, check_lbfgs_outcome, outcome
+
+THREADS = 8
+
+
+@pytest.mark.parametrize("chi", [0.05, 0.45])
+def test_energy_mean_and_grad_max_d3(chi: float) -> None:
+    """4000 lanes, d=3, N=100: mean Phi and max |grad| to 1e-12 (results_cpu.json)."""
+    nv, w, vol, _, s = benchmark_case(3, 100, chi, 4000)
+    phi, g = c_oracle.energy_grad_batch(s, nv, w, vol, threads=THREADS)
+    ref = outcome("value_and_grad", 3, 100, chi, 4000)
+    assert abs(phi.mean() / ref["energy_mean"] - 1.0) <= 1e-12
+    assert abs(np.abs(g).max() / ref["grad_absmax"] - 1.0) <= 1e-12
+
+
+def test_energy_d2_n10000() -> None:
+    """One lane, d=2, N=10^4, chi=0.45 (M=9004): Phi and max |grad| to 1e-12."""
+    nv, w, vol, _, s = benchmark_case(2, 10_000, 0.45, 1)
+    assert nv.shape[0] == 9004
+    phi, g = c_oracle.energy_grad_batch(s, nv, w, vol, threads=1)
+    ref = outcome("value_and_grad", 2, 10_000, 0.45, 1)
+    assert abs(phi[0] / ref["energy_mean"] - 1.0) <= 1e-12
+    assert abs(np.abs(g).max() / ref["grad_absmax"] - 1.0) <= 1e-12
+
+
+@pytest.mark.parametrize("chi,batch", [(0.05, 64), (0.45, 64), (0.05, 1000), (0.45, 1000)])
+def test_descent_400_energy_mean(chi: float, batch: int) -> None:
+    """Mean energy after DescentSettings(max_evals=400), to 1e-11 relative."""
+    nv, w, vol, _, s0 = benchmark_case(3, 100, chi, batch)
+    _, e, ne, _, _ = c_oracle.relax(s0, nv, w, vol, O.DescentSettings(max_evals=400), threads=THREADS)
+    ref = outcome("descent_400", 3, 100, chi, batch)
+    assert int(ne.sum()) == 400 * batch
+    assert abs(e.mean() / ref["energy_mean"] - 1.0) <= 1e-11
+
+
+@pytest.mark.parametrize("chi,batch", [(0.05, 64), (0.45, 64), (0.05, 1000), (0.1, 1000)])
+def test_lbfgs_eval_totals(chi: float, batch: int) -> None:
+    """Total n_evals and mean energy of LBFGSSettings(max_evals_per_pass=200)."""
+    nv, w, vol, _, s0 = benchmark_case(3, 100, chi, batch)
+    _, e, ne, _, st = c_oracle.relax(
+        s0, nv, w, vol, O.LBFGSSettings(max_evals_per_pass=200), threads=THREADS
+    )
+    print(check_lbfgs_outcome(chi, batch, ne, e))
+
+
+def test_lbfgs_old_bracket_floor_is_rejected(tmp_path: Path) -> None:
+    """The same check rejects the oracle rebuilt with a 1e-5 zoom bracket floor."""
+    src = Path(c_oracle.__file__).with_name("cco_oracle.c")
+    so = tmp_path / "libcco_oracle_floor.so"
+    subprocess.run(
+        ["gcc", "-O3", "-fPIC", "-fopenmp", "-ffp-contract=off", "-std=c99", "-shared",
+         "-DLS_INTERVAL_MIN=1e-5", "-o", str(so), str(src), "-lm"],
+        check=True,
+    )
+    handle = c_oracle.load(so)
+    nv, w, vol, _, s0 = benchmark_case(3, 100, 0.05, 64)
+    _, e, ne, _, _ = c_oracle.relax(
+        s0, nv, w, vol, O.LBFGSSettings(max_evals_per_pass=200), threads=THREADS, handle=handle
+    )
+    with pytest.raises(AssertionError):
+        check_lbfgs_outcome(0.05, 64, ne, e)
+
+
+def test_python_and_c_oracles_agree_on_lbfgs() -> None:
+    """The NumPy restatement and the C restatement take the same decisions (4 lanes)."""
+    nv, w, vol, _, s0 = benchmark_case(3, 100, 0.2, 4)
+    vg = O.make_value_grad(nv, w, vol)
+    st = O.LBFGSSettings(max_evals_per_pass=40, rescales=(1e4, 1.0))
+    _, e, ne, ni, sts = c_oracle.relax(s0, nv, w, vol, st, threads=4)
+    for i in range(4):
+        r = O.lbfgs(vg, s0[i], st)
+        assert (r.n_evals, r.n_iters, r.status) == (ne[i], ni[i], sts[i])
+        # final energies ~1e-18 carry the absolute rounding of Phi(s0) ~ 1e2
+        assert abs(r.energy - e[i]) <= 1e-15 * vg(s0[i])[0]

@@ -184,7 +184,9 @@ constexpr double kSlopeRtol = 1e-4;
 constexpr double kCurvRtol = 0.9;
 constexpr double kApproxDecRtol = 1e-6;
 constexpr double kIncrease = 2.0;
-constexpr double kIntervalMin = 1e-5;
+// The zoom phase stops once the bracket |high - low| is no wider than this: optax's
+// zoom search threshold for 64-bit step sizes.
+constexpr double kIntervalMin = 1e-10;
 
 __device__ double ls_decrease_error(double a, double va, double sa, double v0, double s0) {
   double dec = va - (v0 + kSlopeRtol * a * s0);
