@@ -69,12 +69,34 @@ def benchmark_case(d: int, n: int, chi: float, batch: int):
 
 
 # Relative bounds on the total L-BFGS evaluation count against the reference's trig
-# row.  Below the round-off floor the line search runs on energies that are pure
+# row.  Below the round-off floor the line search runs on energies that are mostly
 # rounding noise (about 90% of the evaluations at chi=0.05), so totals follow the
-# reduction order of the energy: the reference's own trig and table paths differ by
-# 0.01-0.3% at chi <= 0.3 and by 2.4-6% at chi = 0.45.
+# rounding of the energy: the reference's own trig and table paths differ by
+# 0.01-0.3% at chi <= 0.3 and by 2.4% (64 lanes) and 6.2% (1000 lanes) at chi = 0.45.
+# At chi = 0.45 the count falls with the floor the relax reaches: the reference's
+# trig path ends at mean Phi 1.09e-27 after 557k evaluations, its table path at
+# 5.8e-28 after 522k; a more accurate energy ends lower still and sooner, hence the
+# wider bound there (checked together with the floor itself, see floor_error).
 LBFGS_EVAL_RTOL = {(1000, 0.05): 0.015, (1000, 0.1): 0.015, (1000, 0.2): 0.015,
-                   (1000, 0.3): 0.015, (1000, 0.45): 0.04, (64, 0.05): 0.015, (64, 0.45): 0.07}
+                   (1000, 0.3): 0.015, (1000, 0.45): 0.12, (64, 0.05): 0.015, (64, 0.45): 0.12}
+
+
+def energy_extended(s: np.ndarray, nv: np.ndarray, w: np.ndarray, volume: float) -> np.ndarray:
+    """Phi of (B, N, d) configurations in 80-bit long double, phases reduced mod 1.
+
+    A check of energies at the round-off floor (Phi ~ 1e-28), where float64
+    evaluations of rho carry relative errors of order one.
+    """
+    n = nv.astype(np.longdouble)
+    two_pi = np.longdouble("6.28318530717958647692528676655900577")
+    out = np.empty(s.shape[0])
+    for i in range(s.shape[0]):
+        dot = s[i].astype(np.longdouble) @ n.T
+        th = two_pi * (dot - np.rint(dot))
+        re = np.cos(th).sum(axis=0)
+        im = np.sin(th).sum(axis=0)
+        out[i] = float((w * (re * re + im * im)).sum() / volume)
+    return out
 
 
 def check_lbfgs_outcome(chi: float, batch: int, n_evals: np.ndarray, energy: np.ndarray) -> str:
