@@ -82,6 +82,22 @@ int cco_value_grad_density_dev(cco_energy* e, const double* s_dev, int B, int N,
                                double* phi_dev, double* grad_dev, double* re_dev,
                                double* im_dev);
 
+/* Allocation-free variant of cco_value_grad_density_dev.  No counterpart in
+ * the reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, kernels and results as
+ * cco_value_grad_density_dev (FP64 only: an energy set to split precision
+ * gives CCO_ERR_STATE); the caller passes every input and output, and the
+ * partial sums live in a fixed device array of cco_noalloc_scratch_bytes()
+ * bytes that the library holds on each GPU until the process ends.  A batch
+ * needing more (about 16 B x N/128 x box entries for the rho partials) gives
+ * CCO_ERR_ARG before any work.  Every call on a GPU shares that GPU's array:
+ * calls that overlap on one GPU (two contexts, two threads) race. */
+int cco_value_grad_density_dev_noalloc(cco_energy* e, const double* s_dev, int B, int N,
+                                       double* phi_dev, double* grad_dev, double* re_dev,
+                                       double* im_dev);
+size_t cco_noalloc_scratch_bytes(void);
+
 /* Settings of the two minimizers (src/xla_cco/minimize/descent.py:52-63,
  * src/xla_cco/minimize/lbfgs.py:55-67). */
 typedef struct {

@@ -4,6 +4,7 @@
 #include <cmath>
 #include <cstring>
 #include <map>
+#include <mutex>
 #include <thread>
 #include <tuple>
 #include <vector>
@@ -26,6 +27,38 @@ static int fail(int code, const std::string& msg) {
 }  // namespace cco
 
 using namespace cco;
+
+// Device scratch of cco_value_grad_density_dev_noalloc: a file-scope array that
+// every GPU the library runs on holds from module load to process end.  Host
+// code must not pass `scratch` itself to a kernel (on the host it names a
+// stand-in, not device memory): scratch_ptr() gives the device's own address,
+// looked up once per device by scratch_setup() when a context is created.
+constexpr size_t kScratchBytes = size_t(64) << 20;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+void* g_scratch_ptr[kMaxDevices] = {};
+
+// With `device` current: record the device address of `scratch` on it.
+int scratch_setup(int device) {
+  if (device < 0 || device >= kMaxDevices) return fail(CCO_ERR_ARG, "device index beyond the scratch table");
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (g_scratch_ptr[device] != nullptr) return CCO_OK;
+  void* ptr = nullptr;
+  const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+  if (e != hipSuccess || ptr == nullptr)
+    return fail(CCO_ERR_DEVICE, std::string("hipGetSymbolAddress(scratch): ") + hipGetErrorString(e));
+  g_scratch_ptr[device] = ptr;
+  return CCO_OK;
+}
+
+void* scratch_ptr(int device) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  return (device >= 0 && device < kMaxDevices) ? g_scratch_ptr[device] : nullptr;
+}
+}  // namespace
 
 struct cco_ctx {
   int device;
@@ -319,15 +352,63 @@ __global__ void iota_kernel(int* a, int n, int* count) {
   if (i == 0) *count = n;
 }
 
+// Scratch of one batched evaluation: rho partials, C on the box, energy
+// partials of the tiled reduce and the identity lane list.
+struct EnergyWs {
+  double2* part;
+  double2* cbox;
+  double* rphi;
+  int* active;
+  int* n_active;
+};
+
+// Bytes of each EnergyWs region for B lanes of N particles (FP64 path).
+struct EnergyWsSizes {
+  size_t part, cbox, rphi, active, n_active;
+};
+
+EnergyWsSizes energy_ws_sizes(const Geo& g, int B, int N) {
+  EnergyWsSizes z;
+  z.part = sizeof(double2) * (size_t)B * n_splits_for(N) * g.Mbox;
+  z.cbox = sizeof(double2) * (size_t)B * g.Mbox;
+  z.rphi = sizeof(double) * (size_t)B * g.n_rtiles;
+  z.active = sizeof(int) * (size_t)B;
+  z.n_active = sizeof(int);
+  return z;
+}
+
+Batch energy_batch_desc(const Geo& g, const EnergyWs& ws, const double* s_dev, int B, int N,
+                        double* phi_dev, double* grad_dev, double* re_dev, double* im_dev) {
+  (void)B;
+  Batch b;
+  b.active = ws.active;
+  b.n_active = ws.n_active;
+  b.N = N;
+  b.lane_stride = N * g.d;
+  b.n_splits = n_splits_for(N);
+  b.split_len = split_len_for(N);
+  b.n_ftiles = n_ftiles_for(N);
+  b.s = s_dev;
+  b.grad = grad_dev;
+  b.phi = phi_dev;
+  b.re = re_dev;
+  b.im = im_dev;
+  b.part = ws.part;
+  b.cbox = ws.cbox;
+  b.phi_part = ws.rphi;
+  b.prec = nullptr;
+  return b;
+}
+
 int energy_batch(cco_energy* E, const double* s_dev, int B, int N, double* phi_dev,
                  double* grad_dev, double* re_dev, double* im_dev) {
   const Geo& g = E->geo;
-  const int nsp = n_splits_for(N);
+  const EnergyWsSizes z = energy_ws_sizes(g, B, N);
   int rc;
-  if ((rc = dev_reserve(E->part, sizeof(double2) * (size_t)B * nsp * g.Mbox))) return rc;
-  if ((rc = dev_reserve(E->cbox, sizeof(double2) * (size_t)B * g.Mbox))) return rc;
-  if ((rc = dev_reserve(E->active, sizeof(int) * (size_t)B))) return rc;
-  if ((rc = dev_reserve(E->n_active, sizeof(int)))) return rc;
+  if ((rc = dev_reserve(E->part, z.part))) return rc;
+  if ((rc = dev_reserve(E->cbox, z.cbox))) return rc;
+  if ((rc = dev_reserve(E->active, z.active))) return rc;
+  if ((rc = dev_reserve(E->n_active, z.n_active))) return rc;
   hipStream_t st = E->ctx->stream;
   if ((rc = reserve_reduce(E->rphi, B, g))) return rc;
   if (E->ident_B != B) {
@@ -336,23 +417,9 @@ int energy_batch(cco_energy* E, const double* s_dev, int B, int N, double* phi_d
     CCO_HIP(hipGetLastError());
     E->ident_B = B;
   }
-  Batch b;
-  b.active = (const int*)E->active.p;
-  b.n_active = (const int*)E->n_active.p;
-  b.N = N;
-  b.lane_stride = N * g.d;
-  b.n_splits = nsp;
-  b.split_len = split_len_for(N);
-  b.n_ftiles = n_ftiles_for(N);
-  b.s = s_dev;
-  b.grad = grad_dev;
-  b.phi = phi_dev;
-  b.re = re_dev;
-  b.im = im_dev;
-  b.part = (double2*)E->part.p;
-  b.cbox = (double2*)E->cbox.p;
-  b.phi_part = (double*)E->rphi.p;
-  b.prec = nullptr;
+  EnergyWs ws{(double2*)E->part.p, (double2*)E->cbox.p, (double*)E->rphi.p, (int*)E->active.p,
+              (int*)E->n_active.p};
+  Batch b = energy_batch_desc(g, ws, s_dev, B, N, phi_dev, grad_dev, re_dev, im_dev);
   if (E->precision == 1 && g.mp_ok) {
     if ((rc = dev_reserve(E->prec, sizeof(int) * (size_t)B))) return rc;
     if ((rc = dev_reserve(E->cpl, mp_cplane_bytes(g) * (size_t)B))) return rc;
@@ -787,6 +854,7 @@ int cco_ctx_create(int device, cco_ctx** out) {
     return fail(CCO_ERR_DEVICE, "no GPU visible to the HIP runtime");
   if (device < 0 || device >= c) return fail(CCO_ERR_ARG, "device index out of range");
   CCO_HIP(hipSetDevice(device));
+  if (const int rc = scratch_setup(device)) return rc;
   cco_ctx* ctx = new cco_ctx();
   ctx->device = device;
   hipError_t e = hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking);
@@ -961,6 +1029,48 @@ int cco_value_grad_density_dev(cco_energy* E, const double* s_dev, int B, int N,
     return fail(CCO_ERR_ARG, "re and im must both be given or both be NULL");
   DevScope ds(E->ctx->device);
   return energy_batch(E, s_dev, B, N, phi_dev, grad_dev, re_dev, im_dev);
+}
+
+size_t cco_noalloc_scratch_bytes(void) { return kScratchBytes; }
+
+int cco_value_grad_density_dev_noalloc(cco_energy* E, const double* s_dev, int B, int N,
+                                       double* phi_dev, double* grad_dev, double* re_dev,
+                                       double* im_dev) {
+  if (!E || !s_dev || !phi_dev) return fail(CCO_ERR_ARG, "NULL argument");
+  if (B < 1 || N < 1) return fail(CCO_ERR_ARG, "need B >= 1 and N >= 1");
+  if ((re_dev == nullptr) != (im_dev == nullptr))
+    return fail(CCO_ERR_ARG, "re and im must both be given or both be NULL");
+  const Geo& g = E->geo;
+  if (E->precision == 1 && g.mp_ok)
+    return fail(CCO_ERR_STATE, "the allocation-free evaluation is FP64 only: set the energy's precision to fp64");
+  const EnergyWsSizes z = energy_ws_sizes(g, B, N);
+  auto up = [](size_t v) { return (v + 255) & ~size_t(255); };
+  const size_t need = up(z.part) + up(z.cbox) + up(z.rphi) + up(z.active) + up(z.n_active);
+  if (need > kScratchBytes)
+    return fail(CCO_ERR_ARG, "batch needs " + std::to_string(need) + " bytes of scratch, more than the " +
+                                 std::to_string(kScratchBytes) + " of the allocation-free evaluation");
+  DevScope ds(E->ctx->device);
+  unsigned char* base = (unsigned char*)scratch_ptr(E->ctx->device);
+  if (base == nullptr) return fail(CCO_ERR_STATE, "scratch not set up on this device");
+  EnergyWs ws;
+  size_t at = 0;
+  ws.part = (double2*)(base + at);
+  at += up(z.part);
+  ws.cbox = (double2*)(base + at);
+  at += up(z.cbox);
+  ws.rphi = (double*)(base + at);
+  at += up(z.rphi);
+  ws.active = (int*)(base + at);
+  at += up(z.active);
+  ws.n_active = (int*)(base + at);
+  hipStream_t st = E->ctx->stream;
+  // the lane list and its count are rewritten on every call: the previous
+  // call's contents (another B) must not be read
+  hipLaunchKernelGGL(iota_kernel, dim3((B + 255) / 256), dim3(256), 0, st, ws.active, B, ws.n_active);
+  Batch b = energy_batch_desc(g, ws, s_dev, B, N, phi_dev, grad_dev, re_dev, im_dev);
+  launch_energy(g, b, B, grad_dev != nullptr, st);
+  CCO_HIP(hipGetLastError());
+  return CCO_OK;
 }
 
 int cco_value_grad_density(cco_energy* E, const double* s_host, int B, int N, double* phi_host,

@@ -149,6 +149,8 @@ SIGNATURES = {
     "cco_energy_info": (C.c_int, [_P, _PI, _PI, _PI, _PD, _PI]),
     "cco_value_grad_density": (C.c_int, [_P, _PD, C.c_int, C.c_int, _PD, _PD, _PD, _PD]),
     "cco_value_grad_density_dev": (C.c_int, [_P, _P, C.c_int, C.c_int, _P, _P, _P, _P]),
+    "cco_value_grad_density_dev_noalloc": (C.c_int, [_P, _P, C.c_int, C.c_int, _P, _P, _P, _P]),
+    "cco_noalloc_scratch_bytes": (C.c_size_t, []),
     "cco_relax_create": (
         C.c_int,
         [
