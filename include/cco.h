@@ -193,7 +193,9 @@ int cco_relax_get_option(cco_relax* r, int key, int64_t* value);
  * bits) and FP32 accumulation; the density partials, Phi and every minimizer
  * and integrator update stay FP64.  L-BFGS lanes start split and turn FP64 for
  * good at the first iteration whose energy is <= switch_phi (1e-8 by default;
- * 0 keeps them split); MD lanes stay split.  mode 0: FP64 (the default).
+ * 0 keeps them split); DOPRI5 gradient-flow lanes likewise at the first step
+ * that starts at energy <= switch_phi (ROS2 flows stay FP64); MD lanes stay
+ * split.  mode 0: FP64 (the default).
  * Takes effect at the next init; a k-set that does not fit the MFMA kernels
  * (more than 48 rows per column) runs FP64. */
 int cco_relax_set_precision(cco_relax* r, int mode, double switch_phi);

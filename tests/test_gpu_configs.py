@@ -1,0 +1,237 @@
+"""The batching, precision and MD workloads at their full sizes.
+
+* 512 uneven lanes (N ~ U{100..2000}, d ~ U{2,3}, chi ~ U{0.1,0.2,0.3,0.45}) in one
+  FP64 session: success count, the oracle's energy at every sampled output, and
+  lane-by-lane agreement with the oracle's L-BFGS on a short budget;
+* split precision at d=3, N=4000, chi=0.3, 64 lanes: the FP64 path's success set,
+  and the error of the split evaluation against the FP64 path and the oracle at the
+  start, mid-relax and just above the 1e-8 switch;
+* split precision at d=2, N=1000, chi=0.45, 256 lanes: the FP64 path's success set
+  and ensemble S(k) and g2 within their standard errors;
+* split-precision DOPRI5 flow and NVE Verlet at d=3, N=10^4, chi=0.45, 32 lanes: the
+  flow against the FP64 flow, then NVE from relaxed states with the energy drift of
+  both paths.
+"""
+
+from __future__ import annotations
+
+import math
+
+import numpy as np
+import pytest
+
+from oracle import c_oracle
+from oracle import oracle as O
+from xla_cco import kvectors as K
+from xla_cco.analysis.pairs import g2_from_histogram, pair_distance_histogram
+from xla_cco.analysis.structure import isotropic_structure_factor
+from xla_cco.box import cubic_box
+from xla_cco.energy import make_stealthy_energy
+from xla_cco.minimize import LBFGSSettings, relax, relax_uneven
+from xla_cco.minimize.flow import FlowSettings, gradient_flow_batch
+
+pytestmark = pytest.mark.gpu
+
+TOL = 9.99999968e-21
+
+
+def _system(n: int, d: int, chi: float):
+    box = cubic_box(n, d)
+    ks = K.enumerate_kvectors(box.basis, 0.0, K.k_outer_from_ka(K.ka_from_chi(chi, 0.05, d), 0.05, d))
+    return box, ks
+
+
+# ---------------- 512 uneven lanes ----------------
+
+
+def _uneven_specs(b: int = 512, seed: int = 2024):
+    rng = np.random.default_rng(seed)
+    return [(int(rng.integers(100, 2001)), int(rng.integers(2, 4)), float(rng.choice([0.1, 0.2, 0.3, 0.45])))
+            for _ in range(b)]
+
+
+def _uneven_batch():
+    specs = _uneven_specs()
+    cache = {}
+    es, s0 = [], []
+    for i, (n, d, chi) in enumerate(specs):
+        if (n, d, chi) not in cache:
+            box, ks = _system(n, d, chi)
+            cache[(n, d, chi)] = (make_stealthy_energy(box, ks), box, ks)
+        es.append(cache[(n, d, chi)][0])
+        s0.append(np.random.default_rng(5000 + i).uniform(size=(n, d)))
+    return specs, cache, es, s0
+
+
+def _sample(specs, count: int, max_n: int, seed: int):
+    rng = np.random.default_rng(seed)
+    idx = [i for i in range(len(specs)) if specs[i][0] <= max_n]
+    return sorted(rng.choice(idx, size=count, replace=False).tolist())
+
+
+def test_uneven_512_lanes_converge_and_match_the_oracle(gpu_ctx) -> None:
+    specs, cache, es, s0 = _uneven_batch()
+    out = relax_uneven(es, s0, LBFGSSettings(energy_tol=TOL))
+    ok = np.array([float(r.energy) < TOL for r in out])
+    print(f"uneven 512: {ok.sum()}/512 below {TOL:g}; statuses "
+          f"{np.bincount([int(r.status) for r in out], minlength=8)}", flush=True)
+    assert ok.sum() >= 500
+    # the oracle's energy at sampled outputs (any size) confirms the success flags
+    for i in _sample(specs, 12, 2000, 3):
+        n, d, chi = specs[i]
+        _, box, ks = cache[(n, d, chi)]
+        phi, _ = c_oracle.energy_grad_batch(out[i].s[None], ks.n, ks.weights, box.volume, threads=8)
+        assert (phi[0] < 10 * TOL) == bool(ok[i]), (i, specs[i], phi[0], float(out[i].energy))
+
+
+def test_uneven_512_lanes_follow_the_oracle_lane_by_lane(gpu_ctx) -> None:
+    """A short budget (8 evaluations, one pass) keeps energies far above round-off,
+    where the device takes the oracle's line-search decisions."""
+    specs, cache, es, s0 = _uneven_batch()
+    st = LBFGSSettings(max_evals_per_pass=8, rescales=(1e4,))
+    out = relax_uneven(es, s0, st)
+    ost = O.LBFGSSettings(max_evals_per_pass=8, rescales=(1e4,))
+    for i in _sample(specs, 16, 900, 4):
+        n, d, chi = specs[i]
+        _, box, ks = cache[(n, d, chi)]
+        _, e, ne, ni, sts = c_oracle.relax(s0[i][None], ks.n, ks.weights, box.volume, ost, threads=1)
+        r = out[i]
+        assert (int(r.n_evals), int(r.n_iters), int(r.status)) == (ne[0], ni[0], sts[0]), (i, specs[i])
+        assert abs(float(r.energy) - e[0]) <= 1e-9 * e[0], (i, specs[i], float(r.energy), e[0])
+
+
+# ---------------- split precision, d=3 N=4000 ----------------
+
+
+def _mixed_errors(box, ks, s: np.ndarray):
+    e64 = make_stealthy_energy(box, ks)
+    emp = make_stealthy_energy(box, ks)
+    assert emp.set_precision("mixed")
+    p64, g64 = e64.value_and_grad_batch(s)
+    pmp, gmp = emp.value_and_grad_batch(s)
+    po, go = c_oracle.energy_grad_batch(s, ks.n, ks.weights, box.volume, threads=8)
+    return p64, g64, pmp, gmp, po, go
+
+
+def test_split_precision_d3_n4000_success_set_and_switch_errors(gpu_ctx) -> None:
+    box, ks = _system(4000, 3, 0.3)
+    assert ks.n.shape[0] == 3561
+    e = make_stealthy_energy(box, ks)
+    s0 = np.random.default_rng(44).uniform(size=(64, 4000, 3))
+    r64 = relax(e, s0, LBFGSSettings(energy_tol=TOL))
+    rmp = relax(e, s0, LBFGSSettings(energy_tol=TOL, precision="mixed"))
+    ok64, okmp = r64.energy < TOL, rmp.energy < TOL
+    print(f"d=3 N=4000 x64: FP64 {ok64.sum()}/64, split {okmp.sum()}/64 below {TOL:g}; "
+          f"evaluations {int(r64.n_evals.sum())} / {int(rmp.n_evals.sum())}", flush=True)
+    assert np.array_equal(ok64, okmp) and ok64.all()
+    # configurations of lane 0 along its FP64 relax: start, Phi ~ 1e-2 and just above the switch
+    one = relax(e, s0[:1], LBFGSSettings(energy_tol=TOL, record_every=4, max_records=600))
+    tr = one.trajectory
+    energies = np.asarray(tr.energy[0][: int(tr.count[0])])
+    picks = [0, int(np.argmax(energies < 1e-2)), int(np.argmax(energies < 1e-7))]
+    s = np.stack([np.asarray(tr.s[0][j]) for j in picks])
+    p64, g64, pmp, gmp, po, go = _mixed_errors(box, ks, s)
+    m, vol, n = ks.n.shape[0], box.volume, 4000
+    # absolute density errors per k: split ~1e-6 sqrt(N) (fp16 hi/lo operands, FP32
+    # sums), FP64 and the trig oracle ~1e-11; Phi's error is then at most
+    # (2 sqrt(Phi V M) drho + M drho^2) / V (Cauchy-Schwarz over the k-set)
+    def bound(phi: float, drho: float) -> float:
+        return (2.0 * math.sqrt(phi * vol * m) * drho + m * drho * drho) / vol
+
+    for j, (a, b, o) in enumerate(zip(p64, pmp, po)):
+        gerr = np.max(np.abs(gmp[j] - g64[j])) / np.max(np.abs(g64[j]))
+        print(f"  Phi_fp64 {a:.3e}: split rel. error {abs(b - a) / a:.2e} (vs oracle {abs(b - o) / o:.2e}, "
+              f"model bound {bound(a, 1e-6 * math.sqrt(n)) / a:.1e}), force error / max|g| {gerr:.2e}; "
+              f"FP64 vs oracle {abs(a - o) / o:.1e}", flush=True)
+        assert abs(a - o) <= 1e-12 * o + bound(a, 1e-11)
+        assert abs(b - a) <= 4.0 * bound(a, 1e-6 * math.sqrt(n))
+        if j == 0:
+            assert abs(b - a) <= 2e-5 * a and gerr <= 2e-4
+
+
+# ---------------- split precision ensemble at d=2 N=1000 ----------------
+
+
+def test_split_precision_ensemble_sk_and_g2_match_fp64(gpu_ctx) -> None:
+    box, ks = _system(1000, 2, 0.45)
+    e = make_stealthy_energy(box, ks)
+    s0 = np.random.default_rng(99).uniform(size=(256, 1000, 2))
+    r64 = relax(e, s0, LBFGSSettings(energy_tol=TOL))
+    rmp = relax(e, s0, LBFGSSettings(energy_tol=TOL, precision="mixed"))
+    assert np.array_equal(r64.energy < TOL, rmp.energy < TOL)
+    assert np.all(r64.energy < TOL)
+    # S(k): per-bin means within their combined standard errors
+    k64, s64, _, es64 = isotropic_structure_factor(r64.s, box, 5.0, 40.0, 0.05)
+    kmp, smp, _, esmp = isotropic_structure_factor(rmp.s, box, 5.0, 40.0, 0.05)
+    assert np.allclose(k64, kmp)
+    se = np.sqrt(es64**2 + esmp**2)
+    big = se > 0
+    z = np.abs(s64 - smp)[big] / se[big]
+    print(f"S(k): {big.sum()} bins, max |z| {z.max():.2f}, share |z| > 3: {(z > 3).mean():.4f}", flush=True)
+    assert (z > 3).mean() <= 0.02 and np.median(z) < 1.0
+    # below the constraint radius S = 0 on both paths (Phi < 1e-20)
+    inside = k64 < 0.95 * K.k_outer_from_ka(K.ka_from_chi(0.45, 0.05, 2), 0.05, 2)
+    assert np.all(s64[inside] < 1e-20) and np.all(smp[inside] < 1e-20)
+    # g2 on [0, 0.2 L] with 200 bins: per-configuration histograms, mean and standard error
+    L = box.basis[0, 0]
+    edges = np.linspace(0.0, 0.2 * L, 201)
+
+    def g2_lanes(s):
+        return np.stack([g2_from_histogram(pair_distance_histogram(s[i:i + 1], box, edges), edges, 1000, 1,
+                                           box.volume, 2) for i in range(s.shape[0])])
+
+    a, b = g2_lanes(r64.s), g2_lanes(rmp.s)
+    se = np.sqrt(a.var(axis=0, ddof=1) / a.shape[0] + b.var(axis=0, ddof=1) / b.shape[0])
+    nz = se > 0
+    z = np.abs(a.mean(axis=0) - b.mean(axis=0))[nz] / se[nz]
+    print(f"g2: {nz.sum()} bins, max |z| {z.max():.2f}, share |z| > 3: {(z > 3).mean():.4f}", flush=True)
+    assert (z > 3).mean() <= 0.03 and np.median(z) < 1.0
+
+
+# ---------------- split precision flow + NVE at d=3 N=10^4 ----------------
+
+
+def test_split_precision_flow_and_nve_d3_n10000(gpu_ctx) -> None:
+    from xla_cco.dynamics import MDSettings, Waves, init_state, prng_key, simulate
+
+    box, ks = _system(10_000, 3, 0.45)
+    assert ks.n.shape[0] == 13540
+    e = make_stealthy_energy(box, ks)
+    s0 = np.random.default_rng(41).uniform(size=(32, 10_000, 3))
+    # the step-size control reacts to evaluation errors, so the two paths are
+    # compared at equal flow time: t_max = the shortest time 30 FP64 steps reach
+    first = gradient_flow_batch(e, s0, box.basis, FlowSettings(max_steps=30))
+    t_end = float(first.t.min())
+    fl = {}
+    for prec in ("fp64", "mixed"):
+        fl[prec] = gradient_flow_batch(e, s0, box.basis, FlowSettings(t_max=t_end, precision=prec))
+    a, b = fl["fp64"], fl["mixed"]
+    rel = np.abs(b.energy - a.energy) / a.energy
+    print(f"flow to t={t_end:.4e}: Phi {a.energy.mean():.6e} (FP64) vs {b.energy.mean():.6e} (split), "
+          f"max rel. difference {rel.max():.2e}; steps {int(a.n_steps.sum())} / {int(b.n_steps.sum())}",
+          flush=True)
+    assert np.all(a.t == t_end) and np.all(b.t == t_end)
+    assert rel.max() <= 1e-6
+    # ground states from the split flow's end points, then NVE at T = 1e-4
+    gs = relax(e, b.s, LBFGSSettings(energy_tol=TOL))
+    assert np.all(gs.energy < TOL)
+    r0 = gs.s @ box.basis
+    v0 = np.random.default_rng(42).normal(scale=math.sqrt(1e-4), size=r0.shape)
+    nv = np.asarray(ks.n, dtype=np.float64)
+    waves = Waves(n_vectors=nv, kvec=nv @ (2.0 * np.pi * np.linalg.inv(box.basis).T),
+                  constrained=np.ones(nv.shape[0], bool), n_constrained=nv.shape[0])
+    keys = np.stack([prng_key(i) for i in range(32)])
+    md = {}
+    for prec in ("fp64", "mixed"):
+        st = MDSettings(dt=0.01, remove_com=False, drift_tol=math.inf, precision=prec)
+        lanes = init_state(e, waves, box.basis, r0, v0, 1e-4, keys, st)
+        md[prec] = simulate(e, waves, box.basis, st, lanes, 300).state
+    a, m = md["fp64"], md["mixed"]
+    kin = 0.5 * np.sum(v0**2, axis=(1, 2))
+    print(f"NVE 300 steps: max drift / KE0 {np.max(a.max_drift / kin):.2e} (FP64), "
+          f"{np.max(m.max_drift / kin):.2e} (split); max |r_split - r_fp64| {np.max(np.abs(m.r - a.r)):.2e}",
+          flush=True)
+    assert np.all(m.step == 300) and np.all(m.status == 0) and np.all(a.status == 0)
+    assert np.all(a.max_drift < 1e-3 * kin)
+    assert np.all(m.max_drift < 1e-2 * kin)
+    assert np.max(np.abs(m.r - a.r)) < 1e-3

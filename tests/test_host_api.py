@@ -6,6 +6,8 @@ names a user of the reference imports; each must resolve here (extras allowed).
 
 from __future__ import annotations
 
+import pytest
+
 import xla_cco
 import xla_cco.dynamics as dyn
 
@@ -54,3 +56,18 @@ def test_auto_chunk_size_and_cli_helpers() -> None:
     assert np.array_equal(w, [[[0.5, 0.75], [0.0, 0.999]]])
     with pytest.raises(ValueError):
         select_device("cpu")
+
+
+def test_flow_precision_settings_are_validated_before_device_work() -> None:
+    """precision='mixed' needs DOPRI5; bad names and a negative switch raise ValueError."""
+    import numpy as np
+
+    from xla_cco.minimize.flow import FlowSettings, _validate
+
+    s0 = np.zeros((1, 4, 2))
+    basis = np.eye(2)
+    _validate(s0, basis, FlowSettings(precision="mixed"))
+    for bad in (FlowSettings(precision="half"), FlowSettings(precision="mixed", method="rosenbrock"),
+                FlowSettings(precision="mixed", precision_switch=-1.0)):
+        with pytest.raises(ValueError):
+            _validate(s0, basis, bad)

@@ -725,7 +725,8 @@ int reserve_mp_planes(cco_relax* R) {
 
 int* resolve_prec(const cco_relax* R) {
   if (R->precision != 1 || R->E == nullptr || !R->E->geo.mp_ok || R->uneven) return nullptr;
-  if (R->method != CCO_METHOD_LBFGS && R->method != METHOD_MD) return nullptr;
+  if (R->method != CCO_METHOD_LBFGS && R->method != METHOD_MD && R->method != METHOD_FLOW) return nullptr;
+  if (R->method == METHOD_FLOW && R->ros2) return nullptr;  // ROS2 evaluates in FP64
   return R->prec_store;
 }
 
@@ -1519,11 +1520,12 @@ int cco_relax_step(cco_relax* R, int64_t n_evals, int* n_active) {
 int cco_relax_set_precision(cco_relax* R, int mode, double switch_phi) {
   if (!R) return fail(CCO_ERR_ARG, "relax is NULL");
   if (mode != 0 && mode != 1) return fail(CCO_ERR_ARG, "precision mode must be 0 (FP64) or 1 (split)");
-  if (mode == 1 && R->method != CCO_METHOD_LBFGS && R->method != METHOD_MD)
-    return fail(CCO_ERR_ARG, "split precision is available for L-BFGS and MD sessions");
+  if (mode == 1 && R->method != CCO_METHOD_LBFGS && R->method != METHOD_MD && R->method != METHOD_FLOW)
+    return fail(CCO_ERR_ARG, "split precision is available for L-BFGS, gradient-flow and MD sessions");
   if (!(switch_phi >= 0.0)) return fail(CCO_ERR_ARG, "switch_phi must be >= 0");
   R->precision = mode;
   R->lc.mixed_switch = switch_phi;
+  R->fc.mixed_switch = switch_phi;
   return CCO_OK;
 }
 

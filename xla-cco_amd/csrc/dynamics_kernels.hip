@@ -614,6 +614,9 @@ __global__ __launch_bounds__(kDynBlock) void flow_advance_kernel(RelaxDev R, Flo
   if (threadIdx.x == 0) {
     R.fl[lane] = L;
     R.status[lane] = code;
+    // a split-precision lane turns FP64 for good when a step starts below the switch
+    if (start_attempt && R.prec != nullptr && c.mixed_switch > 0.0 && L.f <= c.mixed_switch)
+      R.prec[lane] = 0;
   }
 }
 

@@ -134,6 +134,7 @@ struct FlowCfg {
   int d, arclength;
   int record_every, record_cap;
   const double* dt0;     // optional per-lane first step (nullptr: dt_init)
+  double mixed_switch;   // split-precision lanes turn FP64 when a step starts at Phi <= this (0: never)
 };
 
 struct FlowLane {

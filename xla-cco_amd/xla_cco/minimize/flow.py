@@ -48,6 +48,11 @@ class FlowSettings:
     grad_tol: float = 0.0
     record_every: int = 0
     max_records: int = 0
+    # Split precision (no reference counterpart; the reference is FP64): "mixed"
+    # evaluates rho and the forces on the matrix cores (fp16 hi/lo operands, FP32
+    # accumulation) until a step starts at Phi <= precision_switch, then FP64.
+    precision: str = "fp64"
+    precision_switch: float = 1e-8
 
 
 @dataclasses.dataclass
@@ -104,6 +109,12 @@ def _validate(s0: np.ndarray, basis: np.ndarray, settings: FlowSettings) -> None
         raise ValueError("record_every and max_records must be >= 0")
     if (settings.record_every > 0) != (settings.max_records > 1):
         raise ValueError("recording needs record_every > 0 and max_records >= 2")
+    if settings.precision not in ("fp64", "mixed"):
+        raise ValueError(f"precision must be 'fp64' or 'mixed', got {settings.precision!r}")
+    if settings.precision == "mixed" and settings.method != "dopri5":
+        raise ValueError("precision='mixed' is available for method='dopri5'")
+    if not settings.precision_switch >= 0.0:
+        raise ValueError("precision_switch must be >= 0")
 
 
 
@@ -146,6 +157,10 @@ def gradient_flow_batch(
                     f"{type(energy).__name__} is evaluated on the host"
                 )
             _native.check(L.cco_flow_set_method(h, 1))
+        if settings.precision == "mixed":
+            if not builtin:
+                raise ValueError("precision='mixed' needs a StealthyEnergy (its kernels run on the GPU)")
+            _native.check(L.cco_relax_set_precision(h, 1, float(settings.precision_switch)))
         dt0a = None
         if dt0 is not None:
             dt0a = np.ascontiguousarray(np.broadcast_to(np.asarray(dt0, dtype=np.float64), (b,)))
