@@ -1,12 +1,17 @@
-"""Single MD steps on the host with a caller-supplied force function.
+"""Single MD steps on the host, with a caller-supplied force function.
 
-src/xla_cco/dynamics/integrators.py:83-176.  ``verlet_step``: v' = v + dt/2 F;
-r' = r + dt v'; F' = F(r'); v'' = v' + dt/2 F'.  ``andersen_reset`` and
-``baoab_step`` thread a Philox key (xla_cco.dynamics.rng) as the reference
-threads its JAX key.  The batched device loop lives in
-``xla_cco.dynamics.run.simulate`` and applies the same rules with the same
-draws; these host forms serve single steps with ``make_force_fn`` (GPU
-forces) or any other force function.
+The steps are compositions of three maps on (r, v) with unit masses:
+
+* kick  B(h): v <- v + h F
+* drift A(h): r <- r + h v
+* Ornstein-Uhlenbeck O: v <- c v + sqrt((1 - c^2) T) xi, c = exp(-gamma dt)
+
+velocity Verlet is B(dt/2) A(dt) [force] B(dt/2); BAOAB is B(dt/2) A(dt/2) O A(dt/2)
+[force] B(dt/2).  The reference's names and arguments are kept
+(src/xla_cco/dynamics/integrators.py:83-176).  Random draws come from the Philox
+streams of xla_cco.dynamics.rng in the order the MD kernels use, so a host step and
+a device step of one lane draw the same numbers; the reference threads a JAX key the
+same way.  The batched device loop is ``xla_cco.dynamics.run.simulate``.
 """
 
 from __future__ import annotations
@@ -24,62 +29,74 @@ ForceOut = Tuple[float, np.ndarray, np.ndarray, np.ndarray]
 ForceFn = Callable[[np.ndarray], ForceOut]
 
 
-def _check_phase_space(r: np.ndarray, v: np.ndarray, f: np.ndarray, dt: float) -> None:
-    for x, name in ((r, "r"), (v, "v"), (f, "f")):
-        require_float64(x, name)
-    if r.ndim != 2 or v.shape != r.shape or f.shape != r.shape:
-        raise ValueError(f"r, v, f must share an (n, d) shape: {r.shape}, {v.shape}, {f.shape}")
-    if not (math.isfinite(dt) and dt > 0.0):
-        raise ValueError(f"dt must be finite and positive, got {dt}")
+def _kick(v: np.ndarray, f: np.ndarray, h: float) -> np.ndarray:
+    return v + h * f
+
+
+def _drift(r: np.ndarray, v: np.ndarray, h: float) -> np.ndarray:
+    return r + h * v
+
+
+def _forces(force_fn: ForceFn, r: np.ndarray) -> ForceOut:
+    out = force_fn(r)
+    if len(out) != 4:
+        raise ValueError("a force function returns (Phi, F, re, im)")
+    return out
+
+
+def _phase_space(r: np.ndarray, v: np.ndarray, f: np.ndarray, dt: float) -> None:
+    """float64 (n, d) positions, velocities and forces of one shape; finite dt > 0."""
+    require_float64(r, "r")
+    require_float64(v, "v")
+    require_float64(f, "f")
+    if r.ndim != 2 or r.shape != v.shape or r.shape != f.shape:
+        raise ValueError(f"r, v and f must all be (n, d); got {r.shape}, {v.shape}, {f.shape}")
+    if not (dt > 0.0 and math.isfinite(dt)):
+        raise ValueError(f"the time step must be positive and finite, got {dt}")
 
 
 def verlet_step(
     r: np.ndarray, v: np.ndarray, f: np.ndarray, force_fn: ForceFn, dt: float
 ) -> Tuple[np.ndarray, np.ndarray, ForceOut]:
-    """One velocity-Verlet step; returns (r', v', force_fn(r'))."""
-    _check_phase_space(r, v, f, dt)
-    half = 0.5 * dt
-    v_half = v + half * f
-    r_new = r + dt * v_half
-    out = force_fn(r_new)
-    assert len(out) == 4, "force_fn must return (Phi, F, re, im)"
-    v_new = v_half + half * out[1]
-    return r_new, v_new, out
+    """Velocity Verlet: (r', v', force_fn(r'))."""
+    _phase_space(r, v, f, dt)
+    v = _kick(v, f, 0.5 * dt)
+    r = _drift(r, v, dt)
+    out = _forces(force_fn, r)
+    return r, _kick(v, out[1], 0.5 * dt), out
 
 
 def andersen_reset(
     key: np.ndarray, v: np.ndarray, reset_acc: float, temperature: float, p: float, mode: str
 ) -> Tuple[np.ndarray, np.ndarray, float, int]:
-    """Andersen resets after a step; returns (key', v', reset_acc', n_reset) (integrators.py:99-140).
+    """Andersen thermostat after a step: (key', v', reset_acc', number of resets).
 
-    Keys: ``key', k_pick, k_draw, k_coin = split(key, 4)``.  "count": the
-    accumulator gains p and fires (losing 1) when it reaches 1; "bernoulli":
-    fires when uniform(k_coin) < p, the accumulator unchanged.  A firing
-    redraws particle randint(k_pick, n) as sqrt(T) normals(k_draw, d).  The
-    MD kernels draw the same numbers (xla_cco.dynamics.rng).
+    The key splits into (key', pick, draw, coin).  Mode "count" fires
+    deterministically every 1/p steps (an accumulator gains p per step and gives
+    up 1 when it fires); mode "bernoulli" fires when uniform(coin) < p.  A firing
+    gives particle randint(pick, n) the velocity sqrt(T) normals(draw, d).
     """
     if mode not in ANDERSEN_MODES:
-        raise ValueError(f"mode must be one of {ANDERSEN_MODES}, got {mode!r}")
-    if not 0.0 <= p <= 1.0:
-        raise ValueError(f"p must lie in [0, 1], got {p}")
+        raise ValueError(f"unknown Andersen mode {mode!r} (expected one of {ANDERSEN_MODES})")
+    if not (0.0 <= p <= 1.0):
+        raise ValueError(f"the reset probability must be in [0, 1], got {p}")
     rng.require_key(key)
     require_float64(v, "v")
     if v.ndim != 2:
-        raise ValueError(f"v must have shape (n, d), got {v.shape}")
-    n, d = v.shape
+        raise ValueError(f"v must be (n, d), got {v.shape}")
     key, k_pick, k_draw, k_coin = rng.split(key, 4)
     acc = float(reset_acc)
     if mode == "count":
-        acc = acc + p
+        acc += p
         fire = acc >= 1.0
-        if fire:
-            acc = acc - 1.0
+        acc = acc - 1.0 if fire else acc
     else:
-        fire = rng.uniform(k_coin) < p
-    v_new = np.array(v, copy=True)
+        fire = bool(rng.uniform(k_coin) < p)
+    out = v.copy()
     if fire:
-        v_new[rng.randint(k_pick, n)] = math.sqrt(float(temperature)) * rng.normals(k_draw, d)
-    return key, v_new, acc, int(fire)
+        n, d = v.shape
+        out[rng.randint(k_pick, n)] = math.sqrt(float(temperature)) * rng.normals(k_draw, d)
+    return key, out, acc, int(fire)
 
 
 def baoab_step(
@@ -92,26 +109,22 @@ def baoab_step(
     gamma: float,
     temperature: float,
 ) -> Tuple[np.ndarray, np.ndarray, np.ndarray, ForceOut]:
-    """One BAOAB Langevin step; returns (key', r', v', force_fn(r')) (integrators.py:143-176).
+    """BAOAB Langevin step: (key', r', v', force_fn(r')).
 
-    B (half kick), A (half drift), O (v = c v + sqrt((1 - c^2) T) xi with
-    c = exp(-gamma dt)), A, force, B; ``key', k_xi = split(key)`` and xi the
-    first n d normals of k_xi in row-major (particle, axis) order.
+    The key splits into (key', xi); the O map uses the first n d normals of xi in
+    (particle, axis) order, and 1 - c^2 is taken as -expm1(-2 gamma dt).
     """
-    _check_phase_space(r, v, f, dt)
-    if not (math.isfinite(gamma) and gamma >= 0.0):
-        raise ValueError(f"gamma must be finite and >= 0, got {gamma}")
+    _phase_space(r, v, f, dt)
+    if not (gamma >= 0.0 and math.isfinite(gamma)):
+        raise ValueError(f"the friction must be finite and >= 0, got {gamma}")
     rng.require_key(key)
-    c = math.exp(-gamma * dt)
-    noise_var = -math.expm1(-2.0 * gamma * dt)  # 1 - c^2 without cancellation
-    half = 0.5 * dt
     key, k_xi = rng.split(key, 2)
-    v = v + half * f
-    r = r + half * v
-    xi = rng.normals(k_xi, r.size).reshape(r.shape)
-    v = c * v + math.sqrt(noise_var * float(temperature)) * xi
-    r = r + half * v
-    out = force_fn(r)
-    assert len(out) == 4, "force_fn must return (Phi, F, re, im)"
-    v = v + half * out[1]
-    return key, r, v, out
+    c = math.exp(-gamma * dt)
+    sigma = math.sqrt(-math.expm1(-2.0 * gamma * dt) * float(temperature))
+    h = 0.5 * dt
+    v = _kick(v, f, h)
+    r = _drift(r, v, h)
+    v = c * v + sigma * rng.normals(k_xi, r.size).reshape(r.shape)
+    r = _drift(r, v, h)
+    out = _forces(force_fn, r)
+    return key, r, _kick(v, out[1], h), out

@@ -1,10 +1,16 @@
-"""Density waves along MD trajectories: rho_k, centre-of-mass correction, winding.
+"""Host forms of the MD density-wave observables (one lane).
 
-src/xla_cco/dynamics/observables.py:37-125, one lane on the host.  The MD
-kernels apply the same rules to every lane on the device each step; these
-functions serve the host-side bookkeeping (initial rho, pending rows) and
-users.  rho_k = sum_j exp(i k . r_j) for Cartesian r and k, stored as
-(..., 2) = (Re, Im).
+The MD kernels evaluate these quantities for every lane on the device each step;
+the host forms serve the set-up of a run (initial densities, pending rows) and
+users of the reference's API (src/xla_cco/dynamics/observables.py:37-125, same
+names and arguments).  They are written with complex arithmetic:
+
+* rho_k = sum_j exp(i k . r_j) for Cartesian r and k, returned as (w, 2) = (Re, Im);
+* the centre-of-mass correction multiplies rho_k by exp(-i k . R_cm), which makes
+  it invariant under a rigid translation of every particle;
+* the winding of wave k over a step is the angle of rho_new conj(rho_old), in
+  (-pi, pi], counted only while S_k = |rho_k|^2 / n stays above a floor at both ends
+  (below it the phase carries no signal).
 """
 
 from __future__ import annotations
@@ -16,36 +22,46 @@ import numpy as np
 from xla_cco._types import require_float64
 
 
+def _as_complex(rho: np.ndarray) -> np.ndarray:
+    return rho[:, 0] + 1j * rho[:, 1]
+
+
+def _as_pairs(z: np.ndarray) -> np.ndarray:
+    return np.stack([z.real, z.imag], axis=-1)
+
+
 def wave_densities(r: np.ndarray, kvec: np.ndarray, chunk_size: Optional[int] = None) -> np.ndarray:
-    """rho_k (Re, Im) of every wave at Cartesian positions r: (w, 2) (observables.py:56-78)."""
+    """rho_k of every wave k (rows of ``kvec``, (w, d)) at positions r (n, d): (w, 2).
+
+    ``chunk_size`` bounds the number of waves whose phases are held at once.
+    """
     require_float64(r, "r")
     require_float64(kvec, "kvec")
-    assert r.ndim == 2 and kvec.ndim == 2 and r.shape[1] == kvec.shape[1], "shapes"
-    assert chunk_size is None or chunk_size >= 1, "chunk_size must be >= 1"
-    step = kvec.shape[0] if chunk_size is None else int(chunk_size)
-    out = np.empty((kvec.shape[0], 2))
-    for lo in range(0, kvec.shape[0], max(step, 1)):
-        th = kvec[lo : lo + step] @ r.T
-        out[lo : lo + step, 0] = np.cos(th).sum(axis=1)
-        out[lo : lo + step, 1] = np.sin(th).sum(axis=1)
-    return out
+    if r.ndim != 2 or kvec.ndim != 2 or r.shape[1] != kvec.shape[1]:
+        raise ValueError(f"need r (n, d) and kvec (w, d) of one d, got {r.shape} and {kvec.shape}")
+    if chunk_size is not None and chunk_size < 1:
+        raise ValueError(f"chunk_size must be >= 1, got {chunk_size}")
+    w = kvec.shape[0]
+    width = w if chunk_size is None else int(chunk_size)
+    rho = np.empty(w, dtype=np.complex128)
+    for lo in range(0, w, max(width, 1)):
+        rho[lo: lo + width] = np.exp(1j * (kvec[lo: lo + width] @ r.T)).sum(axis=1)
+    return _as_pairs(rho)
 
 
 def center_of_mass(r: np.ndarray) -> np.ndarray:
-    """Mean position (mass 1, unwrapped)."""
+    """Mean of the (unwrapped) positions, unit masses."""
     require_float64(r, "r")
-    assert r.ndim == 2 and r.shape[0] >= 1, "r must have shape (n, d)"
-    return np.mean(r, axis=0)
+    if r.ndim != 2 or r.shape[0] == 0:
+        raise ValueError(f"r must be (n, d) with n >= 1, got {r.shape}")
+    return r.sum(axis=0) / r.shape[0]
 
 
 def com_correct(rho: np.ndarray, kvec: np.ndarray, r_cm: np.ndarray) -> np.ndarray:
-    """rho_k exp(-i k . R_cm): translation-invariant densities (observables.py:89-98)."""
-    assert rho.ndim == 2 and rho.shape[1] == 2, "rho must have shape (w, 2)"
-    assert kvec.shape == (rho.shape[0], r_cm.shape[0]), "kvec must be (w, d)"
-    phase = kvec @ r_cm
-    c, s = np.cos(phase), np.sin(phase)
-    re, im = rho[:, 0], rho[:, 1]
-    return np.stack([re * c + im * s, im * c - re * s], axis=-1)
+    """rho_k exp(-i k . R_cm) as (w, 2)."""
+    if rho.ndim != 2 or rho.shape[1] != 2 or kvec.shape != (rho.shape[0], r_cm.shape[0]):
+        raise ValueError(f"need rho (w, 2), kvec (w, d), r_cm (d,), got {rho.shape}, {kvec.shape}, {r_cm.shape}")
+    return _as_pairs(_as_complex(rho) * np.exp(-1j * (kvec @ r_cm)))
 
 
 def winding_update(
@@ -58,23 +74,25 @@ def winding_update(
     n_part: int,
     phase_floor: float,
 ) -> Tuple[np.ndarray, np.ndarray, np.ndarray, np.ndarray]:
-    """(rho_prev', wind_abs', wind_net', n_valid') (observables.py:101-125).
+    """One step of the winding tallies: (rho_prev', wind_abs', wind_net', n_valid').
 
-    Delta = atan2(Im p, Re p) with p = rho_new conj(rho_old); a wave counts
-    where ``active`` and S = |rho|^2 / n_part > ``phase_floor`` at both ends;
-    rho_prev is replaced where ``active``.
+    Waves where ``active`` is set take rho_new as their new reference; of those, the
+    ones with S above ``phase_floor`` before and after add |angle| to ``wind_abs``,
+    the signed angle to ``wind_net`` and one to ``n_valid``.
     """
-    assert rho_prev.shape == rho_new.shape == (wind_abs.shape[0], 2), "shapes"
-    assert n_part >= 1, "n_part must be positive"
-    o_re, o_im = rho_prev[:, 0], rho_prev[:, 1]
-    n_re, n_im = rho_new[:, 0], rho_new[:, 1]
-    delta = np.arctan2(n_im * o_re - n_re * o_im, n_re * o_re + n_im * o_im)
-    s_old = (o_re * o_re + o_im * o_im) / n_part
-    s_new = (n_re * n_re + n_im * n_im) / n_part
-    ok = active & (s_old > phase_floor) & (s_new > phase_floor)
+    w = wind_abs.shape[0]
+    if rho_prev.shape != (w, 2) or rho_new.shape != (w, 2):
+        raise ValueError(f"rho_prev and rho_new must be ({w}, 2)")
+    if n_part < 1:
+        raise ValueError(f"n_part must be >= 1, got {n_part}")
+    old, new = _as_complex(rho_prev), _as_complex(rho_new)
+    turn = np.angle(new * np.conj(old))
+    s_old = (old.real**2 + old.imag**2) / n_part
+    s_new = (new.real**2 + new.imag**2) / n_part
+    lit = active & (s_old > phase_floor) & (s_new > phase_floor)
     return (
         np.where(active[:, None], rho_new, rho_prev),
-        wind_abs + np.where(ok, np.abs(delta), 0.0),
-        wind_net + np.where(ok, delta, 0.0),
-        n_valid + ok.astype(n_valid.dtype),
+        wind_abs + np.where(lit, np.abs(turn), 0.0),
+        wind_net + np.where(lit, turn, 0.0),
+        n_valid + lit.astype(n_valid.dtype),
     )
