@@ -82,6 +82,16 @@ int cco_value_grad_density_dev(cco_energy* e, const double* s_dev, int B, int N,
                                double* phi_dev, double* grad_dev, double* re_dev,
                                double* im_dev);
 
+/* Tile knob of an energy's FP64 density kernel (no reference counterpart; the
+ * autotuner's).  CCO_EOPT_RHO_CHUNK: particles whose phase tables a rho
+ * workgroup stages in LDS at a time, 16, 32 or 64 (default: the longest whose
+ * tables fit 80 KB).  Each thread sums its particles in the same order for any
+ * chunk, so results are bit-identical; shorter chunks leave room for more
+ * workgroups per CU.  Applies to later evaluations and sessions. */
+#define CCO_EOPT_RHO_CHUNK 1
+int cco_energy_set_option(cco_energy* e, int key, int64_t value);
+int cco_energy_get_option(const cco_energy* e, int key, int64_t* value);
+
 /* Allocation-free variant of cco_value_grad_density_dev.  No counterpart in
  * the reference: it is for callers that cannot allocate device memory (code
  * under HIP graph capture, tight serving loops, harnesses that forbid
@@ -182,10 +192,22 @@ int cco_relax_counters(cco_relax* r, int64_t* out4);
  * lane's results do not depend on the shard count.  Takes effect at the next
  * init. */
 #define CCO_OPT_SHARDS 2
+/* CCO_OPT_GRAPH: built-in-energy descent and L-BFGS sessions queue each
+ * shard's evaluation steps as replays of a HIP graph holding this many steps
+ * (0: plain launches, the default; at most 64).  A graph is instantiated per
+ * shard and launch bound (rounded up to a power of two) and kept until the
+ * next init; profiled steps and a step count that is not a multiple left in
+ * a cco_relax_step call use plain launches.  Results are bit-identical to
+ * plain launches.  Takes effect at the next step. */
+#define CCO_OPT_GRAPH 3
+/* Read-only (cco_relax_get_option): graph replays since the session was
+ * created. */
+#define CCO_OPT_GRAPH_LAUNCHES 4
 int cco_relax_set_option(cco_relax* r, int key, int64_t value);
 /* The value in force: for CCO_OPT_ADVANCE_VARIANT the variant latched by the
  * last cco_relax_init (1, 2 or 3; 0 before the first init); for
- * CCO_OPT_SHARDS the shards of the last init. */
+ * CCO_OPT_SHARDS the shards of the last init; for CCO_OPT_GRAPH the steps per
+ * graph. */
 int cco_relax_get_option(cco_relax* r, int key, int64_t* value);
 /* Split-precision evaluations (no reference counterpart; the precision row of
  * the target design).  mode 1: rho and the forces run as complex GEMMs on the

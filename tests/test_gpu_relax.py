@@ -404,3 +404,41 @@ def test_relax_long_splits_initial_energy(gpu_ctx) -> None:
     for i in range(2):
         want, _ = vg(s0[i])
         assert out.trajectory.energy[i, 0] == pytest.approx(want, rel=1e-12)
+
+
+@pytest.mark.parametrize("method", ["lbfgs", "descent"])
+def test_relax_graph_replays_bitwise(gpu_ctx, method: str) -> None:
+    """Steps queued as HIP-graph replays (4 or 16 steps per graph, 1 or 3 shards)
+    give the plain launches' results and trajectories bit for bit."""
+    import ctypes as C
+
+    from xla_cco import _native
+    from xla_cco.minimize import RelaxSession
+
+    n, d, b = 48, 2, 100
+    e, _, _ = _poisson(n, d, 0.25)
+    s0 = np.random.default_rng(31).uniform(size=(b, n, d))
+    if method == "lbfgs":
+        st = LBFGSSettings(record_every=3, max_records=40)
+    else:
+        st = DescentSettings(grow=2.0, max_evals=4000, energy_tol=1e-20, record_every=50, max_records=40)
+    L = _native.lib()
+    outs = []
+    for shards, graph in ((1, 0), (1, 4), (3, 16), (3, 0)):
+        sess = RelaxSession(e, b, n, d, st)
+        _native.check(L.cco_relax_set_option(sess.handle, 2, shards))
+        _native.check(L.cco_relax_set_option(sess.handle, 3, graph))
+        sess.init(s0)
+        sess.run(1_000_000, 256)
+        outs.append(sess.result())
+        v = C.c_int64(0)
+        _native.check(L.cco_relax_get_option(sess.handle, 4, C.byref(v)))
+        assert (int(v.value) > 0) == (graph > 0)
+        sess.close()
+    a = outs[0]
+    for o in outs[1:]:
+        for f in ("s", "energy", "grad_norm", "n_evals", "n_iters", "status"):
+            assert np.array_equal(getattr(a, f), getattr(o, f)), f
+        assert np.array_equal(a.trajectory.count, o.trajectory.count)
+        assert np.array_equal(a.trajectory.energy, o.trajectory.energy)
+        assert np.array_equal(a.trajectory.s, o.trajectory.s)

@@ -114,6 +114,8 @@ struct cco_energy {
 namespace {
 
 constexpr int kSplitLen = 128;  // particles per rho split up to N = 2048 and in uneven batches
+// rho LDS tables: at most half the CU's 160 KB, so two workgroups share a CU
+constexpr size_t kRhoLdsMax = 80 * 1024;
 
 // Particles per rho split of a uniform batch: 128 up to N = 2048, then longer
 // (a multiple of 128) so that a lane has at most 16 splits.  The partial sums
@@ -291,8 +293,6 @@ int build_geometry(cco_energy* E, const int64_t* n, const double* w, int M, int 
       g.fpart[p] = make_int2(begin, u);
     }
   }
-  // at most half the CU's 160 KB of LDS, so two workgroups share a CU
-  constexpr size_t kRhoLdsMax = 80 * 1024;
   g.rho_chunk = kRhoChunkMax;
   while (g.rho_chunk > 16 && rho_lds_bytes(g) > kRhoLdsMax) g.rho_chunk /= 2;
   if (rho_lds_bytes(g) > kRhoLdsMax)
@@ -493,6 +493,12 @@ struct cco_relax {
   hipStream_t extra_streams[kMaxShards] = {};   // streams of shards 1 ..
   hipEvent_t fork_ev = nullptr;
   int shard_req = 0;        // requested shards: 0 auto
+  unsigned* shard_steps = nullptr;  // [kMaxShards] device step counters (compact_kernel)
+  // HIP graphs of graph_steps evaluation steps per shard (0: plain launches),
+  // instantiated per (shard, launch-bound bucket) and replayed
+  int graph_steps = 0;
+  std::map<std::pair<int, int>, hipGraphExec_t> graphs;
+  int64_t graph_builds = 0, graph_launches = 0;
   int64_t launches = 0;     // evaluation steps queued over all shards since the last init
   // optional per-stage timing: events around the launches of every
   // profile_every-th step of each shard (0: off)
@@ -759,7 +765,14 @@ int alloc_host_words(cco_relax* R) {
     return fail(CCO_ERR_DEVICE, "hipHostGetDevicePointer failed");
   R->progress_dev = (unsigned long long*)dp;
   R->dev.progress = nullptr;
+  R->dev.step_ctr = nullptr;
+  if (int rc = alloc_into(R, R->shard_steps, kMaxShards)) return rc;
   return alloc_into(R, R->shard_counts, kMaxShards);
+}
+
+void drop_graphs(cco_relax* R) {
+  for (auto& kv : R->graphs) (void)hipGraphExecDestroy(kv.second);
+  R->graphs.clear();
 }
 
 // Split the lanes into shards for the run being initialised and queue each
@@ -768,6 +781,7 @@ int alloc_host_words(cco_relax* R) {
 // 2 kShardMinLanes lanes on (or the requested count); the others one.
 int setup_shards(cco_relax* R) {
   int n = 1;
+  drop_graphs(R);  // their launches capture this run's shard layout and kernels
   if (R->E != nullptr && (R->method == CCO_METHOD_DESCENT || R->method == CCO_METHOD_LBFGS))
     n = R->shard_req > 0 ? R->shard_req : std::min(3, R->B / kShardMinLanes);
   n = std::max(1, std::min(n, std::min(kMaxShards, R->B)));
@@ -787,12 +801,16 @@ int setup_shards(cco_relax* R) {
     sh.dev.active = R->dev.active + sh.lo;
     sh.dev.n_active = R->shard_counts + k;
     sh.dev.progress = R->progress_dev + k;
+    sh.dev.step_ctr = nullptr;
     sh.progress_host = R->progress_host + k;
     sh.slots_ub = sh.count;
     sh.steps = 0;
     *sh.progress_host = ~0ull;  // no step of this run has finished
     launch_compact(sh.dev, R->ctx->stream, sh.lo, sh.count, 0u);  // tag 0 is never read
+    sh.dev.step_ctr = R->shard_steps + k;  // the run's steps count from 1 on the device too
   }
+  hipLaunchKernelGGL(fill_int_kernel, dim3(1), dim3(kMaxShards), 0, R->ctx->stream, (int*)R->shard_steps,
+                     kMaxShards, 0);
   if (R->fork_ev == nullptr) CCO_HIP(hipEventCreateWithFlags(&R->fork_ev, hipEventDisableTiming));
   CCO_HIP(hipGetLastError());
   R->steps = 0;
@@ -1352,6 +1370,8 @@ int cco_relax_destroy(cco_relax* R) {
   if (!R) return CCO_OK;
   DevScope ds(R->ctx->device);
   (void)hipStreamSynchronize(R->ctx->stream);
+  for (const Shard& sh : R->shards) (void)hipStreamSynchronize(sh.stream);
+  drop_graphs(R);
   for (DevBuf& b : R->bufs) dev_release(b);
   for (DevBuf& b : R->md_bufs) dev_release(b);
   if (R->trk) cco_energy_destroy(R->trk);
@@ -1412,6 +1432,54 @@ int cco_relax_init(cco_relax* R, const double* s0_host) {
   return CCO_OK;
 }
 
+namespace {
+
+// Launch bound of a graph: the shard's bound rounded up to a power of two (at
+// most the shard's lanes), so a run instantiates a handful of graphs per shard
+// while its lanes stop; surplus workgroups exit at once, as for any bound.
+int graph_bucket(const Shard& sh) {
+  int b = 1;
+  while (b < sh.slots_ub) b <<= 1;
+  return std::min(b, sh.count);
+}
+
+bool graph_eligible(const cco_relax* R) {
+  return R->graph_steps > 0 && R->E != nullptr &&
+         (R->method == CCO_METHOD_DESCENT || R->method == CCO_METHOD_LBFGS);
+}
+
+// The HIP graph of R->graph_steps evaluation steps of shard k at launch bound
+// ub: the stage launches of each step, captured from the shard's stream.  The
+// compaction kernel tags the progress word from the shard's device step
+// counter, so every replay reports its own step numbers.
+int step_graph(cco_relax* R, int k, int ub, hipGraphExec_t* out) {
+  const auto key = std::make_pair(k, ub);
+  const auto it = R->graphs.find(key);
+  if (it != R->graphs.end()) {
+    *out = it->second;
+    return CCO_OK;
+  }
+  Shard sh = R->shards[k];
+  sh.slots_ub = ub;
+  int seq[kStages];
+  const int ns = step_stages(R, seq);
+  CCO_HIP(hipStreamBeginCapture(sh.stream, hipStreamCaptureModeRelaxed));
+  for (int g = 0; g < R->graph_steps; ++g)
+    for (int q = 0; q < ns; ++q) relax_stage(R, sh, seq[q]);
+  hipGraph_t graph = nullptr;
+  CCO_HIP(hipStreamEndCapture(sh.stream, &graph));
+  hipGraphExec_t exec = nullptr;
+  const hipError_t e = hipGraphInstantiate(&exec, graph, nullptr, nullptr, 0);
+  (void)hipGraphDestroy(graph);
+  if (e != hipSuccess) return fail(CCO_ERR_DEVICE, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+  R->graphs[key] = exec;
+  R->graph_builds += 1;
+  *out = exec;
+  return CCO_OK;
+}
+
+}  // namespace
+
 int cco_relax_step(cco_relax* R, int64_t n_evals, int* n_active) {
   if (!R) return fail(CCO_ERR_ARG, "relax is NULL");
   if (!R->ready) return fail(CCO_ERR_STATE, "session not initialised");
@@ -1454,6 +1522,16 @@ int cco_relax_step(cco_relax* R, int64_t n_evals, int* n_active) {
       if (in_flight >= kRunAhead && !force[k]) continue;
       force[k] = false;
       const bool prof = R->profile_every > 0 && sh.steps % R->profile_every == 0;
+      if (!prof && graph_eligible(R) && n_evals - (sh.steps - first[k]) >= R->graph_steps) {
+        hipGraphExec_t exec = nullptr;
+        if (int rc = step_graph(R, k, graph_bucket(sh), &exec)) return rc;
+        CCO_HIP(hipGraphLaunch(exec, sh.stream));
+        sh.steps += R->graph_steps;
+        R->launches += R->graph_steps;
+        R->graph_launches += 1;
+        queued = true;
+        continue;
+      }
       while (prof && R->evs.size() < kEv * (n_prof + 1)) {
         hipEvent_t ev;
         CCO_HIP(hipEventCreate(&ev));
@@ -1543,6 +1621,30 @@ int cco_relax_precision_state(cco_relax* R, int* active_mode, int* lanes_split) 
   return CCO_OK;
 }
 
+int cco_energy_set_option(cco_energy* E, int key, int64_t value) {
+  if (!E) return fail(CCO_ERR_ARG, "energy is NULL");
+  if (key == CCO_EOPT_RHO_CHUNK) {
+    if (value != 16 && value != 32 && value != 64)
+      return fail(CCO_ERR_ARG, "rho chunk must be 16, 32 or 64 particles");
+    Geo g = E->geo;
+    g.rho_chunk = (int)value;
+    if (rho_lds_bytes(g) > kRhoLdsMax)
+      return fail(CCO_ERR_ARG, "rho chunk too long for this k-set's LDS tables");
+    E->geo.rho_chunk = (int)value;
+    return CCO_OK;
+  }
+  return fail(CCO_ERR_ARG, "unknown energy option key");
+}
+
+int cco_energy_get_option(const cco_energy* E, int key, int64_t* value) {
+  if (!E || !value) return fail(CCO_ERR_ARG, "NULL argument");
+  if (key == CCO_EOPT_RHO_CHUNK) {
+    *value = E->geo.rho_chunk;
+    return CCO_OK;
+  }
+  return fail(CCO_ERR_ARG, "unknown energy option key");
+}
+
 int cco_energy_set_precision(cco_energy* E, int mode, int* applied) {
   if (!E) return fail(CCO_ERR_ARG, "energy is NULL");
   if (mode != 0 && mode != 1) return fail(CCO_ERR_ARG, "precision mode must be 0 (FP64) or 1 (split)");
@@ -1559,6 +1661,14 @@ int cco_relax_get_option(cco_relax* R, int key, int64_t* value) {
   }
   if (key == CCO_OPT_SHARDS) {
     *value = (int64_t)R->shards.size();
+    return CCO_OK;
+  }
+  if (key == CCO_OPT_GRAPH) {
+    *value = R->graph_steps;
+    return CCO_OK;
+  }
+  if (key == CCO_OPT_GRAPH_LAUNCHES) {
+    *value = R->graph_launches;
     return CCO_OK;
   }
   return fail(CCO_ERR_ARG, "unknown option key");
@@ -1579,6 +1689,12 @@ int cco_relax_set_option(cco_relax* R, int key, int64_t value) {
   if (key == CCO_OPT_SHARDS) {
     if (value < 0 || value > kMaxShards) return fail(CCO_ERR_ARG, "shards must be 0 (auto) to 4");
     R->shard_req = (int)value;
+    return CCO_OK;
+  }
+  if (key == CCO_OPT_GRAPH) {
+    if (value < 0 || value > 64) return fail(CCO_ERR_ARG, "graph steps must be 0 (off) to 64");
+    if (value != R->graph_steps) drop_graphs(R);
+    R->graph_steps = (int)value;
     return CCO_OK;
   }
   return fail(CCO_ERR_ARG, "unknown option key");

@@ -164,6 +164,10 @@ struct RelaxDev {
   // mapped host word (null: none) that compact_kernel sets to
   // (completed steps << 32) | running lanes after every step
   unsigned long long* progress;
+  // device step counter of the shard (null: the host passes each step's tag):
+  // compact_kernel increments it and tags the progress word with the new
+  // value, so a step launched from a replayed HIP graph carries its own number
+  unsigned* step_ctr;
   double* s_eval; // [B][stride] trial point (energy input, s coordinates)
   double* g_eval; // [B][stride] energy gradient at the trial point
   double* phi_eval;  // [B]

@@ -43,7 +43,7 @@ __global__ __launch_bounds__(1024) void compact_kernel(const int* __restrict__ s
                                                        int* __restrict__ active,
                                                        int* __restrict__ n_active, int lo,
                                                        int count, unsigned long long* progress,
-                                                       unsigned step_tag) {
+                                                       unsigned* step_ctr, unsigned step_tag) {
   __shared__ int cnt[1024];
   const int t = threadIdx.x;
   const int per = (count + 1023) / 1024;
@@ -64,16 +64,21 @@ __global__ __launch_bounds__(1024) void compact_kernel(const int* __restrict__ s
     if (status[i] == ST_RUNNING) active[pos++] = i;
   if (t == 1023) {
     *n_active = cnt[1023];
+    unsigned tag = step_tag;
+    if (step_ctr != nullptr) {  // this kernel alone writes the counter, in stream order
+      tag = *step_ctr + 1u;
+      *step_ctr = tag;
+    }
     // the host reads this word while later steps are still queued
     if (progress != nullptr)
-      __hip_atomic_store(progress, ((unsigned long long)step_tag << 32) | (unsigned)cnt[1023],
+      __hip_atomic_store(progress, ((unsigned long long)tag << 32) | (unsigned)cnt[1023],
                          __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_SYSTEM);
   }
 }
 
 void launch_compact(const RelaxDev& r, hipStream_t st, int lo, int count, unsigned step_tag) {
   hipLaunchKernelGGL(compact_kernel, dim3(1), dim3(1024), 0, st, r.status, r.active, r.n_active,
-                     lo, count, r.progress, step_tag);
+                     lo, count, r.progress, r.step_ctr, step_tag);
 }
 
 // ---------------------------------------------------------------------------

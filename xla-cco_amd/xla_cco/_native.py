@@ -151,6 +151,8 @@ SIGNATURES = {
     "cco_value_grad_density_dev": (C.c_int, [_P, _P, C.c_int, C.c_int, _P, _P, _P, _P]),
     "cco_value_grad_density_dev_noalloc": (C.c_int, [_P, _P, C.c_int, C.c_int, _P, _P, _P, _P]),
     "cco_noalloc_scratch_bytes": (C.c_size_t, []),
+    "cco_energy_set_option": (C.c_int, [_P, C.c_int, C.c_int64]),
+    "cco_energy_get_option": (C.c_int, [_P, C.c_int, C.POINTER(C.c_int64)]),
     "cco_relax_create": (
         C.c_int,
         [

@@ -256,8 +256,13 @@ def relax(
     s0: np.ndarray,
     settings: Settings,
     chunk_iters: int = 2_000,
+    autotune: bool = False,
 ) -> RelaxResult:
     """Relax a batch (b, n, d) of configurations; leaves have a leading lane axis.
+
+    ``autotune`` (no reference counterpart) runs the built-in energy with the
+    launch and tile knobs xla_cco.minimize.tuning picked for this workload class,
+    timing the candidates on ``s0`` the first time; results are bit-identical.
 
     Raises TypeError if ``s0`` is not float64 or the settings type is unknown,
     RuntimeError if lanes are still running after the round bound.
@@ -272,8 +277,15 @@ def relax(
     if isinstance(energy, StealthyEnergy) and s0.shape[2] != energy.dim:
         raise ValueError("dimension mismatch")
     b, n, d = s0.shape
+    choice = None
+    if autotune and isinstance(energy, StealthyEnergy):
+        from xla_cco.minimize import tuning
+
+        choice = tuning.choice_for(energy, s0, settings)
     sess = RelaxSession(energy, b, n, d, settings)
     try:
+        if choice is not None:
+            tuning.apply(sess, choice)
         sess.init(s0)
         sess.run(_eval_bound(settings, chunk_iters), sync_every=max(1, min(chunk_iters, 256)))
         return sess.result()
