@@ -312,6 +312,8 @@ int build_geometry(cco_energy* E, const int64_t* n, const double* w, int M, int 
   }
   g.mp_ok = (g.RP <= 48 && g.wmin > 0.0 && mp_force_batch_for(g) >= 1 &&
               mp_lds_bytes(g) <= 64 * 1024) ? 1 : 0;
+  g.mp_pg = g.mp_cb = 0;
+  if (g.mp_ok) mp_force_plan(g, &g.mp_pg, &g.mp_cb);
   // one device allocation holding every table
   auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
   const size_t o_un = 0;

@@ -52,6 +52,8 @@ struct Geo {
   int mp_ok;      // the k-set fits the MFMA kernels (rows <= 48, LDS <= 64 KB)
   int RP;         // rows padded to a multiple of 16
   int n_mp_ctiles;  // 64-column workgroup tiles of the rho MFMA kernel
+  int mp_pg;      // force MFMA kernel: 64-particle groups per workgroup (1 or 2)
+  int mp_cb;      // force MFMA kernel: 16-column tiles of C per LDS batch at mp_pg groups
   double wmax, wmin;  // largest and smallest positive box weight (C scaling)
   const int4* gcol;   // per column (rho order): {box offset, L, n2, n3}
   int n_u;      // force traversal: u = 0 .. n_u-1 (u = |n2|)
@@ -167,6 +169,9 @@ void launch_pair(const double* s, int B, int N, int d, const double* basis, cons
 size_t rho_lds_bytes(const Geo& g);
 size_t mp_lds_bytes(const Geo& g);
 int mp_force_batch_for(const Geo& g);  // 16-column tiles per force batch (0: does not fit)
+// The force MFMA kernel's particle groups per workgroup and C tiles per batch
+// for a geometry (two groups when their LDS fits half a CU, else one).
+void mp_force_plan(const Geo& g, int* pg, int* cb);
 void launch_energy_mixed(int stage, const Geo& g, const Batch& b, int slots_ub, hipStream_t st);
 
 // Closed-form Hessians of B configurations (hessian.hip).  kn: n_k1, n_k2,

@@ -26,6 +26,8 @@
 //
 // Lanes take this path while their precision flag is set (b.prec[lane] != 0);
 // the FP64 kernels skip those lanes and these kernels skip the others.
+#include <algorithm>
+
 #include "cco_internal.h"
 
 namespace cco {
@@ -440,13 +442,48 @@ __global__ __launch_bounds__(256) void mp_cplanes_kernel(Geo g, Batch b) {
   }
 }
 
-// grid: slots_ub x ceil(N / 64); block: 4 waves, wave w owns particles
-// [64 pt + 16 w, +16).  C is converted once per workgroup into LDS operand
-// planes, a batch of 16-column tiles at a time.
-template <int D, int NRT>
+// LDS of the force kernel with pg particle groups and cb C tiles per batch
+// (pg = 1 with cb = mp_force_batch is the one-group layout within 64 KB):
+// the z1 planes (one group at a time), the z2 / z3 power tables of every
+// group, the fill's unit phases, then the C batch.
+__host__ __device__ inline size_t mp_force_group_bytes(int P2, int P3) {
+  return (size_t)kMpChunk * (mp_pad(P2 + 1) + mp_pad(P3 + 1)) * sizeof(float2);
+}
+__host__ __device__ inline size_t mp_force_zu_offset(int RP, int P2, int P3, int pg) {
+  const size_t end = (size_t)4 * RP * kZ1Pitch * sizeof(_Float16) + (size_t)pg * mp_force_group_bytes(P2, P3);
+  return (end + 15) & ~(size_t)15;
+}
+__host__ __device__ inline size_t mp_force_lds(int RP, int P2, int P3, int pg, int cb) {
+  return mp_force_zu_offset(RP, P2, P3, pg) + (size_t)3 * kMpChunk * sizeof(double2) +
+         (size_t)8 * 16 * cb * mp_cpitch(RP) * sizeof(_Float16);
+}
+
+void mp_force_plan(const Geo& g, int* pg, int* cb) {
+  // two particle groups per workgroup when their tables and one C tile fit
+  // half a CU's LDS (two workgroups per CU), else one group in 64 KB
+  const size_t per_tile = (size_t)8 * 16 * mp_cpitch(g.RP) * sizeof(_Float16);
+  const size_t budget = 78 * 1024;
+  const size_t fixed = mp_force_lds(g.RP, g.P2, g.P3, 2, 0);
+  if (fixed + per_tile <= budget) {
+    *pg = 2;
+    *cb = (int)std::min<size_t>(8, (budget - fixed) / per_tile);
+    return;
+  }
+  *pg = 1;
+  *cb = mp_force_batch(g.RP, g.P2, g.P3);
+}
+
+// grid: slots_ub x ceil(N / (64 PG)); block: 4 waves.  Group p of the
+// workgroup holds particles [64 (PG pt + p), +64); wave w owns particles
+// 16 w .. 16 w + 15 of every group.  C is copied once per workgroup into LDS
+// operand planes, a batch of 16-column tiles at a time, and each C operand
+// read from LDS serves the PG groups.  Per particle the MFMA and FP64
+// accumulation order is that of a single group.
+template <int D, int NRT, int PG>
 __global__ __launch_bounds__(256) void force_mp_kernel(Geo g, Batch b) {
   extern __shared__ char smem[];
-  const int n_pt = (b.N + 63) / 64;
+  constexpr int PW = 64 * PG;
+  const int n_pt = (b.N + PW - 1) / PW;
   const int slot = blockIdx.x / n_pt;
   if (slot >= *b.n_active) return;
   const int pt = blockIdx.x % n_pt;
@@ -457,33 +494,55 @@ __global__ __launch_bounds__(256) void force_mp_kernel(Geo g, Batch b) {
   const int g16 = l / 16;
   const int m16 = l % 16;
   const double* s = b.s + (size_t)lane * b.lane_stride;
-  const MpTables T = mp_tables(smem, g);
-  _Float16* cpl = (_Float16*)(smem + mp_table_bytes(g.RP, g.P2, g.P3));
   const int RP = g.RP;
   const int CP = mp_cpitch(RP);
-  const int CB = mp_force_batch(RP, g.P2, g.P3);
+  const int CB = g.mp_cb;
   const size_t plane = (size_t)16 * CB * CP;
-  const int jc = pt * 64;
-  const int nj = min(64, b.N - jc);
-  mp_zero_pad_rows(T, g.P1 + 1, RP);  // ordered before use by mp_fill's barrier
-  mp_fill<D>(T, s, jc, nj, g.P1 + 1, RP, g.P2, g.P3);
-  const double sigma = b.csig[lane];  // written with the C planes (mp_cplanes_kernel)
-  __syncthreads();
-  // A operand: particle 16 wave + m16 (m), rows 16 rk + 4 g16 + i (k)
-  half4_t are_h[NRT], are_l[NRT], aim_h[NRT], aim_l[NRT];
+  const size_t z1_bytes = (size_t)4 * RP * kZ1Pitch * sizeof(_Float16);
+  const size_t grp_bytes = mp_force_group_bytes(g.P2, g.P3);
+  const size_t zu_off = mp_force_zu_offset(RP, g.P2, g.P3, PG);
+  _Float16* cpl = (_Float16*)(smem + zu_off + (size_t)3 * kMpChunk * sizeof(double2));
+  auto tables = [&](int p) {
+    MpTables T;
+    T.z1p = (_Float16*)smem;
+    T.w2p = mp_pad(g.P2 + 1);
+    T.w3p = mp_pad(g.P3 + 1);
+    T.t2 = (float2*)(smem + z1_bytes + (size_t)p * grp_bytes);
+    T.t3 = T.t2 + (size_t)kMpChunk * T.w2p;
+    T.zu = (double2*)(smem + zu_off);
+    T.t2p = nullptr;
+    return T;
+  };
+  // A operands: particle 16 wave + m16 of group p (m), rows 16 rk + 4 g16 + i (k)
+  half4_t are_h[PG][NRT], are_l[PG][NRT], aim_h[PG][NRT], aim_l[PG][NRT];
   const int jA = wave * 16 + m16;
+  mp_zero_pad_rows(tables(0), g.P1 + 1, RP);  // ordered before use by mp_fill's barrier
 #pragma unroll
-  for (int rk = 0; rk < NRT; ++rk) {
+  for (int p = 0; p < PG; ++p) {
+    const int jc = pt * PW + p * kMpChunk;
+    const int nj = max(0, min(kMpChunk, b.N - jc));
+    mp_fill<D>(tables(p), s, jc, nj, g.P1 + 1, RP, g.P2, g.P3);
+    __syncthreads();
+    const _Float16* z1p = tables(p).z1p;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int r = rk * 16 + 4 * g16 + i;
-      are_h[rk][i] = T.z1p[(0 * RP + r) * kZ1Pitch + jA];
-      are_l[rk][i] = T.z1p[(1 * RP + r) * kZ1Pitch + jA];
-      aim_h[rk][i] = T.z1p[(2 * RP + r) * kZ1Pitch + jA];
-      aim_l[rk][i] = T.z1p[(3 * RP + r) * kZ1Pitch + jA];
+    for (int rk = 0; rk < NRT; ++rk) {
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int r = rk * 16 + 4 * g16 + i;
+        are_h[p][rk][i] = z1p[(0 * RP + r) * kZ1Pitch + jA];
+        are_l[p][rk][i] = z1p[(1 * RP + r) * kZ1Pitch + jA];
+        aim_h[p][rk][i] = z1p[(2 * RP + r) * kZ1Pitch + jA];
+        aim_l[p][rk][i] = z1p[(3 * RP + r) * kZ1Pitch + jA];
+      }
     }
+    __syncthreads();  // the next group's fill rewrites the z1 planes
   }
-  double g1[4] = {0.0, 0.0, 0.0, 0.0}, g2[4] = {0.0, 0.0, 0.0, 0.0}, g3[4] = {0.0, 0.0, 0.0, 0.0};
+  const double sigma = b.csig[lane];  // written with the C planes (mp_cplanes_kernel)
+  double g1[PG][4], g2[PG][4], g3[PG][4];
+#pragma unroll
+  for (int p = 0; p < PG; ++p)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) g1[p][i] = g2[p][i] = g3[p][i] = 0.0;
   const int n_ct = (g.Cn + 15) / 16;
   for (int cb0 = 0; cb0 < n_ct; cb0 += CB) {
     const int nb = min(CB, n_ct - cb0);
@@ -494,17 +553,18 @@ __global__ __launch_bounds__(256) void force_mp_kernel(Geo g, Batch b) {
       const size_t gplane = (size_t)n_ct * 16 * CP;
       const _Float16* src = (const _Float16*)b.cpl + (size_t)lane * 8 * gplane + (size_t)cb0 * 16 * CP;
       for (int e = threadIdx.x; e < 8 * per; e += blockDim.x) {
-        const int p = e / per;
-        const int o = e - p * per;
-        reinterpret_cast<uint2*>(cpl + (size_t)p * plane)[o] =
-            reinterpret_cast<const uint2*>(src + (size_t)p * gplane)[o];
+        const int q = e / per;
+        const int o = e - q * per;
+        reinterpret_cast<uint2*>(cpl + (size_t)q * plane)[o] =
+            reinterpret_cast<const uint2*>(src + (size_t)q * gplane)[o];
       }
     }
     __syncthreads();
     for (int tb = 0; tb < nb; ++tb) {
       const int ct = cb0 + tb;
-      float4_t hre = {0.f, 0.f, 0.f, 0.f}, him = {0.f, 0.f, 0.f, 0.f};
-      float4_t h1re = {0.f, 0.f, 0.f, 0.f}, h1im = {0.f, 0.f, 0.f, 0.f};
+      float4_t hre[PG], him[PG], h1re[PG], h1im[PG];
+#pragma unroll
+      for (int p = 0; p < PG; ++p) hre[p] = him[p] = h1re[p] = h1im[p] = float4_t{0.f, 0.f, 0.f, 0.f};
 #pragma unroll
       for (int rk = 0; rk < NRT; ++rk) {
         // B operand: rows 16 rk + 4 g16 + i of column 16 tb + m16
@@ -517,69 +577,86 @@ __global__ __launch_bounds__(256) void force_mp_kernel(Geo g, Batch b) {
         const half4_t dre_l = *(const half4_t*)(q + 5 * plane);
         const half4_t dim_h = *(const half4_t*)(q + 6 * plane);
         const half4_t dim_l = *(const half4_t*)(q + 7 * plane);
-        const half4_t naim_h = -aim_h[rk];
-        const half4_t naim_l = -aim_l[rk];
-        // conj(a) c: Re = ar cr + ai ci, Im = ar ci - ai cr
-        hre = MFMA16(are_h[rk], cre_h, hre);
-        hre = MFMA16(are_h[rk], cre_l, hre);
-        hre = MFMA16(are_l[rk], cre_h, hre);
-        hre = MFMA16(aim_h[rk], cim_h, hre);
-        hre = MFMA16(aim_h[rk], cim_l, hre);
-        hre = MFMA16(aim_l[rk], cim_h, hre);
-        him = MFMA16(are_h[rk], cim_h, him);
-        him = MFMA16(are_h[rk], cim_l, him);
-        him = MFMA16(are_l[rk], cim_h, him);
-        him = MFMA16(naim_h, cre_h, him);
-        him = MFMA16(naim_h, cre_l, him);
-        him = MFMA16(naim_l, cre_h, him);
-        h1re = MFMA16(are_h[rk], dre_h, h1re);
-        h1re = MFMA16(are_h[rk], dre_l, h1re);
-        h1re = MFMA16(are_l[rk], dre_h, h1re);
-        h1re = MFMA16(aim_h[rk], dim_h, h1re);
-        h1re = MFMA16(aim_h[rk], dim_l, h1re);
-        h1re = MFMA16(aim_l[rk], dim_h, h1re);
-        h1im = MFMA16(are_h[rk], dim_h, h1im);
-        h1im = MFMA16(are_h[rk], dim_l, h1im);
-        h1im = MFMA16(are_l[rk], dim_h, h1im);
-        h1im = MFMA16(naim_h, dre_h, h1im);
-        h1im = MFMA16(naim_h, dre_l, h1im);
-        h1im = MFMA16(naim_l, dre_h, h1im);
+#pragma unroll
+        for (int p = 0; p < PG; ++p) {
+          const half4_t ah = are_h[p][rk], al = are_l[p][rk], ih = aim_h[p][rk], il = aim_l[p][rk];
+          const half4_t nih = -ih, nil = -il;
+          // conj(a) c: Re = ar cr + ai ci, Im = ar ci - ai cr
+          hre[p] = MFMA16(ah, cre_h, hre[p]);
+          hre[p] = MFMA16(ah, cre_l, hre[p]);
+          hre[p] = MFMA16(al, cre_h, hre[p]);
+          hre[p] = MFMA16(ih, cim_h, hre[p]);
+          hre[p] = MFMA16(ih, cim_l, hre[p]);
+          hre[p] = MFMA16(il, cim_h, hre[p]);
+          him[p] = MFMA16(ah, cim_h, him[p]);
+          him[p] = MFMA16(ah, cim_l, him[p]);
+          him[p] = MFMA16(al, cim_h, him[p]);
+          him[p] = MFMA16(nih, cre_h, him[p]);
+          him[p] = MFMA16(nih, cre_l, him[p]);
+          him[p] = MFMA16(nil, cre_h, him[p]);
+          h1re[p] = MFMA16(ah, dre_h, h1re[p]);
+          h1re[p] = MFMA16(ah, dre_l, h1re[p]);
+          h1re[p] = MFMA16(al, dre_h, h1re[p]);
+          h1re[p] = MFMA16(ih, dim_h, h1re[p]);
+          h1re[p] = MFMA16(ih, dim_l, h1re[p]);
+          h1re[p] = MFMA16(il, dim_h, h1re[p]);
+          h1im[p] = MFMA16(ah, dim_h, h1im[p]);
+          h1im[p] = MFMA16(ah, dim_l, h1im[p]);
+          h1im[p] = MFMA16(al, dim_h, h1im[p]);
+          h1im[p] = MFMA16(nih, dre_h, h1im[p]);
+          h1im[p] = MFMA16(nih, dre_l, h1im[p]);
+          h1im[p] = MFMA16(nil, dre_h, h1im[p]);
+        }
       }
-      // lane l holds H[particle 16 wave + 4 g16 + i][column 16 ct + m16]
+      // lane l holds H[particle 16 wave + 4 g16 + i of group p][column 16 ct + m16]
       const int4 col = g.gcol[ct * 16 + m16];
       const int a2 = abs(col.z), a3 = abs(col.w);
       const float sg2 = col.z < 0 ? -1.0f : 1.0f;
       const float sg3 = col.w < 0 ? -1.0f : 1.0f;
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const float2 B = mp_bcol<D>(T, wave * 16 + 4 * g16 + i, a2, sg2, a3, sg3);
-        // Im(conj(B) H) = B.x H.im - B.y H.re
-        const double p = (double)(B.x * him[i] - B.y * hre[i]);
-        const double p1 = (double)(B.x * h1im[i] - B.y * h1re[i]);
-        g1[i] += p1;
-        if (D >= 2) g2[i] += (double)col.z * p;
-        if (D >= 3) g3[i] += (double)col.w * p;
+      for (int p = 0; p < PG; ++p) {
+        const MpTables T = tables(p);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const float2 B = mp_bcol<D>(T, wave * 16 + 4 * g16 + i, a2, sg2, a3, sg3);
+          // Im(conj(B) H) = B.x H.im - B.y H.re
+          const double v = (double)(B.x * him[p][i] - B.y * hre[p][i]);
+          const double v1 = (double)(B.x * h1im[p][i] - B.y * h1re[p][i]);
+          g1[p][i] += v1;
+          if (D >= 2) g2[p][i] += (double)col.z * v;
+          if (D >= 3) g3[p][i] += (double)col.w * v;
+        }
       }
     }
   }
 #pragma unroll
-  for (int i = 0; i < 4; ++i) {
-    g1[i] = row16_sum(g1[i]);
-    if (D >= 2) g2[i] = row16_sum(g2[i]);
-    if (D >= 3) g3[i] = row16_sum(g3[i]);
-  }
+  for (int p = 0; p < PG; ++p)
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      g1[p][i] = row16_sum(g1[p][i]);
+      if (D >= 2) g2[p][i] = row16_sum(g2[p][i]);
+      if (D >= 3) g3[p][i] = row16_sum(g3[p][i]);
+    }
   if (m16 == 15 && b.grad != nullptr) {
     const double scale = g.pref / (sigma * kPhaseScale * kPhaseScale);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-      const int jj = wave * 16 + 4 * g16 + i;
-      if (jj >= nj) continue;
-      double* o = b.grad + (size_t)lane * b.lane_stride + (size_t)(jc + jj) * D;
-      o[0] = scale * g1[i];
-      if (D >= 2) o[1] = scale * g2[i];
-      if (D >= 3) o[2] = scale * g3[i];
-    }
+    for (int p = 0; p < PG; ++p)
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int j = pt * PW + p * kMpChunk + wave * 16 + 4 * g16 + i;
+        if (j >= b.N) continue;
+        double* o = b.grad + (size_t)lane * b.lane_stride + (size_t)j * D;
+        o[0] = scale * g1[p][i];
+        if (D >= 2) o[1] = scale * g2[p][i];
+        if (D >= 3) o[2] = scale * g3[p][i];
+      }
   }
+}
+
+template <int D, int NRT, int PG>
+static void launch_force_mp(const Geo& g, const Batch& b, int slots_ub, hipStream_t st) {
+  hipLaunchKernelGGL((force_mp_kernel<D, NRT, PG>), dim3(slots_ub * ((b.N + 64 * PG - 1) / (64 * PG))),
+                     dim3(256), mp_force_lds(g.RP, g.P2, g.P3, PG, g.mp_cb), st, g, b);
 }
 
 template <int D, int NRT>
@@ -590,8 +667,16 @@ static void launch_mp(int stage, const Geo& g, const Batch& b, int slots_ub, hip
   } else {
     hipLaunchKernelGGL(mp_cplanes_kernel, dim3(slots_ub * ((g.Cn + 15) / 16)), dim3(256), 0, st, g,
                        b);
-    hipLaunchKernelGGL((force_mp_kernel<D, NRT>), dim3(slots_ub * ((b.N + 63) / 64)), dim3(256),
-                       mp_lds_bytes(g), st, g, b);
+    // two groups per workgroup halve the C copies per particle; below about
+    // four workgroups per CU the launch keeps one group (more workgroups)
+    const long wg2 = (long)slots_ub * ((b.N + 127) / 128);
+    if (g.mp_pg == 2 && wg2 >= 1024) {
+      launch_force_mp<D, NRT, 2>(g, b, slots_ub, st);
+    } else {
+      Geo g1 = g;
+      g1.mp_cb = mp_force_batch(g.RP, g.P2, g.P3);
+      launch_force_mp<D, NRT, 1>(g1, b, slots_ub, st);
+    }
   }
 }
 
