@@ -548,15 +548,18 @@ __global__ __launch_bounds__(256) void force_mp_kernel(Geo g, Batch b) {
     const int nb = min(CB, n_ct - cb0);
     __syncthreads();  // the previous batch's planes are no longer read
     {
-      // the batch's planes: per plane, nb x 16 contiguous columns of CP halves
-      const int per = nb * 16 * CP / 4;  // 8-byte units (CP = RP + 4 is even)
+      // the batch's planes: per plane, nb x 16 contiguous columns of CP halves,
+      // moved in 16-byte units (16 columns are 32 CP bytes with CP = RP + 4 and
+      // RP a multiple of 16, so every plane and batch offset, in HBM and in LDS,
+      // is a multiple of 16 bytes)
+      const int per = nb * 16 * CP / 8;
       const size_t gplane = (size_t)n_ct * 16 * CP;
       const _Float16* src = (const _Float16*)b.cpl + (size_t)lane * 8 * gplane + (size_t)cb0 * 16 * CP;
       for (int e = threadIdx.x; e < 8 * per; e += blockDim.x) {
         const int q = e / per;
         const int o = e - q * per;
-        reinterpret_cast<uint2*>(cpl + (size_t)q * plane)[o] =
-            reinterpret_cast<const uint2*>(src + (size_t)q * gplane)[o];
+        reinterpret_cast<uint4*>(cpl + (size_t)q * plane)[o] =
+            reinterpret_cast<const uint4*>(src + (size_t)q * gplane)[o];
       }
     }
     __syncthreads();
