@@ -402,6 +402,16 @@ Batch energy_batch_desc(const Geo& g, const EnergyWs& ws, const double* s_dev, i
   return b;
 }
 
+// The split-precision planes of B lanes share one buffer: the C planes of every
+// lane, then the z1 planes of every lane.
+size_t mp_buffer_bytes(const Geo& g, int B, int N) {
+  return (mp_cplane_bytes(g) + mp_z1g_bytes(g, N)) * (size_t)B;
+}
+void mp_bind(Batch& b, const Geo& g, void* base, int B) {
+  b.cpl = base;
+  b.z1g = base != nullptr ? (char*)base + mp_cplane_bytes(g) * (size_t)B : nullptr;
+}
+
 int energy_batch(cco_energy* E, const double* s_dev, int B, int N, double* phi_dev,
                  double* grad_dev, double* re_dev, double* im_dev) {
   const Geo& g = E->geo;
@@ -424,9 +434,9 @@ int energy_batch(cco_energy* E, const double* s_dev, int B, int N, double* phi_d
   Batch b = energy_batch_desc(g, ws, s_dev, B, N, phi_dev, grad_dev, re_dev, im_dev);
   if (E->precision == 1 && g.mp_ok) {
     if ((rc = dev_reserve(E->prec, sizeof(int) * (size_t)B))) return rc;
-    if ((rc = dev_reserve(E->cpl, mp_cplane_bytes(g) * (size_t)B))) return rc;
+    if ((rc = dev_reserve(E->cpl, mp_buffer_bytes(g, B, N)))) return rc;
     if ((rc = dev_reserve(E->csig, sizeof(double) * (size_t)B))) return rc;
-    b.cpl = E->cpl.p;
+    mp_bind(b, g, E->cpl.p, B);
     b.csig = (double*)E->csig.p;
     if (E->prec_B < B) {
       hipLaunchKernelGGL(fill_int_kernel, dim3((B + 255) / 256), dim3(256), 0, st, (int*)E->prec.p,
@@ -612,7 +622,7 @@ Batch relax_batch(cco_relax* R, const RelaxDev& dv) {
   b.cbox = (double2*)R->cbox.p;
   b.phi_part = (double*)R->rphi.p;
   b.prec = R->dev.prec;
-  b.cpl = R->cpl.p;
+  mp_bind(b, R->E->geo, R->cpl.p, R->B);
   b.csig = (double*)R->csig.p;
   if (R->method == METHOD_MD && R->mc.w > 0) {  // the tracked constrained waves read re / im
     b.rbox = R->md_rbox;  // md_waves_kernel gathers the constrained waves from it
@@ -726,7 +736,7 @@ int resolve_variant(const cco_relax* R) {
 // Device buffers of the split-precision C planes, when the run uses them.
 int reserve_mp_planes(cco_relax* R) {
   if (R->dev.prec == nullptr) return CCO_OK;
-  int rc = dev_reserve(R->cpl, mp_cplane_bytes(R->E->geo) * (size_t)R->B);
+  int rc = dev_reserve(R->cpl, mp_buffer_bytes(R->E->geo, R->B, R->N));
   if (rc == CCO_OK) rc = dev_reserve(R->csig, sizeof(double) * (size_t)R->B);
   return rc;
 }
@@ -1794,7 +1804,7 @@ int cco_energy_profile(cco_energy* E, const double* s_dev, int B, int N, int rep
   b.cbox = (double2*)E->cbox.p;
   b.phi_part = (double*)E->rphi.p;
   b.prec = (E->precision == 1 && g.mp_ok) ? (const int*)E->prec.p : nullptr;
-  b.cpl = E->cpl.p;
+  mp_bind(b, g, E->cpl.p, B);
   b.csig = (double*)E->csig.p;
   hipStream_t st = E->ctx->stream;
   std::vector<hipEvent_t> ev(4 * (size_t)reps);

@@ -100,6 +100,7 @@ struct Batch {
   const int* skip = nullptr;  // [lane] nonzero: rho and reduce leave the lane alone (null: none)
   void* cpl = nullptr;        // split-precision C planes [lane][8][Cn padded to 16][RP + 4] fp16
   double* csig = nullptr;     // [lane] the C scale of those planes
+  void* z1g = nullptr;        // split-precision z1 planes [lane][64-particle chunk][4][P1 + 1][64] fp16
   // uneven batches (null: every lane uses the launch's Geo and the strides above)
   const LaneLayout* lay = nullptr;  // [lane]
   const Geo* geos = nullptr;        // [lane] geometry (rho: UT = 64 units, two threads per unit)
@@ -160,6 +161,7 @@ void launch_energy(const Geo& g, const Batch& b, int slots_ub, bool with_grad,
 void launch_rho_reduce(const Geo& g, const Batch& b, int slots_ub, hipStream_t stream);
 void launch_energy_stage(int stage, const Geo& g, const Batch& b, int slots_ub, hipStream_t st);
 size_t mp_cplane_bytes(const Geo& g);  // split-precision C planes of one lane
+size_t mp_z1g_bytes(const Geo& g, int N);  // split-precision z1 planes of one lane
 void launch_copy(const void* src, void* dst, size_t bytes, hipStream_t st);
 void launch_sk(const double* s, int n_cfg, int N, int d, const int4* dirs, const int* koff, int D,
                int Mk, int max_L, double* out, hipStream_t st);

@@ -122,18 +122,14 @@ __device__ __forceinline__ void split_phase(double x, _Float16& hi, _Float16& lo
   lo = (_Float16)(f - (float)hi);
 }
 
-// Tables of particles [jc, jc + nj), called by every thread of the block:
-// the unit phases z_a = exp(2 pi i s_a) once per particle and axis, then the
-// power tables in blocks of 8 powers spread over all threads, each block
-// seeded by z^{8q} (repeated squaring) and continued by recurrence, so no
-// thread runs a serial chain longer than 8 + q steps.  z1 powers are built for
-// particle pairs (packed 32-bit LDS stores) and rows < RL only.
+// Unit phases z_a = exp(2 pi i s_a) of particles [jc, jc + nj) for axes
+// [a0, a1), zeros past the chunk's end (every thread of the block).
 template <int D>
-__device__ void mp_fill(const MpTables& T, const double* s, int jc, int nj, int RL, int RP, int P2,
-                        int P3) {
-  for (int idx = threadIdx.x; idx < kMpChunk * D; idx += blockDim.x) {
+__device__ void mp_units(const MpTables& T, const double* s, int jc, int nj, int a0, int a1) {
+  const int n = kMpChunk * (a1 - a0);
+  for (int idx = threadIdx.x; idx < n; idx += blockDim.x) {
     const int jj = idx % kMpChunk;
-    const int ax = idx / kMpChunk;
+    const int ax = a0 + idx / kMpChunk;
     double2 z = make_double2(0.0, 0.0);
     if (jj < nj) {
       double sn, cs;
@@ -142,79 +138,137 @@ __device__ void mp_fill(const MpTables& T, const double* s, int jc, int nj, int 
     }
     T.zu[ax * kMpChunk + jj] = z;
   }
+}
+
+// The z1 planes of one chunk in HBM, [4][RL][kMpChunk] halves (re_hi, re_lo,
+// im_hi, im_lo; rows past RL are not stored): written once per evaluation by
+// mp_z1_kernel, read by every rho workgroup of the lane's column tiles and by
+// the force workgroups.
+__host__ __device__ inline size_t mp_z1_chunk_halves(int RL) { return (size_t)4 * RL * kMpChunk; }
+
+size_t mp_z1g_bytes(const Geo& g, int N) {
+  return (size_t)((N + kMpChunk - 1) / kMpChunk) * mp_z1_chunk_halves(g.P1 + 1) * sizeof(_Float16);
+}
+
+// LDS rows [0, RL) of the z1 planes from a chunk's HBM planes, in 16-byte
+// units (a row is 128 bytes in HBM, kZ1Pitch halves = 144 bytes in LDS);
+// zeros for a null source (a force group past the lane's last particle).
+__device__ void mp_z1_load(const MpTables& T, const _Float16* src, int RL, int RP) {
+  const int per = RL * (kMpChunk / 8);
+  for (int e = threadIdx.x; e < 4 * per; e += blockDim.x) {
+    const int plane = e / per;
+    const int rem = e - plane * per;
+    const int r = rem / (kMpChunk / 8);
+    const int c = rem % (kMpChunk / 8);
+    reinterpret_cast<uint4*>(T.z1p + (size_t)(plane * RP + r) * kZ1Pitch)[c] =
+        src != nullptr ? reinterpret_cast<const uint4*>(src)[e] : make_uint4(0u, 0u, 0u, 0u);
+  }
+}
+
+// Tables of particles [jc, jc + nj), called by every thread of the block: the
+// z1 planes copied from the chunk's HBM planes z1src (null: nj = 0, zeros),
+// then the unit phases of axes 2 and 3 and their power tables z2^p (p <= P2),
+// z3^p (p <= P3) in blocks of 8 powers, seeded and continued like the z1 rows.
+template <int D>
+__device__ void mp_fill(const MpTables& T, const double* s, const _Float16* z1src, int jc, int nj,
+                        int RL, int RP, int P2, int P3) {
+  mp_z1_load(T, z1src, RL, RP);
+  if (D == 1) return;  // the caller's barrier orders the copy before use
+  mp_units<D>(T, s, jc, nj, 1, D);
   __syncthreads();
   const double tscale = D == 2 ? 4096.0 : 64.0;
-  constexpr int kPairs = kMpChunk / 2;
-  const int b1 = (RL + 7) / 8;
   const int b2 = D >= 2 ? (P2 + 8) / 8 : 0;  // powers 0 .. P2
   const int b3 = D >= 3 ? (P3 + 8) / 8 : 0;
-  const int n1 = kPairs * b1;
-  for (int idx = threadIdx.x; idx < n1 + kMpChunk * (b2 + b3); idx += blockDim.x) {
-    if (idx < n1) {
-      const int jp = idx % kPairs;
-      const int q = idx / kPairs;
-      const double2 za = T.zu[2 * jp], zb = T.zu[2 * jp + 1];
-      double2 pa = make_double2(2 * jp < nj ? 1.0 : 0.0, 0.0);  // particles past the chunk: zeros
-      double2 pb = make_double2(2 * jp + 1 < nj ? 1.0 : 0.0, 0.0);
-      if (q > 0) {
-        double2 a8 = cmul(za, za), b8 = cmul(zb, zb);
-        a8 = cmul(a8, a8);
-        b8 = cmul(b8, b8);
-        a8 = cmul(a8, a8);
-        b8 = cmul(b8, b8);
-        for (int i = 0; i < q; ++i) {
-          pa = cmul(pa, a8);
-          pb = cmul(pb, b8);
-        }
+  for (int e = threadIdx.x; e < kMpChunk * (b2 + b3); e += blockDim.x) {
+    const int jj = e % kMpChunk;
+    int q = e / kMpChunk;
+    const int ax = q < b2 ? 1 : 2;
+    if (ax == 2) q -= b2;
+    const double2 z = T.zu[ax * kMpChunk + jj];
+    double2 p = make_double2(jj < nj ? 1.0 : 0.0, 0.0);
+    if (q > 0) {
+      double2 z8 = cmul(z, z);
+      z8 = cmul(z8, z8);
+      z8 = cmul(z8, z8);
+      for (int i = 0; i < q; ++i) p = cmul(p, z8);
+    }
+    float2* t = (ax == 1) ? (T.t2 + jj * T.w2p) : (T.t3 + jj * T.w3p);
+    const int r1 = min(8 * q + 8, (ax == 1 ? P2 : P3) + 1);
+    const bool planes = ax == 1 && T.t2p != nullptr;  // d = 2: also the fp16 operand planes
+    const size_t pl = (size_t)(P2 + 1) * kZ1Pitch;
+    for (int r = 8 * q; r < r1; ++r) {
+      t[r] = make_float2((float)(tscale * p.x), (float)(tscale * p.y));
+      if (planes) {
+        _Float16 rh, rl, ih, il;
+        split_phase(p.x, rh, rl);
+        split_phase(p.y, ih, il);
+        _Float16* o = T.t2p + (size_t)r * kZ1Pitch + jj;
+        o[0] = rh;
+        o[pl] = rl;
+        o[2 * pl] = ih;
+        o[3 * pl] = il;
       }
-      const int r1 = min(8 * q + 8, RL);
-      for (int r = 8 * q; r < r1; ++r) {
-        _Float16 ah, al, bh, bl, ch, cl, dh, dl;
-        split_phase(pa.x, ah, al);
-        split_phase(pb.x, bh, bl);
-        split_phase(pa.y, ch, cl);
-        split_phase(pb.y, dh, dl);
-        typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
-        half2_t* row = reinterpret_cast<half2_t*>(T.z1p + (size_t)r * kZ1Pitch) + jp;
-        const size_t pl = (size_t)RP * kZ1Pitch / 2;  // one plane in half2 units
-        row[0] = half2_t{ah, bh};
-        row[pl] = half2_t{al, bl};
-        row[2 * pl] = half2_t{ch, dh};
-        row[3 * pl] = half2_t{cl, dl};
-        pa = cmul(pa, za);
-        pb = cmul(pb, zb);
-      }
-    } else {
-      const int e = idx - n1;
-      const int jj = e % kMpChunk;
-      int q = e / kMpChunk;
-      const int ax = q < b2 ? 1 : 2;
-      if (ax == 2) q -= b2;
-      const double2 z = T.zu[ax * kMpChunk + jj];
-      double2 p = make_double2(jj < nj ? 1.0 : 0.0, 0.0);
-      if (q > 0) {
-        double2 z8 = cmul(z, z);
-        z8 = cmul(z8, z8);
-        z8 = cmul(z8, z8);
-        for (int i = 0; i < q; ++i) p = cmul(p, z8);
-      }
-      float2* t = (ax == 1) ? (T.t2 + jj * T.w2p) : (T.t3 + jj * T.w3p);
-      const int r1 = min(8 * q + 8, (ax == 1 ? P2 : P3) + 1);
-      const bool planes = ax == 1 && T.t2p != nullptr;  // d = 2: also the fp16 operand planes
-      const size_t pl = (size_t)(P2 + 1) * kZ1Pitch;
-      for (int r = 8 * q; r < r1; ++r) {
-        t[r] = make_float2((float)(tscale * p.x), (float)(tscale * p.y));
-        if (planes) {
-          _Float16 rh, rl, ih, il;
-          split_phase(p.x, rh, rl);
-          split_phase(p.y, ih, il);
-          _Float16* o = T.t2p + (size_t)r * kZ1Pitch + jj;
-          o[0] = rh;
-          o[pl] = rl;
-          o[2 * pl] = ih;
-          o[3 * pl] = il;
-        }
-        p = cmul(p, z);
+      p = cmul(p, z);
+    }
+  }
+}
+
+constexpr int kZ1Chunks = 4;  // chunks per mp_z1_kernel workgroup (one thread per particle)
+
+// grid: slots_ub x ceil(N / (64 kZ1Chunks)); block 256.  The z1 planes of a
+// split-precision lane, once per evaluation: the rho workgroups of all the
+// lane's column tiles, and the force workgroups, copy them instead of each
+// rebuilding them.  One thread per particle builds all its rows: z^{8q} by
+// repeated squaring and q multiplications, rows 8 q .. 8 q + 7 from it by
+// recurrence (the chain of every row block starts again from its seed, so no
+// error grows over more than 8 steps).  The row blocks advance together, one
+// row of each per step, so a thread has up to 6 independent chains in flight;
+// each row is stored straight to HBM (a wave's store covers one 128-byte
+// plane row).
+template <int D>
+__global__ __launch_bounds__(256) void mp_z1_kernel(Geo g, Batch b) {
+  constexpr int kBlocks = 6;  // row blocks of 8 for up to 48 rows
+  const int n_ch = (b.N + kMpChunk - 1) / kMpChunk;
+  const int n_wg = (n_ch + kZ1Chunks - 1) / kZ1Chunks;
+  const int slot = blockIdx.x / n_wg;
+  if (slot >= *b.n_active) return;
+  const int lane = b.active[slot];
+  if (b.prec[lane] == 0) return;
+  const int ch = (blockIdx.x % n_wg) * kZ1Chunks + (int)threadIdx.x / kMpChunk;
+  if (ch >= n_ch) return;  // no barrier below
+  const int jj = (int)threadIdx.x % kMpChunk;
+  const int j = ch * kMpChunk + jj;
+  const int RL = g.P1 + 1;
+  double2 z = make_double2(0.0, 0.0);
+  if (j < b.N) {
+    double sn, cs;
+    sincospi(2.0 * b.s[(size_t)lane * b.lane_stride + (size_t)j * D], &sn, &cs);
+    z = make_double2(cs, sn);
+  }
+  double2 z8 = cmul(z, z);
+  z8 = cmul(z8, z8);
+  z8 = cmul(z8, z8);
+  double2 p[kBlocks];
+  p[0] = make_double2(j < b.N ? 1.0 : 0.0, 0.0);  // particles past the lane's end: zeros
+#pragma unroll
+  for (int q = 1; q < kBlocks; ++q) p[q] = cmul(p[q - 1], z8);
+  _Float16* dst = (_Float16*)b.z1g + ((size_t)lane * n_ch + ch) * mp_z1_chunk_halves(RL) + jj;
+  const size_t pl = (size_t)RL * kMpChunk;  // one plane in halves
+#pragma unroll
+  for (int r = 0; r < 8; ++r) {
+#pragma unroll
+    for (int q = 0; q < kBlocks; ++q) {
+      const int row = 8 * q + r;
+      if (row < RL) {
+        _Float16 rh, rl, ih, il;
+        split_phase(p[q].x, rh, rl);
+        split_phase(p[q].y, ih, il);
+        _Float16* o = dst + (size_t)row * kMpChunk;
+        o[0] = rh;
+        o[pl] = rl;
+        o[2 * pl] = ih;
+        o[3 * pl] = il;
+        p[q] = cmul(p[q], z);
       }
     }
   }
@@ -295,10 +349,13 @@ __global__ __launch_bounds__(256) void rho_mp_kernel(Geo g, Batch b) {
       sum_im[rt][i] = 0.0;
     }
   const float4_t z4 = {0.f, 0.f, 0.f, 0.f};
-  mp_zero_pad_rows(T, g.P1 + 1, g.RP);  // ordered before use by mp_fill's barrier
+  const int n_ch = (b.N + kMpChunk - 1) / kMpChunk;
+  const _Float16* z1_lane = (const _Float16*)b.z1g + (size_t)lane * n_ch * mp_z1_chunk_halves(g.P1 + 1);
+  mp_zero_pad_rows(T, g.P1 + 1, g.RP);  // ordered before use by the barrier after mp_fill
   for (int jc = j0; jc < j1; jc += kMpChunk) {
     const int nj = min(kMpChunk, j1 - jc);
-    mp_fill<D>(T, s, jc, nj, g.P1 + 1, g.RP, g.P2, g.P3);
+    mp_fill<D>(T, s, z1_lane + (size_t)(jc / kMpChunk) * mp_z1_chunk_halves(g.P1 + 1), jc, nj,
+               g.P1 + 1, g.RP, g.P2, g.P3);
     __syncthreads();
     // FP32 accumulators over the chunk: hi*hi and cross terms apart
     float4_t rb[NRT], rs[NRT], ib[NRT], is[NRT];
@@ -516,12 +573,16 @@ __global__ __launch_bounds__(256) void force_mp_kernel(Geo g, Batch b) {
   // A operands: particle 16 wave + m16 of group p (m), rows 16 rk + 4 g16 + i (k)
   half4_t are_h[PG][NRT], are_l[PG][NRT], aim_h[PG][NRT], aim_l[PG][NRT];
   const int jA = wave * 16 + m16;
-  mp_zero_pad_rows(tables(0), g.P1 + 1, RP);  // ordered before use by mp_fill's barrier
+  const int n_ch = (b.N + kMpChunk - 1) / kMpChunk;
+  const _Float16* z1_lane = (const _Float16*)b.z1g + (size_t)lane * n_ch * mp_z1_chunk_halves(g.P1 + 1);
+  mp_zero_pad_rows(tables(0), g.P1 + 1, RP);  // ordered before use by the barrier after mp_fill
 #pragma unroll
   for (int p = 0; p < PG; ++p) {
     const int jc = pt * PW + p * kMpChunk;
     const int nj = max(0, min(kMpChunk, b.N - jc));
-    mp_fill<D>(tables(p), s, jc, nj, g.P1 + 1, RP, g.P2, g.P3);
+    const _Float16* z1src =
+        nj > 0 ? z1_lane + (size_t)(jc / kMpChunk) * mp_z1_chunk_halves(g.P1 + 1) : nullptr;
+    mp_fill<D>(tables(p), s, z1src, jc, nj, g.P1 + 1, RP, g.P2, g.P3);
     __syncthreads();
     const _Float16* z1p = tables(p).z1p;
 #pragma unroll
@@ -665,6 +726,9 @@ static void launch_force_mp(const Geo& g, const Batch& b, int slots_ub, hipStrea
 template <int D, int NRT>
 static void launch_mp(int stage, const Geo& g, const Batch& b, int slots_ub, hipStream_t st) {
   if (stage == 0) {
+    const int n_ch = (b.N + kMpChunk - 1) / kMpChunk;
+    hipLaunchKernelGGL((mp_z1_kernel<D>), dim3(slots_ub * ((n_ch + kZ1Chunks - 1) / kZ1Chunks)),
+                       dim3(kMpChunk * kZ1Chunks), 0, st, g, b);
     hipLaunchKernelGGL((rho_mp_kernel<D, NRT>), dim3(slots_ub * g.n_mp_ctiles * b.n_splits),
                        dim3(256), mp_table_bytes(g.RP, g.P2, g.P3, D == 2), st, g, b);
   } else {
