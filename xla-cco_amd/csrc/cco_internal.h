@@ -98,7 +98,7 @@ struct Batch {
   double2* rbox = nullptr;  // rho per box entry [lane][Mbox] (null: not written; uniform only)
   const int* prec = nullptr;  // [lane] 1 = split-precision MFMA path (null: every lane FP64)
   const int* skip = nullptr;  // [lane] nonzero: rho and reduce leave the lane alone (null: none)
-  void* cpl = nullptr;        // split-precision C planes [lane][8][Cn padded to 16][RP + 4] fp16
+  void* cpl = nullptr;        // split-precision C planes [lane][8][Cn padded to 16][RP + 8] fp16
   double* csig = nullptr;     // [lane] the C scale of those planes
   void* z1g = nullptr;        // split-precision z1 planes [lane][64-particle chunk][4][P1 + 1][64] fp16
   // uneven batches (null: every lane uses the launch's Geo and the strides above)

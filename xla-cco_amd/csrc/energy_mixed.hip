@@ -35,6 +35,8 @@ namespace cco {
 typedef _Float16 half4_t __attribute__((ext_vector_type(4)));
 typedef _Float16 half8_t __attribute__((ext_vector_type(8)));
 typedef float float4_t __attribute__((ext_vector_type(4)));
+typedef unsigned u32x4_t __attribute__((ext_vector_type(4)));
+typedef int i32x4_t __attribute__((ext_vector_type(4)));
 
 constexpr double kPhaseScale = 4096.0;  // 2^12 on every phase operand
 constexpr double kCScale = 1024.0;      // 2^10 on C (and r C, r < 48)
@@ -288,21 +290,52 @@ __device__ __forceinline__ float2 mp_bcol(const MpTables& T, int jj, int a2, flo
   return e;
 }
 
-// force: C planes for a batch of 16-column tiles, [8][16 cb][RP + 4] halves:
-// C re_hi, re_lo, im_hi, im_lo, then r C in the same order.
-__host__ __device__ inline int mp_cpitch(int RP) { return RP + 4; }
+constexpr int kForcePf = 6;  // 16-byte units of the next C batch each force thread holds
 
-__host__ __device__ inline int mp_force_batch(int RP, int P2, int P3) {
-  const size_t avail = 64 * 1024 - mp_table_bytes(RP, P2, P3);
-  const size_t per = (size_t)8 * 16 * mp_cpitch(RP) * sizeof(_Float16);
-  const int cb = (int)(avail / per);
-  return cb < 1 ? 0 : (cb > 8 ? 8 : cb);
+// force: C planes for a batch of 16-column tiles, [8][16 cb][RP + 8] halves:
+// C re_hi, re_lo, im_hi, im_lo, then r C in the same order.  A column is 16
+// bytes longer than its rows, so the 16-byte operand reads of 8 lanes with
+// consecutive columns fall on distinct LDS banks.
+__host__ __device__ inline int mp_cpitch(int RP) { return RP + 8; }
+
+// LDS of the force kernel with pg particle groups and cb C tiles per batch:
+// the z2 / z3 power tables of every group, then one region used first by the
+// fills (z1 planes and unit phases, one group at a time) and then, once the
+// z1 operands are in registers, by the C batches.
+__host__ __device__ inline size_t mp_force_group_bytes(int P2, int P3) {
+  return (size_t)kMpChunk * (mp_pad(P2 + 1) + mp_pad(P3 + 1)) * sizeof(float2);
+}
+__host__ __device__ inline size_t mp_force_fill_bytes(int RP) {
+  return (size_t)4 * RP * kZ1Pitch * sizeof(_Float16) + (size_t)3 * kMpChunk * sizeof(double2);
+}
+// a C tile in LDS: its 8 planes and its 16 column descriptors
+__host__ __device__ inline size_t mp_force_tile_bytes(int RP) {
+  return (size_t)8 * 16 * mp_cpitch(RP) * sizeof(_Float16) + 16 * sizeof(int4);
+}
+__host__ __device__ inline size_t mp_force_lds(int RP, int P2, int P3, int pg, int cb) {
+  const size_t batch = (size_t)cb * mp_force_tile_bytes(RP);
+  const size_t fill = mp_force_fill_bytes(RP);
+  return (size_t)pg * mp_force_group_bytes(P2, P3) + (batch > fill ? batch : fill);
 }
 
+// C tiles per batch that the force threads' prefetch registers hold
+__host__ __device__ inline int mp_force_cb_cap(int RP) {
+  const int cap = 16 * kForcePf / mp_cpitch(RP);
+  return cap < 1 ? 1 : cap;
+}
+
+// C tiles per batch of the one-group layout within 64 KB (0: it does not fit)
+__host__ __device__ inline int mp_force_batch(int RP, int P2, int P3) {
+  const size_t grp = mp_force_group_bytes(P2, P3);
+  if (grp + mp_force_fill_bytes(RP) > 64 * 1024) return 0;
+  const int cb = (int)((64 * 1024 - grp) / mp_force_tile_bytes(RP));
+  const int cap = mp_force_cb_cap(RP);
+  return cb < 1 ? 0 : (cb > cap ? cap : cb);
+}
+
+// LDS of the one-group force layout
 size_t mp_lds_bytes(const Geo& g) {
-  const int cb = mp_force_batch(g.RP, g.P2, g.P3);
-  return mp_table_bytes(g.RP, g.P2, g.P3) +
-         (size_t)8 * 16 * cb * mp_cpitch(g.RP) * sizeof(_Float16);
+  return mp_force_lds(g.RP, g.P2, g.P3, 1, mp_force_batch(g.RP, g.P2, g.P3));
 }
 
 // The C planes of a lane, converted once per evaluation by mp_cplanes_kernel:
@@ -499,35 +532,79 @@ __global__ __launch_bounds__(256) void mp_cplanes_kernel(Geo g, Batch b) {
   }
 }
 
-// LDS of the force kernel with pg particle groups and cb C tiles per batch
-// (pg = 1 with cb = mp_force_batch is the one-group layout within 64 KB):
-// the z1 planes (one group at a time), the z2 / z3 power tables of every
-// group, the fill's unit phases, then the C batch.
-__host__ __device__ inline size_t mp_force_group_bytes(int P2, int P3) {
-  return (size_t)kMpChunk * (mp_pad(P2 + 1) + mp_pad(P3 + 1)) * sizeof(float2);
-}
-__host__ __device__ inline size_t mp_force_zu_offset(int RP, int P2, int P3, int pg) {
-  const size_t end = (size_t)4 * RP * kZ1Pitch * sizeof(_Float16) + (size_t)pg * mp_force_group_bytes(P2, P3);
-  return (end + 15) & ~(size_t)15;
-}
-__host__ __device__ inline size_t mp_force_lds(int RP, int P2, int P3, int pg, int cb) {
-  return mp_force_zu_offset(RP, P2, P3, pg) + (size_t)3 * kMpChunk * sizeof(double2) +
-         (size_t)8 * 16 * cb * mp_cpitch(RP) * sizeof(_Float16);
-}
-
 void mp_force_plan(const Geo& g, int* pg, int* cb) {
-  // two particle groups per workgroup when their tables and one C tile fit
-  // half a CU's LDS (two workgroups per CU), else one group in 64 KB
-  const size_t per_tile = (size_t)8 * 16 * mp_cpitch(g.RP) * sizeof(_Float16);
+  // two particle groups per workgroup when their tables, the fill and one C
+  // tile fit half a CU's LDS (two workgroups per CU), else one group in 64 KB
   const size_t budget = 78 * 1024;
-  const size_t fixed = mp_force_lds(g.RP, g.P2, g.P3, 2, 0);
-  if (fixed + per_tile <= budget) {
+  const size_t tables = 2 * mp_force_group_bytes(g.P2, g.P3);
+  if (mp_force_lds(g.RP, g.P2, g.P3, 2, 1) <= budget) {
     *pg = 2;
-    *cb = (int)std::min<size_t>(8, (budget - fixed) / per_tile);
+    *cb = (int)std::min<size_t>(mp_force_cb_cap(g.RP), (budget - tables) / mp_force_tile_bytes(g.RP));
     return;
   }
   *pg = 1;
   *cb = mp_force_batch(g.RP, g.P2, g.P3);
+}
+
+// The 24 split products of one K tile of the force GEMMs: H += conj(A) C and
+// H1 += conj(A) (r C), Re = ar cr + ai ci, Im = ar ci - ai cr, each real
+// product as hi*hi + hi*lo + lo*hi.  a = z1 planes (re_hi, re_lo, im_hi,
+// im_lo), c = C planes then r C planes in the same order; HV = half8_t for a
+// 32-row tile (16x16x32 MFMA), half4_t for a 16-row tile (16x16x16).
+template <typename HV>
+__device__ __forceinline__ void mp_force_macs(const HV (&a)[4], const HV (&c)[8], float4_t& hre,
+                                              float4_t& him, float4_t& h1re, float4_t& h1im) {
+  auto mac = [](HV x, HV y, float4_t acc) {
+    if constexpr (sizeof(HV) == 16)
+      return MFMA32(x, y, acc);
+    else
+      return MFMA16(x, y, acc);
+  };
+  const HV ah = a[0], al = a[1], ih = a[2], il = a[3];
+  const HV nih = -ih, nil = -il;
+  hre = mac(ah, c[0], hre);
+  hre = mac(ah, c[1], hre);
+  hre = mac(al, c[0], hre);
+  hre = mac(ih, c[2], hre);
+  hre = mac(ih, c[3], hre);
+  hre = mac(il, c[2], hre);
+  him = mac(ah, c[2], him);
+  him = mac(ah, c[3], him);
+  him = mac(al, c[2], him);
+  him = mac(nih, c[0], him);
+  him = mac(nih, c[1], him);
+  him = mac(nil, c[0], him);
+  h1re = mac(ah, c[4], h1re);
+  h1re = mac(ah, c[5], h1re);
+  h1re = mac(al, c[4], h1re);
+  h1re = mac(ih, c[6], h1re);
+  h1re = mac(ih, c[7], h1re);
+  h1re = mac(il, c[6], h1re);
+  h1im = mac(ah, c[6], h1im);
+  h1im = mac(ah, c[7], h1im);
+  h1im = mac(al, c[6], h1im);
+  h1im = mac(nih, c[4], h1im);
+  h1im = mac(nih, c[5], h1im);
+  h1im = mac(nil, c[4], h1im);
+}
+
+// Loads C batch cb0 of a lane (its planes and column descriptors) into the
+// thread's prefetch registers.
+__device__ __forceinline__ void mp_force_fetch(const Geo& g, const u32x4_t* src_lane, size_t gplane,
+                                               int CP, int CB, int n_ct, int cb0, u32x4_t (&pf)[kForcePf],
+                                               i32x4_t& pcol) {
+  const int nb = min(CB, n_ct - cb0);
+  const int per = nb * 16 * CP / 8;
+  if ((int)threadIdx.x < 16 * nb) pcol = reinterpret_cast<const i32x4_t*>(g.gcol)[cb0 * 16 + (int)threadIdx.x];
+#pragma unroll
+  for (int k = 0; k < kForcePf; ++k) {
+    const int e = (int)threadIdx.x + k * 256;
+    if (e < 8 * per) {
+      const int q = e / per;
+      const int o = e - q * per;
+      pf[k] = src_lane[(q * gplane + (size_t)cb0 * 16 * CP) / 8 + o];
+    }
+  }
 }
 
 // grid: slots_ub x ceil(N / (64 PG)); block: 4 waves.  Group p of the
@@ -555,23 +632,27 @@ __global__ __launch_bounds__(256) void force_mp_kernel(Geo g, Batch b) {
   const int CP = mp_cpitch(RP);
   const int CB = g.mp_cb;
   const size_t plane = (size_t)16 * CB * CP;
-  const size_t z1_bytes = (size_t)4 * RP * kZ1Pitch * sizeof(_Float16);
   const size_t grp_bytes = mp_force_group_bytes(g.P2, g.P3);
-  const size_t zu_off = mp_force_zu_offset(RP, g.P2, g.P3, PG);
-  _Float16* cpl = (_Float16*)(smem + zu_off + (size_t)3 * kMpChunk * sizeof(double2));
+  char* region = smem + (size_t)PG * grp_bytes;  // fills, then C batches
+  _Float16* cpl = (_Float16*)region;
   auto tables = [&](int p) {
     MpTables T;
-    T.z1p = (_Float16*)smem;
+    T.z1p = (_Float16*)region;
     T.w2p = mp_pad(g.P2 + 1);
     T.w3p = mp_pad(g.P3 + 1);
-    T.t2 = (float2*)(smem + z1_bytes + (size_t)p * grp_bytes);
+    T.t2 = (float2*)(smem + (size_t)p * grp_bytes);
     T.t3 = T.t2 + (size_t)kMpChunk * T.w2p;
-    T.zu = (double2*)(smem + zu_off);
+    T.zu = (double2*)(region + (size_t)4 * RP * kZ1Pitch * sizeof(_Float16));
     T.t2p = nullptr;
     return T;
   };
-  // A operands: particle 16 wave + m16 of group p (m), rows 16 rk + 4 g16 + i (k)
-  half4_t are_h[PG][NRT], are_l[PG][NRT], aim_h[PG][NRT], aim_l[PG][NRT];
+  // A operands: particle 16 wave + m16 of group p (m) and the rows (k) of
+  // K32 tiles of 32 rows (rows 32 t + 8 g16 + i, 16x16x32 MFMA), then K16
+  // tiles of 16 (rows 32 K32 + 4 g16 + i, 16x16x16 MFMA)
+  constexpr int K32 = NRT / 2;
+  constexpr int K16 = NRT % 2;
+  half8_t a8[PG][K32 > 0 ? K32 : 1][4];  // [group][tile][re_hi, re_lo, im_hi, im_lo]
+  half4_t a4[PG][4];
   const int jA = wave * 16 + m16;
   const int n_ch = (b.N + kMpChunk - 1) / kMpChunk;
   const _Float16* z1_lane = (const _Float16*)b.z1g + (size_t)lane * n_ch * mp_z1_chunk_halves(g.P1 + 1);
@@ -586,14 +667,14 @@ __global__ __launch_bounds__(256) void force_mp_kernel(Geo g, Batch b) {
     __syncthreads();
     const _Float16* z1p = tables(p).z1p;
 #pragma unroll
-    for (int rk = 0; rk < NRT; ++rk) {
+    for (int c = 0; c < 4; ++c) {
 #pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int r = rk * 16 + 4 * g16 + i;
-        are_h[p][rk][i] = z1p[(0 * RP + r) * kZ1Pitch + jA];
-        are_l[p][rk][i] = z1p[(1 * RP + r) * kZ1Pitch + jA];
-        aim_h[p][rk][i] = z1p[(2 * RP + r) * kZ1Pitch + jA];
-        aim_l[p][rk][i] = z1p[(3 * RP + r) * kZ1Pitch + jA];
+      for (int t = 0; t < K32; ++t)
+#pragma unroll
+        for (int i = 0; i < 8; ++i) a8[p][t][c][i] = z1p[(c * RP + 32 * t + 8 * g16 + i) * kZ1Pitch + jA];
+      if constexpr (K16 > 0) {
+#pragma unroll
+        for (int i = 0; i < 4; ++i) a4[p][c][i] = z1p[(c * RP + 32 * K32 + 4 * g16 + i) * kZ1Pitch + jA];
       }
     }
     __syncthreads();  // the next group's fill rewrites the z1 planes
@@ -604,76 +685,65 @@ __global__ __launch_bounds__(256) void force_mp_kernel(Geo g, Batch b) {
   for (int p = 0; p < PG; ++p)
 #pragma unroll
     for (int i = 0; i < 4; ++i) g1[p][i] = g2[p][i] = g3[p][i] = 0.0;
+  // C batches: per plane, nb x 16 contiguous columns of CP halves, moved in
+  // 16-byte units (16 columns are 32 CP bytes with CP = RP + 8 and RP a
+  // multiple of 16, so every plane and batch offset, in HBM and in LDS, is a
+  // multiple of 16 bytes).  The next batch is loaded into registers while the
+  // current one is in use (at most kForcePf units per thread: the plan keeps
+  // 8 cb 16 CP / 8 <= 256 kForcePf).
   const int n_ct = (g.Cn + 15) / 16;
+  const size_t gplane = (size_t)n_ct * 16 * CP;
+  const u32x4_t* src_lane = reinterpret_cast<const u32x4_t*>((const _Float16*)b.cpl + (size_t)lane * 8 * gplane);
+  // the batch's column descriptors follow its planes in LDS, so the tile loop
+  // issues no global load (which would wait for the prefetch in flight)
+  int4* cols = reinterpret_cast<int4*>(cpl + 8 * plane);
+  u32x4_t pf[kForcePf];
+  i32x4_t pcol = {0, 0, 0, 0};
+  mp_force_fetch(g, src_lane, gplane, CP, CB, n_ct, 0, pf, pcol);
   for (int cb0 = 0; cb0 < n_ct; cb0 += CB) {
     const int nb = min(CB, n_ct - cb0);
     __syncthreads();  // the previous batch's planes are no longer read
     {
-      // the batch's planes: per plane, nb x 16 contiguous columns of CP halves,
-      // moved in 16-byte units (16 columns are 32 CP bytes with CP = RP + 4 and
-      // RP a multiple of 16, so every plane and batch offset, in HBM and in LDS,
-      // is a multiple of 16 bytes)
       const int per = nb * 16 * CP / 8;
-      const size_t gplane = (size_t)n_ct * 16 * CP;
-      const _Float16* src = (const _Float16*)b.cpl + (size_t)lane * 8 * gplane + (size_t)cb0 * 16 * CP;
-      for (int e = threadIdx.x; e < 8 * per; e += blockDim.x) {
-        const int q = e / per;
-        const int o = e - q * per;
-        reinterpret_cast<uint4*>(cpl + (size_t)q * plane)[o] =
-            reinterpret_cast<const uint4*>(src + (size_t)q * gplane)[o];
+#pragma unroll
+      for (int k = 0; k < kForcePf; ++k) {
+        const int e = (int)threadIdx.x + k * 256;
+        if (e < 8 * per) {
+          const int q = e / per;
+          const int o = e - q * per;
+          reinterpret_cast<u32x4_t*>(cpl + (size_t)q * plane)[o] = pf[k];
+        }
       }
+      if ((int)threadIdx.x < 16 * nb) reinterpret_cast<i32x4_t*>(cols)[threadIdx.x] = pcol;
     }
     __syncthreads();
+    if (cb0 + CB < n_ct) mp_force_fetch(g, src_lane, gplane, CP, CB, n_ct, cb0 + CB, pf, pcol);
     for (int tb = 0; tb < nb; ++tb) {
-      const int ct = cb0 + tb;
       float4_t hre[PG], him[PG], h1re[PG], h1im[PG];
 #pragma unroll
       for (int p = 0; p < PG; ++p) hre[p] = him[p] = h1re[p] = h1im[p] = float4_t{0.f, 0.f, 0.f, 0.f};
+      const _Float16* qc = cpl + (size_t)(tb * 16 + m16) * CP;  // column 16 tb + m16
 #pragma unroll
-      for (int rk = 0; rk < NRT; ++rk) {
-        // B operand: rows 16 rk + 4 g16 + i of column 16 tb + m16
-        const _Float16* q = cpl + (size_t)(tb * 16 + m16) * CP + rk * 16 + 4 * g16;
-        const half4_t cre_h = *(const half4_t*)(q + 0 * plane);
-        const half4_t cre_l = *(const half4_t*)(q + 1 * plane);
-        const half4_t cim_h = *(const half4_t*)(q + 2 * plane);
-        const half4_t cim_l = *(const half4_t*)(q + 3 * plane);
-        const half4_t dre_h = *(const half4_t*)(q + 4 * plane);
-        const half4_t dre_l = *(const half4_t*)(q + 5 * plane);
-        const half4_t dim_h = *(const half4_t*)(q + 6 * plane);
-        const half4_t dim_l = *(const half4_t*)(q + 7 * plane);
+      for (int t = 0; t < K32; ++t) {
+        // B operands: rows 32 t + 8 g16 + i, one 16-byte read per plane
+        const _Float16* q = qc + 32 * t + 8 * g16;
+        half8_t c[8];
 #pragma unroll
-        for (int p = 0; p < PG; ++p) {
-          const half4_t ah = are_h[p][rk], al = are_l[p][rk], ih = aim_h[p][rk], il = aim_l[p][rk];
-          const half4_t nih = -ih, nil = -il;
-          // conj(a) c: Re = ar cr + ai ci, Im = ar ci - ai cr
-          hre[p] = MFMA16(ah, cre_h, hre[p]);
-          hre[p] = MFMA16(ah, cre_l, hre[p]);
-          hre[p] = MFMA16(al, cre_h, hre[p]);
-          hre[p] = MFMA16(ih, cim_h, hre[p]);
-          hre[p] = MFMA16(ih, cim_l, hre[p]);
-          hre[p] = MFMA16(il, cim_h, hre[p]);
-          him[p] = MFMA16(ah, cim_h, him[p]);
-          him[p] = MFMA16(ah, cim_l, him[p]);
-          him[p] = MFMA16(al, cim_h, him[p]);
-          him[p] = MFMA16(nih, cre_h, him[p]);
-          him[p] = MFMA16(nih, cre_l, him[p]);
-          him[p] = MFMA16(nil, cre_h, him[p]);
-          h1re[p] = MFMA16(ah, dre_h, h1re[p]);
-          h1re[p] = MFMA16(ah, dre_l, h1re[p]);
-          h1re[p] = MFMA16(al, dre_h, h1re[p]);
-          h1re[p] = MFMA16(ih, dim_h, h1re[p]);
-          h1re[p] = MFMA16(ih, dim_l, h1re[p]);
-          h1re[p] = MFMA16(il, dim_h, h1re[p]);
-          h1im[p] = MFMA16(ah, dim_h, h1im[p]);
-          h1im[p] = MFMA16(ah, dim_l, h1im[p]);
-          h1im[p] = MFMA16(al, dim_h, h1im[p]);
-          h1im[p] = MFMA16(nih, dre_h, h1im[p]);
-          h1im[p] = MFMA16(nih, dre_l, h1im[p]);
-          h1im[p] = MFMA16(nil, dre_h, h1im[p]);
-        }
+        for (int k = 0; k < 8; ++k) c[k] = *(const half8_t*)(q + k * plane);
+#pragma unroll
+        for (int p = 0; p < PG; ++p)
+          mp_force_macs<half8_t>(a8[p][t], c, hre[p], him[p], h1re[p], h1im[p]);
       }
-      // lane l holds H[particle 16 wave + 4 g16 + i of group p][column 16 ct + m16]
-      const int4 col = g.gcol[ct * 16 + m16];
+      if constexpr (K16 > 0) {
+        const _Float16* q = qc + 32 * K32 + 4 * g16;
+        half4_t c[8];
+#pragma unroll
+        for (int k = 0; k < 8; ++k) c[k] = *(const half4_t*)(q + k * plane);
+#pragma unroll
+        for (int p = 0; p < PG; ++p) mp_force_macs<half4_t>(a4[p], c, hre[p], him[p], h1re[p], h1im[p]);
+      }
+      // lane l holds H[particle 16 wave + 4 g16 + i of group p][column 16 (cb0 + tb) + m16]
+      const int4 col = cols[tb * 16 + m16];
       const int a2 = abs(col.z), a3 = abs(col.w);
       const float sg2 = col.z < 0 ? -1.0f : 1.0f;
       const float sg3 = col.w < 0 ? -1.0f : 1.0f;
