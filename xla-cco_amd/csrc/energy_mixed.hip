@@ -167,17 +167,11 @@ __device__ void mp_z1_load(const MpTables& T, const _Float16* src, int RL, int R
   }
 }
 
-// Tables of particles [jc, jc + nj), called by every thread of the block: the
-// z1 planes copied from the chunk's HBM planes z1src (null: nj = 0, zeros),
-// then the unit phases of axes 2 and 3 and their power tables z2^p (p <= P2),
-// z3^p (p <= P3) in blocks of 8 powers, seeded and continued like the z1 rows.
+// Power tables z2^p (p <= P2), z3^p (p <= P3) of the chunk's particles from
+// the unit phases of axes 2 and 3, in blocks of 8 powers seeded by z^{8q}
+// (repeated squaring) and continued by recurrence (every thread of the block).
 template <int D>
-__device__ void mp_fill(const MpTables& T, const double* s, const _Float16* z1src, int jc, int nj,
-                        int RL, int RP, int P2, int P3) {
-  mp_z1_load(T, z1src, RL, RP);
-  if (D == 1) return;  // the caller's barrier orders the copy before use
-  mp_units<D>(T, s, jc, nj, 1, D);
-  __syncthreads();
+__device__ void mp_powers(const MpTables& T, int nj, int P2, int P3) {
   const double tscale = D == 2 ? 4096.0 : 64.0;
   const int b2 = D >= 2 ? (P2 + 8) / 8 : 0;  // powers 0 .. P2
   const int b3 = D >= 3 ? (P3 + 8) / 8 : 0;
@@ -213,6 +207,19 @@ __device__ void mp_fill(const MpTables& T, const double* s, const _Float16* z1sr
       p = cmul(p, z);
     }
   }
+}
+
+// Tables of particles [jc, jc + nj), called by every thread of the block: the
+// z1 planes copied from the chunk's HBM planes z1src (null: nj = 0, zeros),
+// then the unit phases of axes 2 and 3 and their power tables.
+template <int D>
+__device__ void mp_fill(const MpTables& T, const double* s, const _Float16* z1src, int jc, int nj,
+                        int RL, int RP, int P2, int P3) {
+  mp_z1_load(T, z1src, RL, RP);
+  if (D == 1) return;  // the caller's barrier orders the copy before use
+  mp_units<D>(T, s, jc, nj, 1, D);
+  __syncthreads();
+  mp_powers<D>(T, nj, P2, P3);
 }
 
 constexpr int kZ1Chunks = 4;  // chunks per mp_z1_kernel workgroup (one thread per particle)
@@ -588,6 +595,11 @@ __device__ __forceinline__ void mp_force_macs(const HV (&a)[4], const HV (&c)[8]
   h1im = mac(nil, c[4], h1im);
 }
 
+// ceil(2^32 / d) for 2 <= d < 2^16: __umulhi(e, mp_recip(d)) == e / d for every
+// e < 2^16 (d is wave-uniform, so this is scalar work; the quotient is one
+// v_mul_hi_u32 instead of a software division per unit)
+__device__ __forceinline__ unsigned mp_recip(int d) { return (unsigned)(4294967296ull / (unsigned)d) + 1u; }
+
 // Loads C batch cb0 of a lane (its planes and column descriptors) into the
 // thread's prefetch registers.
 __device__ __forceinline__ void mp_force_fetch(const Geo& g, const u32x4_t* src_lane, size_t gplane,
@@ -595,12 +607,13 @@ __device__ __forceinline__ void mp_force_fetch(const Geo& g, const u32x4_t* src_
                                                i32x4_t& pcol) {
   const int nb = min(CB, n_ct - cb0);
   const int per = nb * 16 * CP / 8;
+  const unsigned inv = mp_recip(per);
   if ((int)threadIdx.x < 16 * nb) pcol = reinterpret_cast<const i32x4_t*>(g.gcol)[cb0 * 16 + (int)threadIdx.x];
 #pragma unroll
   for (int k = 0; k < kForcePf; ++k) {
     const int e = (int)threadIdx.x + k * 256;
     if (e < 8 * per) {
-      const int q = e / per;
+      const int q = (int)__umulhi((unsigned)e, inv);  // e / per
       const int o = e - q * per;
       pf[k] = src_lane[(q * gplane + (size_t)cb0 * 16 * CP) / 8 + o];
     }
@@ -705,11 +718,12 @@ __global__ __launch_bounds__(256) void force_mp_kernel(Geo g, Batch b) {
     __syncthreads();  // the previous batch's planes are no longer read
     {
       const int per = nb * 16 * CP / 8;
+      const unsigned inv = mp_recip(per);
 #pragma unroll
       for (int k = 0; k < kForcePf; ++k) {
         const int e = (int)threadIdx.x + k * 256;
         if (e < 8 * per) {
-          const int q = e / per;
+          const int q = (int)__umulhi((unsigned)e, inv);  // e / per
           const int o = e - q * per;
           reinterpret_cast<u32x4_t*>(cpl + (size_t)q * plane)[o] = pf[k];
         }
