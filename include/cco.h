@@ -89,6 +89,12 @@ int cco_value_grad_density_dev(cco_energy* e, const double* s_dev, int B, int N,
  * chunk, so results are bit-identical; shorter chunks leave room for more
  * workgroups per CU.  Applies to later evaluations and sessions. */
 #define CCO_EOPT_RHO_CHUNK 1
+/* CCO_EOPT_MP_LANE: 1 (default where the k-set fits) evaluates split-precision
+ * lanes of up to about 4M column-row-particle products in one workgroup per
+ * lane (rho, Phi, C and the forces in one launch, nothing staged in HBM);
+ * 0 keeps them on the tiled split-precision kernels.  The two differ in
+ * summation order (within the split-precision error bound, not bitwise). */
+#define CCO_EOPT_MP_LANE 2
 int cco_energy_set_option(cco_energy* e, int key, int64_t value);
 int cco_energy_get_option(const cco_energy* e, int key, int64_t* value);
 

@@ -8,10 +8,13 @@ oracle, and check that a mixed relax ends with the FP64 path's success set.
 
 from __future__ import annotations
 
+import ctypes as C
+
 import numpy as np
 import pytest
 
 from oracle import oracle as O
+from xla_cco import _native
 from xla_cco import kvectors as K
 from xla_cco.box import cubic_box
 from xla_cco.energy import make_stealthy_energy
@@ -61,6 +64,50 @@ def test_mixed_energy_forces_density_errors(gpu_ctx, n: int, d: int, chi: float)
     phi_o, g_o = vg(s[0])
     assert abs(pmp[0] - phi_o) <= MIXED_RTOL_PHI * phi_o
     assert np.max(np.abs(gmp[0] - g_o)) <= MIXED_RTOL_GRAD * np.max(np.abs(g_o))
+
+
+EOPT_MP_LANE = 2  # CCO_EOPT_MP_LANE
+
+
+def _mp_lane(e, on: bool | None = None) -> int:
+    """Set (on given) and read the energy's one-workgroup-per-lane option."""
+    L = _native.lib()
+    if on is not None:
+        _native.check(L.cco_energy_set_option(e.handle(), EOPT_MP_LANE, 1 if on else 0))
+    v = C.c_int64()
+    _native.check(L.cco_energy_get_option(e.handle(), EOPT_MP_LANE, C.byref(v)))
+    return int(v.value)
+
+
+@pytest.mark.parametrize(
+    "n,d,chi",
+    [(64, 1, 0.3), (256, 2, 0.25), (1000, 2, 0.45), (1500, 2, 0.45), (77, 2, 0.45), (100, 3, 0.45),
+     (257, 3, 0.2)],
+)
+def test_lane_kernel_errors_and_tiled_agreement(gpu_ctx, n: int, d: int, chi: float) -> None:
+    """Small lanes evaluate in one workgroup each (rho, Phi, C and the forces in
+    LDS); the tiled split-precision kernels stay selectable.  Both are held to
+    the split-precision bounds against FP64, and the energy-only call returns
+    the same Phi bits as the value-and-gradient call."""
+    e64, emp, ks, box = _pair(n, d, chi)
+    assert _mp_lane(emp) == 1
+    s = np.random.default_rng(n * d + 5).uniform(size=(5, n, d))
+    p64, g64, re64, im64 = e64.value_grad_density_batch(s)
+    lane = emp.value_grad_density_batch(s)
+    assert np.array_equal(emp.energy_batch(s), lane[0])
+    assert _mp_lane(emp, False) == 0
+    tiled = emp.value_grad_density_batch(s)
+    scale = np.sqrt(n)
+    for pmp, gmp, remp, immp in (lane, tiled):
+        assert np.all(np.abs(pmp - p64) <= MIXED_RTOL_PHI * p64)
+        for i in range(5):
+            assert np.max(np.abs(gmp[i] - g64[i])) <= MIXED_RTOL_GRAD * np.max(np.abs(g64[i]))
+        assert np.max(np.abs(remp - re64)) <= MIXED_RTOL_RHO * scale
+        assert np.max(np.abs(immp - im64)) <= MIXED_RTOL_RHO * scale
+    assert _mp_lane(emp, True) == 1
+    again = emp.value_grad_density_batch(s)
+    for a, b in zip(lane, again):
+        assert np.array_equal(a, b)
 
 
 def test_mixed_error_is_absolute_near_ground_state(gpu_ctx) -> None:

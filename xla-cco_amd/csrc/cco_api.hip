@@ -314,6 +314,7 @@ int build_geometry(cco_energy* E, const int64_t* n, const double* w, int M, int 
               mp_lds_bytes(g) <= 64 * 1024) ? 1 : 0;
   g.mp_pg = g.mp_cb = 0;
   if (g.mp_ok) mp_force_plan(g, &g.mp_pg, &g.mp_cb);
+  g.mp_lane = (g.mp_ok && mp_lane_tiles_ok(g)) ? 1 : 0;
   // one device allocation holding every table
   auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
   const size_t o_un = 0;
@@ -1645,6 +1646,11 @@ int cco_energy_set_option(cco_energy* E, int key, int64_t value) {
     E->geo.rho_chunk = (int)value;
     return CCO_OK;
   }
+  if (key == CCO_EOPT_MP_LANE) {
+    if (value != 0 && value != 1) return fail(CCO_ERR_ARG, "split-precision lane kernel option must be 0 or 1");
+    E->geo.mp_lane = (value == 1 && E->geo.mp_ok && mp_lane_tiles_ok(E->geo)) ? 1 : 0;
+    return CCO_OK;
+  }
   return fail(CCO_ERR_ARG, "unknown energy option key");
 }
 
@@ -1652,6 +1658,10 @@ int cco_energy_get_option(const cco_energy* E, int key, int64_t* value) {
   if (!E || !value) return fail(CCO_ERR_ARG, "NULL argument");
   if (key == CCO_EOPT_RHO_CHUNK) {
     *value = E->geo.rho_chunk;
+    return CCO_OK;
+  }
+  if (key == CCO_EOPT_MP_LANE) {
+    *value = E->geo.mp_lane;
     return CCO_OK;
   }
   return fail(CCO_ERR_ARG, "unknown energy option key");

@@ -54,6 +54,7 @@ struct Geo {
   int n_mp_ctiles;  // 64-column workgroup tiles of the rho MFMA kernel
   int mp_pg;      // force MFMA kernel: 64-particle groups per workgroup (1 or 2)
   int mp_cb;      // force MFMA kernel: 16-column tiles of C per LDS batch at mp_pg groups
+  int mp_lane;    // split-precision lanes may run the one-workgroup-per-lane kernel (fits its LDS)
   double wmax, wmin;  // largest and smallest positive box weight (C scaling)
   const int4* gcol;   // per column (rho order): {box offset, L, n2, n3}
   int n_u;      // force traversal: u = 0 .. n_u-1 (u = |n2|)
@@ -107,6 +108,7 @@ struct Batch {
   int tiles_rho = 0, tiles_reduce = 0, tiles_force = 0;  // workgroups per lane (max)
   int rho_lds = 0;                  // dynamic LDS of the rho kernel (bytes, max over lanes)
   int dmask = 0;                    // bit d set when some lane has dimension d
+  int mp_lane = 0;  // set at launch: the split-precision lanes run mp_lane_kernel (reduce skips them)
 };
 
 // One lane's geometry and buffers inside a launch.
@@ -175,6 +177,10 @@ int mp_force_batch_for(const Geo& g);  // 16-column tiles per force batch (0: do
 // for a geometry (two groups when their LDS fits half a CU, else one).
 void mp_force_plan(const Geo& g, int* pg, int* cb);
 void launch_energy_mixed(int stage, const Geo& g, const Batch& b, int slots_ub, hipStream_t st);
+// 1 when a geometry's rho tiles fit mp_lane_kernel (the batch's N decides the rest)
+int mp_lane_tiles_ok(const Geo& g);
+// The split-precision lanes of this launch run mp_lane_kernel (one workgroup per lane).
+bool mp_lane_used(const Geo& g, const Batch& b);
 
 // Closed-form Hessians of B configurations (hessian.hip).  kn: n_k1, n_k2,
 // n_k3, w_k per k (M rows); workspaces X, Xw: B x 2M x N, G: B x N x N,

@@ -296,6 +296,7 @@ __global__ __launch_bounds__(kReduceTile) void reduce_kernel(Geo g0, Batch b) {
   const int tile = blockIdx.x % tiles_per_lane;
   const int lane = b.active[slot];
   if (b.skip != nullptr && b.skip[lane]) return;  // uniform per workgroup, before any barrier
+  if (b.mp_lane && b.prec[lane] != 0) return;      // evaluated whole by mp_lane_kernel
   const Geo g = b.lay != nullptr ? b.geos[lane] : g0;
   if (tile >= g.n_rtiles) return;  // uniform per workgroup, before any barrier
   const LaneIO io = lane_io(g, b, lane);
@@ -536,9 +537,11 @@ void launch_energy_stage(int stage, const Geo& g, const Batch& b, int slots_ub, 
     if (b.dmask & 8) launch_stage<3>(stage, g, b, slots_ub, st);
     return;
   }
-  if (g.d == 1) launch_stage<1>(stage, g, b, slots_ub, st);
-  if (g.d == 2) launch_stage<2>(stage, g, b, slots_ub, st);
-  if (g.d == 3) launch_stage<3>(stage, g, b, slots_ub, st);
+  Batch bs = b;
+  bs.mp_lane = stage == 1 && mp_lane_used(g, b) ? 1 : 0;  // the reduce skips those lanes
+  if (g.d == 1) launch_stage<1>(stage, g, bs, slots_ub, st);
+  if (g.d == 2) launch_stage<2>(stage, g, bs, slots_ub, st);
+  if (g.d == 3) launch_stage<3>(stage, g, bs, slots_ub, st);
   if (stage != 1) launch_energy_mixed(stage, g, b, slots_ub, st);
 }
 
