@@ -138,30 +138,38 @@ def published_shape(chi: float, reps: int = 2, autotune: bool = False) -> dict:
             "vs_reference": (ok / best) / ref["value"]}
 
 
-def bench_energy(args) -> int:
-    """Batched energy+force evaluations at one shape, FP64 and split precision: lane-evals/s."""
+def energy_times(d: int, n: int, chi: float, B: int, steps: int, warmup: int):
+    """Seconds per batched value_and_grad of B device-resident configurations, FP64 and
+    split precision; (times, M, split precision applied)."""
     from xla_cco import _native
 
     ctx = _native.context(0)
-    d, n, chi, B = args.d, args.n, args.chi, args.lanes
-    box, ks, e64 = build_workload(d, n, chi)
+    _, ks, e64 = build_workload(d, n, chi)
     _, _, emp = build_workload(d, n, chi)
     mixed_ok = emp.set_precision("mixed")
-    M = int(ks.n.shape[0])
     s = _native.DeviceArray.from_host(ctx, np.random.default_rng(17).uniform(size=(B, n, d)))
     phi = _native.DeviceArray(ctx, (B,))
     grad = _native.DeviceArray(ctx, (B, n, d))
     L = _native.lib()
     out = {}
     for name, e in (("fp64", e64), ("mixed", emp)):
-        for _ in range(args.warmup):
+        for _ in range(warmup):
             _native.check(L.cco_value_grad_density_dev(e.handle(), s.ptr, B, n, phi.ptr, grad.ptr, None, None))
         ctx.synchronize()
         t0 = time.perf_counter()
-        for _ in range(args.steps):
+        for _ in range(steps):
             _native.check(L.cco_value_grad_density_dev(e.handle(), s.ptr, B, n, phi.ptr, grad.ptr, None, None))
         ctx.synchronize()
-        out[name] = (time.perf_counter() - t0) / args.steps
+        out[name] = (time.perf_counter() - t0) / steps
+    for a in (s, phi, grad):
+        a.free()
+    return out, int(ks.n.shape[0]), mixed_ok
+
+
+def bench_energy(args) -> int:
+    """Batched energy+force evaluations at one shape, FP64 and split precision: lane-evals/s."""
+    d, n, chi, B = args.d, args.n, args.chi, args.lanes
+    out, M, mixed_ok = energy_times(d, n, chi, B, args.steps, args.warmup)
     flop = 8.0 * M * n * (1 + d) * B
     line = {
         "metric": "energy+force lane-evaluations/s (batched value_and_grad on device-resident positions)",
@@ -469,6 +477,8 @@ def main() -> int:
                          "(1: the constrained waves only; the reference MD command line tracks 2K)")
     ap.add_argument("--no-mixed", action="store_true",
                     help="skip the split-precision relax reported beside the FP64 value")
+    ap.add_argument("--no-energy-compare", action="store_true",
+                    help="skip the FP64 vs split-precision energy+force evaluation timings")
     ap.add_argument("--no-published", action="store_true",
                     help="skip the reference's published d=3 N=100 x 1200 workload reported beside the value")
     ap.add_argument(
@@ -690,6 +700,17 @@ def main() -> int:
         energy_ms = sum(main_run["ms"][STAGES.index(k)] for k in ("rho_kernel", "reduce_kernel", "force_kernel"))
         mixed_bound = sum(main_run["ms"]) / max(sum(main_run["ms"]) - energy_ms, 1e-12)
 
+    # energy+force evaluations alone, FP64 against split precision, at the relax
+    # workload's shape and at the MD / flow lanes' shape (d=3 N=10^4 chi=0.45 x32)
+    evals = None
+    if not args.no_energy_compare:
+        evals = {}
+        for key, (ed, en, echi, eB) in (("relax_shape", (d, n, chi, B)), ("d3_n10000_b32", (3, 10000, 0.45, 32))):
+            tt, eM, ok = energy_times(ed, en, echi, eB, 10, 2)
+            evals[key] = {"workload": f"value_and_grad of {eB} configurations, d={ed} N={en} chi={echi} (M={eM})",
+                          "fp64_ms": tt["fp64"] * 1e3, "mixed_ms": tt["mixed"] * 1e3,
+                          "mixed_vs_fp64": tt["fp64"] / tt["mixed"] if ok else None}
+
     published = None
     if not args.no_published:
         published = {f"chi{c}": published_shape(c, autotune=args.autotune) for c in (0.35, 0.45)}
@@ -740,6 +761,7 @@ def main() -> int:
                             "evaluations would speed the relax up by at most mixed_speedup_bound "
                             "(event-timed kernel time of a step / its non-energy part)",
         "mixed_speedup_bound": mixed_bound,
+        "energy_evals_fp64_vs_mixed": evals,
         "published_shape": published,
         "cpu_baseline": cpu,
     }

@@ -298,12 +298,11 @@ int build_geometry(cco_energy* E, const int64_t* n, const double* w, int M, int 
   if (rho_lds_bytes(g) > kRhoLdsMax)
     return fail(CCO_ERR_ARG, "k-set too wide for the rho kernel's LDS tables (max |n| too large)");
   // split-precision path: per-column table, padded rows, weight range
-  // padded to whole 64-column tiles with empty columns (L = 0)
-  std::vector<int4> gcol_tab(((cols.size() + 63) / 64) * 64, make_int4(0, 0, 0, 0));
+  // padded to whole 128-column tiles with empty columns (L = 0)
+  std::vector<int4> gcol_tab(((cols.size() + 127) / 128) * 128, make_int4(0, 0, 0, 0));
   for (size_t c = 0; c < cols.size(); ++c)
     gcol_tab[c] = make_int4(cols[c].off, cols[c].L, cols[c].n2, cols[c].n3);
   g.RP = ((P1 + 1 + 15) / 16) * 16;
-  g.n_mp_ctiles = (g.Cn + 63) / 64;
   g.wmax = 0.0;
   g.wmin = 0.0;
   for (double wv : wbox) {

@@ -51,7 +51,6 @@ struct Geo {
   // split-precision (MFMA) path
   int mp_ok;      // the k-set fits the MFMA kernels (rows <= 48, LDS <= 64 KB)
   int RP;         // rows padded to a multiple of 16
-  int n_mp_ctiles;  // 64-column workgroup tiles of the rho MFMA kernel
   int mp_pg;      // force MFMA kernel: 64-particle groups per workgroup (1 or 2)
   int mp_cb;      // force MFMA kernel: 16-column tiles of C per LDS batch at mp_pg groups
   int mp_lane;    // split-precision lanes may run the one-workgroup-per-lane kernel (fits its LDS)
