@@ -245,6 +245,25 @@ def bench_uneven(args) -> int:
     _native.check(L.cco_relax_profile(sess.handle, ms, launches))
     stage_avg = {STAGES[i]: ms[i] / launches[i] for i in range(len(STAGES)) if launches[i] > 0}
     flops = sum(8.0 * m_real[i] * n * (1 + d) for i, (n, d, _) in enumerate(specs))
+    sess.close()
+    # the same batch with split precision (lanes that fit the one-workgroup-per-
+    # lane kernel start split, all turn FP64 at E <= 1e-8): one warm, one timed relax
+    mixed = None
+    if not args.no_mixed:
+        ms_ = UnevenSession(energies, [(n, d) for n, d, _ in specs], LBFGSSettings(energy_tol=tol, precision="mixed"))
+        _native.check(L.cco_relax_set_option(ms_.handle, 2, args.shards))
+        for _ in range(2):
+            ms_.init(inputs[args.warmup])
+            split0 = ms_.precision_state()[1]
+            ctx.synchronize()
+            t0 = time.perf_counter()
+            ms_.run(bound, 256)
+            ctx.synchronize()
+            t_m = time.perf_counter() - t0
+        conv_m = sum(1 for r in ms_.result() if float(r.energy) < tol)
+        ms_.close()
+        mixed = {"value": conv_m / t_m, "unit": "configs/s", "converged": f"{conv_m}/{B}",
+                 "lanes_split_at_start": split0, "vs_fp64": (conv_m / t_m) / (converged / elapsed)}
     line = {
         "metric": METRIC,
         "value": converged / elapsed,
@@ -269,10 +288,10 @@ def bench_uneven(args) -> int:
         "energy_force_evals_per_s": lane_evals / elapsed,
         "mean_flop_per_lane_eval": flops / B,
         "kernel_avg_ms": stage_avg,
+        "mixed_vs_fp64": mixed,
         "cpu_baseline": None,
     }
     print(json.dumps(line))
-    sess.close()
     return 0
 
 

@@ -84,6 +84,23 @@ def test_uneven_512_lanes_converge_and_match_the_oracle(gpu_ctx) -> None:
         assert (phi[0] < 10 * TOL) == bool(ok[i]), (i, specs[i], phi[0], float(out[i].energy))
 
 
+def test_uneven_512_lanes_split_precision_same_success_as_fp64(gpu_ctx) -> None:
+    """The same 512 lanes in split precision (the lanes that fit the
+    one-workgroup-per-lane kernel start split, turn FP64 at Phi <= 1e-8):
+    the success set is the FP64 batch's."""
+    specs, cache, es, s0 = _uneven_batch()
+    r64 = relax_uneven(es, s0, LBFGSSettings(energy_tol=TOL))
+    rmp = relax_uneven(es, s0, LBFGSSettings(energy_tol=TOL, precision="mixed"))
+    ok64 = np.array([float(r.energy) < TOL for r in r64])
+    okmp = np.array([float(r.energy) < TOL for r in rmp])
+    ev64 = sum(int(r.n_evals) for r in r64)
+    evmp = sum(int(r.n_evals) for r in rmp)
+    print(f"uneven 512 split precision: {okmp.sum()}/512 (FP64 {ok64.sum()}/512), "
+          f"evaluations {evmp} (FP64 {ev64})", flush=True)
+    assert okmp.sum() >= ok64.sum() - 2
+    assert np.sum(ok64 != okmp) <= 2
+
+
 def test_uneven_512_lanes_follow_the_oracle_lane_by_lane(gpu_ctx) -> None:
     """A short budget (8 evaluations, one pass) keeps energies far above round-off,
     where the device takes the oracle's line-search decisions."""

@@ -93,6 +93,59 @@ def test_uneven_rejects_bad_input(gpu_ctx) -> None:
     with pytest.raises(ValueError):
         relax_uneven(es, [s0[0], s0[0]], LBFGSSettings())  # d = 2 positions for a d = 3 energy
     with pytest.raises(ValueError):
-        relax_uneven(es, s0, LBFGSSettings(precision="mixed"))
+        relax_uneven(es, s0, LBFGSSettings(precision="half"))
     with pytest.raises(TypeError):
         relax_uneven(es, [s0[0].astype(np.float32), s0[1]], LBFGSSettings())
+
+
+def test_uneven_split_precision_success_and_switch(gpu_ctx) -> None:
+    """Split precision in an uneven batch: lanes whose k-set and N fit the
+    one-workgroup-per-lane kernel start split and turn FP64 below the switch
+    (every lane of this batch fits); the success set equals the FP64 batch's,
+    and the converged configurations are checked in FP64."""
+    from xla_cco.minimize import UnevenSession
+
+    specs = SMALL + [(1000, 2, 0.45), (300, 3, 0.2), (757, 2, 0.3)]
+    es, s0 = _batch(specs, 9)
+    tol = 9.99999968e-21
+    r64 = relax_uneven(es, s0, LBFGSSettings(energy_tol=tol))
+    st = LBFGSSettings(energy_tol=tol, precision="mixed")
+    sess = UnevenSession(es, [np.shape(s) for s in s0], st)
+    try:
+        sess.init(s0)
+        on, split = sess.precision_state()
+        assert on and split == len(specs)
+        sess.run(10_000_000, 64)
+        on, split = sess.precision_state()
+        assert on and split == 0
+        rmp = sess.result()
+    finally:
+        sess.close()
+    ok64 = np.array([float(r.energy) < tol for r in r64])
+    okmp = np.array([float(r.energy) < tol for r in rmp])
+    assert np.array_equal(ok64, okmp) and okmp.all()
+    for e, r in zip(es, rmp):
+        assert e.energy_batch(r.s[None])[0] < 1e-19
+
+
+def test_uneven_split_precision_lanes_that_do_not_fit_stay_fp64(gpu_ctx) -> None:
+    """A lane whose column box is too wide for the lane kernel (d=3, N=4000)
+    runs FP64 from the start; the small lanes beside it run split."""
+    from xla_cco.minimize import UnevenSession
+
+    specs = [(100, 2, 0.3), (4000, 3, 0.3), (200, 3, 0.2)]
+    es, s0 = _batch(specs, 10)
+    st = LBFGSSettings(energy_tol=1e-6, precision="mixed", precision_switch=0.0)
+    sess = UnevenSession(es, [np.shape(s) for s in s0], st)
+    try:
+        sess.init(s0)
+        assert sess.precision_state() == (True, 2)
+        sess.run(10_000_000, 64)
+        out = sess.result()
+    finally:
+        sess.close()
+    # the FP64 lane is the FP64 batch's lane bit for bit
+    r64 = relax_uneven(es, s0, LBFGSSettings(energy_tol=1e-6))
+    assert np.array_equal(out[1].s, r64[1].s)
+    assert int(out[1].n_evals) == int(r64[1].n_evals)
+    assert all(float(r.energy) < 1e-6 for r in out)

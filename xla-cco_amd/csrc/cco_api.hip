@@ -519,6 +519,8 @@ struct cco_relax {
   int variant_run = 0;      // the variant latched by the last init (never 0)
   int precision = 0;        // requested: 1 = split-precision evaluations
   int* prec_store = nullptr;  // [B] per-lane precision flags
+  int* prec_ok = nullptr;     // [B] uneven sessions: lanes that fit the split-precision lane kernel
+  int lane_lds[4][4] = {};    // uneven sessions: that kernel's LDS per (d, row tiles) class
   // uneven batch (per-lane energy and N): layouts and geometry copies on device
   bool uneven = false;
   size_t vec_total = 0;     // doubles of one position array (B N d when uniform)
@@ -585,7 +587,7 @@ __global__ void scatter_eval_kernel(const double* __restrict__ phi_p,
 __global__ void relax_init_kernel(RelaxDev r, int method) {
   const int lane = blockIdx.x * blockDim.x + threadIdx.x;
   if (lane >= r.B) return;
-  if (r.prec != nullptr) r.prec[lane] = 1;
+  if (r.prec != nullptr) r.prec[lane] = r.prec_ok != nullptr ? r.prec_ok[lane] : 1;
   r.status[lane] = ST_RUNNING;
   r.active[lane] = lane;
   if (lane == 0) *r.n_active = r.B;
@@ -635,6 +637,7 @@ Batch relax_batch(cco_relax* R, const RelaxDev& dv) {
     b.tiles_force = R->tiles_force;
     b.rho_lds = R->rho_lds;
     b.dmask = R->dmask;
+    std::memcpy(b.lane_lds, R->lane_lds, sizeof(b.lane_lds));
   }
   return b;
 }
@@ -735,14 +738,16 @@ int resolve_variant(const cco_relax* R) {
 // when asked for, the k-set fits the MFMA kernels and the method uses it.
 // Device buffers of the split-precision C planes, when the run uses them.
 int reserve_mp_planes(cco_relax* R) {
-  if (R->dev.prec == nullptr) return CCO_OK;
+  if (R->dev.prec == nullptr || R->uneven) return CCO_OK;  // uneven: one workgroup per lane, no planes in HBM
   int rc = dev_reserve(R->cpl, mp_buffer_bytes(R->E->geo, R->B, R->N));
   if (rc == CCO_OK) rc = dev_reserve(R->csig, sizeof(double) * (size_t)R->B);
   return rc;
 }
 
 int* resolve_prec(const cco_relax* R) {
-  if (R->precision != 1 || R->E == nullptr || !R->E->geo.mp_ok || R->uneven) return nullptr;
+  if (R->precision != 1) return nullptr;
+  if (R->uneven) return (R->method == CCO_METHOD_LBFGS && R->prec_ok != nullptr) ? R->prec_store : nullptr;
+  if (R->E == nullptr || !R->E->geo.mp_ok) return nullptr;
   if (R->method != CCO_METHOD_LBFGS && R->method != METHOD_MD && R->method != METHOD_FLOW) return nullptr;
   if (R->method == METHOD_FLOW && R->ros2) return nullptr;  // ROS2 evaluates in FP64
   return R->prec_store;
@@ -1250,6 +1255,7 @@ static int relax_create_impl(cco_ctx* ctx, cco_energy* E, int B, int N, int d, i
   std::vector<int> h_lnd;
   std::vector<LaneLayout> h_lay;
   std::vector<Geo> h_geo;
+  std::vector<int> h_ok;
   size_t part_total = 0, cbox_total = 0, rphi_total = 0;
   if (un) {
     R->uneven = true;
@@ -1259,6 +1265,7 @@ static int relax_create_impl(cco_ctx* ctx, cco_energy* E, int B, int N, int d, i
     h_lnd.resize(B);
     h_lay.resize(B);
     h_geo.resize(B);
+    h_ok.assign(B, 0);
     long long vt = 0, ht = 0;
     int maxnd = 0;
     for (int i = 0; i < B; ++i) {
@@ -1289,6 +1296,14 @@ static int relax_create_impl(cco_ctx* ctx, cco_energy* E, int B, int N, int d, i
       R->dmask |= 1 << g.d;
       maxnd = std::max(maxnd, nd);
       h_geo[i] = g;
+      // split precision (when asked for at init) only for lanes that fit the
+      // one-workgroup-per-lane kernel
+      const size_t lds = method == CCO_METHOD_LBFGS ? mp_lane_fit(g, un->N[i]) : 0;
+      h_ok[i] = lds > 0 ? 1 : 0;
+      if (lds > 0) {
+        int& cls = R->lane_lds[g.d][g.RP / 16];
+        cls = std::max(cls, (int)lds);
+      }
     }
     v.nd = maxnd;
     v.stride = maxnd;
@@ -1350,6 +1365,8 @@ static int relax_create_impl(cco_ctx* ctx, cco_energy* E, int B, int N, int d, i
     if (rc == 0) rc = dev_reserve(R->rphi, sizeof(double) * rphi_total);
     if (rc == 0) rc = dev_reserve(R->un_lay, sizeof(LaneLayout) * (size_t)B);
     if (rc == 0) rc = dev_reserve(R->un_geo, sizeof(Geo) * (size_t)B);
+    const bool any_ok = std::find(h_ok.begin(), h_ok.end(), 1) != h_ok.end();
+    if (any_ok) A(R->prec_ok, B);
     if (rc == 0) {
       hipStream_t st = ctx->stream;
       if (hipMemcpyAsync(d_loff, h_loff.data(), sizeof(long long) * B, hipMemcpyHostToDevice, st) !=
@@ -1362,12 +1379,16 @@ static int relax_create_impl(cco_ctx* ctx, cco_energy* E, int B, int N, int d, i
                          st) != hipSuccess ||
           hipMemcpyAsync(R->un_geo.p, h_geo.data(), sizeof(Geo) * B, hipMemcpyHostToDevice, st) !=
               hipSuccess ||
+          (R->prec_ok != nullptr &&
+           hipMemcpyAsync(R->prec_ok, h_ok.data(), sizeof(int) * B, hipMemcpyHostToDevice, st) !=
+               hipSuccess) ||
           hipStreamSynchronize(st) != hipSuccess)
         rc = fail(CCO_ERR_DEVICE, "uploading the uneven layout failed");
     }
     v.loff = d_loff;
     v.lhoff = d_lhoff;
     v.lnd = d_lnd;
+    v.prec_ok = R->prec_ok;
   }
   if (rc == 0) rc = alloc_host_words(R);
   if (rc) {

@@ -108,6 +108,9 @@ struct Batch {
   int rho_lds = 0;                  // dynamic LDS of the rho kernel (bytes, max over lanes)
   int dmask = 0;                    // bit d set when some lane has dimension d
   int mp_lane = 0;  // set at launch: the split-precision lanes run mp_lane_kernel (reduce skips them)
+  // uneven batches: LDS of mp_lane_kernel per (d, row tiles) class of the
+  // lanes that may run split precision (0: no such lane)
+  int lane_lds[4][4] = {};
 };
 
 // One lane's geometry and buffers inside a launch.
@@ -178,6 +181,8 @@ void mp_force_plan(const Geo& g, int* pg, int* cb);
 void launch_energy_mixed(int stage, const Geo& g, const Batch& b, int slots_ub, hipStream_t st);
 // 1 when a geometry's rho tiles fit mp_lane_kernel (the batch's N decides the rest)
 int mp_lane_tiles_ok(const Geo& g);
+// LDS bytes of mp_lane_kernel for one lane of this geometry and N (0: the lane does not fit it)
+size_t mp_lane_fit(const Geo& g, int N);
 // The split-precision lanes of this launch run mp_lane_kernel (one workgroup per lane).
 bool mp_lane_used(const Geo& g, const Batch& b);
 

@@ -529,12 +529,15 @@ void launch_energy_stage(int stage, const Geo& g, const Batch& b, int slots_ub, 
   if (slots_ub <= 0) return;
   if (b.lay != nullptr) {  // uneven: reduce once, rho and forces once per dimension present
     if (stage == 1) {
-      launch_stage<1>(stage, g, b, slots_ub, st);
+      Batch bs = b;
+      bs.mp_lane = mp_lane_used(g, b) ? 1 : 0;  // the split-precision lanes are evaluated whole
+      launch_stage<1>(stage, g, bs, slots_ub, st);
       return;
     }
     if (b.dmask & 2) launch_stage<1>(stage, g, b, slots_ub, st);
     if (b.dmask & 4) launch_stage<2>(stage, g, b, slots_ub, st);
     if (b.dmask & 8) launch_stage<3>(stage, g, b, slots_ub, st);
+    launch_energy_mixed(stage, g, b, slots_ub, st);
     return;
   }
   Batch bs = b;

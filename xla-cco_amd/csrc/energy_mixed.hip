@@ -999,10 +999,24 @@ int mp_lane_tiles_ok(const Geo& g) {
   return (NRT >= 1 && NRT <= 3 && CT * NRT <= 2 * kLaneTw) ? 1 : 0;
 }
 
+size_t mp_lane_fit(const Geo& g, int N) {
+  if (!(g.mp_ok && g.mp_lane && mp_lane_tiles_ok(g))) return 0;
+  if ((long long)16 * ((g.Cn + 15) / 16) * g.RP * N > kLaneWorkMax) return 0;
+  const size_t lds = mp_lane_layout(g, N).total;
+  return lds <= kLaneLdsMax ? lds : 0;
+}
+
+// Uneven batches: every split-precision lane runs mp_lane_kernel (the lanes
+// that do not fit it start FP64).
 bool mp_lane_used(const Geo& g, const Batch& b) {
-  return b.prec != nullptr && b.lay == nullptr && g.mp_ok && g.mp_lane && mp_lane_tiles_ok(g) &&
-         (long long)16 * ((g.Cn + 15) / 16) * g.RP * b.N <= kLaneWorkMax &&
-         mp_lane_layout(g, b.N).total <= kLaneLdsMax;
+  if (b.prec == nullptr) return false;
+  if (b.lay != nullptr) {
+    for (int d = 1; d <= 3; ++d)
+      for (int r = 1; r <= 3; ++r)
+        if (b.lane_lds[d][r] > 0) return true;
+    return false;
+  }
+  return mp_lane_fit(g, b.N) > 0;
 }
 
 typedef _Float16 half2_t __attribute__((ext_vector_type(2)));
@@ -1143,15 +1157,19 @@ __device__ void lane_powers(float2* ta, int wa, const double2* zua, int na, floa
   }
 }
 
-// grid: slots_ub; block: kLaneThreads.  One split-precision lane per workgroup.
+// grid: slots_ub; block: kLaneThreads.  One split-precision lane per
+// workgroup; in uneven batches, the lanes of dimension D and NRT row tiles.
 template <int D, int NRT>
-__global__ __launch_bounds__(kLaneThreads) void mp_lane_kernel(Geo g, Batch b) {
+__global__ __launch_bounds__(kLaneThreads) void mp_lane_kernel(Geo g0, Batch b) {
   extern __shared__ char smem[];
   const int slot = blockIdx.x;
   if (slot >= *b.n_active) return;
   const int lane = b.active[slot];
   if (b.prec[lane] == 0) return;  // uniform per workgroup, before any barrier
-  const int N = b.N;
+  const Geo g = b.lay != nullptr ? b.geos[lane] : g0;
+  if (g.d != D || g.RP != 16 * NRT) return;  // another launch's lane (uneven batches)
+  const LaneIO io = lane_io(g, b, lane);
+  const int N = io.N;
   const LaneLds L = mp_lane_layout(g, N);
   const int tid = (int)threadIdx.x;
   const int wave = tid / kWave;
@@ -1168,7 +1186,7 @@ __global__ __launch_bounds__(kLaneThreads) void mp_lane_kernel(Geo g, Batch b) {
   const int npad = nch * kMpChunk;
   const int w2p = mp_pad(g.P2 + 1);
   const int w3p = mp_pad(g.P3 + 1);
-  const double* s = b.s + (size_t)lane * b.lane_stride;
+  const double* s = io.s;
   double2* zu = (double2*)(smem + L.zu);  // [D][npad]
   int4* cols = (int4*)(smem + L.col);
   double* wbx = (double*)(smem + L.wbx);  // box weights
@@ -1362,11 +1380,11 @@ __global__ __launch_bounds__(kLaneThreads) void mp_lane_kernel(Geo g, Batch b) {
   }
   const double phiV = block_sum<kLaneThreads>(acc, (double*)(smem + L.red));
   if (tid == 0) b.phi[lane] = phiV / g.volume;
-  double* pp = b.phi_part + (size_t)lane * g.n_rtiles;  // as the tiled reduce leaves them
+  double* pp = io.phi_part;  // as the tiled reduce leaves them
   for (int t = tid; t < g.n_rtiles; t += kLaneThreads) pp[t] = t == 0 ? phiV : 0.0;
-  if (b.rbox != nullptr)
-    for (int i = tid; i < g.Mbox; i += kLaneThreads) b.rbox[(size_t)lane * g.Mbox + i] = box[i];
-  if (b.re != nullptr) {
+  if (io.rbox != nullptr)
+    for (int i = tid; i < g.Mbox; i += kLaneThreads) io.rbox[i] = box[i];
+  if (io.re != nullptr) {
     for (int k = tid; k < g.M_pad; k += kLaneThreads) {
       double re = (double)N;
       double im = 0.0;
@@ -1376,11 +1394,11 @@ __global__ __launch_bounds__(kLaneThreads) void mp_lane_kernel(Geo g, Batch b) {
         re = v.x;
         im = code < 0 ? -v.y : v.y;
       }
-      b.re[(size_t)lane * g.M_pad + k] = re;
-      b.im[(size_t)lane * g.M_pad + k] = im;
+      io.re[k] = re;
+      io.im[k] = im;
     }
   }
-  if (b.grad == nullptr) return;  // uniform
+  if (io.grad == nullptr) return;  // uniform
 
   // ---------------- C planes ----------------
   const double bound = g.wmax * sqrt(fmax(phiV, 0.0) / g.wmin);
@@ -1519,7 +1537,7 @@ __global__ __launch_bounds__(kLaneThreads) void mp_lane_kernel(Geo g, Batch b) {
         for (int i = 0; i < 4; ++i) {
           const int j = c * kMpChunk + (wave & 3) * 16 + 4 * g16 + i;
           if (j >= N) continue;
-          double* o = b.grad + (size_t)lane * b.lane_stride + (size_t)j * D;
+          double* o = io.grad + (size_t)j * D;
           o[0] = scale * g1[i];
           if (D >= 2) o[1] = scale * g2[i];
           if (D >= 3) o[2] = scale * g3[i];
@@ -1531,9 +1549,7 @@ __global__ __launch_bounds__(kLaneThreads) void mp_lane_kernel(Geo g, Batch b) {
 }
 
 template <int D>
-static void launch_lane_d(const Geo& g, const Batch& b, int slots_ub, hipStream_t st) {
-  const size_t lds = mp_lane_layout(g, b.N).total;
-  const int nrt = g.RP / 16;
+static void launch_lane_nrt(int nrt, size_t lds, const Geo& g, const Batch& b, int slots_ub, hipStream_t st) {
   if (nrt == 1)
     hipLaunchKernelGGL((mp_lane_kernel<D, 1>), dim3(slots_ub), dim3(kLaneThreads), lds, st, g, b);
   if (nrt == 2)
@@ -1542,10 +1558,25 @@ static void launch_lane_d(const Geo& g, const Batch& b, int slots_ub, hipStream_
     hipLaunchKernelGGL((mp_lane_kernel<D, 3>), dim3(slots_ub), dim3(kLaneThreads), lds, st, g, b);
 }
 
+template <int D>
+static void launch_lane_d(const Geo& g, const Batch& b, int slots_ub, hipStream_t st) {
+  launch_lane_nrt<D>(g.RP / 16, mp_lane_layout(g, b.N).total, g, b, slots_ub, st);
+}
+
 // stage 0: rho (before reduce_kernel), stage 2: forces (after it).  Lanes
 // of mp_lane_kernel are evaluated whole in stage 0.
 void launch_energy_mixed(int stage, const Geo& g, const Batch& b, int slots_ub, hipStream_t st) {
-  if (slots_ub <= 0 || b.prec == nullptr || !g.mp_ok) return;
+  if (slots_ub <= 0 || b.prec == nullptr) return;
+  if (b.lay != nullptr) {  // uneven: one launch per (d, row tiles) class of split-capable lanes
+    if (stage != 0) return;
+    for (int nrt = 1; nrt <= 3; ++nrt) {
+      if (b.lane_lds[1][nrt] > 0) launch_lane_nrt<1>(nrt, b.lane_lds[1][nrt], g, b, slots_ub, st);
+      if (b.lane_lds[2][nrt] > 0) launch_lane_nrt<2>(nrt, b.lane_lds[2][nrt], g, b, slots_ub, st);
+      if (b.lane_lds[3][nrt] > 0) launch_lane_nrt<3>(nrt, b.lane_lds[3][nrt], g, b, slots_ub, st);
+    }
+    return;
+  }
+  if (!g.mp_ok) return;
   if (mp_lane_used(g, b)) {
     if (stage != 0) return;
     if (g.d == 1) launch_lane_d<1>(g, b, slots_ub, st);

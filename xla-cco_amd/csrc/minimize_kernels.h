@@ -159,6 +159,7 @@ struct RelaxDev {
   const long long* lhoff;
   int* status;    // [B]
   int* prec;      // [B] 1 = split-precision evaluations (null: all FP64)
+  const int* prec_ok;  // [B] lanes that may start split (null: every lane)
   int* active;    // [B]
   int* n_active;  // [1]
   // mapped host word (null: none) that compact_kernel sets to

@@ -307,8 +307,11 @@ class UnevenSession:
             raise ValueError("need one energy per configuration and at least one configuration")
         if not isinstance(settings, (DescentSettings, LBFGSSettings)):
             raise TypeError(f"unsupported settings type {type(settings).__name__}")
-        if isinstance(settings, LBFGSSettings) and settings.precision != "fp64":
-            raise ValueError("uneven batches run FP64")
+        if isinstance(settings, LBFGSSettings):
+            if settings.precision not in ("fp64", "mixed"):
+                raise ValueError(f"precision must be 'fp64' or 'mixed', got {settings.precision!r}")
+            if not settings.precision_switch >= 0.0:
+                raise ValueError("precision_switch must be >= 0")
         uniq: List[StealthyEnergy] = []
         index = {}
         self.b = len(shapes)
@@ -349,6 +352,18 @@ class UnevenSession:
             )
         )
         self.handle = h
+        # split precision: lanes whose k-set and N fit the one-workgroup-per-lane
+        # kernel start split and turn FP64 below the switch; the others run FP64
+        if isinstance(settings, LBFGSSettings) and settings.precision == "mixed":
+            _native.check(_native.lib().cco_relax_set_precision(h, 1, float(settings.precision_switch)))
+
+    def precision_state(self) -> Tuple[bool, int]:
+        """(split precision enabled by the last init, lanes still split now)."""
+        mode, lanes = C.c_int(0), C.c_int(0)
+        _native.check(
+            _native.lib().cco_relax_precision_state(self.handle, C.byref(mode), C.byref(lanes))
+        )
+        return bool(mode.value), int(lanes.value)
 
     def pack(self, s0: Sequence[np.ndarray]) -> np.ndarray:
         if len(s0) != self.b:
