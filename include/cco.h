@@ -95,6 +95,12 @@ int cco_value_grad_density_dev(cco_energy* e, const double* s_dev, int B, int N,
  * 0 keeps them on the tiled split-precision kernels.  The two differ in
  * summation order (within the split-precision error bound, not bitwise). */
 #define CCO_EOPT_MP_LANE 2
+/* CCO_EOPT_MP_LAYOUT (d = 3): 1 (default where it fits) runs the tiled
+ * split-precision kernels on the pair layout (columns (n1, n2), rows n3), about
+ * half the operand products of the column layout (columns (n2, n3), rows n1);
+ * 0 keeps the column layout.  Summation orders differ (within the
+ * split-precision error bound, not bitwise); get returns the layout in use. */
+#define CCO_EOPT_MP_LAYOUT 3
 int cco_energy_set_option(cco_energy* e, int key, int64_t value);
 int cco_energy_get_option(const cco_energy* e, int key, int64_t* value);
 

@@ -110,6 +110,38 @@ def test_lane_kernel_errors_and_tiled_agreement(gpu_ctx, n: int, d: int, chi: fl
         assert np.array_equal(a, b)
 
 
+EOPT_MP_LAYOUT = 3  # CCO_EOPT_MP_LAYOUT
+
+
+@pytest.mark.parametrize("n,d,chi", [(700, 3, 0.3), (2000, 3, 0.2), (4000, 3, 0.45), (1500, 3, 0.1)])
+def test_pair_layout_errors_and_column_layout(gpu_ctx, n: int, d: int, chi: float) -> None:
+    """At d = 3 the tiled split-precision kernels run on the pair layout
+    (columns (n1, n2), rows n3 + P3); the column layout stays selectable.  Both
+    are held to the split-precision bounds against FP64, densities included, and
+    the energy-only call returns the value-and-gradient call's Phi bits."""
+    e64, emp, ks, box = _pair(n, d, chi)
+    _mp_lane(emp, False)
+    L = _native.lib()
+    v = C.c_int64()
+    _native.check(L.cco_energy_get_option(emp.handle(), EOPT_MP_LAYOUT, C.byref(v)))
+    assert v.value == 1
+    s = np.random.default_rng(n + 31).uniform(size=(4, n, d))
+    p64, g64, re64, im64 = e64.value_grad_density_batch(s)
+    pair = emp.value_grad_density_batch(s)
+    assert np.array_equal(emp.energy_batch(s), pair[0])
+    _native.check(L.cco_energy_set_option(emp.handle(), EOPT_MP_LAYOUT, 0))
+    _native.check(L.cco_energy_get_option(emp.handle(), EOPT_MP_LAYOUT, C.byref(v)))
+    assert v.value == 0
+    col = emp.value_grad_density_batch(s)
+    scale = np.sqrt(n)
+    for pmp, gmp, remp, immp in (pair, col):
+        assert np.all(np.abs(pmp - p64) <= MIXED_RTOL_PHI * p64)
+        for i in range(4):
+            assert np.max(np.abs(gmp[i] - g64[i])) <= MIXED_RTOL_GRAD * np.max(np.abs(g64[i]))
+        assert np.max(np.abs(remp - re64)) <= MIXED_RTOL_RHO * scale
+        assert np.max(np.abs(immp - im64)) <= MIXED_RTOL_RHO * scale
+
+
 def test_mixed_error_is_absolute_near_ground_state(gpu_ctx) -> None:
     """The split-precision density error is absolute (|d rho_k| <~ 1e-6 sqrt(N) at
     any configuration), so close to a ground state the mixed path stops being

@@ -98,8 +98,10 @@ static void dev_release(DevBuf& b) {
 struct cco_energy {
   cco_ctx* ctx;
   Geo geo;
+  Geo geo_mp;     // d = 3: the pair layout of the split-precision tiled kernels (mp_ok 0: unused)
   int nmax[3];
   DevBuf tables;
+  DevBuf tables_mp;
   // batch workspace for cco_value_grad_density*
   DevBuf s, phi, grad, re, im, part, cbox, active, n_active, rphi, prec, cpl, csig;
   int ident_B = 0;
@@ -140,6 +142,119 @@ struct ColKey {
   int n2, n3;
   bool operator<(const ColKey& o) const { return std::tie(n2, n3) < std::tie(o.n2, o.n3); }
 };
+
+struct BoxCol {
+  int n2, n3, L, off;
+};
+
+// The pair layout of a d = 3 k-set for the split-precision tiled kernels:
+// columns (n1, n2) with n1 >= 0, each holding the contiguous rows
+// r = n3 + P3 in [a, a + L) of its k-vectors, the column phase carrying
+// z3^-P3.  It is used when every column's rows are contiguous, its box is no
+// larger than the column box (whose Mbox stays the stride of the partial sums
+// and of C), it has fewer columns and its tiles fit the kernels.
+int build_pair_geometry(cco_energy* E, const std::vector<int>& n1, const std::vector<int>& n2,
+                        const std::vector<int>& n3, const std::vector<char>& flip, const double* w,
+                        const std::vector<BoxCol>& cols, const std::map<ColKey, int>& col_index) {
+  const Geo& g = E->geo;
+  Geo& p = E->geo_mp;
+  p = g;
+  p.mp_ok = 0;
+  const int M = g.M;
+  const int shift = g.P3;
+  std::map<ColKey, std::pair<int, int>> range;  // (n1, n2) -> rows [lo, hi]
+  for (int k = 0; k < M; ++k) {
+    const int r = n3[k] + shift;
+    auto it = range.find({n1[k], n2[k]});
+    if (it == range.end())
+      range[{n1[k], n2[k]}] = {r, r};
+    else {
+      it->second.first = std::min(it->second.first, r);
+      it->second.second = std::max(it->second.second, r);
+    }
+  }
+  std::vector<int4> pc;  // {off - a, a << 16 | (a + L), n1, n2}
+  std::map<ColKey, int> poff;
+  int off = 0;
+  for (auto& kv : range) {
+    const int a = kv.second.first, L = kv.second.second - a + 1;
+    pc.push_back(make_int4(off - a, (a << 16) | (a + L), kv.first.n2, kv.first.n3));
+    poff[kv.first] = off - a;
+    off += L;
+  }
+  if (off > g.Mbox) return CCO_OK;  // larger than the column box: not used
+  std::vector<int> kmap(M);
+  std::vector<double> wp(off, 0.0);
+  std::vector<char> hit(off, 0);
+  for (int k = 0; k < M; ++k) {
+    const int i = poff[{n1[k], n2[k]}] + n3[k] + shift;
+    kmap[k] = flip[k] ? ~i : i;
+    wp[i] += w[k];
+    hit[i] = 1;
+  }
+  for (int i = 0; i < off; ++i)
+    if (!hit[i]) return CCO_OK;  // a gap inside a column's rows: not used
+  // pair-layout entry -> column-box entry of the same (n1, n2, n3)
+  std::vector<int> p2b(off, -1);
+  for (auto& kv : range) {
+    const int base = poff[kv.first];
+    for (int r = kv.second.first; r <= kv.second.second; ++r) {
+      auto c = col_index.find({kv.first.n3, r - shift});
+      if (c != col_index.end() && kv.first.n2 < cols[c->second].L) p2b[base + r] = cols[c->second].off + kv.first.n2;
+    }
+  }
+  p.mp_perm = 1;
+  p.mp_shift = shift;
+  p.Cn = (int)pc.size();
+  p.P1 = 2 * shift;  // row powers 0 .. 2 P3
+  p.P2 = g.P1;       // column tables: n1 >= 0, |n2|
+  p.P3 = g.P2;
+  p.RP = ((p.P1 + 1 + 15) / 16) * 16;
+  p.Mbox = g.Mbox;   // the stride of the partial sums and of C
+  p.mp_Mbox = off;
+  p.mp_geo = nullptr;
+  // fewer columns means fewer column-operand products on the VALU, which bound
+  // the tiled kernels more than the MFMA row padding does
+  p.mp_ok = (p.RP <= 48 && p.wmin > 0.0 && mp_force_batch_for(p) >= 1 && mp_lds_bytes(p) <= 64 * 1024 &&
+             p.Cn < g.Cn) ? 1 : 0;
+  if (!p.mp_ok) return CCO_OK;
+  p.mp_lane = 0;
+  mp_force_plan(p, &p.mp_pg, &p.mp_cb);
+  pc.resize(((pc.size() + 127) / 128) * 128, make_int4(0, 0, 0, 0));
+  auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
+  const size_t o_gc = 0;
+  const size_t o_kb = align(sizeof(int4) * pc.size());
+  const size_t o_wb = align(o_kb + sizeof(int) * kmap.size());
+  const size_t o_pb = align(o_wb + sizeof(double) * wp.size());
+  const size_t total = align(o_pb + sizeof(int) * p2b.size());
+  std::vector<char> host(total, 0);
+  std::memcpy(host.data() + o_gc, pc.data(), sizeof(int4) * pc.size());
+  std::memcpy(host.data() + o_kb, kmap.data(), sizeof(int) * kmap.size());
+  std::memcpy(host.data() + o_wb, wp.data(), sizeof(double) * wp.size());
+  std::memcpy(host.data() + o_pb, p2b.data(), sizeof(int) * p2b.size());
+  if (int rc = dev_reserve(E->tables_mp, total)) return rc;
+  CCO_HIP(hipMemcpy(E->tables_mp.p, host.data(), total, hipMemcpyHostToDevice));
+  char* base = (char*)E->tables_mp.p;
+  p.gcol = (const int4*)(base + o_gc);
+  p.kbox = (const int*)(base + o_kb);
+  p.wbox = (const double*)(base + o_wb);
+  p.mp_kbox = p.kbox;
+  p.mp_wbox = p.wbox;
+  p.mp_p2b = (const int*)(base + o_pb);
+  return CCO_OK;
+}
+
+// Point the column-box Geo at the pair layout (on: when it was built) or not.
+void use_pair_layout(cco_energy* E, bool on) {
+  Geo& g = E->geo;
+  const Geo& p = E->geo_mp;
+  const bool use = on && p.mp_ok && p.mp_perm;
+  g.mp_geo = use ? &E->geo_mp : nullptr;
+  g.mp_Mbox = use ? p.mp_Mbox : 0;
+  g.mp_wbox = use ? p.mp_wbox : nullptr;
+  g.mp_kbox = use ? p.mp_kbox : nullptr;
+  g.mp_p2b = use ? p.mp_p2b : nullptr;
+}
 
 // Build the column box and traversal tables of a k-set (host), upload them.
 int build_geometry(cco_energy* E, const int64_t* n, const double* w, int M, int d, double volume,
@@ -186,9 +301,7 @@ int build_geometry(cco_energy* E, const int64_t* n, const double* w, int M, int 
     else
       it->second = std::max(it->second, n1[k] + 1);
   }
-  struct Col {
-    int n2, n3, L, off;
-  };
+  using Col = BoxCol;
   std::vector<Col> cols;
   for (auto& kv : colL) cols.push_back({kv.first.n2, kv.first.n3, kv.second, 0});
   std::stable_sort(cols.begin(), cols.end(), [](const Col& a, const Col& b) {
@@ -314,6 +427,8 @@ int build_geometry(cco_energy* E, const int64_t* n, const double* w, int M, int 
   g.mp_pg = g.mp_cb = 0;
   if (g.mp_ok) mp_force_plan(g, &g.mp_pg, &g.mp_cb);
   g.mp_lane = (g.mp_ok && mp_lane_tiles_ok(g)) ? 1 : 0;
+  g.mp_geo = nullptr;
+  g.mp_perm = 0;
   // one device allocation holding every table
   auto align = [](size_t x) { return (x + 255) & ~(size_t)255; };
   const size_t o_un = 0;
@@ -340,6 +455,10 @@ int build_geometry(cco_energy* E, const int64_t* n, const double* w, int M, int 
   g.wbox = (const double*)(base + o_wb);
   g.ustart = (const int2*)(base + o_us);
   g.gmem = (const int4*)(base + o_gm);
+  if (d == 3 && g.mp_ok) {
+    if ((rc = build_pair_geometry(E, n1, n2, n3, flip, w, cols, col_index))) return rc;
+    use_pair_layout(E, true);
+  }
   return CCO_OK;
 }
 
@@ -404,10 +523,13 @@ Batch energy_batch_desc(const Geo& g, const EnergyWs& ws, const double* s_dev, i
 
 // The split-precision planes of B lanes share one buffer: the C planes of every
 // lane, then the z1 planes of every lane.
-size_t mp_buffer_bytes(const Geo& g, int B, int N) {
+// (sized for the layout the tiled kernels use: the pair layout when set)
+size_t mp_buffer_bytes(const Geo& g0, int B, int N) {
+  const Geo& g = g0.mp_geo != nullptr ? *g0.mp_geo : g0;
   return (mp_cplane_bytes(g) + mp_z1g_bytes(g, N)) * (size_t)B;
 }
-void mp_bind(Batch& b, const Geo& g, void* base, int B) {
+void mp_bind(Batch& b, const Geo& g0, void* base, int B) {
+  const Geo& g = g0.mp_geo != nullptr ? *g0.mp_geo : g0;
   b.cpl = base;
   b.z1g = base != nullptr ? (char*)base + mp_cplane_bytes(g) * (size_t)B : nullptr;
 }
@@ -969,6 +1091,7 @@ int cco_energy_create(cco_ctx* ctx, const int64_t* n, const double* w, int M, in
   int rc = build_geometry(E, n, w, M, d, volume, pad_to);
   if (rc) {
     dev_release(E->tables);
+    dev_release(E->tables_mp);
     delete E;
     return rc;
   }
@@ -980,7 +1103,7 @@ int cco_energy_destroy(cco_energy* E) {
   if (!E) return CCO_OK;
   DevScope ds(E->ctx->device);
   (void)hipStreamSynchronize(E->ctx->stream);
-  for (DevBuf* b : {&E->tables, &E->s, &E->phi, &E->grad, &E->re, &E->im, &E->part, &E->cbox,
+  for (DevBuf* b : {&E->tables, &E->tables_mp, &E->s, &E->phi, &E->grad, &E->re, &E->im, &E->part, &E->cbox,
                     &E->active, &E->n_active, &E->rphi, &E->prec, &E->cpl, &E->csig, &E->kn, &E->hs, &E->hX, &E->hXw,
                     &E->hG, &E->hrho, &E->hD, &E->hH})
     dev_release(*b);
@@ -1666,6 +1789,11 @@ int cco_energy_set_option(cco_energy* E, int key, int64_t value) {
     E->geo.rho_chunk = (int)value;
     return CCO_OK;
   }
+  if (key == CCO_EOPT_MP_LAYOUT) {
+    if (value != 0 && value != 1) return fail(CCO_ERR_ARG, "split-precision layout option must be 0 or 1");
+    use_pair_layout(E, value == 1);
+    return CCO_OK;
+  }
   if (key == CCO_EOPT_MP_LANE) {
     if (value != 0 && value != 1) return fail(CCO_ERR_ARG, "split-precision lane kernel option must be 0 or 1");
     E->geo.mp_lane = (value == 1 && E->geo.mp_ok && mp_lane_tiles_ok(E->geo)) ? 1 : 0;
@@ -1682,6 +1810,10 @@ int cco_energy_get_option(const cco_energy* E, int key, int64_t* value) {
   }
   if (key == CCO_EOPT_MP_LANE) {
     *value = E->geo.mp_lane;
+    return CCO_OK;
+  }
+  if (key == CCO_EOPT_MP_LAYOUT) {
+    *value = E->geo.mp_geo != nullptr ? 1 : 0;
     return CCO_OK;
   }
   return fail(CCO_ERR_ARG, "unknown energy option key");

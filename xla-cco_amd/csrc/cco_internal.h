@@ -66,6 +66,19 @@ struct Geo {
   const double* wbox;    // summed weight of the k rows on each box entry (Mbox)
   const int2* ustart;    // per u: {first group, number of v values}
   const int4* gmem;      // per group: 4 members {box offset, L, n2, n3}
+  // Split precision on the pair layout (d = 3): columns are (n1, n2) pairs and
+  // rows r = n3 + mp_shift, the column phase carrying z3^-mp_shift.  In the
+  // column-box Geo: the pair layout's Geo for the MFMA kernels (a host pointer,
+  // null when not used) and, for the reduce of those lanes, its entry count,
+  // weights, k map and map to this box (-1: no entry here).  Both layouts use
+  // this box's Mbox as the stride of the partial sums and of C.
+  const Geo* mp_geo;
+  int mp_perm;   // this Geo is a pair layout
+  int mp_shift;  // max |n3| of the k-set (pair layout)
+  int mp_Mbox;
+  const double* mp_wbox;
+  const int* mp_kbox;
+  const int* mp_p2b;
 };
 
 // Uneven batches: where one lane's data lives (offsets in elements).

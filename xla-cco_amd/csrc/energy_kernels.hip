@@ -303,24 +303,33 @@ __global__ __launch_bounds__(kReduceTile) void reduce_kernel(Geo g0, Batch b) {
   const double2* part = io.part;
   double2* cbox = io.cbox;
   const int idx = tile * kReduceTile + (int)threadIdx.x;
+  // split-precision lanes of the pair layout: its entries, weights and k map
+  // (the column box's Mbox stays the stride)
+  const bool pair = g.mp_geo != nullptr && b.prec != nullptr && b.prec[lane] != 0;
+  const int nbox = pair ? g.mp_Mbox : g.Mbox;
+  const double* wbox = pair ? g.mp_wbox : g.wbox;
+  const int* kbox = pair ? g.mp_kbox : g.kbox;
   double acc = 0.0;
-  if (idx < g.Mbox) {
+  if (idx < nbox) {
     double2 rho = part[idx];
     for (int sp = 1; sp < io.n_splits; ++sp) {
       const double2 v = part[(size_t)sp * g.Mbox + idx];
       rho.x += v.x;
       rho.y += v.y;
     }
-    const double w = g.wbox[idx];
+    const double w = wbox[idx];
     cbox[idx] = make_double2(w * rho.x, w * rho.y);
-    if (io.rbox != nullptr) io.rbox[idx] = rho;  // box order: coalesced, gathered by k later
+    if (io.rbox != nullptr) {  // column-box order: coalesced, gathered by k later
+      const int o = pair ? g.mp_p2b[idx] : idx;
+      if (o >= 0) io.rbox[o] = rho;
+    }
     acc = w * (rho.x * rho.x + rho.y * rho.y);
   }
   if (io.re != nullptr && idx < g.M_pad) {
     double re = (double)io.N;
     double im = 0.0;
     if (idx < g.M) {
-      const int code = g.kbox[idx];
+      const int code = kbox[idx];
       const int i = code < 0 ? ~code : code;
       double2 rho = part[i];
       for (int sp = 1; sp < io.n_splits; ++sp) {

@@ -161,7 +161,8 @@ __host__ __device__ inline size_t mp_z1_plane_bytes(const Geo& g, int N) {
 }
 __host__ __device__ inline size_t mp_z1_lane_bytes(const Geo& g, int N) {
   const size_t npad = (size_t)((N + kMpChunk - 1) / kMpChunk) * kMpChunk;
-  return mp_z1_plane_bytes(g, N) + (size_t)(g.d - 1) * npad * sizeof(double2);
+  // pair layout: the unit phases of axes 1 and 2 and z3^-shift
+  return mp_z1_plane_bytes(g, N) + (size_t)(g.d - 1 + g.mp_perm) * npad * sizeof(double2);
 }
 size_t mp_z1g_bytes(const Geo& g, int N) { return mp_z1_lane_bytes(g, N); }
 
@@ -184,7 +185,7 @@ __device__ void mp_z1_load(const MpTables& T, const _Float16* src, int RL, int R
 // the unit phases of axes 2 and 3, in blocks of 8 powers seeded by z^{8q}
 // (repeated squaring) and continued by recurrence (every thread of the block).
 template <int D>
-__device__ void mp_powers(const MpTables& T, int nj, int P2, int P3) {
+__device__ void mp_powers(const MpTables& T, int nj, int P2, int P3, bool perm = false) {
   const double tscale = D == 2 ? 4096.0 : 64.0;
   const int b2 = D >= 2 ? (P2 + 8) / 8 : 0;  // powers 0 .. P2
   const int b3 = D >= 3 ? (P3 + 8) / 8 : 0;
@@ -201,6 +202,7 @@ __device__ void mp_powers(const MpTables& T, int nj, int P2, int P3) {
       z8 = cmul(z8, z8);
       for (int i = 0; i < q; ++i) sd = cmul(sd, z8);
     }
+    if (perm && ax == 1) sd = cmul(sd, T.zu[jj]);  // pair layout: n1 powers carry z3^-shift
     // the run's powers by FP32 recurrence from the FP64 seed (at most 7 steps)
     float2 p = make_float2((float)sd.x, (float)sd.y);
     const float2 zf = make_float2((float)z.x, (float)z.y);
@@ -231,17 +233,20 @@ __device__ void mp_powers(const MpTables& T, int nj, int P2, int P3) {
 // ([axis - 1][npad]), then their power tables.
 template <int D>
 __device__ void mp_fill(const MpTables& T, const double2* zusrc, int npad, const _Float16* z1src, int jc,
-                        int nj, int RL, int RP, int P2, int P3) {
+                        int nj, int RL, int RP, int P2, int P3, bool perm) {
   mp_z1_load(T, z1src, RL, RP);
   if (D == 1) return;  // the caller's barrier orders the copy before use
-  for (int idx = threadIdx.x; idx < kMpChunk * (D - 1); idx += blockDim.x) {
+  // column layout: slots 1, 2 <- axes 2, 3; pair layout: slots 1, 2 <- axes 1, 2
+  // and slot 0 <- z3^-shift
+  const int na = D - 1 + (perm ? 1 : 0);
+  for (int idx = threadIdx.x; idx < kMpChunk * na; idx += blockDim.x) {
     const int jj = idx % kMpChunk;
-    const int ax = 1 + idx / kMpChunk;
-    T.zu[ax * kMpChunk + jj] =
-        z1src != nullptr ? zusrc[(size_t)(ax - 1) * npad + jc + jj] : make_double2(0.0, 0.0);
+    const int a = idx / kMpChunk;  // source array
+    const int slot = perm ? (a + 1) % 3 : a + 1;
+    T.zu[slot * kMpChunk + jj] = z1src != nullptr ? zusrc[(size_t)a * npad + jc + jj] : make_double2(0.0, 0.0);
   }
   __syncthreads();
-  mp_powers<D>(T, nj, P2, P3);
+  mp_powers<D>(T, nj, P2, P3, perm);
 }
 
 constexpr int kZ1Chunks = 4;  // chunks per mp_z1_kernel workgroup (one thread per particle)
@@ -270,10 +275,11 @@ __global__ __launch_bounds__(256) void mp_z1_kernel(Geo g, Batch b) {
   const int jj = (int)threadIdx.x % kMpChunk;
   const int j = ch * kMpChunk + jj;
   const int RL = g.P1 + 1;
+  const int rax = g.mp_perm ? D - 1 : 0;  // row axis: n1, or n3 on the pair layout
   double2 z = make_double2(0.0, 0.0);
   if (j < b.N) {
     double sn, cs;
-    sincospi(2.0 * b.s[(size_t)lane * b.lane_stride + (size_t)j * D], &sn, &cs);
+    sincospi(2.0 * b.s[(size_t)lane * b.lane_stride + (size_t)j * D + rax], &sn, &cs);
     z = make_double2(cs, sn);
   }
   double2 z8 = cmul(z, z);
@@ -284,16 +290,25 @@ __global__ __launch_bounds__(256) void mp_z1_kernel(Geo g, Batch b) {
 #pragma unroll
   for (int q = 1; q < kBlocks; ++q) p[q] = cmul(p[q - 1], z8);
   char* lane_base = (char*)b.z1g + (size_t)lane * mp_z1_lane_bytes(g, b.N);
-  if (D > 1) {  // unit phases of the other axes, for the rho and force workgroups' tables
+  if (D > 1) {  // unit phases of the column axes, for the rho and force workgroups' tables
     double2* zo = (double2*)(lane_base + mp_z1_plane_bytes(g, b.N));
-    for (int ax = 1; ax < D; ++ax) {
+    for (int a = 0; a < D - 1; ++a) {
+      const int ax = g.mp_perm ? a : a + 1;
       double2 za = make_double2(0.0, 0.0);
       if (j < b.N) {
         double sn, cs;
         sincospi(2.0 * b.s[(size_t)lane * b.lane_stride + (size_t)j * D + ax], &sn, &cs);
         za = make_double2(cs, sn);
       }
-      zo[(size_t)(ax - 1) * n_ch * kMpChunk + j] = za;
+      zo[(size_t)a * n_ch * kMpChunk + j] = za;
+    }
+    if (g.mp_perm) {  // z3^-shift = conj(z3)^shift, binary powering
+      double2 c = make_double2(j < b.N ? 1.0 : 0.0, 0.0), q = cconj(z);
+      for (int e = g.mp_shift; e > 0; e >>= 1) {
+        if (e & 1) c = cmul(c, q);
+        q = cmul(q, q);
+      }
+      zo[(size_t)(D - 1) * n_ch * kMpChunk + j] = c;
     }
   }
   _Float16* dst = (_Float16*)lane_base + (size_t)ch * mp_z1_chunk_halves(RL) + jj;
@@ -433,7 +448,7 @@ __global__ __launch_bounds__(64 * W) void rho_mp_kernel(Geo g, Batch b) {
   for (int jc = j0; jc < j1; jc += kMpChunk) {
     const int nj = min(kMpChunk, j1 - jc);
     mp_fill<D>(T, zu_lane, n_ch * kMpChunk, z1_lane + (size_t)(jc / kMpChunk) * mp_z1_chunk_halves(g.P1 + 1),
-               jc, nj, g.P1 + 1, g.RP, g.P2, g.P3);
+               jc, nj, g.P1 + 1, g.RP, g.P2, g.P3, g.mp_perm != 0);
     __syncthreads();
     // FP32 accumulators over the chunk: hi*hi and cross terms apart
     float4_t rb[NRT], rs[NRT], ib[NRT], is[NRT];
@@ -543,11 +558,11 @@ __global__ __launch_bounds__(64 * W) void rho_mp_kernel(Geo g, Batch b) {
   double2* out = b.part + ((size_t)lane * b.n_splits + split) * g.Mbox;
 #pragma unroll
   for (int i = 0; i < 4; ++i) {
-    const int4 oc = g.gcol[c0 + 4 * g16 + i];
+    const int4 oc = g.gcol[c0 + 4 * g16 + i];  // rows [y >> 16, y & 0xffff) at x + r
 #pragma unroll
     for (int rt = 0; rt < NRT; ++rt) {
       const int r = rt * 16 + m16;
-      if (r < oc.y)
+      if (r >= (oc.y >> 16) && r < (oc.y & 0xffff))
         out[oc.x + r] = make_double2(kRhoUnscale * sum_re[rt][i], kRhoUnscale * sum_im[rt][i]);
     }
   }
@@ -594,7 +609,7 @@ __global__ __launch_bounds__(256) void mp_cplanes_kernel(Geo g, Batch b) {
     const int r = e - cl * CP;
     const int4 oc = g.gcol[ct * 16 + cl];
     double2 v = make_double2(0.0, 0.0);
-    if (r < oc.y) v = C[oc.x + r];
+    if (r >= (oc.y >> 16) && r < (oc.y & 0xffff)) v = C[oc.x + r];
     const float xr = (float)(v.x * sigma);
     const float xi = (float)(v.y * sigma);
     _Float16 h, lo;
@@ -753,7 +768,7 @@ __global__ __launch_bounds__(256) void force_mp_kernel(Geo g, Batch b) {
     const int nj = max(0, min(kMpChunk, b.N - jc));
     const _Float16* z1src =
         nj > 0 ? z1_lane + (size_t)(jc / kMpChunk) * mp_z1_chunk_halves(g.P1 + 1) : nullptr;
-    mp_fill<D>(tables(p), zu_lane, n_ch * kMpChunk, z1src, jc, nj, g.P1 + 1, RP, g.P2, g.P3);
+    mp_fill<D>(tables(p), zu_lane, n_ch * kMpChunk, z1src, jc, nj, g.P1 + 1, RP, g.P2, g.P3, g.mp_perm != 0);
     __syncthreads();
     const _Float16* z1p = tables(p).z1p;
 #pragma unroll
@@ -770,6 +785,7 @@ __global__ __launch_bounds__(256) void force_mp_kernel(Geo g, Batch b) {
     __syncthreads();  // the next group's fill rewrites the z1 planes
   }
   const double sigma = b.csig[lane];  // written with the C planes (mp_cplanes_kernel)
+  const double shift = (double)g.mp_shift;  // pair layout: rows r = n3 + shift
   double g1[PG][4], g2[PG][4], g3[PG][4];
 #pragma unroll
   for (int p = 0; p < PG; ++p)
@@ -847,7 +863,8 @@ __global__ __launch_bounds__(256) void force_mp_kernel(Geo g, Batch b) {
           // Im(conj(B) H) = B.x H.im - B.y H.re
           const double v = (double)(B.x * him[p][i] - B.y * hre[p][i]);
           const double v1 = (double)(B.x * h1im[p][i] - B.y * h1re[p][i]);
-          g1[p][i] += v1;
+          // row-axis component: sum of (r - shift) Im(...) (shift = 0 on the column layout)
+          g1[p][i] += v1 - shift * v;
           if (D >= 2) g2[p][i] += (double)col.z * v;
           if (D >= 3) g3[p][i] += (double)col.w * v;
         }
@@ -871,9 +888,15 @@ __global__ __launch_bounds__(256) void force_mp_kernel(Geo g, Batch b) {
         const int j = pt * PW + p * kMpChunk + wave * 16 + 4 * g16 + i;
         if (j >= b.N) continue;
         double* o = b.grad + (size_t)lane * b.lane_stride + (size_t)j * D;
-        o[0] = scale * g1[p][i];
-        if (D >= 2) o[1] = scale * g2[p][i];
-        if (D >= 3) o[2] = scale * g3[p][i];
+        if (D == 3 && g.mp_perm) {  // pair layout: columns carry (n1, n2), rows n3
+          o[0] = scale * g2[p][i];
+          o[1] = scale * g3[p][i];
+          o[2] = scale * g1[p][i];
+        } else {
+          o[0] = scale * g1[p][i];
+          if (D >= 2) o[1] = scale * g2[p][i];
+          if (D >= 3) o[2] = scale * g3[p][i];
+        }
       }
   }
 }
@@ -1586,7 +1609,7 @@ void launch_energy_mixed(int stage, const Geo& g, const Batch& b, int slots_ub, 
   }
   if (g.d == 1) launch_mp_d<1>(stage, g, b, slots_ub, st);
   if (g.d == 2) launch_mp_d<2>(stage, g, b, slots_ub, st);
-  if (g.d == 3) launch_mp_d<3>(stage, g, b, slots_ub, st);
+  if (g.d == 3) launch_mp_d<3>(stage, g.mp_geo != nullptr ? *g.mp_geo : g, b, slots_ub, st);
 }
 
 int mp_force_batch_for(const Geo& g) { return mp_force_batch(g.RP, g.P2, g.P3); }
