@@ -314,26 +314,37 @@ def bench_flow(args) -> int:
     fl = FlowSettings()
     fs = _native.FlowSettingsC(fl.rtol, fl.atol, fl.dt_init, fl.dt_min, fl.dt_max, fl.t_max, fl.energy_tol,
                                fl.grad_tol, int(fl.max_steps), 0, 0, 0)
-    h = C.c_void_p()
     bas = np.ascontiguousarray(box.basis)
-    _native.check(L.cco_flow_create(ctx.handle, energy.handle(), B, n, d, C.byref(fs), _native.dptr(bas),
-                                    C.byref(h)))
-    try:
-        _native.check(L.cco_flow_init(h, _native.dptr(s0), None))
-        na = C.c_int(0)
-        _native.check(L.cco_relax_step(h, args.warmup, C.byref(na)))  # evaluations, lanes resident
-        ctx.synchronize()
-        _native.check(L.cco_relax_set_profiling(h, 4))
-        t0 = time.perf_counter()
-        _native.check(L.cco_relax_step(h, args.steps, C.byref(na)))
-        ctx.synchronize()
-        wall = time.perf_counter() - t0
-        ms = (C.c_double * len(STAGES))()
-        launches = (C.c_int64 * len(STAGES))()
-        _native.check(L.cco_relax_profile(h, ms, launches))
-        running = int(na.value)
-    finally:
-        L.cco_relax_destroy(h)
+
+    def run(mixed: bool):
+        """Timed evaluation steps of a flow session (split precision: lanes turn FP64 at Phi <= 1e-8)."""
+        h = C.c_void_p()
+        _native.check(L.cco_flow_create(ctx.handle, energy.handle(), B, n, d, C.byref(fs), _native.dptr(bas),
+                                        C.byref(h)))
+        try:
+            if mixed:
+                _native.check(L.cco_relax_set_precision(h, 1, 1e-8))
+            _native.check(L.cco_flow_init(h, _native.dptr(s0), None))
+            na = C.c_int(0)
+            _native.check(L.cco_relax_step(h, args.warmup, C.byref(na)))  # evaluations, lanes resident
+            ctx.synchronize()
+            _native.check(L.cco_relax_set_profiling(h, 4))
+            t0 = time.perf_counter()
+            _native.check(L.cco_relax_step(h, args.steps, C.byref(na)))
+            ctx.synchronize()
+            wall = time.perf_counter() - t0
+            ms = (C.c_double * len(STAGES))()
+            launches = (C.c_int64 * len(STAGES))()
+            _native.check(L.cco_relax_profile(h, ms, launches))
+            return wall, ms, launches, int(na.value)
+        finally:
+            L.cco_relax_destroy(h)
+
+    wall, ms, launches, running = run(False)
+    mixed = None
+    if not args.no_mixed:
+        wall_m, _, _, _ = run(True)
+        mixed = {"value": B * args.steps / wall_m, "vs_fp64": wall / wall_m}
     stage_avg = {STAGES[i]: ms[i] / launches[i] for i in range(len(STAGES)) if launches[i] > 0}
     if "lbfgs_advance_kernel" in stage_avg:  # the advance stage of a flow session
         stage_avg["flow_advance_kernel"] = stage_avg.pop("lbfgs_advance_kernel")
@@ -361,6 +372,7 @@ def bench_flow(args) -> int:
         # (rocprofv3 instruction counts give the executed FP64 work)
         "algorithmic_frac_of_fp64_peak": flop / wall / 1e12 / PEAK_FP64_TFLOPS,
         "kernel_avg_ms": stage_avg,
+        "mixed": mixed,
         "cpu_baseline": None,
     }
     print(json.dumps(line))
