@@ -320,7 +320,9 @@ def test_lbfgs_advance_variants_agree(gpu_ctx, n: int, d: int) -> None:
 
 
 def test_lbfgs_advance_variant_selection(gpu_ctx) -> None:
-    """Auto runs the register two-loop; the Gram form is opt-in and needs memory <= 128."""
+    """Auto runs the register two-loop where N d fits it, else the Gram form (memory
+    <= 128) or the global-memory two-loop; the Gram form can be asked for at any
+    N d with memory <= 128."""
     import ctypes as C
 
     from xla_cco import _native
@@ -344,6 +346,16 @@ def test_lbfgs_advance_variant_selection(gpu_ctx) -> None:
         assert np.all(res.energy < 1e-20)
         sess.close()
     assert got == {100: 2, 150: 2}
+    # N d = 8400: past the register form (8192)
+    e2, _, _ = _poisson(2800, 3, 0.1)
+    s2 = np.random.default_rng(13).uniform(size=(1, 2800, 3))
+    for mem, want in ((100, 3), (150, 1)):
+        sess = RelaxSession(e2, 1, 2800, 3, LBFGSSettings(memory=mem))
+        sess.init(s2)
+        v = C.c_int64(0)
+        _native.check(L.cco_relax_get_option(sess.handle, 1, C.byref(v)))
+        assert int(v.value) == want
+        sess.close()
 
 
 @pytest.mark.parametrize("method", ["lbfgs", "descent"])
