@@ -491,6 +491,9 @@ def main() -> int:
     ap.add_argument("--autotune", action="store_true",
                     help="pick rho chunk, shards and graph steps with xla_cco.minimize.tuning "
                          "(timed on the first warmup input, outside the timed region)")
+    ap.add_argument("--tune-rounding", action="store_true",
+                    help="with --autotune, also pick the rho split length and the L-BFGS "
+                         "advance variant (results change at rounding level)")
     ap.add_argument(
         "--uneven",
         action="store_true",
@@ -563,7 +566,7 @@ def main() -> int:
         from xla_cco.minimize import tuning
 
         choice = tuning.tune_relax(energy, np.random.default_rng(1000).uniform(size=(B, n, d)), settings,
-                                   store=False)
+                                   store=False, exact=not args.tune_rounding)
     sess = RelaxSession(energy, B, n, d, settings)
     _native.check(L.cco_relax_set_option(sess.handle, 1, args.advance_variant))
     _native.check(L.cco_relax_set_option(sess.handle, 2, args.shards))

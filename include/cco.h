@@ -101,6 +101,14 @@ int cco_value_grad_density_dev(cco_energy* e, const double* s_dev, int B, int N,
  * 0 keeps the column layout.  Summation orders differ (within the
  * split-precision error bound, not bitwise); get returns the layout in use. */
 #define CCO_EOPT_MP_LAYOUT 3
+/* CCO_EOPT_RHO_SPLIT: particles per density split of a uniform batch, 0 (the
+ * default: 128 up to N = 2048, else the multiple of 128 giving at most 16
+ * splits) or a multiple of 128 up to 65536.  A value longer than the default
+ * gives fewer, longer splits (less partial-sum traffic, fewer workgroups); a
+ * shorter one acts as the default, whose split count the partial-sum buffers
+ * hold.  The splits' sums are added in a different grouping, so results
+ * change at rounding level (not bitwise).  Get returns the requested value. */
+#define CCO_EOPT_RHO_SPLIT 4
 int cco_energy_set_option(cco_energy* e, int key, int64_t value);
 int cco_energy_get_option(const cco_energy* e, int key, int64_t* value);
 

@@ -130,6 +130,13 @@ int split_len_for(int N) {
   return ((N + kMaxSplits * kSplitLen - 1) / (kMaxSplits * kSplitLen)) * kSplitLen;
 }
 int n_splits_for(int N) { return (N + split_len_for(N) - 1) / split_len_for(N); }
+// The split length a uniform batch runs: the energy's request when it is longer
+// than the default (never more splits than the buffers sized by n_splits_for).
+int split_len_of(const Geo& g, int N) {
+  const int def = split_len_for(N);
+  return g.rho_split > def ? g.rho_split : def;
+}
+int n_splits_of(const Geo& g, int N) { return (N + split_len_of(g, N) - 1) / split_len_of(g, N); }
 int n_splits_uneven(int N) { return (N + kSplitLen - 1) / kSplitLen; }
 int n_ftiles_for(int N) { return (N + kForceTile - 1) / kForceTile; }
 
@@ -506,8 +513,8 @@ Batch energy_batch_desc(const Geo& g, const EnergyWs& ws, const double* s_dev, i
   b.n_active = ws.n_active;
   b.N = N;
   b.lane_stride = N * g.d;
-  b.n_splits = n_splits_for(N);
-  b.split_len = split_len_for(N);
+  b.n_splits = n_splits_of(g, N);
+  b.split_len = split_len_of(g, N);
   b.n_ftiles = n_ftiles_for(N);
   b.s = s_dev;
   b.grad = grad_dev;
@@ -734,8 +741,8 @@ Batch relax_batch(cco_relax* R, const RelaxDev& dv) {
   b.n_active = dv.n_active;
   b.N = R->N;
   b.lane_stride = R->dev.stride;
-  b.n_splits = n_splits_for(R->N);
-  b.split_len = R->uneven ? kSplitLen : split_len_for(R->N);
+  b.n_splits = R->uneven ? n_splits_for(R->N) : n_splits_of(R->E->geo, R->N);
+  b.split_len = R->uneven ? kSplitLen : split_len_of(R->E->geo, R->N);
   b.n_ftiles = n_ftiles_for(R->N);
   b.s = R->dev.s_eval;
   b.grad = R->dev.g_eval;
@@ -1794,6 +1801,12 @@ int cco_energy_set_option(cco_energy* E, int key, int64_t value) {
     E->geo.rho_chunk = (int)value;
     return CCO_OK;
   }
+  if (key == CCO_EOPT_RHO_SPLIT) {
+    if (value < 0 || value > 65536 || value % kSplitLen != 0)
+      return fail(CCO_ERR_ARG, "rho split must be 0 or a multiple of 128 up to 65536 particles");
+    E->geo.rho_split = (int)value;
+    return CCO_OK;
+  }
   if (key == CCO_EOPT_MP_LAYOUT) {
     if (value != 0 && value != 1) return fail(CCO_ERR_ARG, "split-precision layout option must be 0 or 1");
     use_pair_layout(E, value == 1);
@@ -1815,6 +1828,10 @@ int cco_energy_get_option(const cco_energy* E, int key, int64_t* value) {
   }
   if (key == CCO_EOPT_MP_LANE) {
     *value = E->geo.mp_lane;
+    return CCO_OK;
+  }
+  if (key == CCO_EOPT_RHO_SPLIT) {
+    *value = E->geo.rho_split;
     return CCO_OK;
   }
   if (key == CCO_EOPT_MP_LAYOUT) {
@@ -1959,8 +1976,8 @@ int cco_energy_profile(cco_energy* E, const double* s_dev, int B, int N, int rep
   b.n_active = (const int*)E->n_active.p;
   b.N = N;
   b.lane_stride = N * g.d;
-  b.n_splits = n_splits_for(N);
-  b.split_len = split_len_for(N);
+  b.n_splits = n_splits_of(g, N);
+  b.split_len = split_len_of(g, N);
   b.n_ftiles = n_ftiles_for(N);
   b.s = s_dev;
   b.grad = (double*)E->grad.p;

@@ -46,6 +46,7 @@ struct Geo {
   int n_utiles; // rho workgroups per (lane, split)
   int Q;        // row-run seeds per particle: z1^(q kRowSeg), q < Q
   int rho_chunk;  // particles staged in LDS per rho chunk (16, 32 or 64)
+  int rho_split;  // requested particles per rho split (0: the default rule)
   int n_rtiles;   // reduce workgroups per lane (kReduceTile box / k entries each)
   int rho_halves; // threads per rho unit (1 or 2)
   // split-precision (MFMA) path

@@ -257,12 +257,16 @@ def relax(
     settings: Settings,
     chunk_iters: int = 2_000,
     autotune: bool = False,
+    tune_exact: bool = True,
 ) -> RelaxResult:
     """Relax a batch (b, n, d) of configurations; leaves have a leading lane axis.
 
     ``autotune`` (no reference counterpart) runs the built-in energy with the
     launch and tile knobs xla_cco.minimize.tuning picked for this workload class,
     timing the candidates on ``s0`` the first time; results are bit-identical.
+    ``tune_exact=False`` lets it also pick the rho split length and the L-BFGS
+    advance variant, which change results at rounding level (the tolerance is
+    stated in xla_cco.minimize.tuning); the energy's options are restored after.
 
     Raises TypeError if ``s0`` is not float64 or the settings type is unknown,
     RuntimeError if lanes are still running after the round bound.
@@ -281,16 +285,20 @@ def relax(
     if autotune and isinstance(energy, StealthyEnergy):
         from xla_cco.minimize import tuning
 
-        choice = tuning.choice_for(energy, s0, settings)
+        choice = tuning.choice_for(energy, s0, settings, exact=tune_exact)
     sess = RelaxSession(energy, b, n, d, settings)
+    saved = None
     try:
         if choice is not None:
+            saved = tuning.energy_options(energy)
             tuning.apply(sess, choice)
         sess.init(s0)
         sess.run(_eval_bound(settings, chunk_iters), sync_every=max(1, min(chunk_iters, 256)))
         return sess.result()
     finally:
         sess.close()
+        if saved is not None:
+            tuning.restore_energy(energy, saved)
 
 
 class UnevenSession:
