@@ -24,11 +24,11 @@ def load(path: Path) -> C.CDLL:
     """Open a build of cco_oracle.c and declare its entry points."""
     h = C.CDLL(str(path))
     h.oracle_energy_grad_batch.argtypes = [
-        _PD, C.c_int, C.c_int, _PD, _PD, C.c_int, C.c_int, C.c_double, _PD, _PD, C.c_int
+        _PD, C.c_int, C.c_int, _PD, _PD, C.c_int, C.c_int, C.c_double, _PD, _PD, C.c_int, C.c_int
     ]
     h.oracle_relax.argtypes = [
         _PD, C.c_int, C.c_int, _PD, _PD, C.c_int, C.c_int, C.c_double, C.c_int,
-        _PD, _PI64, _PD, _PI64, C.c_int64, _PD, _PD, _PI64, _PI64, _PI64, C.c_int,
+        _PD, _PI64, _PD, _PI64, C.c_int64, _PD, _PD, _PI64, _PI64, _PI64, C.c_int, C.c_int,
     ]
     return h
 
@@ -47,15 +47,21 @@ def _d(a: np.ndarray):
 
 
 def energy_grad_batch(
-    s: np.ndarray, n: np.ndarray, w: np.ndarray, volume: float, threads: int = 1
+    s: np.ndarray, n: np.ndarray, w: np.ndarray, volume: float, threads: int = 1, phases: int = 0
 ) -> Tuple[np.ndarray, np.ndarray]:
+    """Phi and dPhi/ds of (B, N, d) configurations.
+
+    ``phases`` 0 evaluates cos/sin of 2 pi n.s per (k, j) as the reference does;
+    1 multiplies per-axis power tables (the accuracy class of the HIP kernels).
+    """
     s = np.ascontiguousarray(s, dtype=np.float64)
     nf = np.ascontiguousarray(n, dtype=np.float64)
     w = np.ascontiguousarray(w, dtype=np.float64)
     b, nn, d = s.shape
     phi = np.empty(b)
     g = np.empty_like(s)
-    lib().oracle_energy_grad_batch(_d(s), b, nn, _d(nf), _d(w), nf.shape[0], d, volume, _d(phi), _d(g), threads)
+    lib().oracle_energy_grad_batch(_d(s), b, nn, _d(nf), _d(w), nf.shape[0], d, volume, _d(phi), _d(g),
+                                   threads, int(phases))
     return phi, g
 
 
@@ -68,8 +74,11 @@ def relax(
     eval_cap: int = 2**62,
     threads: int = 1,
     handle: Optional[C.CDLL] = None,
+    phases: int = 0,
 ):
     """Batched relax with an oracle.DescentSettings / oracle.LBFGSSettings.
+
+    ``phases`` selects the energy's phase evaluation as in energy_grad_batch.
 
     ``handle`` selects another build of cco_oracle.c (default: the Makefile's).
     """
@@ -105,6 +114,6 @@ def relax(
         _d(s0), b, nn, _d(nf), _d(w), nf.shape[0], d, volume, method,
         _d(dcfg), dint.ctypes.data_as(_PI64), _d(lcfg), lint.ctypes.data_as(_PI64), int(eval_cap),
         _d(s), _d(e), ne.ctypes.data_as(_PI64), ni.ctypes.data_as(_PI64), st.ctypes.data_as(_PI64),
-        threads,
+        threads, int(phases),
     )
     return s, e, ne, ni, st

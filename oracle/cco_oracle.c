@@ -8,6 +8,14 @@
  * Follows (paths in the sdawley1/xla-cco tree):
  *   energy + gradient : src/xla_cco/energy.py:45-50, 206-225, 301-324
  *                       (trig per (k, j): theta = 2 pi n.s_j, cos and sin)
+ *                       phases = 1 forms exp(2 pi i n.s_j) instead as the
+ *                       product over axes of z_a^{n_a}, z_a = exp(2 pi i s_a)
+ *                       rounded once from 80-bit cosl/sinl and raised to
+ *                       integer powers by repeated multiplication: the
+ *                       accuracy class of the factorised phase tables (errors
+ *                       of a few ulps per power step, against 2 pi |n.s| ulps
+ *                       of the rounded argument in the trig form), which
+ *                       matters only at the round-off floor (Phi ~ 1e-27)
  *   adaptive descent  : src/xla_cco/minimize/descent.py:92-197
  *   L-BFGS            : src/xla_cco/minimize/lbfgs.py:163-329 with the optax
  *                       0.2.8 lbfgs + zoom line search restated in
@@ -38,7 +46,49 @@ typedef struct {
   const double* w; /* M */
   int M, d;
   double volume;
+  int phases; /* 0: trig of the rounded argument, 1: per-axis power tables */
 } kset_t;
+
+/* Per-axis power tables of one configuration: tab[(j*d + a)*(P+1) + p] =
+   z_{j,a}^p, z = exp(2 pi i s_{j,a}), p = 0..P (complex, re/im interleaved). */
+static double* phase_tables(const double* s, int N, const kset_t* K, int* P_out) {
+  int P = 0;
+  for (size_t i = 0; i < (size_t)K->M * K->d; ++i) {
+    const int e = (int)fabs(K->n[i]);
+    if (e > P) P = e;
+  }
+  const size_t per = (size_t)P + 1;
+  double* tab = (double*)malloc(sizeof(double) * 2 * per * (size_t)N * K->d);
+  for (size_t ja = 0; ja < (size_t)N * K->d; ++ja) {
+    const long double th = 6.28318530717958647692528676655900577L * (long double)s[ja];
+    const double zr = (double)cosl(th), zi = (double)sinl(th);
+    double* t = tab + 2 * per * ja;
+    t[0] = 1.0;
+    t[1] = 0.0;
+    for (size_t p = 1; p < per; ++p) {
+      t[2 * p] = t[2 * p - 2] * zr - t[2 * p - 1] * zi;
+      t[2 * p + 1] = t[2 * p - 2] * zi + t[2 * p - 1] * zr;
+    }
+  }
+  *P_out = P;
+  return tab;
+}
+
+/* exp(2 pi i n.s_j) as the product of the axes' table entries (conjugate for n_a < 0). */
+static void table_phase(const double* tab, int P, int d, int j, const double* nk, double* cr,
+                        double* ci) {
+  double pr = 1.0, pi_ = 0.0;
+  for (int a = 0; a < d; ++a) {
+    const int e = (int)nk[a];
+    const double* t = tab + 2 * ((size_t)P + 1) * ((size_t)j * d + a) + 2 * (size_t)(e < 0 ? -e : e);
+    const double er = t[0], ei = e < 0 ? -t[1] : t[1];
+    const double tr = pr * er - pi_ * ei;
+    pi_ = pr * ei + pi_ * er;
+    pr = tr;
+  }
+  *cr = pr;
+  *ci = pi_;
+}
 
 /* Phi, dPhi/ds (N x d), Re/Im rho (M, may be NULL). */
 void oracle_energy_grad(const double* s, int N, const kset_t* K, double* phi, double* grad,
@@ -46,10 +96,19 @@ void oracle_energy_grad(const double* s, int N, const kset_t* K, double* phi, do
   const int d = K->d, M = K->M;
   double* re = (double*)malloc(sizeof(double) * (size_t)M);
   double* im = (double*)malloc(sizeof(double) * (size_t)M);
+  int P = 0;
+  double* tab = K->phases ? phase_tables(s, N, K, &P) : NULL;
   double value = 0.0;
   for (int k = 0; k < M; ++k) {
     double a = 0.0, b = 0.0;
     for (int j = 0; j < N; ++j) {
+      if (tab) {
+        double cr, ci;
+        table_phase(tab, P, d, j, K->n + (size_t)k * d, &cr, &ci);
+        a += cr;
+        b += ci;
+        continue;
+      }
       double dot = 0.0;
       for (int c = 0; c < d; ++c) dot += K->n[(size_t)k * d + c] * s[(size_t)j * d + c];
       const double th = TWO_PI * dot;
@@ -66,10 +125,17 @@ void oracle_energy_grad(const double* s, int N, const kset_t* K, double* phi, do
     for (int j = 0; j < N; ++j) {
       double acc[3] = {0.0, 0.0, 0.0};
       for (int k = 0; k < M; ++k) {
-        double dot = 0.0;
-        for (int c = 0; c < d; ++c) dot += s[(size_t)j * d + c] * K->n[(size_t)k * d + c];
-        const double th = TWO_PI * dot;
-        const double coef = K->w[k] * (im[k] * cos(th) - re[k] * sin(th));
+        double cs, sn;
+        if (tab) {
+          table_phase(tab, P, d, j, K->n + (size_t)k * d, &cs, &sn);
+        } else {
+          double dot = 0.0;
+          for (int c = 0; c < d; ++c) dot += s[(size_t)j * d + c] * K->n[(size_t)k * d + c];
+          const double th = TWO_PI * dot;
+          cs = cos(th);
+          sn = sin(th);
+        }
+        const double coef = K->w[k] * (im[k] * cs - re[k] * sn);
         for (int c = 0; c < d; ++c) acc[c] += coef * K->n[(size_t)k * d + c];
       }
       for (int c = 0; c < d; ++c) grad[(size_t)j * d + c] = pref * acc[c];
@@ -77,6 +143,7 @@ void oracle_energy_grad(const double* s, int N, const kset_t* K, double* phi, do
   }
   if (re_out) memcpy(re_out, re, sizeof(double) * (size_t)M);
   if (im_out) memcpy(im_out, im, sizeof(double) * (size_t)M);
+  free(tab);
   free(re);
   free(im);
 }
@@ -84,8 +151,8 @@ void oracle_energy_grad(const double* s, int N, const kset_t* K, double* phi, do
 /* Batched energy+force over B lanes (OpenMP over lanes). */
 void oracle_energy_grad_batch(const double* s, int B, int N, const double* n, const double* w,
                               int M, int d, double volume, double* phi, double* grad,
-                              int threads) {
-  kset_t K = {n, w, M, d, volume};
+                              int threads, int phases) {
+  kset_t K = {n, w, M, d, volume, phases};
   const size_t nd = (size_t)N * d;
 #ifdef _OPENMP
   if (threads > 0) omp_set_num_threads(threads);
@@ -449,8 +516,8 @@ void oracle_relax(const double* s0, int B, int N, const double* n, const double*
                   double volume, int method, const double* dcfg, const int64_t* dint,
                   const double* lcfg, const int64_t* lint, int64_t eval_cap, double* s_out,
                   double* e_out, int64_t* evals_out, int64_t* iters_out, int64_t* status_out,
-                  int threads) {
-  kset_t K = {n, w, M, d, volume};
+                  int threads, int phases) {
+  kset_t K = {n, w, M, d, volume, phases};
   const size_t nd = (size_t)N * d;
   descent_cfg dc;
   lbfgs_cfg lc;
