@@ -141,12 +141,25 @@ def published_shape(chi: float, reps: int = 2, autotune: bool = False) -> dict:
     energies = np.concatenate([p.energy for p in out])
     evals = int(sum(int(p.n_evals.sum()) for p in out))
     ok = int(np.sum(energies < tol))
+    # the same 1200 configurations as one batch (its L-BFGS history, 0.6 GB, is
+    # far inside one GPU's 288 GB): no second, mostly idle batch of 176 lanes
+    one, one_out = float("inf"), None
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        res = relax(energy, s0, LBFGSSettings(), autotune=autotune)
+        wall = time.perf_counter() - t0
+        if wall < one:
+            one, one_out = wall, res
+    ok_one = int(np.sum(one_out.energy < tol))
     ref = PUBLISHED_SHAPE[chi]
     return {"value": ok / best, "unit": "configs/s", "converged": f"{ok}/1200", "wall_s": best,
             "total_evals": evals, "M": int(ks.n.shape[0]),
             "reference": {"value": ref["value"], "hardware": "A100 80GB PCIe", "total_evals": ref["total_evals"],
                           "source": "benchmarks/RESULTS.md:116-120"},
-            "vs_reference": (ok / best) / ref["value"]}
+            "vs_reference": (ok / best) / ref["value"],
+            "one_batch": {"value": ok_one / one, "converged": f"{ok_one}/1200", "wall_s": one,
+                          "total_evals": int(one_out.n_evals.sum()),
+                          "note": "all 1200 configurations in one relax call instead of batches of 1024"}}
 
 
 def energy_times(d: int, n: int, chi: float, B: int, steps: int, warmup: int):
