@@ -403,6 +403,103 @@ def bench_flow(args) -> int:
     return 0
 
 
+def bench_md_drift(args) -> int:
+    """The long NVE run of the large MD shape: split-precision flow, ground states, then
+    args.md_steps velocity-Verlet steps in FP64 and in split precision from the same
+    states; energy drift of each and their difference."""
+    from xla_cco import _native, kvectors as K
+    from xla_cco.box import cubic_box
+    from xla_cco.dynamics import MDSettings, Waves, init_state, prng_key
+    from xla_cco.dynamics.run import MDSession
+    from xla_cco.energy import make_stealthy_energy
+    from xla_cco.minimize import LBFGSSettings, relax
+    from xla_cco.minimize.flow import FlowSettings, gradient_flow_batch
+
+    ctx = _native.context(0)
+    d, n, chi, B = 3, 10000, 0.45, 32
+    box = cubic_box(n, d)
+    ks = K.enumerate_kvectors(box.basis, 0.0, K.k_outer_from_ka(K.ka_from_chi(chi, 0.05, d), 0.05, d))
+    M = int(ks.n.shape[0])
+    energy = make_stealthy_energy(box, ks)
+    s0 = np.random.default_rng(41).uniform(size=(B, n, d))
+
+    def note(msg: str) -> None:
+        print(msg, file=sys.stderr, flush=True)
+
+    t0 = time.perf_counter()
+    fl = gradient_flow_batch(energy, s0, box.basis, FlowSettings(max_steps=args.flow_steps, precision="mixed"))
+    t_flow = time.perf_counter() - t0
+    note(f"split flow: {args.flow_steps} steps in {t_flow:.1f} s, mean Phi {fl.energy.mean():.4e}")
+    t0 = time.perf_counter()
+    gs = relax(energy, fl.s, LBFGSSettings(energy_tol=1e-20, precision="mixed"))
+    t_relax = time.perf_counter() - t0
+    ok = int(np.sum(gs.energy < 1e-20))
+    note(f"split relax to E < 1e-20: {ok}/{B} in {t_relax:.1f} s")
+    r0 = gs.s @ box.basis
+    v0 = np.random.default_rng(42).normal(scale=np.sqrt(1e-4), size=r0.shape)
+    kin0 = 0.5 * np.sum(v0 * v0, axis=(1, 2))
+    nv = np.asarray(ks.n, dtype=np.float64)
+    waves = Waves(n_vectors=nv, kvec=nv @ (2.0 * np.pi * np.linalg.inv(box.basis).T),
+                  constrained=np.ones(M, bool), n_constrained=M)
+    keys = np.stack([prng_key(i) for i in range(B)])
+    chunk = 10_000
+    out = {}
+    for prec in ("fp64", "mixed"):
+        st = MDSettings(dt=0.01, remove_com=False, drift_tol=float("inf"), precision=prec, chunk_steps=chunk)
+        lanes = init_state(energy, waves, box.basis, r0, v0, 1e-4, keys, st)
+        dev = MDSession(energy, waves, box.basis, st, B, n, d)
+        dev.put(lanes)
+        ctx.synchronize()
+        t0 = time.perf_counter()
+        done = 0
+        while done < args.md_steps:
+            k = min(chunk, args.md_steps - done)
+            dev.run(k)
+            done += k
+            ctx.synchronize()
+            note(f"{prec}: {done} steps, {time.perf_counter() - t0:.1f} s")
+        wall = time.perf_counter() - t0
+        out[prec] = (wall, dev.get(lanes))
+        dev.close()
+    (w64, a), (wmx, m) = out["fp64"], out["mixed"]
+    e_a = a.phi + 0.5 * np.sum(a.v * a.v, axis=(1, 2))
+    e_m = m.phi + 0.5 * np.sum(m.v * m.v, axis=(1, 2))
+    line = {
+        "metric": "NVE MD energy drift over a long run, FP64 vs split-precision forces",
+        "value": B * args.md_steps / wmx,
+        "unit": "lane-steps/s (split precision)",
+        "n_gpus": 1,
+        "steps": args.md_steps,
+        "warmup": 0,
+        "ms_per_step": wmx / args.md_steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64 positions/velocities, split-precision (fp16 hi+lo MFMA) forces",
+        "data": "synthetic",
+        "config": {"workload": f"split-precision DOPRI5 flow ({args.flow_steps} steps) and L-BFGS relax of {B} "
+                               f"uniform-random configurations, d={d} N={n} chi={chi} (M={M}), then NVE velocity "
+                               f"Verlet from the relaxed states, v ~ N(0, 1e-4), dt=0.01, {args.md_steps} steps",
+                   "lanes": B, "M": M},
+        "flow": {"seconds": t_flow, "mean_phi": float(fl.energy.mean())},
+        "relax": {"seconds": t_relax, "converged": f"{ok}/{B}"},
+        "fp64": {"lane_steps_per_s": B * args.md_steps / w64, "ms_per_step": w64 / args.md_steps * 1e3,
+                 "max_drift_over_ke0": float(np.max(a.max_drift / kin0)),
+                 "final_energy_change_over_ke0": float(np.max(np.abs(e_a - a.e0) / kin0)),
+                 "status_ok": int(np.sum(a.status == 0))},
+        "mixed": {"lane_steps_per_s": B * args.md_steps / wmx, "ms_per_step": wmx / args.md_steps * 1e3,
+                  "vs_fp64": w64 / wmx,
+                  "max_drift_over_ke0": float(np.max(m.max_drift / kin0)),
+                  "final_energy_change_over_ke0": float(np.max(np.abs(e_m - m.e0) / kin0)),
+                  "status_ok": int(np.sum(m.status == 0))},
+        "mixed_vs_fp64_final": {"energy_rel": float(np.max(np.abs(e_m - e_a) / np.abs(e_a))),
+                                "max_abs_dr": float(np.max(np.abs(m.r - a.r)))},
+        "cpu_baseline": None,
+    }
+    print(json.dumps(line))
+    return 0
+
+
 def bench_md(args) -> int:
     """NVE MD steps of 32 lanes at d=3, N=10000, chi=0.45 (the large MD shape): lane-steps/s."""
     from xla_cco import _native, kvectors as K
@@ -530,6 +627,11 @@ def main() -> int:
                     help="time NVE MD at d=3 N=10000 chi=0.45 with 32 lanes instead (steps = MD steps)")
     ap.add_argument("--flow", action="store_true",
                     help="time the DOPRI5 gradient flow at d=3 N=10000 chi=0.45 with 32 lanes (steps = evaluations)")
+    ap.add_argument("--md-drift", action="store_true",
+                    help="the long NVE run of the large MD shape: split flow, relax, then --md-steps "
+                         "steps in FP64 and split precision; energy drift of each")
+    ap.add_argument("--md-steps", type=int, default=100_000)
+    ap.add_argument("--flow-steps", type=int, default=60)
     ap.add_argument("--md-track", type=float, default=1.0,
                     help="with --md: track every wave with |k| < md_track K, wound every step "
                          "(1: the constrained waves only; the reference MD command line tracks 2K)")
@@ -564,6 +666,8 @@ def main() -> int:
         return bench_energy(args)
     if args.flow:
         return bench_flow(args)
+    if args.md_drift:
+        return bench_md_drift(args)
     if args.md:
         return bench_md(args)
 
