@@ -265,6 +265,45 @@ def test_tracked_pass_at_full_shape(gpu_ctx) -> None:
         assert np.all(out.n_valid[i, pick] == 2)
 
 
+@pytest.mark.parametrize("shape", ["d3-n300", "d3-n10000"])
+def test_tracked_pass_split_precision(gpu_ctx, shape) -> None:
+    """A split-precision MD session sums the tracked waves (every |k| < 2K) on the MFMA
+    kernels as well: after a few steps their densities equal direct sums over the
+    returned positions within the split-precision bound 2e-5 sqrt(N), while the FP64
+    session's equal them to round-off (so the two sessions took different paths)."""
+    if shape == "d3-n300":
+        e, basis, waves, _ = _system(n=300, d=3, chi=0.1, track=2.0)
+        n, d, b, steps = 300, 3, 2, 4
+    else:
+        d, n, b, steps = 3, 10000, 32, 2
+        box = cubic_box(n, d)
+        k_c = K.k_outer_from_ka(K.ka_from_chi(0.45, 0.05, d), 0.05, d)
+        e = make_stealthy_energy(box, K.enumerate_kvectors(box.basis, 0.0, k_c))
+        basis = box.basis
+        waves = make_waves(basis, k_c, 2.0 * k_c)
+    rng = np.random.default_rng(9)
+    r0 = rng.uniform(size=(b, n, d)) @ basis
+    v0 = rng.normal(scale=1e-2, size=(b, n, d))
+    keys = np.stack([prng_key(i) for i in range(b)])
+    nc, w = waves.n_constrained, waves.kvec.shape[0]
+    pick = np.r_[nc, rng.choice(np.arange(nc + 1, w - 1), min(40, w - nc - 2), replace=False), w - 1]
+    err = {}
+    for prec in ("fp64", "mixed"):
+        st = MDSettings(dt=0.01, drift_tol=math.inf, wind_every=1, precision=prec)
+        out = simulate(e, waves, basis, st, init_state(e, waves, basis, r0, v0, 1e-4, keys, st), steps).state
+        worst = 0.0
+        for i in (0, b - 1):
+            th = out.r[i] @ waves.kvec[pick].T
+            want = np.stack([np.cos(th).sum(axis=0), np.sin(th).sum(axis=0)], axis=-1)
+            worst = max(worst, float(np.max(np.abs(out.rho_prev[i, pick] - want))))
+            assert np.all(out.n_valid[i, pick] >= 1)
+        err[prec] = worst
+    print(f"{shape}: tracked rho vs direct sums, FP64 {err['fp64']:.2e}, split {err['mixed']:.2e}", flush=True)
+    assert err["fp64"] <= 1e-9 * n
+    assert err["mixed"] <= 2e-5 * math.sqrt(n)
+    assert err["mixed"] > 100.0 * err["fp64"]  # the split-precision kernels summed them
+
+
 def test_session_call_order_and_counters(gpu_ctx) -> None:
     """cco_md_run / cco_md_get before a state is put are STATE errors; counters that
     would index past the snapshot or trace rows are rejected before any copy."""

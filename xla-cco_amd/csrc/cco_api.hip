@@ -670,6 +670,7 @@ struct cco_relax {
   cco_energy* trk = nullptr;
   int trk_splits = 0, trk_split_len = 0;  // its particle splits (fewer than the energy's)
   double2 *trk_part = nullptr, *trk_cbox = nullptr, *trk_rbox = nullptr;
+  unsigned char* trk_z1g = nullptr;  // split precision: that pass's z1 planes (null: FP64 pass)
   double2* md_rbox = nullptr;  // the force pass's rho per box entry (md_obs.rbox)
   double* trk_rphi = nullptr;
   std::vector<hipEvent_t> evs;
@@ -793,6 +794,10 @@ Batch track_batch(cco_relax* R, const RelaxDev& dv) {
   b.cbox = R->trk_cbox;
   b.phi_part = R->trk_rphi;
   b.skip = R->mc.trk_skip;
+  if (R->trk_z1g != nullptr && R->dev.prec != nullptr) {  // split-precision lanes: MFMA rho
+    b.prec = R->dev.prec;
+    b.z1g = R->trk_z1g;
+  }
   return b;
 }
 
@@ -814,7 +819,9 @@ void relax_stage(cco_relax* R, const Shard& sh, int stage) {
   } else if (stage == STG_COMPACT) {
     launch_compact(dv, st, sh.lo, sh.count, (unsigned)(sh.steps + 1));
   } else if (stage == STG_TRACK) {
-    launch_rho_reduce(R->trk->geo, track_batch(R, dv), ub, st);
+    const Batch tb = track_batch(R, dv);
+    launch_rho_mixed(R->trk->geo, tb, ub, st);  // split-precision lanes (the FP64 rho skips them)
+    launch_rho_reduce(R->trk->geo, tb, ub, st);
   } else if (stage >= STG_GRAM_DOT) {
     launch_lbfgs_gram(dv, R->lc, ub, stage - STG_GRAM_DOT + 1, st);
   } else {
@@ -2642,6 +2649,11 @@ int md_tracked_pass(cco_relax* R, int n_waves, int n_constrained, const double* 
   go(R->trk_rphi, B * g.n_rtiles);
   go(R->trk_rbox, B * g.Mbox);
   go(R->mc.trk_skip, B);
+  // a split-precision session sums the tracked waves on the MFMA kernels too
+  // when their k-set fits them (row tables, LDS): their z1 planes
+  const Geo& gm = g.mp_geo != nullptr ? *g.mp_geo : g;
+  if (R->precision == 1 && R->E->geo.mp_ok && (gm.mp_ok || mp_rho_fits(gm)))
+    go(R->trk_z1g, B * mp_z1g_bytes(gm, R->N));
   if (rc) return rc;
   R->mc.trk_mbox = g.Mbox;
   o.trk_rbox = R->trk_rbox;
@@ -2681,6 +2693,7 @@ int cco_md_configure(cco_relax* R, const cco_md_settings* ms, int n_waves, int n
   R->trk = nullptr;
   R->trk_part = R->trk_cbox = nullptr;
   R->trk_rphi = nullptr;
+  R->trk_z1g = nullptr;
   MdCfg& c = R->mc;
   c.trk_skip = nullptr;
   c.trk_mbox = 0;

@@ -193,6 +193,11 @@ int mp_force_batch_for(const Geo& g);  // 16-column tiles per force batch (0: do
 // for a geometry (two groups when their LDS fits half a CU, else one).
 void mp_force_plan(const Geo& g, int* pg, int* cb);
 void launch_energy_mixed(int stage, const Geo& g, const Batch& b, int slots_ub, hipStream_t st);
+// Density only, split precision, tiled kernels (z1 planes + rho_mp) for the
+// batch's split-precision lanes; its reduce is the FP64 reduce_kernel.
+void launch_rho_mixed(const Geo& g, const Batch& b, int slots_ub, hipStream_t st);
+// 1 when a geometry's split-precision rho tables fit the rho kernel (density-only passes)
+int mp_rho_fits(const Geo& g);
 // 1 when a geometry's rho tiles fit mp_lane_kernel (the batch's N decides the rest)
 int mp_lane_tiles_ok(const Geo& g);
 // LDS bytes of mp_lane_kernel for one lane of this geometry and N (0: the lane does not fit it)

@@ -270,6 +270,7 @@ __global__ __launch_bounds__(256) void mp_z1_kernel(Geo g, Batch b) {
   if (slot >= *b.n_active) return;
   const int lane = b.active[slot];
   if (b.prec[lane] == 0) return;
+  if (b.skip != nullptr && b.skip[lane]) return;
   const int ch = (blockIdx.x % n_wg) * kZ1Chunks + (int)threadIdx.x / kMpChunk;
   if (ch >= n_ch) return;  // no barrier below
   const int jj = (int)threadIdx.x % kMpChunk;
@@ -418,6 +419,7 @@ __global__ __launch_bounds__(64 * W) void rho_mp_kernel(Geo g, Batch b) {
   const int ct = t / b.n_splits;
   const int lane = b.active[slot];
   if (b.prec[lane] == 0) return;  // uniform per workgroup, before any barrier
+  if (b.skip != nullptr && b.skip[lane]) return;
   const int wave = (int)threadIdx.x / kWave;
   const int l = (int)threadIdx.x % kWave;
   const int g16 = l / 16;
@@ -1610,6 +1612,24 @@ void launch_energy_mixed(int stage, const Geo& g, const Batch& b, int slots_ub, 
   if (g.d == 1) launch_mp_d<1>(stage, g, b, slots_ub, st);
   if (g.d == 2) launch_mp_d<2>(stage, g, b, slots_ub, st);
   if (g.d == 3) launch_mp_d<3>(stage, g.mp_geo != nullptr ? *g.mp_geo : g, b, slots_ub, st);
+}
+
+// A density-only pass needs the rho kernel's tables alone (not the force
+// kernel's LDS plan that mp_ok also checks): rows <= 48 and tables within
+// half a CU's LDS, so two 512-thread rho workgroups share a CU.
+constexpr size_t kMpRhoLdsMax = 80 * 1024;
+int mp_rho_fits(const Geo& g) {
+  return g.RP <= 48 && mp_table_bytes(g.RP, g.P2, g.P3, g.d == 2) <= kMpRhoLdsMax ? 1 : 0;
+}
+
+void launch_rho_mixed(const Geo& g, const Batch& b, int slots_ub, hipStream_t st) {
+  if (slots_ub <= 0 || b.prec == nullptr || b.lay != nullptr) return;
+  // the pair layout exists only for mp_ok geometries; else the column layout
+  const Geo& gm = (g.d == 3 && g.mp_geo != nullptr) ? *g.mp_geo : g;
+  if (!gm.mp_ok && !mp_rho_fits(gm)) return;
+  if (g.d == 1) launch_mp_d<1>(0, gm, b, slots_ub, st);
+  if (g.d == 2) launch_mp_d<2>(0, gm, b, slots_ub, st);
+  if (g.d == 3) launch_mp_d<3>(0, gm, b, slots_ub, st);
 }
 
 int mp_force_batch_for(const Geo& g) { return mp_force_batch(g.RP, g.P2, g.P3); }
