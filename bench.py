@@ -162,6 +162,49 @@ def published_shape(chi: float, reps: int = 2, autotune: bool = False) -> dict:
                           "note": "all 1200 configurations in one relax call instead of batches of 1024"}}
 
 
+# evals/s the reference published for relax on an A100-SXM4-80GB (benchmarks/RESULTS.md:55-66):
+# d=3, N=100, 1000 configurations of default_rng(0), per-(k,j) trig path, warm wall time
+PUBLISHED_RELAX_RATES = {
+    ("lbfgs", 0.05): {"evals": 455419, "wall_s": 1.58, "evals_per_s": 2.88e5},
+    ("lbfgs", 0.45): {"evals": 557049, "wall_s": 15.50, "evals_per_s": 3.594e4},
+    ("descent_400", 0.05): {"evals": 400000, "wall_s": 0.10, "evals_per_s": 3.998e6},
+    ("descent_400", 0.45): {"evals": 400000, "wall_s": 0.16, "evals_per_s": 2.477e6},
+}
+
+
+def published_relax_rates(reps: int = 2) -> dict:
+    """The reference's published relax benchmark rows, on the same inputs.
+
+    relax of default_rng(0).random((1000, 100, 3)) at chi 0.05 and 0.45 (phi=0.05) with
+    LBFGSSettings(max_evals_per_pass=200) and DescentSettings(max_evals=400), as
+    benchmarks/bench_energy.py:147-170 times them: host arrays in, results out, best of
+    ``reps`` warm runs.  evaluations/s = total n_evals / wall seconds.
+    """
+    from xla_cco.minimize import DescentSettings, LBFGSSettings, relax
+
+    out = {}
+    s0 = np.random.default_rng(0).random((1000, 100, 3))
+    for chi in (0.05, 0.45):
+        _, _, energy = build_workload(3, 100, chi)
+        for name, st in (("lbfgs", LBFGSSettings(max_evals_per_pass=200)),
+                         ("descent_400", DescentSettings(max_evals=400))):
+            relax(energy, s0[:64], st)  # warm the session paths
+            best, evals = float("inf"), 0
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                res = relax(energy, s0, st)
+                wall = time.perf_counter() - t0
+                if wall < best:
+                    best, evals = wall, int(res.n_evals.sum())
+            ref = PUBLISHED_RELAX_RATES[(name, chi)]
+            out[f"{name}_chi{chi}"] = {
+                "evals": evals, "wall_s": best, "evals_per_s": evals / best,
+                "reference": dict(ref, hardware="A100-SXM4-80GB", source="benchmarks/RESULTS.md:55-66"),
+                "wall_vs_reference": ref["wall_s"] / best,
+            }
+    return out
+
+
 def energy_times(d: int, n: int, chi: float, B: int, steps: int, warmup: int):
     """Seconds per batched value_and_grad of B device-resident configurations, FP64 and
     split precision; (times, M, split precision applied)."""
@@ -876,6 +919,7 @@ def main() -> int:
     published = None
     if not args.no_published:
         published = {f"chi{c}": published_shape(c, autotune=args.autotune) for c in (0.35, 0.45)}
+        published["relax_rates"] = published_relax_rates()
 
     cpu = None
     if not args.no_cpu:
