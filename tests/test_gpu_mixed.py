@@ -181,6 +181,31 @@ def test_mixed_relax_same_success_as_fp64(gpu_ctx) -> None:
     assert np.all(e.energy_batch(rmp.s) < 1e-19)
 
 
+@pytest.mark.parametrize("n,d,chi,b", [(300, 2, 0.3, 12), (3000, 3, 0.2, 4)])
+def test_mixed_rescue_turns_failing_lanes_fp64(gpu_ctx, n, d, chi, b) -> None:
+    """With a switch energy no lane reaches in split precision (1e-30), every lane's
+    line search fails or stagnates on split-precision energies near the floor; the lane
+    then turns FP64, re-evaluates its point and goes on, so the relax still reaches
+    E < tol on FP64's lanes (register two-loop at N d = 600, Gram form at 9000)."""
+    box, ks = _system(n, d, chi)
+    e = make_stealthy_energy(box, ks)
+    s0 = np.random.default_rng(17).uniform(size=(b, n, d))
+    tol = 9.99999968e-21
+    r64 = relax(e, s0, LBFGSSettings(energy_tol=tol))
+    sess = RelaxSession(e, b, n, d, LBFGSSettings(energy_tol=tol, precision="mixed", precision_switch=1e-30))
+    sess.init(s0)
+    assert sess.precision_state() == (True, b)
+    sess.run(10_000_000, 64)
+    out = sess.result()
+    assert sess.precision_state() == (True, 0)  # every lane was rescued into FP64
+    sess.close()
+    print(f"N={n} d={d}: evaluations {int(r64.n_evals.sum())} FP64, {int(out.n_evals.sum())} split + rescue",
+          flush=True)
+    assert np.array_equal(r64.energy < tol, out.energy < tol)
+    assert np.all(out.energy < tol)
+    assert np.all(e.energy_batch(out.s) < 1e-19)
+
+
 def test_mixed_session_switches_every_lane(gpu_ctx) -> None:
     box, ks = _system(200, 3, 0.2)
     e = make_stealthy_energy(box, ks)

@@ -8,7 +8,10 @@ namespace cco {
 constexpr int kAdvBlock = 256;
 constexpr int kMaxRescales = 8;
 
-enum LaneMode : int { MODE_INIT = 0, MODE_EVAL = 1 };
+// MODE_REEVAL (L-BFGS, split precision): the lane's last search failed or stagnated
+// on split-precision energies; it turned FP64 and its current point is being
+// evaluated again before its iteration restarts.
+enum LaneMode : int { MODE_INIT = 0, MODE_EVAL = 1, MODE_REEVAL = 2 };
 
 enum Status : int {
   ST_RUNNING = 0,

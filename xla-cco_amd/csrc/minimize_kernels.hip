@@ -484,8 +484,29 @@ __global__ __launch_bounds__(BLOCK) void lbfgs_advance_kernel(RelaxDev r, LbfgsC
   const double fe = r.phi_eval[lane];
   int code = ST_RUNNING;
   bool started = false;
+  const bool split = r.prec != nullptr && r.prec[lane] != 0;
+  bool rescue = false;
 
-  if (L.mode == MODE_INIT) {
+  if (L.mode == MODE_REEVAL) {
+    // the FP64 evaluation at the current point after a split-precision rescue:
+    // the iteration restarts from it with an empty history
+    const double rs = c.rescales[L.pass_idx];
+    for (int i = threadIdx.x; i < nd; i += blockDim.x) gx[i] = ge[i] / rs;
+    L.f = fe;
+    L.n_evals += 1;
+    L.pass_evals += 1;
+    L.mode = MODE_EVAL;
+    code = (fe < c.energy_tol) ? ST_ENERGY_TOL : ST_RUNNING;
+    __syncthreads();
+    if (code == ST_RUNNING) {
+      if constexpr (GRAM) {
+        lbfgs_gram_begin(r, c, lane, L);
+        started = true;
+      } else {
+        lbfgs_start_iteration<BLOCK>(r, c, lane, L, red);
+      }
+    }
+  } else if (L.mode == MODE_INIT) {
     // init_one (lbfgs.py:163-197)
     const double r0 = c.rescales[0];
     for (int i = threadIdx.x; i < nd; i += blockDim.x) {
@@ -569,6 +590,11 @@ __global__ __launch_bounds__(BLOCK) void lbfgs_advance_kernel(RelaxDev r, LbfgsC
       int local = (L.pass_evals >= c.max_evals_per_pass) ? ST_BUDGET : ST_RUNNING;
       if (L.stag >= c.stagnation_patience) local = ST_STAGNATED;
       if (fail) local = ST_LINESEARCH_FAIL;
+      // split precision: a search that fails, or a lane that stagnates, on
+      // split-precision energies turns the lane FP64 and re-evaluates its
+      // current point instead of ending the pass
+      rescue = split && glob == ST_RUNNING && (local == ST_LINESEARCH_FAIL || local == ST_STAGNATED);
+      if (rescue) local = ST_RUNNING;
       const bool last = L.pass_idx >= c.n_rescales - 1;
       const bool pass_end = (local != ST_RUNNING) && (glob == ST_RUNNING);
       const bool advance = pass_end && !last;
@@ -601,7 +627,13 @@ __global__ __launch_bounds__(BLOCK) void lbfgs_advance_kernel(RelaxDev r, LbfgsC
         L.rec_count += 1;
       }
       __syncthreads();
-      if (code == ST_RUNNING) {
+      if (rescue && code == ST_RUNNING) {
+        double* se = r.s_eval + off;
+        for (int i = threadIdx.x; i < nd; i += blockDim.x) se[i] = x[i] / rs;
+        L.mode = MODE_REEVAL;
+        L.stag = 0;
+        L.opt_count = 0;
+      } else if (code == ST_RUNNING) {
         if constexpr (GRAM) {
           lbfgs_gram_begin(r, c, lane, L);
           started = true;
@@ -620,7 +652,8 @@ __global__ __launch_bounds__(BLOCK) void lbfgs_advance_kernel(RelaxDev r, LbfgsC
     r.ll[lane] = L;
     r.status[lane] = code;
     if constexpr (GRAM) r.dir[lane] = make_int2(started ? 1 : 0, L.opt_count);
-    if (code == ST_RUNNING && r.prec != nullptr && c.mixed_switch > 0.0 && L.f <= c.mixed_switch)
+    if (code == ST_RUNNING && r.prec != nullptr &&
+        (rescue || (c.mixed_switch > 0.0 && L.f <= c.mixed_switch)))
       r.prec[lane] = 0;
   }
 }
@@ -977,8 +1010,25 @@ __global__ __launch_bounds__(NT) void lbfgs_advance_reg_kernel(RelaxDev r, Lbfgs
   const double fe = r.phi_eval[lane];
   int code = ST_RUNNING;
   bool start = false;
+  const bool split = r.prec != nullptr && r.prec[lane] != 0;
+  bool rescue = false;
 
-  if (L.mode == MODE_INIT) {
+  if (L.mode == MODE_REEVAL) {
+    // the FP64 evaluation at the current point after a split-precision rescue:
+    // the iteration restarts from it with an empty history
+    const double rs = c.rescales[L.pass_idx];
+#pragma unroll
+    for (int e = 0; e < EPT; ++e) {
+      const int i = reg_elem<NT, V>(tid, e);
+      if (i < nd) gx[i] = ge[i] / rs;
+    }
+    L.f = fe;
+    L.n_evals += 1;
+    L.pass_evals += 1;
+    L.mode = MODE_EVAL;
+    code = (fe < c.energy_tol) ? ST_ENERGY_TOL : ST_RUNNING;
+    start = code == ST_RUNNING;
+  } else if (L.mode == MODE_INIT) {
     const double r0 = c.rescales[0];
 #pragma unroll
     for (int e = 0; e < EPT; ++e) {
@@ -1075,6 +1125,10 @@ __global__ __launch_bounds__(NT) void lbfgs_advance_reg_kernel(RelaxDev r, Lbfgs
       int local = (L.pass_evals >= c.max_evals_per_pass) ? ST_BUDGET : ST_RUNNING;
       if (L.stag >= c.stagnation_patience) local = ST_STAGNATED;
       if (fail) local = ST_LINESEARCH_FAIL;
+      // split precision: a failed search or a stagnating lane turns FP64 and
+      // re-evaluates its current point instead of ending the pass
+      rescue = split && glob == ST_RUNNING && (local == ST_LINESEARCH_FAIL || local == ST_STAGNATED);
+      if (rescue) local = ST_RUNNING;
       const bool last = L.pass_idx >= c.n_rescales - 1;
       const bool pass_end = (local != ST_RUNNING) && (glob == ST_RUNNING);
       const bool advance = pass_end && !last;
@@ -1114,7 +1168,18 @@ __global__ __launch_bounds__(NT) void lbfgs_advance_reg_kernel(RelaxDev r, Lbfgs
         }
         L.rec_count += 1;
       }
-      start = code == ST_RUNNING;
+      start = code == ST_RUNNING && !rescue;
+      if (rescue && code == ST_RUNNING) {
+        double* se = r.s_eval + off;
+#pragma unroll
+        for (int e = 0; e < EPT; ++e) {
+          const int i = reg_elem<NT, V>(tid, e);
+          if (i < nd) se[i] = x[i] / rs;
+        }
+        L.mode = MODE_REEVAL;
+        L.stag = 0;
+        L.opt_count = 0;
+      }
     } else {
       const double a = L.ls.a;
       double* se = r.s_eval + off;
@@ -1129,7 +1194,8 @@ __global__ __launch_bounds__(NT) void lbfgs_advance_reg_kernel(RelaxDev r, Lbfgs
   if (tid == 0) {
     r.ll[lane] = L;
     r.status[lane] = code;
-    if (code == ST_RUNNING && r.prec != nullptr && c.mixed_switch > 0.0 && L.f <= c.mixed_switch)
+    if (code == ST_RUNNING && r.prec != nullptr &&
+        (rescue || (c.mixed_switch > 0.0 && L.f <= c.mixed_switch)))
       r.prec[lane] = 0;
   }
 }
