@@ -94,8 +94,11 @@ __device__ inline void rng_unif_words(const unsigned key[2], unsigned c[4]) {
 // r <- r + dt/2 v, v <- c v + sqrt((1 - c^2) T) xi, r <- r + dt/2 v, with
 // (key', k_xi) = split(key) and xi the normal stream of k_xi in (j, a)
 // order.  Then s_eval = r inv(B).
-__global__ __launch_bounds__(kDynBlock) void md_prepare_kernel(RelaxDev R, MdCfg c) {
-  const int slot = blockIdx.x;
+// grid: slots_ub x chunks (workgroups per lane; one for baoab, whose draw keys
+// the lane's workgroup reads and then replaces).
+__global__ __launch_bounds__(kDynBlock) void md_prepare_kernel(RelaxDev R, MdCfg c, int chunks) {
+  const int slot = blockIdx.x / chunks;
+  const int part = blockIdx.x % chunks;
   if (slot >= *R.n_active) return;
   const int lane = R.active[slot];
   const size_t off = (size_t)lane * R.stride;
@@ -111,7 +114,7 @@ __global__ __launch_bounds__(kDynBlock) void md_prepare_kernel(RelaxDev R, MdCfg
     rng_split(L.key, 2, ks);
     amp = sqrt(c.ou_var * L.temperature);
   }
-  for (int j = threadIdx.x; j < c.n; j += blockDim.x) {
+  for (int j = part * blockDim.x + threadIdx.x; j < c.n; j += chunks * blockDim.x) {
     double rn[3];
     for (int a = 0; a < d; ++a) {
       const int e = j * d + a;
@@ -142,8 +145,31 @@ __global__ __launch_bounds__(kDynBlock) void md_prepare_kernel(RelaxDev R, MdCfg
   }
   // the step md_finish is about to complete is L.step + 1: a cadence step
   // sums the unconstrained waves (the same test as md_finish's)
-  if (c.trk_skip != nullptr && threadIdx.x == 0)
+  if (c.trk_skip != nullptr && part == 0 && threadIdx.x == 0)
     c.trk_skip[lane] = (c.wind_every == 1 || (L.step + 1) % c.wind_every == 0) ? 0 : 1;
+}
+
+// The second half kick's elementwise part, F = -g B^-T and v += dt/2 F
+// (grid: slots_ub x chunks); md_finish_kernel does the rest of the step.
+__global__ __launch_bounds__(kDynBlock) void md_kick_kernel(RelaxDev R, MdCfg c, int chunks) {
+  const int slot = blockIdx.x / chunks;
+  const int part = blockIdx.x % chunks;
+  if (slot >= *R.n_active) return;
+  const int lane = R.active[slot];
+  const size_t off = (size_t)lane * R.stride;
+  double* v = R.v + off;
+  double* f = R.f + off;
+  const double* g = R.g_eval + off;
+  const int d = c.d;
+  for (int j = part * blockDim.x + threadIdx.x; j < c.n; j += chunks * blockDim.x) {
+    for (int a = 0; a < d; ++a) {
+      double acc = 0.0;
+      for (int b = 0; b < d; ++b) acc = acc + g[j * d + b] * c.inv_t[b * d + a];
+      const double fa = -acc;
+      f[j * d + a] = fa;
+      v[j * d + a] = v[j * d + a] + c.half * fa;
+    }
+  }
 }
 
 // Second half kick with the new forces; andersen: reset one particle's
@@ -160,19 +186,8 @@ __global__ __launch_bounds__(kDynBlock) void md_finish_kernel(RelaxDev R, MdCfg 
   const size_t off = (size_t)lane * R.stride;
   double* r = R.r + off;
   double* v = R.v + off;
-  double* f = R.f + off;
-  const double* g = R.g_eval + off;
   MdLane L = R.ml[lane];
   const int d = c.d;
-  for (int j = threadIdx.x; j < c.n; j += blockDim.x) {
-    for (int a = 0; a < d; ++a) {
-      double acc = 0.0;
-      for (int b = 0; b < d; ++b) acc = acc + g[j * d + b] * c.inv_t[b * d + a];
-      const double fa = -acc;
-      f[j * d + a] = fa;
-      v[j * d + a] = v[j * d + a] + c.half * fa;
-    }
-  }
   if (c.scheme == 1) {
     unsigned ks[4][2];
     rng_split(L.key, 4, ks);
@@ -350,14 +365,20 @@ __global__ __launch_bounds__(kDynBlock) void md_waves_kernel(RelaxDev R, MdCfg c
   }
 }
 
+// Workgroups per lane of the elementwise MD kernels: one per kDynBlock particles.
+int md_chunks(const MdCfg& c) { return (c.n + kDynBlock - 1) / kDynBlock; }
+
 void launch_md_prepare(const RelaxDev& r, const MdCfg& c, int slots_ub, hipStream_t st) {
   if (slots_ub <= 0) return;
-  hipLaunchKernelGGL(md_prepare_kernel, dim3(slots_ub), dim3(kDynBlock), 0, st, r, c);
+  const int chunks = c.scheme == 2 ? 1 : md_chunks(c);
+  hipLaunchKernelGGL(md_prepare_kernel, dim3(slots_ub * chunks), dim3(kDynBlock), 0, st, r, c, chunks);
 }
 
 void launch_md_finish(const RelaxDev& r, const MdCfg& c, const MdObs& o, int slots_ub,
                       hipStream_t st) {
   if (slots_ub <= 0) return;
+  const int chunks = md_chunks(c);
+  hipLaunchKernelGGL(md_kick_kernel, dim3(slots_ub * chunks), dim3(kDynBlock), 0, st, r, c, chunks);
   hipLaunchKernelGGL(md_finish_kernel, dim3(slots_ub), dim3(kDynBlock), 0, st, r, c, o);
   if (c.w > 0) {
     const int n_wtiles = (c.w + kDynBlock - 1) / kDynBlock;
