@@ -1539,9 +1539,11 @@ __global__ __launch_bounds__(kLaneThreads) void mp_lane_kernel(Geo g0, Batch b) 
         for (int t = 0; t < K32; ++t) a8[t][q] = *(const half8_t*)(za + 32 * t + 8 * g16);
         if constexpr (K16 > 0) a4[q] = *(const half4_t*)(za + 32 * K32 + 4 * g16);
       }
-      double g1[4], g2[4], g3[4];
+      // the column terms of the lane's (at most 8) column tiles summed in FP32,
+      // then across the 16 columns of a row in FP64
+      float f1[4], f2[4], f3[4];
 #pragma unroll
-      for (int i = 0; i < 4; ++i) g1[i] = g2[i] = g3[i] = 0.0;
+      for (int i = 0; i < 4; ++i) f1[i] = f2[i] = f3[i] = 0.0f;
       for (int ct = 0; ct < CT; ++ct) {
         float4_t hre = z4, him = z4, h1re = z4, h1im = z4;
         const _Float16* qc = cpl + (size_t)(ct * 16 + m16) * CP;
@@ -1569,18 +1571,19 @@ __global__ __launch_bounds__(kLaneThreads) void mp_lane_kernel(Geo g0, Batch b) 
         for (int i = 0; i < 4; ++i) {
           const float2 B = mp_bcol<D>(TF, (wave & 3) * 16 + 4 * g16 + i, a2, sg2, a3, sg3);
           // Im(conj(B) H) = B.x H.im - B.y H.re
-          const double v = (double)(B.x * him[i] - B.y * hre[i]);
-          const double v1 = (double)(B.x * h1im[i] - B.y * h1re[i]);
-          g1[i] += v1;
-          if (D >= 2) g2[i] += (double)col.z * v;
-          if (D >= 3) g3[i] += (double)col.w * v;
+          const float v = B.x * him[i] - B.y * hre[i];
+          const float v1 = B.x * h1im[i] - B.y * h1re[i];
+          f1[i] += v1;
+          if (D >= 2) f2[i] += (float)col.z * v;
+          if (D >= 3) f3[i] += (float)col.w * v;
         }
       }
+      double g1[4], g2[4], g3[4];
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        g1[i] = row16_sum(g1[i]);
-        if (D >= 2) g2[i] = row16_sum(g2[i]);
-        if (D >= 3) g3[i] = row16_sum(g3[i]);
+        g1[i] = row16_sum((double)f1[i]);
+        g2[i] = D >= 2 ? row16_sum((double)f2[i]) : 0.0;
+        g3[i] = D >= 3 ? row16_sum((double)f3[i]) : 0.0;
       }
       if (m16 == 15) {
 #pragma unroll
