@@ -638,6 +638,8 @@ def main() -> int:
     ap.add_argument("--precision", choices=("fp64", "mixed"), default="fp64",
                     help="mixed: split-precision MFMA evaluations while E > --switch, FP64 below")
     ap.add_argument("--switch", type=float, default=1e-8, help="energy below which mixed lanes turn FP64")
+    ap.add_argument("--split-history", type=int, choices=(0, 1), default=0,
+                    help="split-precision L-BFGS lanes keep FP64 (s, y) pairs (0) or FP32 pairs (1)")
     ap.add_argument(
         "--advance-variant",
         type=int,
@@ -742,6 +744,7 @@ def main() -> int:
     _native.check(L.cco_relax_set_option(sess.handle, 1, args.advance_variant))
     _native.check(L.cco_relax_set_option(sess.handle, 2, args.shards))
     _native.check(L.cco_relax_set_option(sess.handle, 3, args.graph))
+    _native.check(L.cco_relax_set_option(sess.handle, 5, args.split_history))
     if choice is not None:
         tuning.apply(sess, choice)
         args.shards, args.graph = choice.shards, choice.graph
@@ -885,6 +888,7 @@ def main() -> int:
         ms_ = RelaxSession(energy, B, n, d, LBFGSSettings(energy_tol=tol, precision="mixed",
                                                            precision_switch=args.switch))
         _native.check(L.cco_relax_set_option(ms_.handle, 2, args.shards))
+        _native.check(L.cco_relax_set_option(ms_.handle, 5, args.split_history))
         for _ in range(2):
             ctx.synchronize()
             t0 = time.perf_counter()
@@ -897,7 +901,10 @@ def main() -> int:
         conv_m = int(np.sum(out_e.to_host() < tol))
         mixed = {"value": conv_m / t_mixed, "unit": "configs/s", "converged": f"{conv_m}/{B}",
                  "vs_fp64": (conv_m / t_mixed) / value, "switch": args.switch,
-                 "note": "split-precision rho and forces (fp16 hi+lo MFMA) until E <= switch, then FP64"}
+                 "split_history": "fp32" if args.split_history else "fp64",
+                 "note": "split-precision rho and forces (fp16 hi+lo MFMA) and FP32 L-BFGS pairs until "
+                         "E <= switch, then FP64" if args.split_history else
+                         "split-precision rho and forces (fp16 hi+lo MFMA) until E <= switch, then FP64"}
         ms_.close()
 
     mixed_bound = None

@@ -223,6 +223,13 @@ int cco_relax_counters(cco_relax* r, int64_t* out4);
 /* Read-only (cco_relax_get_option): graph replays since the session was
  * created. */
 #define CCO_OPT_GRAPH_LAUNCHES 4
+/* CCO_OPT_SPLIT_HISTORY: L-BFGS sessions in split precision on the
+ * register-resident advance: 0 (default) keeps every (s, y) pair in FP64;
+ * 1 stores a split-precision lane's pairs in FP32 (the two-loop arithmetic
+ * stays FP64), which halves the history the two-loop streams, and widens
+ * them to FP64 when the lane turns FP64 (a rescue empties them instead).
+ * FP64 lanes are unaffected.  Takes effect at the next init. */
+#define CCO_OPT_SPLIT_HISTORY 5
 int cco_relax_set_option(cco_relax* r, int key, int64_t value);
 /* The value in force: for CCO_OPT_ADVANCE_VARIANT the variant latched by the
  * last cco_relax_init (1, 2 or 3; 0 before the first init); for

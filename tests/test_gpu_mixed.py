@@ -206,6 +206,50 @@ def test_mixed_rescue_turns_failing_lanes_fp64(gpu_ctx, n, d, chi, b) -> None:
     assert np.all(e.energy_batch(out.s) < 1e-19)
 
 
+OPT_ADVANCE_VARIANT = 1  # CCO_OPT_ADVANCE_VARIANT
+OPT_SPLIT_HISTORY = 5  # CCO_OPT_SPLIT_HISTORY
+
+
+@pytest.mark.parametrize("n,d,chi,b", [(300, 2, 0.3, 12), (1001, 3, 0.2, 6), (1500, 3, 0.2, 4)])
+def test_mixed_fp32_history(gpu_ctx, n, d, chi, b) -> None:
+    """Split-precision lanes on the register two-loop with CCO_OPT_SPLIT_HISTORY = 1 keep
+    FP32 (s, y) pairs and widen them to FP64 when they turn FP64: every lane converges as
+    in FP64, with evaluation totals within 10% of the same relax on FP64 pairs (the
+    default).  The three shapes take
+    the 512-thread paired loads (N d = 600), the one-pair-per-element loads (3003) and the
+    eight-element loop (4500)."""
+    box, ks = _system(n, d, chi)
+    e = make_stealthy_energy(box, ks)
+    s0 = np.random.default_rng(23).uniform(size=(b, n, d))
+    tol = 9.99999968e-21
+    r64 = relax(e, s0, LBFGSSettings(energy_tol=tol))
+    L = _native.lib()
+    outs = {}
+    for hist in (1, 0):
+        sess = RelaxSession(e, b, n, d, LBFGSSettings(energy_tol=tol, precision="mixed"))
+        v = C.c_int64(-1)
+        _native.check(L.cco_relax_get_option(sess.handle, OPT_SPLIT_HISTORY, C.byref(v)))
+        assert v.value == 0  # the default: FP64 pairs
+        _native.check(L.cco_relax_set_option(sess.handle, OPT_SPLIT_HISTORY, hist))
+        sess.init(s0)
+        _native.check(L.cco_relax_get_option(sess.handle, OPT_ADVANCE_VARIANT, C.byref(v)))
+        assert v.value == 2  # the register two-loop
+        assert sess.precision_state() == (True, b)
+        sess.run(10_000_000, 64)
+        outs[hist] = sess.result()
+        assert sess.precision_state() == (True, 0)  # every lane turned FP64
+        assert L.cco_relax_set_option(sess.handle, OPT_SPLIT_HISTORY, 2) != 0
+        sess.close()
+    e32, e64 = int(outs[1].n_evals.sum()), int(outs[0].n_evals.sum())
+    print(f"N={n} d={d}: evaluations {int(r64.n_evals.sum())} FP64, {e64} split + FP64 pairs, "
+          f"{e32} split + FP32 pairs", flush=True)
+    for out in outs.values():
+        assert np.array_equal(r64.energy < tol, out.energy < tol)
+        assert np.all(out.energy < tol)
+        assert np.all(e.energy_batch(out.s) < 1e-19)
+    assert abs(e32 / e64 - 1.0) <= 0.10
+
+
 def test_mixed_session_switches_every_lane(gpu_ctx) -> None:
     box, ks = _system(200, 3, 0.2)
     e = make_stealthy_energy(box, ks)

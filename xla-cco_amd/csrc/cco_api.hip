@@ -621,7 +621,8 @@ struct cco_relax {
   double* flow_dt0 = nullptr;  // device copy of per-lane first steps
   RelaxDev dev;
   std::vector<DevBuf> bufs;
-  DevBuf part, cbox, rphi, out_s, out_scalars, pack, pack_idx, cpl, csig;
+  DevBuf part, cbox, rphi, out_s, out_scalars, pack, pack_idx, cpl, csig, hist32;
+  int split_history = 0;    // 1: split-precision lanes keep FP32 L-BFGS history
   int slots_ub = 0;
   int64_t steps = 0;
   int64_t lane_evals = 0;
@@ -878,7 +879,25 @@ int resolve_variant(const cco_relax* R) {
 // Per-lane precision flags for the run being initialised: split precision
 // when asked for, the k-set fits the MFMA kernels and the method uses it.
 // Device buffers of the split-precision C planes, when the run uses them.
+// FP32 history rows for the split-precision lanes of an L-BFGS run on the
+// register-resident two-loop (the other advance variants keep FP64 rows).
+int reserve_split_history(cco_relax* R) {
+  R->dev.S32 = nullptr;
+  R->dev.Y32 = nullptr;
+  if (R->dev.prec == nullptr || R->method != CCO_METHOD_LBFGS || R->variant_run != 2 ||
+      !R->split_history)
+    return CCO_OK;
+  const size_t n = R->vec_total * (size_t)R->lc.memory;
+  const size_t half = (n + 63) / 64 * 64;  // Y rows start 256-byte aligned
+  int rc = dev_reserve(R->hist32, sizeof(float) * 2 * half);
+  if (rc != CCO_OK) return rc;
+  R->dev.S32 = (float*)R->hist32.p;
+  R->dev.Y32 = R->dev.S32 + half;
+  return CCO_OK;
+}
+
 int reserve_mp_planes(cco_relax* R) {
+  if (int rc = reserve_split_history(R)) return rc;
   if (R->dev.prec == nullptr || R->uneven) return CCO_OK;  // uneven: one workgroup per lane, no planes in HBM
   int rc = dev_reserve(R->cpl, mp_buffer_bytes(R->E->geo, R->B, R->N));
   if (rc == CCO_OK) rc = dev_reserve(R->csig, sizeof(double) * (size_t)R->B);
@@ -1550,7 +1569,7 @@ int cco_relax_destroy(cco_relax* R) {
   for (DevBuf& b : R->bufs) dev_release(b);
   for (DevBuf& b : R->md_bufs) dev_release(b);
   if (R->trk) cco_energy_destroy(R->trk);
-  for (DevBuf* b : {&R->cpl, &R->csig, &R->part, &R->cbox, &R->rphi, &R->out_s, &R->out_scalars, &R->pack,
+  for (DevBuf* b : {&R->cpl, &R->csig, &R->hist32, &R->part, &R->cbox, &R->rphi, &R->out_s, &R->out_scalars, &R->pack,
                     &R->pack_idx, &R->un_lay, &R->un_geo, &R->ros_S, &R->ros_H, &R->ros_rhs,
                     &R->ros_dts, &R->ros_ipiv, &R->ros_info, &R->ros_cnt})
     dev_release(*b);
@@ -1874,6 +1893,10 @@ int cco_relax_get_option(cco_relax* R, int key, int64_t* value) {
     *value = R->graph_launches;
     return CCO_OK;
   }
+  if (key == CCO_OPT_SPLIT_HISTORY) {
+    *value = R->split_history;
+    return CCO_OK;
+  }
   return fail(CCO_ERR_ARG, "unknown option key");
 }
 
@@ -1898,6 +1921,11 @@ int cco_relax_set_option(cco_relax* R, int key, int64_t value) {
     if (value < 0 || value > 64) return fail(CCO_ERR_ARG, "graph steps must be 0 (off) to 64");
     if (value != R->graph_steps) drop_graphs(R);
     R->graph_steps = (int)value;
+    return CCO_OK;
+  }
+  if (key == CCO_OPT_SPLIT_HISTORY) {
+    if (value != 0 && value != 1) return fail(CCO_ERR_ARG, "split history must be 0 or 1");
+    R->split_history = (int)value;
     return CCO_OK;
   }
   return fail(CCO_ERR_ARG, "unknown option key");

@@ -185,6 +185,10 @@ struct RelaxDev {
   double* S;      // [B][m][stride]
   double* Y;      // [B][m][stride]
   double* rho;    // [B][m]
+  // FP32 (s, y) rows of split-precision lanes, same offsets as S and Y (null:
+  // every lane keeps FP64 rows); register-resident two-loop only
+  float* S32;
+  float* Y32;
   // L-BFGS in Gram form: per-lane start flag {1, count} and dot-product tables
   int2* dir;       // [B]
   double* gram;    // [B][gram_stride]: SY (m x m), YY (m x m), Sg, Yg, cs, cy (m), misc (8)

@@ -16,6 +16,8 @@
 // rounds exactly like its NumPy statement (x + a*u, q - a*y, ...).
 #pragma clang fp contract(off)
 
+#include <type_traits>
+
 #include "minimize_kernels.h"
 
 namespace cco {
@@ -726,9 +728,13 @@ __device__ __forceinline__ void reg_sum2(double& v0, double& v1, Red2& R) {
   v1 = partials_sum<W>(b + 1, 2);
 }
 
-template <int NT, int EPT, int V>
+// H32: a split-precision lane's (s, y) rows are FP32 (r.S32 / r.Y32, same
+// element offsets as the FP64 rows), which halves the bytes each pair streams;
+// the arithmetic stays FP64.
+template <int NT, int EPT, int V, bool H32>
 __device__ void lbfgs_start_reg(const RelaxDev& r, const LbfgsCfg& c, int lane, LbfgsLane& L,
                                 Red2& R, double* alphas, double* rhos) {
+  using HT = std::conditional_t<H32, float, double>;
   const int nd = rv_nd(r, lane);
   const int tid = threadIdx.x;
   const size_t off = rv_off(r, lane);
@@ -740,6 +746,17 @@ __device__ void lbfgs_start_reg(const RelaxDev& r, const LbfgsCfg& c, int lane, 
   double* u = r.u + off;
   double* S = r.S + rv_hist(r, lane, m);
   double* Y = r.Y + rv_hist(r, lane, m);
+  float* S32 = r.S32 + rv_hist(r, lane, m);  // dereferenced only when H32
+  float* Y32 = r.Y32 + rv_hist(r, lane, m);
+  const HT* Sh;
+  const HT* Yh;
+  if constexpr (H32) {
+    Sh = S32;
+    Yh = Y32;
+  } else {
+    Sh = S;
+    Yh = Y;
+  }
   double* rho = r.rho + (size_t)lane * m;
   const double rs = c.rescales[L.pass_idx];
   const int count = L.opt_count;
@@ -759,10 +776,19 @@ __device__ void lbfgs_start_reg(const RelaxDev& r, const LbfgsCfg& c, int lane, 
       q[e] = 0.0;
       if (i < nd) {
         const double gi = gx[i];
-        const double ds = x[i] - px[i];
-        const double dy = gi - pg[i];
-        Ss[i] = ds;
-        Ys[i] = dy;
+        double ds = x[i] - px[i];
+        double dy = gi - pg[i];
+        if constexpr (H32) {
+          // rho and gamma from the rounded pair the two-loop will read
+          const float fs = (float)ds, fy = (float)dy;
+          S32[(size_t)slot * nd + i] = fs;
+          Y32[(size_t)slot * nd + i] = fy;
+          ds = fs;
+          dy = fy;
+        } else {
+          Ss[i] = ds;
+          Ys[i] = dy;
+        }
         sy += dy * ds;
         yy += dy * dy;
         q[e] = gi;
@@ -801,7 +827,7 @@ __device__ void lbfgs_start_reg(const RelaxDev& r, const LbfgsCfg& c, int lane, 
     // and both loops run to a multiple of D, the extra pairs with zero
     // weight (which leaves q unchanged), so the loop body is straight-line
     // and each use waits for its own pair only, not for the pairs in flight.
-    constexpr int D = NT == 512 ? 4 : (EPT == 1 ? 6 : (EPT == 2 ? 4 : 2));
+    constexpr int D = NT == 512 ? (H32 ? 8 : 4) : (EPT == 1 ? 6 : (EPT == 2 ? 4 : 2));
     int ic[EPT];
     bool mk[EPT];
 #pragma unroll
@@ -811,16 +837,17 @@ __device__ void lbfgs_start_reg(const RelaxDev& r, const LbfgsCfg& c, int lane, 
       // V = 2 keeps pairs whole: an even element and its successor (nd is even)
       ic[e] = mk[e] ? i : (V == 2 ? nd - 2 + (e & 1) : nd - 1);
     }
-    double bs[D][EPT], by[D][EPT];
-    auto load_pair = [&](double (&ds)[EPT], double (&dy)[EPT], int t) {
+    HT bs[D][EPT], by[D][EPT];
+    auto load_pair = [&](HT (&ds)[EPT], HT (&dy)[EPT], int t) {
       const int sl = (midx + t) % m;
-      const double* Si = S + (size_t)sl * nd;
-      const double* Yi = Y + (size_t)sl * nd;
+      const HT* Si = Sh + (size_t)sl * nd;
+      const HT* Yi = Yh + (size_t)sl * nd;
       if constexpr (V == 2) {
+        using HT2 = std::conditional_t<H32, float2, double2>;
 #pragma unroll
         for (int e = 0; e < EPT; e += 2) {
-          const double2 a = *reinterpret_cast<const double2*>(Si + ic[e]);
-          const double2 b = *reinterpret_cast<const double2*>(Yi + ic[e]);
+          const HT2 a = *reinterpret_cast<const HT2*>(Si + ic[e]);
+          const HT2 b = *reinterpret_cast<const HT2*>(Yi + ic[e]);
           ds[e] = a.x;
           ds[e + 1] = a.y;
           dy[e] = b.x;
@@ -895,16 +922,16 @@ __device__ void lbfgs_start_reg(const RelaxDev& r, const LbfgsCfg& c, int lane, 
   } else {
     // One pair per reduction, one pair in flight (vectors of 8 elements per
     // thread leave no registers for more).
-    double bs[EPT], by[EPT];
-    auto load_into = [&](double (&ds)[EPT], double (&dy)[EPT], int t) {
+    HT bs[EPT], by[EPT];
+    auto load_into = [&](HT (&ds)[EPT], HT (&dy)[EPT], int t) {
       const int sl = (midx + t) % m;
-      const double* Si = S + (size_t)sl * nd;
-      const double* Yi = Y + (size_t)sl * nd;
+      const HT* Si = Sh + (size_t)sl * nd;
+      const HT* Yi = Yh + (size_t)sl * nd;
   #pragma unroll
       for (int e = 0; e < EPT; ++e) {
         const int i = reg_elem<NT, V>(tid, e);
-        ds[e] = i < nd ? Si[i] : 0.0;
-        dy[e] = i < nd ? Yi[i] : 0.0;
+        ds[e] = i < nd ? Si[i] : (HT)0;
+        dy[e] = i < nd ? Yi[i] : (HT)0;
       }
     };
     // first loop: newest -> oldest (k = 0 is age m - 1)
@@ -1011,6 +1038,7 @@ __global__ __launch_bounds__(NT) void lbfgs_advance_reg_kernel(RelaxDev r, Lbfgs
   int code = ST_RUNNING;
   bool start = false;
   const bool split = r.prec != nullptr && r.prec[lane] != 0;
+  const bool h32 = split && r.S32 != nullptr;
   bool rescue = false;
 
   if (L.mode == MODE_REEVAL) {
@@ -1190,13 +1218,35 @@ __global__ __launch_bounds__(NT) void lbfgs_advance_reg_kernel(RelaxDev r, Lbfgs
       }
     }
   }
-  if (start) lbfgs_start_reg<NT, EPT, V>(r, c, lane, L, R, alphas, rhos);
+  if (start) {
+    if (h32)
+      lbfgs_start_reg<NT, EPT, V, true>(r, c, lane, L, R, alphas, rhos);
+    else
+      lbfgs_start_reg<NT, EPT, V, false>(r, c, lane, L, R, alphas, rhos);
+  }
+  const bool to_fp64 = code == ST_RUNNING && r.prec != nullptr &&
+                       (rescue || (c.mixed_switch > 0.0 && L.f <= c.mixed_switch));
+  if (h32 && to_fp64 && !rescue) {
+    // the lane's later iterations read FP64 rows: widen its valid FP32 pairs
+    // (slots 0 .. min(count, m) - 1; a rescue empties the history instead)
+    const int nv = min(L.opt_count - 1, c.memory);
+    const size_t hb = rv_hist(r, lane, c.memory);
+    for (int t = 0; t < nv; ++t) {
+      const size_t row = hb + (size_t)t * nd;
+#pragma unroll
+      for (int e = 0; e < EPT; ++e) {
+        const int i = reg_elem<NT, V>(tid, e);
+        if (i < nd) {
+          r.S[row + i] = (double)r.S32[row + i];
+          r.Y[row + i] = (double)r.Y32[row + i];
+        }
+      }
+    }
+  }
   if (tid == 0) {
     r.ll[lane] = L;
     r.status[lane] = code;
-    if (code == ST_RUNNING && r.prec != nullptr &&
-        (rescue || (c.mixed_switch > 0.0 && L.f <= c.mixed_switch)))
-      r.prec[lane] = 0;
+    if (to_fp64) r.prec[lane] = 0;
   }
 }
 
