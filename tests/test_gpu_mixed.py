@@ -250,6 +250,32 @@ def test_mixed_fp32_history(gpu_ctx, n, d, chi, b) -> None:
     assert abs(e32 / e64 - 1.0) <= 0.10
 
 
+def test_mixed_fp32_history_shard_and_graph_invariant(gpu_ctx) -> None:
+    """FP32 (s, y) pairs are per lane: 64 lanes split over 1 or 3 shards, with or without
+    HIP-graph replays, give bit-identical positions, energies and evaluation counts."""
+    n, d, b = 300, 2, 64
+    box, ks = _system(n, d, 0.3)
+    e = make_stealthy_energy(box, ks)
+    s0 = np.random.default_rng(29).uniform(size=(b, n, d))
+    L = _native.lib()
+    outs = []
+    for shards, graph in ((1, 0), (3, 0), (3, 8)):
+        sess = RelaxSession(e, b, n, d, LBFGSSettings(energy_tol=9.99999968e-21, precision="mixed"))
+        _native.check(L.cco_relax_set_option(sess.handle, OPT_SPLIT_HISTORY, 1))
+        _native.check(L.cco_relax_set_option(sess.handle, 2, shards))
+        _native.check(L.cco_relax_set_option(sess.handle, 3, graph))
+        sess.init(s0)
+        sess.run(10_000_000, 64)
+        outs.append(sess.result())
+        assert sess.precision_state() == (True, 0)
+        sess.close()
+    for out in outs[1:]:
+        assert np.array_equal(out.s, outs[0].s)
+        assert np.array_equal(out.energy, outs[0].energy)
+        assert np.array_equal(out.n_evals, outs[0].n_evals)
+    assert np.all(outs[0].energy < 9.99999968e-21)
+
+
 def test_mixed_session_switches_every_lane(gpu_ctx) -> None:
     box, ks = _system(200, 3, 0.2)
     e = make_stealthy_energy(box, ks)
