@@ -98,11 +98,14 @@ def test_uneven_rejects_bad_input(gpu_ctx) -> None:
         relax_uneven(es, [s0[0].astype(np.float32), s0[1]], LBFGSSettings())
 
 
-def test_uneven_split_precision_success_and_switch(gpu_ctx) -> None:
+@pytest.mark.parametrize("split_history", [0, 1])
+def test_uneven_split_precision_success_and_switch(gpu_ctx, split_history: int) -> None:
     """Split precision in an uneven batch: lanes whose k-set and N fit the
     one-workgroup-per-lane kernel start split and turn FP64 below the switch
     (every lane of this batch fits); the success set equals the FP64 batch's,
-    and the converged configurations are checked in FP64."""
+    and the converged configurations are checked in FP64.  With split_history 1
+    the split lanes keep FP32 (s, y) pairs at their per-lane history offsets."""
+    from xla_cco import _native
     from xla_cco.minimize import UnevenSession
 
     specs = SMALL + [(1000, 2, 0.45), (300, 3, 0.2), (757, 2, 0.3)]
@@ -112,6 +115,7 @@ def test_uneven_split_precision_success_and_switch(gpu_ctx) -> None:
     st = LBFGSSettings(energy_tol=tol, precision="mixed")
     sess = UnevenSession(es, [np.shape(s) for s in s0], st)
     try:
+        _native.check(_native.lib().cco_relax_set_option(sess.handle, 5, split_history))  # CCO_OPT_SPLIT_HISTORY
         sess.init(s0)
         on, split = sess.precision_state()
         assert on and split == len(specs)
