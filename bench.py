@@ -319,6 +319,7 @@ def bench_uneven(args) -> int:
     if not args.no_mixed:
         ms_ = UnevenSession(energies, [(n, d) for n, d, _ in specs], LBFGSSettings(energy_tol=tol, precision="mixed"))
         _native.check(L.cco_relax_set_option(ms_.handle, 2, args.shards))
+        _native.check(L.cco_relax_set_option(ms_.handle, 5, args.split_history))
         for _ in range(2):
             ms_.init(inputs[args.warmup])
             split0 = ms_.precision_state()[1]
@@ -330,7 +331,8 @@ def bench_uneven(args) -> int:
         conv_m = sum(1 for r in ms_.result() if float(r.energy) < tol)
         ms_.close()
         mixed = {"value": conv_m / t_m, "unit": "configs/s", "converged": f"{conv_m}/{B}",
-                 "lanes_split_at_start": split0, "vs_fp64": (conv_m / t_m) / (converged / elapsed)}
+                 "lanes_split_at_start": split0, "vs_fp64": (conv_m / t_m) / (converged / elapsed),
+                 "split_history": "fp32" if args.split_history else "fp64"}
     line = {
         "metric": METRIC,
         "value": converged / elapsed,
