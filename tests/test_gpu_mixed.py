@@ -226,11 +226,11 @@ def test_mixed_fp32_history(gpu_ctx, n, d, chi, b) -> None:
     L = _native.lib()
     outs = {}
     for hist in (1, 0):
-        sess = RelaxSession(e, b, n, d, LBFGSSettings(energy_tol=tol, precision="mixed"))
+        sess = RelaxSession(e, b, n, d, LBFGSSettings(energy_tol=tol, precision="mixed",
+                                                      split_history="fp32" if hist else "fp64"))
         v = C.c_int64(-1)
         _native.check(L.cco_relax_get_option(sess.handle, OPT_SPLIT_HISTORY, C.byref(v)))
-        assert v.value == 0  # the default: FP64 pairs
-        _native.check(L.cco_relax_set_option(sess.handle, OPT_SPLIT_HISTORY, hist))
+        assert v.value == hist
         sess.init(s0)
         _native.check(L.cco_relax_get_option(sess.handle, OPT_ADVANCE_VARIANT, C.byref(v)))
         assert v.value == 2  # the register two-loop
@@ -304,6 +304,13 @@ def test_mixed_rejects_bad_settings(gpu_ctx) -> None:
         e.set_precision("fp32")
     with pytest.raises(ValueError):
         RelaxSession(e, 1, 16, 2, LBFGSSettings(precision="half"))
+    with pytest.raises(ValueError):
+        RelaxSession(e, 1, 16, 2, LBFGSSettings(precision="mixed", split_history="half"))
+    sess = RelaxSession(e, 1, 16, 2, LBFGSSettings(precision="mixed"))
+    v = C.c_int64(-1)
+    _native.check(_native.lib().cco_relax_get_option(sess.handle, OPT_SPLIT_HISTORY, C.byref(v)))
+    assert v.value == 0  # the default: FP64 pairs
+    sess.close()
 
 
 def test_mixed_md_energy_drift_vs_fp64(gpu_ctx) -> None:

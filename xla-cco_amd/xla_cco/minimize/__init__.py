@@ -148,6 +148,7 @@ class RelaxSession:
                 _native.check(
                     _native.lib().cco_relax_set_precision(h, 1, float(settings.precision_switch))
                 )
+            _apply_split_history(h, settings)
 
     def precision_state(self) -> Tuple[bool, int]:
         """(split precision enabled by the last init, lanes still split now)."""
@@ -301,6 +302,13 @@ def relax(
             tuning.restore_energy(energy, saved)
 
 
+def _apply_split_history(h, settings: LBFGSSettings) -> None:
+    """CCO_OPT_SPLIT_HISTORY from ``settings.split_history`` ("fp64" or "fp32")."""
+    if settings.split_history not in ("fp64", "fp32"):
+        raise ValueError(f"split_history must be 'fp64' or 'fp32', got {settings.split_history!r}")
+    _native.check(_native.lib().cco_relax_set_option(h, 5, 1 if settings.split_history == "fp32" else 0))
+
+
 class UnevenSession:
     """Device session over configurations of different N, d and k-set.
 
@@ -364,6 +372,8 @@ class UnevenSession:
         # kernel start split and turn FP64 below the switch; the others run FP64
         if isinstance(settings, LBFGSSettings) and settings.precision == "mixed":
             _native.check(_native.lib().cco_relax_set_precision(h, 1, float(settings.precision_switch)))
+        if isinstance(settings, LBFGSSettings):
+            _apply_split_history(h, settings)
 
     def precision_state(self) -> Tuple[bool, int]:
         """(split precision enabled by the last init, lanes still split now)."""

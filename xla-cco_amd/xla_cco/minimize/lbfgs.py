@@ -39,6 +39,9 @@ class LBFGSSettings:
     # (an extension: "fp64" keeps the reference's arithmetic)
     precision: str = "fp64"
     precision_switch: float = 1e-8
+    # (s, y) pairs of split-precision lanes: "fp64", or "fp32" (half the history
+    # bytes per two-loop on the register-resident advance, widened at the switch)
+    split_history: str = "fp64"
 
 
 def update_stagnation(f_new: float, f_old: float, stag: int) -> int:
