@@ -30,7 +30,8 @@ rounding level while every lane still stops on the same tests:
 for each candidate (best of two; the advance variants after ``memory`` steps, when
 the history is full), picks the fastest, and records it under a key of the
 workload's shape class: method, d, N, the k-count M, the lane count rounded up to a
-power of two, the precision, the L-BFGS memory and the exactness.  Choices live in
+power of two, the precision (with the split-precision pair storage), the L-BFGS memory
+and the exactness.  Choices live in
 memory and, when ``XLA_CCO_TUNING_CACHE`` names a file, in that JSON file across
 processes.
 """
@@ -92,6 +93,8 @@ def workload_key(energy: StealthyEnergy, b: int, n: int, d: int, settings: Setti
     """The shape class a choice is kept under."""
     method = "descent" if isinstance(settings, DescentSettings) else "lbfgs"
     precision = getattr(settings, "precision", "fp64")
+    if precision == "mixed" and getattr(settings, "split_history", "fp64") == "fp32":
+        precision = "mixed-fp32pairs"  # a different advance stream, timed apart
     memory = getattr(settings, "memory", 0)
     tail = "" if exact else "/rounding"
     return f"{method}/d{d}/n{n}/m{energy.m}/b{_pow2(b)}/{precision}/mem{memory}{tail}"
